@@ -1,0 +1,227 @@
+/*
+ * lsbflow.h -- C-ABI of the MI355X (gfx950) implementation of lsbflow's
+ * per-time-step hot path.
+ *
+ * Every entry point below replaces one Fortran module procedure of the
+ * reference (cited as file:line).  Arrays use the reference's own memory
+ * layout (Fortran column-major, real(8)):
+ *
+ *   spectral field  qf(nxpl, ny, nzp)   nxpl = nx reals = nx/2 complex (Re,Im)
+ *   physical field  q (nxpp, nypl, nzp) nxpp = nx + 2 (2 pad reals per x-row)
+ *   vector          (..., 3)            three fields back to back
+ *   tensor          (..., 3, 3)         element (l,m) is field l + 3*m
+ *
+ * with one rank (P = 1): nxpl = nx, nypl = ny, nzpl = nzp.
+ *
+ * Two tiers share one set of kernels:
+ *   tier A  lsb_<name>(ctx, host pointers ...)   synchronous drop-in: copies
+ *           in, runs the HIP kernels, copies out (correctness/parity path);
+ *   tier B  lsb_state_* / lsb_timestep           the whole step device
+ *           resident (throughput path), plus lsb_<name>_dev(ctx, device
+ *           pointers ..., stream) kernel-level entry points.
+ *
+ * Errors: every function returns 0 on success or a negative LSB_E* code;
+ * lsb_last_error() returns a message.  The reference has no return codes (it
+ * prints and continues or calls MPI_FINALIZE, horfft.f90:327-330); a Fortran
+ * shim turns a nonzero status into `error stop`.
+ *
+ * Pads of physical outputs are written as zero (the reference leaves FFTW
+ * junk there, horfft.f90:144); pads of physical inputs are ignored, as in the
+ * reference's r2c transform.
+ */
+#ifndef LSBFLOW_H
+#define LSBFLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSB_OK 0
+#define LSB_E_ARG (-1)
+#define LSB_E_HIP (-2)
+#define LSB_E_NOMEM (-3)
+#define LSB_E_STATE (-4)
+#define LSB_E_UNSUPPORTED (-5)
+#define LSB_E_NUMERIC (-6)
+
+#define LSB_FP64 0
+#define LSB_MIXED 1
+
+typedef struct lsb_ctx lsb_ctx;
+
+/* Snapshot of the module globals the hot path reads through `use`
+ * (common.f90:10-197: dim, runparam, parameters, sgsinput, time). */
+typedef struct lsb_config {
+    int nx, ny, nz;                 /* &dimen (common.f90:20)                  */
+    int mapping;                    /* &run  mapping 0..3 (init.f90:188-254)    */
+    int bctype;                     /* 0 homogeneous, 1 Spalart, 3 ceiling V(x) */
+    int forcing;                    /* must be 0 (sponge is outside this path)   */
+    int sgsflag, sgsmodel;          /* sgsterm.f90:21-35                        */
+    int dealiasflag;                /* 0 none, 1 3/2-rule, 2 spectral filter     */
+    int dealias_ref_compat;         /* 1: reference ky placement (dealiasing.f90:329,435) */
+    int ifrz;                       /* navierstokes.f90:53-59                    */
+    int nstart;                     /* 0: Euler/BDF2 start (navierstokes.f90:1507) */
+    int precision;                  /* LSB_FP64 or LSB_MIXED                     */
+    int device;                     /* HIP device ordinal                        */
+    double xlen, ylen, zlen, x0, u0, xnu, blfac, prtl;  /* &param               */
+    double a, b, cs, kc, filt_order, filt_amp;          /* &sgsin               */
+    double samp, istartbound;       /* ceiling amplitude, BC start time          */
+    double dt;                      /* constant time step (dtadapt = 0)          */
+} lsb_config;
+
+/* ---------------------------------------------------------------- context */
+
+/* Builds dims (common.f90:23-54), grid (init.f90:130-273), Chebyshev operators
+ * (chebyshev.f90:9-48, 204-270 as a matrix) and the FP64 eigen-setup
+ * (helmholtz.f90:13-126), and uploads them.  Equivalent of the driver's
+ * initialize() setup_fftw/setup_grid/setup_zder sequence (init.f90:36-97). */
+int lsb_create(const lsb_config* cfg, lsb_ctx** out);
+int lsb_destroy(lsb_ctx* ctx);
+const char* lsb_last_error(void);
+
+/* Override the snapshot with the driver's own module arrays (e.g. after
+ * read_grid/read_paral on a restart, io.f90:134-316).  Lengths: wavx nx,
+ * wavy ny, zpts/gf nzp, xpts nx; d/d2 nzp*nzp; diag nzm, smat/simat nzm*nzm. */
+int lsb_set_grid(lsb_ctx* ctx, const double* wavx, const double* wavy, const double* zpts,
+                 const double* gf, const double* xpts, double dx, double dy, double xl);
+int lsb_set_mats(lsb_ctx* ctx, const double* d, const double* d2, const double* diag,
+                 const double* smat, const double* simat);
+/* paral module (common.f90:151-159): ubar, ubarp, ubarpp, wbar, wbarpp, length nzp. */
+int lsb_set_paral(lsb_ctx* ctx, const double* ubar, const double* ubarp, const double* ubarpp,
+                  const double* wbar, const double* wbarpp);
+/* Host copies of the context's grid/operators (any pointer may be NULL). */
+int lsb_get_grid(const lsb_ctx* ctx, double* wavx, double* wavy, double* zpts, double* gf,
+                 double* xpts, double* dxdyxl /* [3] */);
+int lsb_get_mats(const lsb_ctx* ctx, double* dor, double* d, double* d2, double* mdz,
+                 double* md2, double* diag, double* smat, double* simat);
+
+/* --------------------------------------------- host-side setup (no GPU) */
+
+/* chebyshev.f90:9-48 build_cheb_zder_matrix(dor, d, d2): CGL matrix scaled 2/xl,
+ * d = diag(gf) dor, d2 = d d.  Arrays nzp*nzp, column-major. */
+int lsb_build_cheb_zder_matrix(int nzp, double xl, const double* gf, double* dor, double* d,
+                               double* d2);
+/* chebyshev.f90:204-270 dct_diff(n1, n2, n, y, dydx, p): DCT-I recurrence
+ * derivative along the last (z) dimension, applied p times.  Host FP64. */
+int lsb_dct_diff(int n1, int n2, int n, double xl, const double* gf, const double* y,
+                 double* dydx, int p);
+/* helmholtz.f90:13-126 diagonalize_matrix(D2, n, lamda, e, ei): D2 = E diag(lamda) E^-1
+ * (own Hessenberg + Francis QR + inverse iteration in extended precision;
+ * replaces LAPACK dggev/dgetrf/dgetri).  Column-major n*n. */
+int lsb_diagonalize_matrix(const double* D2, int n, double* lamda, double* e, double* ei);
+
+/* ------------------------------------------------------ tier A: modhorfft */
+
+/* horfft.f90:61-150 horfft(ain, ainf, is): is = -1 physical -> spectral (scaled
+ * 1/(nx*ny), kx-Nyquist dropped); is = +1 spectral -> physical.  nfields
+ * consecutive fields (the reference calls it per field). */
+int lsb_horfft(lsb_ctx* ctx, double* ain, double* ainf, int is, int nfields);
+/* horfft.f90:153-224 norm(uf, vf, wf, tempf). */
+int lsb_norm(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* tempf);
+/* horfft.f90:261-356 / 359-454 at P = 1: the layout reorder between the
+ * x-row (nxpp) and spectral (nxpl) strides. */
+int lsb_dataflip_xtoy(lsb_ctx* ctx, const double* x, double* xf);
+int lsb_dataflip_ytox(lsb_ctx* ctx, double* x, const double* xf);
+
+/* --------------------------------------------------- tier A: derivatives */
+
+int lsb_dfdx(lsb_ctx* ctx, const double* uf, double* dudxf);      /* derivatives.f90:261-280 */
+int lsb_dfdy(lsb_ctx* ctx, const double* uf, double* dudyf);      /* derivatives.f90:283-303 */
+int lsb_dfdz(lsb_ctx* ctx, const double* uf, double* dudzf);      /* derivatives.f90:112-132 */
+int lsb_d2fdz2(lsb_ctx* ctx, const double* uf, double* d2f);      /* derivatives.f90:161-188 */
+/* derivatives.f90:135-158 dpdz(n1, n2, n, u, dudz, p) with n = nzp (device). */
+int lsb_dpdz(lsb_ctx* ctx, int n1, int n2, const double* u, double* dudz, int p);
+int lsb_divuv(lsb_ctx* ctx, const double* uf, const double* vf, double* divf); /* :394-409 */
+int lsb_div(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+            double* divf);                                          /* :329-346 */
+int lsb_lapl(lsb_ctx* ctx, const double* uf, double* lapluf);     /* :306-326 */
+/* derivatives.f90:348-392 divcheck: mean |div u| in physical space. */
+int lsb_divcheck(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+                 double* diverr);
+
+/* ----------------------------------------------------- tier A: helmholtz */
+
+/* helmholtz.f90:130-208 solve_helmholtz(rhs, dtnui, n, diag, smat, simat), n = nzm. */
+int lsb_solve_helmholtz(lsb_ctx* ctx, double* rhs, double dtnui, int n, const double* diag,
+                        const double* smat, const double* simat);
+/* helmholtz.f90:211-358 solve_nhom_helmholtz(rhs, bc, dtnui, diag, smat, simat). */
+int lsb_solve_nhom_helmholtz(lsb_ctx* ctx, double* rhs, const double* bc, double dtnui,
+                             const double* diag, const double* smat, const double* simat);
+
+/* ---------------------------------------------------- tier A: sgs_models */
+
+/* sgsterm.f90:10-37 tau_sgs(ui, uiuj, duidxj, tauij), physical (nxpp,ny,nzp,3[,3]). */
+int lsb_tau_sgs(lsb_ctx* ctx, const double* ui, const double* uiuj, const double* duidxj,
+                double* tauij);
+int lsb_static_smagorinsky(lsb_ctx* ctx, double cs, const double* duidxj, double* tauij); /* :39-120 */
+int lsb_dynamic_smagorinsky(lsb_ctx* ctx, const double* ui, const double* uiuj,
+                            const double* duidxj, double* tauij);                     /* :123-280 */
+int lsb_sigma_model(lsb_ctx* ctx, const double* duidxj, double* tauij);               /* :283-423 */
+int lsb_interscale_model(lsb_ctx* ctx, const double* ui, double* tauij);              /* :426-467 */
+
+/* ------------------------------------------------------- tier A: filters */
+
+/* filters.f90:363-458 filter_q_3pt(q, q_filt, n4, n5, a, b, interpflag). */
+int lsb_filter_q_3pt(lsb_ctx* ctx, const double* q, double* q_filt, int n4, int n5, double a,
+                     double b, int interpflag);
+/* filters.f90:164-192 filterqspec(qf, nq) (in place). */
+int lsb_filterqspec(lsb_ctx* ctx, double* qf, int nq);
+/* filters.f90:281-335 specfilter_exp_xy(uf, uff, kc, p, amp, filtery). */
+int lsb_specfilter_exp_xy(lsb_ctx* ctx, const double* uf, double* uff, double kc, double p,
+                          double amp, int filtery);
+
+/* ------------------------------------------------------- tier A: dealias */
+
+/* dealiasing.f90:87-188 setup_dealiasing: returns nx_lg, ny_lg. */
+int lsb_setup_dealiasing(lsb_ctx* ctx, int* nx_lg, int* ny_lg);
+/* dealiasing.f90:262-447 dealiasing_2d_fftw(a, b, ab): padded product of two
+ * physical fields (ref_compat placement per lsb_config.dealias_ref_compat). */
+int lsb_dealiasing_2d_fftw(lsb_ctx* ctx, const double* a, const double* b, double* ab);
+
+/* ------------------------------------------------- tier A: step routines */
+
+/* navierstokes.f90:658-855 nlterm_skewsym(uif, tempf, nltermif, nltempf, sgsflag)
+ * for tempf == 0 (the temperature path is not restated; see lsb_state_set). */
+int lsb_nlterm_skewsym(lsb_ctx* ctx, const double* uif, double* nltermif);
+
+/* ----------------------------------------------------- tier B: the step */
+
+/* Core state (common.f90:162-175: uf, vf, wf, pf, pnf, unif, unmif, nlnif,
+ * nlnmif) and time module values; any history pointer may be NULL (zero).
+ * tempf must be zero everywhere (buoyancy/temperature is not part of this
+ * path; LSB_E_UNSUPPORTED otherwise). */
+int lsb_state_set(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+                  const double* pf, const double* pnf, const double* unif, const double* unmif,
+                  const double* nlnif, const double* nlnmif, int istep, double t);
+int lsb_state_get(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* pf, double* pnf,
+                  double* unif, double* unmif, double* nlnif, double* nlnmif, int* istep,
+                  double* t);
+/* navierstokes.f90:10-90 timestep() x nsteps (ifilt = 0, dtadapt = 0):
+ * navierstokes_update + norm + ifrz + time update, all on the device. */
+int lsb_timestep(lsb_ctx* ctx, int nsteps);
+/* navierstokes.f90:1157-1496 navierstokes_update (one step, no norm). */
+int lsb_navierstokes_update(lsb_ctx* ctx);
+/* Device pointers of the resident state (for in-HBM benchmarks): fields
+ * uf, vf, wf, pf (each nx*ny*nzp doubles). */
+int lsb_state_device_ptrs(lsb_ctx* ctx, void** uf, void** vf, void** wf, void** pf);
+/* Stream the context launches on (hipStream_t). */
+void* lsb_stream(lsb_ctx* ctx);
+
+/* ------------------------------------- device-pointer kernel entry points */
+/* Same semantics as the tier-A routines, on device memory, asynchronous on
+ * `stream` (NULL = the context stream). */
+int lsb_horfft_dev(lsb_ctx* ctx, double* ain, double* ainf, int is, int nfields, void* stream);
+int lsb_dfdz_dev(lsb_ctx* ctx, const double* uf, double* dudzf, int nfields, void* stream);
+int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields, void* stream);
+
+/* Kernel timing for roofline accounting: average device time (ms) of the
+ * last lsb_profile_* window per kernel class, and the launch counts. */
+int lsb_profile_begin(lsb_ctx* ctx);
+int lsb_profile_end(lsb_ctx* ctx, double* ms_fft, double* ms_zcontract, double* ms_helmholtz,
+                    double* ms_pointwise, int* n_fft, int* n_zcontract, int* n_helmholtz,
+                    int* n_pointwise);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSBFLOW_H */

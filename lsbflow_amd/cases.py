@@ -1,0 +1,225 @@
+"""Benchmark/parity configurations and their seeded initial conditions.
+
+The reference driver has no tanh base flow, no seeded perturbation and no
+steady ceiling suction/blowing profile (its ``ic=1`` hook at init.f90:443-445
+is empty and ``bound`` only knows bctype 1/2, navierstokes.f90:241-329).  This
+module is the driver-side patch that supplies them, identically for the HIP
+library and for the CPU oracle:
+
+* base flow  ``ubar = tanh(z/delta)`` with analytic ``ubarp``/``ubarpp``
+  (delta = 0.2), wall at z = 0 (reference index k = nzp), ceiling at z = zlen;
+* perturbation = curl of a random vector potential
+  ``A_c(x,y,z) = env(z) * sum c * T_n(xi) * exp(i(kx x + ky y))`` with N(0,1)
+  coefficients on |kx index| <= nx/8, |ky index| <= ny/8, Chebyshev degree
+  n <= nz/4 and ``env = (z(zlen-z))^2/(zlen/2)^4`` (A and dA/dz vanish at both
+  walls, so the velocity does too); divergence free to rounding because the
+  curl is formed with commuting spectral operators; scaled to an rms of 1e-3
+  over the physical grid;
+* ceiling profile (bctype 3): ``w(x, zlen) = samp*(-G(0.4 Lx) + G(0.7 Lx))``,
+  ``G(c) = exp(-((x - x0 - c)/(0.05 Lx))^2)`` on the reference's ``xpts``.
+
+The random numbers come from SplitMix64 + Box-Muller so that every consumer
+reproduces the same bits from the same seed.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, asdict, replace
+
+import numpy as np
+from numpy.polynomial import chebyshev as npcheb
+
+PI = 4.0 * math.atan(1.0)
+
+
+@dataclass
+class Case:
+    """Namelist values of one configuration (reference names; common.f90:20-107)."""
+
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    xlen: float = 2.0 * PI
+    ylen: float = PI
+    zlen: float = 2.0
+    x0: float = 0.0
+    u0: float = 1.0
+    xnu: float = 1.0e-3
+    mapping: int = 3
+    bctype: int = 3
+    samp: float = 0.0
+    sgsflag: int = 0
+    sgsmodel: int = 0
+    a: float = 1.0
+    b: float = 2.0
+    cs: float = 0.1
+    kc: float = 0.6
+    filt_order: float = 1.0
+    filt_amp: float = 20.0
+    dealiasflag: int = 2
+    dt: float = 1.0e-3
+    nsteps: int = 10
+    delta: float = 0.2
+    pert_rms: float = 1.0e-3
+    seed: int = 1234
+
+    @property
+    def nzp(self) -> int:
+        return self.nz + 1
+
+    @property
+    def points(self) -> int:
+        """Grid points per field (nx * nzp * ny)."""
+        return self.nx * self.nzp * self.ny
+
+    def with_(self, **kw) -> "Case":
+        return replace(self, **kw)
+
+    def as_dict(self) -> dict:
+        return asdict(self)
+
+
+# Names in BASELINE order: x = reference x (nx), y_wall = reference z (nzp points),
+# z_span = reference y (ny).  "512x193x256" is nx=512, nz=192, ny=256.
+CASES = {
+    "c1_dns": Case("c1_dns", nx=64, ny=32, nz=64, xnu=1.0e-3, dt=1.0e-3, nsteps=10),
+    "c2_dyn_les": Case("c2_dyn_les", nx=256, ny=128, nz=128, xnu=0.2 / 5.0e4, samp=0.3,
+                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=5.0e-4, nsteps=100),
+    "c4_sigma_les": Case("c4_sigma_les", nx=512, ny=256, nz=192, xnu=0.2 / 1.0e5, samp=0.3,
+                         sgsflag=1, sgsmodel=2, dt=2.5e-4, nsteps=100),
+    "c4_dyn_les": Case("c4_dyn_les", nx=512, ny=256, nz=192, xnu=0.2 / 1.0e5, samp=0.3,
+                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=2.5e-4, nsteps=100),
+}
+
+
+# ---------------------------------------------------------------- random numbers
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n successive SplitMix64 outputs (uint64) starting from ``seed``."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def normal_stream(seed: int, n: int) -> np.ndarray:
+    """n N(0,1) doubles by Box-Muller on SplitMix64 uniforms in (0, 1]."""
+    m = (n + 1) // 2
+    r = splitmix64(seed, 2 * m)
+    u = ((r >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[0::2], u[1::2]
+    rad = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m)
+    z[0::2] = rad * np.cos(2.0 * PI * u2)
+    z[1::2] = rad * np.sin(2.0 * PI * u2)
+    return z[:n]
+
+
+# ------------------------------------------------------------------ grid helpers
+def zpts_mapping3(case: Case) -> np.ndarray:
+    """init.f90:231-237 (mapping 3): z = zlen/2 + zlen/2 cos(pi k/nz), k = 0..nz."""
+    xl = case.zlen
+    nz = case.nz
+    return np.array([xl / 2.0 + xl / 2.0 * math.cos(PI * (k + 0.0) / float(nz)) for k in range(nz + 1)])
+
+
+def xpts(case: Case) -> np.ndarray:
+    """init.f90:153-156 (dx = xlen/(nx-1), reference quirk Q4)."""
+    dx = case.xlen / (case.nx - 1.0)
+    return np.array([case.x0 + i * dx for i in range(case.nx)])
+
+
+def base_flow(case: Case):
+    """ubar, ubarp, ubarpp on the Chebyshev points (analytic tanh profile)."""
+    z = zpts_mapping3(case)
+    d = case.delta
+    t = np.tanh(z / d)
+    s2 = 1.0 - t * t
+    return t, s2 / d, -2.0 * t * s2 / (d * d)
+
+
+def ceiling_profile(case: Case) -> np.ndarray:
+    """Physical ceiling w(x) for bctype 3 (length nx)."""
+    L = case.xlen
+    xr = xpts(case) - case.x0
+    g1 = np.exp(-(((xr - 0.4 * L) / (0.05 * L)) ** 2))
+    g2 = np.exp(-(((xr - 0.7 * L) / (0.05 * L)) ** 2))
+    return case.samp * (g2 - g1)
+
+
+def _spectral_to_physical(fh: np.ndarray, nx: int) -> np.ndarray:
+    """(nzp, ny, nx/2) complex coefficients -> (nzp, ny, nx) physical (horfft +1 convention)."""
+    X = np.fft.ifft(fh, axis=-2, norm="forward")
+    Xf = np.zeros(X.shape[:-1] + (nx // 2 + 1,), dtype=np.complex128)
+    Xf[..., : nx // 2] = X
+    return np.fft.irfft(Xf, n=nx, axis=-1, norm="forward")
+
+
+def initial_condition(case: Case):
+    """Seeded divergence-free perturbation -> (uf, vf, wf) spectral (nzp, ny, nx) float64."""
+    if case.mapping != 3:
+        raise ValueError("the seeded perturbation is defined for the channel mapping 3")
+    nx, ny, nz = case.nx, case.ny, case.nz
+    nzp = nz + 1
+    z = zpts_mapping3(case)
+    xl = case.zlen
+    xi = 2.0 * z / xl - 1.0
+    env = (z * (xl - z)) ** 2 / (xl / 2.0) ** 4
+    denv = 2.0 * z * (xl - z) * (xl - 2.0 * z) / (xl / 2.0) ** 4
+    mx, my, nc = max(1, nx // 8), max(1, ny // 8), max(1, nz // 4)
+    alpha = 2.0 * PI / case.xlen
+    beta = 2.0 * PI / case.ylen
+    jys = list(range(-my, my + 1))
+    ncoef = 3 * (mx + 1) * len(jys) * (nc + 1) * 2
+    g = normal_stream(case.seed, ncoef).reshape(3, mx + 1, len(jys), nc + 1, 2)
+    coef = g[..., 0] + 1j * g[..., 1]
+    A = np.zeros((3, nzp, ny, nx // 2), dtype=np.complex128)
+    dA = np.zeros_like(A)
+    for c in range(3):
+        for m in range(mx + 1):
+            for jj, jy in enumerate(jys):
+                cc = coef[c, m, jj]
+                p = npcheb.chebval(xi, cc)
+                dp = npcheb.chebval(xi, npcheb.chebder(cc)) * (2.0 / xl)
+                A[c, :, jy % ny, m] += env * p
+                dA[c, :, jy % ny, m] += denv * p + env * dp
+    # Hermitian symmetry of the kx = 0 column (real field)
+    for arr in (A, dA):
+        for jy in range(1, my + 1):
+            arr[:, :, (-jy) % ny, 0] = np.conj(arr[:, :, jy, 0])
+        arr[:, :, 0, 0] = arr[:, :, 0, 0].real
+    kx = (np.arange(nx // 2) * alpha)[None, None, :]
+    ky = np.array([(j if j < ny // 2 else j - ny) * beta for j in range(ny)])[None, :, None]
+    ax, ay, az = A
+    dax, day, daz = dA
+    uh = 1j * ky * az - day
+    vh = dax - 1j * kx * az
+    wh = 1j * kx * ay - 1j * ky * ax
+    phys = [_spectral_to_physical(f, nx) for f in (uh, vh, wh)]
+    rms = math.sqrt(sum(float(np.mean(p * p)) for p in phys) / 3.0)
+    s = case.pert_rms / rms
+    out = []
+    for f in (uh, vh, wh):
+        f = np.ascontiguousarray(f * s)
+        out.append(f.view(np.float64).reshape(nzp, ny, nx).copy())
+    return tuple(out)
+
+
+def helmholtz_rhs(nzp: int, nx: int, ny: int, nfields: int, seed: int, zlen: float = 2.0):
+    """C3 right-hand sides: Chebyshev coefficients N(0,1) exp(-0.1 n), to collocation.
+    Returns (nfields, nzp, ny, nx) float64 in the spectral layout."""
+    nz = nzp - 1
+    xi = np.array([math.cos(PI * k / nz) for k in range(nzp)])
+    ncol = nx * ny
+    g = normal_stream(seed, nfields * nzp * ncol).reshape(nfields, nzp, ncol)
+    g *= np.exp(-0.1 * np.arange(nzp))[None, :, None]
+    T = npcheb.chebvander(xi, nz)                      # (nzp, nzp): T_n(xi_k)
+    out = np.einsum("kn,fnc->fkc", T, g)
+    return np.ascontiguousarray(out.reshape(nfields, nzp, ny, nx))

@@ -1,0 +1,417 @@
+"""Host-side mirror of the reference's hot-path modules over the C-ABI.
+
+Method names and argument meanings follow the Fortran routines they replace
+(horfft.f90, derivatives.f90, chebyshev.f90, helmholtz.f90, sgsterm.f90,
+filters.f90, dealiasing.f90, navierstokes.f90).  Arrays are numpy float64 in
+the reference's memory layout:
+
+* spectral  ``(nzp, ny, nx)``      (Fortran ``qf(nxpl, ny, nzp)``)
+* physical  ``(nzp, ny, nx + 2)``  (Fortran ``q(nxpp, nypl, nzp)``)
+* vectors   ``(3, nzp, ...)``
+* tensors   ``(3, 3, nzp, ...)`` indexed ``[l][m]``; converted to the
+  reference's ``(..., 3, 3)`` order (element (l,m) = field l + 3m) at the call.
+
+Every call runs the HIP kernels; there is no CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import LsbConfig, check
+
+PI = 3.141592653589793
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _to_fortran_tensor(t):
+    """[l][m] numpy tensor -> contiguous (m, l, ...) buffer (element (l,m) at field l + 3m)."""
+    return np.ascontiguousarray(np.swapaxes(np.asarray(t, dtype=np.float64), 0, 1))
+
+
+def _from_fortran_tensor(buf):
+    return np.ascontiguousarray(np.swapaxes(buf, 0, 1))
+
+
+@dataclass
+class Config:
+    """Namelist snapshot passed to ``lsb_create`` (reference names)."""
+
+    nx: int
+    ny: int
+    nz: int
+    mapping: int = 3
+    bctype: int = 3
+    forcing: int = 0
+    sgsflag: int = 0
+    sgsmodel: int = 0
+    dealiasflag: int = 2
+    dealias_ref_compat: int = 1
+    ifrz: int = 0
+    nstart: int = 0
+    precision: int = 0
+    device: int = 0
+    xlen: float = 2.0 * PI
+    ylen: float = PI
+    zlen: float = 2.0
+    x0: float = 0.0
+    u0: float = 1.0
+    xnu: float = 1.0e-3
+    blfac: float = 8.0
+    prtl: float = 0.71
+    a: float = 1.0
+    b: float = 2.0
+    cs: float = 0.1
+    kc: float = 0.6
+    filt_order: float = 1.0
+    filt_amp: float = 20.0
+    samp: float = 0.0
+    istartbound: float = 0.0
+    dt: float = 1.0e-3
+
+    @classmethod
+    def from_case(cls, case, **kw) -> "Config":
+        keys = {f for f in cls.__dataclass_fields__}
+        d = {k: v for k, v in case.as_dict().items() if k in keys}
+        d.update(kw)
+        return cls(**d)
+
+    def to_c(self) -> LsbConfig:
+        c = LsbConfig()
+        for name, _ in LsbConfig._fields_:
+            setattr(c, name, getattr(self, name))
+        return c
+
+
+# ----------------------------------------------------------- host-only setup
+def build_cheb_zder_matrix(nzp: int, xl: float, gf):
+    """chebyshev.f90:9-48 -> (dor, d, d2) as (nzp, nzp) arrays [row, col]."""
+    gf = _f64(gf)
+    dor = np.zeros((nzp, nzp), order="F")
+    d = np.zeros((nzp, nzp), order="F")
+    d2 = np.zeros((nzp, nzp), order="F")
+    check(_lib.lib().lsb_build_cheb_zder_matrix(nzp, xl, _p(gf), _pf(dor), _pf(d), _pf(d2)), "build_cheb_zder_matrix")
+    return np.ascontiguousarray(dor), np.ascontiguousarray(d), np.ascontiguousarray(d2)
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def dct_diff(y, xl: float, gf, p: int):
+    """chebyshev.f90:204-270 along axis 0 of y (n, ...)."""
+    y = _f64(y)
+    n = y.shape[0]
+    P = int(np.prod(y.shape[1:])) if y.ndim > 1 else 1
+    out = np.empty_like(y)
+    check(_lib.lib().lsb_dct_diff(P, 1, n, xl, _p(_f64(gf)), _p(y), _p(out), p), "dct_diff")
+    return out
+
+
+def diagonalize_matrix(D2):
+    """helmholtz.f90:13-126: returns (lamda, E, E^-1) with E[:, i] the eigenvectors."""
+    D2 = np.asfortranarray(D2, dtype=np.float64)
+    n = D2.shape[0]
+    lam = np.zeros(n)
+    e = np.zeros((n, n), order="F")
+    ei = np.zeros((n, n), order="F")
+    check(_lib.lib().lsb_diagonalize_matrix(_pf(D2), n, _p(lam), _pf(e), _pf(ei)), "diagonalize_matrix")
+    return lam, np.ascontiguousarray(e), np.ascontiguousarray(ei)
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One HIP context: module globals snapshot + device operators + state."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self._L = _lib.lib()
+        h = C.c_void_p()
+        cc = cfg.to_c()
+        check(self._L.lsb_create(C.byref(cc), C.byref(h)), "lsb_create")
+        self.h = h
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        self.nzp = cfg.nz + 1
+        self.nzm = cfg.nz - 1
+        self.nxpp = cfg.nx + 2
+        self.sshape = (self.nzp, self.ny, self.nx)
+        self.pshape = (self.nzp, self.ny, self.nxpp)
+
+    def close(self):
+        if self.h:
+            self._L.lsb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------- snapshot
+    def set_paral(self, ubar, ubarp, ubarpp, wbar=None, wbarpp=None):
+        a = [None if x is None else _f64(x) for x in (ubar, ubarp, ubarpp, wbar, wbarpp)]
+        check(self._L.lsb_set_paral(self.h, *[_p(x) for x in a]), "set_paral")
+
+    def set_grid(self, wavx, wavy, zpts, gf, xpts, dx, dy, xl):
+        a = [_f64(x) for x in (wavx, wavy, zpts, gf, xpts)]
+        check(self._L.lsb_set_grid(self.h, *[_p(x) for x in a], dx, dy, xl), "set_grid")
+
+    def set_mats(self, d=None, d2=None, diag=None, smat=None, simat=None):
+        def fo(x):
+            return None if x is None else np.asfortranarray(x, dtype=np.float64)
+        a = [fo(d), fo(d2), None if diag is None else _f64(diag), fo(smat), fo(simat)]
+        check(self._L.lsb_set_mats(self.h, *[None if x is None else _pf(x) for x in a]), "set_mats")
+
+    def grid(self):
+        wavx, wavy = np.zeros(self.nx), np.zeros(self.ny)
+        zpts, gf, xpts = np.zeros(self.nzp), np.zeros(self.nzp), np.zeros(self.nx)
+        dd = np.zeros(3)
+        check(self._L.lsb_get_grid(self.h, _p(wavx), _p(wavy), _p(zpts), _p(gf), _p(xpts), _p(dd)), "get_grid")
+        return dict(wavx=wavx, wavy=wavy, zpts=zpts, gf=gf, xpts=xpts, dx=dd[0], dy=dd[1], xl=dd[2])
+
+    def mats(self):
+        n, m = self.nzp, self.nzm
+        arrs = {k: np.zeros((n, n), order="F") for k in ("dor", "d", "d2", "mdz", "md2")}
+        diag = np.zeros(m)
+        smat, simat = np.zeros((m, m), order="F"), np.zeros((m, m), order="F")
+        check(self._L.lsb_get_mats(self.h, *[_pf(arrs[k]) for k in ("dor", "d", "d2", "mdz", "md2")],
+                                   _p(diag), _pf(smat), _pf(simat)), "get_mats")
+        out = {k: np.ascontiguousarray(v) for k, v in arrs.items()}
+        out.update(diag=diag, smat=np.ascontiguousarray(smat), simat=np.ascontiguousarray(simat))
+        return out
+
+    # ------------------------------------------------------------ modhorfft
+    def horfft(self, ain, ainf, is_):
+        """horfft.f90:61-150 in place: is_=-1 fills ainf from ain, is_=+1 fills ain from ainf.
+        Leading axes beyond (nzp, ny, nx[pp]) are independent fields."""
+        assert ain.dtype == np.float64 and ainf.dtype == np.float64
+        assert ain.flags.c_contiguous and ainf.flags.c_contiguous
+        nf = int(np.prod(ain.shape[:-3])) if ain.ndim > 3 else 1
+        check(self._L.lsb_horfft(self.h, _p(ain), _p(ainf), is_, nf), "horfft")
+
+    def fwd(self, phys):
+        phys = _f64(phys)
+        out = np.zeros(phys.shape[:-1] + (self.nx,))
+        self.horfft(phys, out, -1)
+        return out
+
+    def inv(self, spec):
+        spec = _f64(spec)
+        out = np.zeros(spec.shape[:-1] + (self.nxpp,))
+        self.horfft(out, spec, 1)
+        return out
+
+    def norm(self, uf, vf, wf, tempf=None):
+        """horfft.f90:153-224 in place."""
+        check(self._L.lsb_norm(self.h, _p(uf), _p(vf), _p(wf), _p(tempf)), "norm")
+
+    # ---------------------------------------------------------- derivatives
+    def _unary(self, fn, uf):
+        uf = _f64(uf)
+        out = np.empty_like(uf)
+        check(fn(self.h, _p(uf), _p(out)), fn.__name__)
+        return out
+
+    def dfdx(self, uf):
+        return self._unary(self._L.lsb_dfdx, uf)
+
+    def dfdy(self, uf):
+        return self._unary(self._L.lsb_dfdy, uf)
+
+    def dfdz(self, uf):
+        return self._unary(self._L.lsb_dfdz, uf)
+
+    def d2fdz2(self, uf):
+        return self._unary(self._L.lsb_d2fdz2, uf)
+
+    def lapl(self, uf):
+        return self._unary(self._L.lsb_lapl, uf)
+
+    def dpdz(self, u, p):
+        """derivatives.f90:135-158 on (nzp, n2, n1) arrays."""
+        u = _f64(u)
+        out = np.empty_like(u)
+        n1 = u.shape[-1]
+        n2 = int(np.prod(u.shape[1:-1])) if u.ndim > 2 else 1
+        check(self._L.lsb_dpdz(self.h, n1, n2, _p(u), _p(out), p), "dpdz")
+        return out
+
+    def divuv(self, uf, vf):
+        uf, vf = _f64(uf), _f64(vf)
+        out = np.empty_like(uf)
+        check(self._L.lsb_divuv(self.h, _p(uf), _p(vf), _p(out)), "divuv")
+        return out
+
+    def div(self, uf, vf, wf):
+        uf, vf, wf = _f64(uf), _f64(vf), _f64(wf)
+        out = np.empty_like(uf)
+        check(self._L.lsb_div(self.h, _p(uf), _p(vf), _p(wf), _p(out)), "div")
+        return out
+
+    def divcheck(self, uf, vf, wf):
+        uf, vf, wf = _f64(uf), _f64(vf), _f64(wf)
+        r = C.c_double()
+        check(self._L.lsb_divcheck(self.h, _p(uf), _p(vf), _p(wf), C.byref(r)), "divcheck")
+        return r.value
+
+    # ------------------------------------------------------------ helmholtz
+    def solve_helmholtz(self, rhs, dtnui, diag=None, smat=None, simat=None):
+        out = np.array(rhs, dtype=np.float64, copy=True, order="C")
+        m = self.mats() if diag is None else None
+        diag = m["diag"] if diag is None else _f64(diag)
+        smat = np.asfortranarray(m["smat"] if smat is None else smat)
+        simat = np.asfortranarray(m["simat"] if simat is None else simat)
+        check(self._L.lsb_solve_helmholtz(self.h, _p(out), dtnui, self.nzm, _p(diag), _pf(smat), _pf(simat)),
+              "solve_helmholtz")
+        return out
+
+    def solve_nhom_helmholtz(self, rhs, bc, dtnui):
+        out = np.array(rhs, dtype=np.float64, copy=True, order="C")
+        bc = _f64(bc)
+        check(self._L.lsb_solve_nhom_helmholtz(self.h, _p(out), _p(bc), dtnui, None, None, None),
+              "solve_nhom_helmholtz")
+        return out
+
+    # ----------------------------------------------------------- sgs_models
+    def _tau(self, fn, *args):
+        out = np.zeros((3, 3) + self.pshape)
+        check(fn(self.h, *args, _p(out)), fn.__name__)
+        return _from_fortran_tensor(out)
+
+    def sigma_model(self, duidxj):
+        g = _to_fortran_tensor(duidxj)
+        return self._tau(self._L.lsb_sigma_model, _p(g))
+
+    def static_smagorinsky(self, cs, duidxj):
+        g = _to_fortran_tensor(duidxj)
+        out = np.zeros((3, 3) + self.pshape)
+        check(self._L.lsb_static_smagorinsky(self.h, cs, _p(g), _p(out)), "static_smagorinsky")
+        return _from_fortran_tensor(out)
+
+    def dynamic_smagorinsky(self, ui, uiuj, duidxj):
+        u = _f64(ui)
+        uu = _to_fortran_tensor(uiuj)
+        g = _to_fortran_tensor(duidxj)
+        return self._tau(self._L.lsb_dynamic_smagorinsky, _p(u), _p(uu), _p(g))
+
+    def interscale_model(self, ui):
+        return self._tau(self._L.lsb_interscale_model, _p(_f64(ui)))
+
+    def tau_sgs(self, ui, uiuj, duidxj):
+        u = _f64(ui)
+        uu = _to_fortran_tensor(uiuj)
+        g = _to_fortran_tensor(duidxj)
+        return self._tau(self._L.lsb_tau_sgs, _p(u), _p(uu), _p(g))
+
+    # -------------------------------------------------------------- filters
+    def filter_q_3pt(self, q, a, b, interpflag=1):
+        """filters.f90:363-458; q (..., nzp, ny, nxpp) with the leading axes as (n5, n4)."""
+        q = _f64(q)
+        nf = int(np.prod(q.shape[:-3])) if q.ndim > 3 else 1
+        out = np.zeros_like(q)
+        check(self._L.lsb_filter_q_3pt(self.h, _p(q), _p(out), nf, 1, a, b, interpflag), "filter_q_3pt")
+        return out
+
+    def specfilter_exp_xy(self, uf, kc, p, amp, filtery=1):
+        uf = _f64(uf)
+        out = np.empty_like(uf)
+        check(self._L.lsb_specfilter_exp_xy(self.h, _p(uf), _p(out), kc, p, amp, filtery), "specfilter_exp_xy")
+        return out
+
+    def filterqspec(self, qf):
+        out = np.array(qf, dtype=np.float64, copy=True, order="C")
+        nq = int(np.prod(out.shape[:-3])) if out.ndim > 3 else 1
+        check(self._L.lsb_filterqspec(self.h, _p(out), nq), "filterqspec")
+        return out
+
+    # --------------------------------------------------------------- dealias
+    def setup_dealiasing(self):
+        a, b = C.c_int(), C.c_int()
+        check(self._L.lsb_setup_dealiasing(self.h, C.byref(a), C.byref(b)), "setup_dealiasing")
+        return a.value, b.value
+
+    def dealiasing_2d_fftw(self, a, b):
+        a, b = _f64(a), _f64(b)
+        out = np.zeros_like(a)
+        check(self._L.lsb_dealiasing_2d_fftw(self.h, _p(a), _p(b), _p(out)), "dealiasing_2d_fftw")
+        return out
+
+    # ------------------------------------------------------------------ step
+    def nlterm_skewsym(self, uif):
+        uif = _f64(uif)
+        out = np.zeros_like(uif)
+        check(self._L.lsb_nlterm_skewsym(self.h, _p(uif), _p(out)), "nlterm_skewsym")
+        return out
+
+    def state_set(self, uf, vf, wf, pf=None, pnf=None, unif=None, unmif=None, nlnif=None, nlnmif=None, istep=0,
+                  t=0.0):
+        arrs = [None if x is None else _f64(x) for x in (uf, vf, wf, pf, pnf, unif, unmif, nlnif, nlnmif)]
+        check(self._L.lsb_state_set(self.h, *[_p(x) for x in arrs], istep, t), "state_set")
+
+    def state_get(self):
+        s = self.sshape
+        d = dict(uf=np.zeros(s), vf=np.zeros(s), wf=np.zeros(s), pf=np.zeros(s), pnf=np.zeros(s),
+                 unif=np.zeros((3,) + s), unmif=np.zeros((3,) + s), nlnif=np.zeros((3,) + s),
+                 nlnmif=np.zeros((3,) + s))
+        ist = C.c_int()
+        t = C.c_double()
+        keys = ("uf", "vf", "wf", "pf", "pnf", "unif", "unmif", "nlnif", "nlnmif")
+        check(self._L.lsb_state_get(self.h, *[_p(d[k]) for k in keys], C.byref(ist), C.byref(t)), "state_get")
+        d["istep"] = ist.value
+        d["t"] = t.value
+        return d
+
+    def timestep(self, nsteps=1):
+        check(self._L.lsb_timestep(self.h, nsteps), "timestep")
+
+    def navierstokes_update(self):
+        check(self._L.lsb_navierstokes_update(self.h), "navierstokes_update")
+
+    def device_ptrs(self):
+        ps = [C.c_void_p() for _ in range(4)]
+        check(self._L.lsb_state_device_ptrs(self.h, *[C.byref(p) for p in ps]), "state_device_ptrs")
+        return [p.value for p in ps]
+
+    def stream(self):
+        return self._L.lsb_stream(self.h)
+
+    def profile_begin(self):
+        check(self._L.lsb_profile_begin(self.h), "profile_begin")
+
+    def profile_end(self):
+        ms = [C.c_double() for _ in range(4)]
+        n = [C.c_int() for _ in range(4)]
+        check(self._L.lsb_profile_end(self.h, *[C.byref(x) for x in ms], *[C.byref(x) for x in n]), "profile_end")
+        keys = ("fft", "zcontract", "helmholtz", "pointwise")
+        return {k: (ms[i].value, n[i].value) for i, k in enumerate(keys)}
+
+
+def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:
+    """Context for a configuration of lsbflow_amd.cases with its base flow installed."""
+    from . import cases
+    cfg = Config.from_case(case, precision=precision, device=device, **kw)
+    ctx = Context(cfg)
+    ub, ubp, ubpp = cases.base_flow(case)
+    ctx.set_paral(ub, ubp, ubpp)
+    return ctx
+
+
+__all__ = ["Config", "Context", "build_cheb_zder_matrix", "dct_diff", "diagonalize_matrix", "setup_case", "field"]

@@ -1,0 +1,1075 @@
+// extern "C" entry points of include/lsbflow.h.
+//
+// Tier A (host pointers) stages the caller's arrays through device buffers the
+// context owns, runs the same kernels as tier B and synchronises before
+// returning (the reference's routines are synchronous, one rank per process).
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lsb_internal.h"
+#include "physical.h"
+#include "spectral.h"
+
+namespace lsb {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+int dev_alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return LSB_E_NOMEM;
+    }
+    return LSB_OK;
+}
+
+int upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    LSB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return LSB_OK;
+}
+
+// Staging buffer `slot` of at least `bytes` (grown on demand, kept for reuse).
+int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p) {
+    if ((int)c->stage.size() <= slot) {
+        c->stage.resize(slot + 1, nullptr);
+        c->stage_bytes.resize(slot + 1, 0);
+    }
+    if (c->stage_bytes[slot] < bytes) {
+        if (c->stage[slot]) {
+            LSB_HIP(hipStreamSynchronize(c->stream));
+            LSB_HIP(hipFree(c->stage[slot]));
+        }
+        c->stage[slot] = nullptr;
+        c->stage_bytes[slot] = 0;
+        LSB_TRY(dev_alloc(&c->stage[slot], bytes));
+        c->stage_bytes[slot] = bytes;
+    }
+    *p = c->stage[slot];
+    return LSB_OK;
+}
+
+int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token) {
+    *token = -1;
+    if (!c->profiling) return LSB_OK;
+    lsb_ctx::ProfEv ev;
+    ev.cls = cls;
+    if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return LSB_OK;
+    (void)hipEventRecord(ev.a, s);
+    c->prof.push_back(ev);
+    *token = (int)c->prof.size() - 1;
+    return LSB_OK;
+}
+int prof_mark_end(lsb_ctx* c, int token, hipStream_t s) {
+    if (token < 0 || !c->profiling) return LSB_OK;
+    (void)hipEventRecord(c->prof[token].b, s);
+    return LSB_OK;
+}
+
+static void pad_matrix(const std::vector<double>& src, int nrow, int ncol, int ldm, std::vector<double>& dst) {
+    dst.assign((size_t)ldm * ncol, 0.0);
+    for (int j = 0; j < ncol; ++j)
+        for (int i = 0; i < nrow; ++i) dst[(size_t)j * ldm + i] = src[(size_t)j * nrow + i];
+}
+
+// Upload grid-, matrix- and parameter-dependent device arrays.
+int upload_context(lsb_ctx* c) {
+    const int nzp = c->nzp, nzm = c->nzm;
+    hipStream_t s = c->stream;
+    auto put = [&](double** dptr, const std::vector<double>& h) -> int {
+        if (!*dptr) LSB_TRY(dev_alloc((void**)dptr, h.size() * sizeof(double)));
+        LSB_HIP(hipMemcpyAsync(*dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        return LSB_OK;
+    };
+    LSB_TRY(put(&c->d_wavx, c->wavx));
+    LSB_TRY(put(&c->d_wavy, c->wavy));
+    LSB_TRY(put(&c->d_zpts, c->zpts));
+    std::vector<double> t;
+    c->ldm_z = ((nzp + 63) / 64) * 64 + 64;
+    pad_matrix(c->mdz, nzp, nzp, c->ldm_z, t);
+    LSB_TRY(put(&c->d_mdz, t));
+    pad_matrix(c->md2, nzp, nzp, c->ldm_z, t);
+    LSB_TRY(put(&c->d_md2, t));
+    pad_matrix(c->d, nzp, nzp, c->ldm_z, t);
+    LSB_TRY(put(&c->d_d, t));
+    {   // rows 0 and nzp-1 of md2, ld 64
+        std::vector<double> b((size_t)64 * nzp, 0.0);
+        for (int j = 0; j < nzp; ++j) {
+            b[(size_t)j * 64 + 0] = c->md2[(size_t)j * nzp + 0];
+            b[(size_t)j * 64 + 1] = c->md2[(size_t)j * nzp + nzp - 1];
+        }
+        LSB_TRY(put(&c->d_md2b, b));
+    }
+    const int ldh = ((nzm + 63) / 64) * 64 + 64;
+    pad_matrix(c->smat, nzm, nzm, ldh, t);
+    LSB_TRY(put(&c->d_smat, t));
+    pad_matrix(c->simat, nzm, nzm, ldh, t);
+    LSB_TRY(put(&c->d_simat, t));
+    LSB_TRY(put(&c->d_diag, c->diag));
+    std::vector<double> lift;
+    lift.insert(lift.end(), c->hbot.begin(), c->hbot.end());
+    lift.insert(lift.end(), c->htop.begin(), c->htop.end());
+    lift.insert(lift.end(), c->d2hbot.begin(), c->d2hbot.end());
+    lift.insert(lift.end(), c->d2htop.begin(), c->d2htop.end());
+    LSB_TRY(put(&c->d_lift, lift));
+    std::vector<double> par;
+    par.insert(par.end(), c->ubar.begin(), c->ubar.end());
+    par.insert(par.end(), c->ubarp.begin(), c->ubarp.end());
+    par.insert(par.end(), c->ubarpp.begin(), c->ubarpp.end());
+    LSB_TRY(put(&c->d_paral, par));
+    std::vector<double> del;
+    del.insert(del.end(), c->delta_static.begin(), c->delta_static.end());
+    del.insert(del.end(), c->delta_dyn.begin(), c->delta_dyn.end());
+    del.insert(del.end(), c->delta_sigma.begin(), c->delta_sigma.end());
+    LSB_TRY(put(&c->d_delta, del));
+    std::vector<double> fz;
+    fz.insert(fz.end(), c->fz_ab.begin(), c->fz_ab.end());
+    fz.insert(fz.end(), c->fz_11.begin(), c->fz_11.end());
+    LSB_TRY(put(&c->d_fz, fz));
+    if (!c->d_smag) LSB_TRY(dev_alloc((void**)&c->d_smag, (size_t)nzp * sizeof(double)));
+    if (!c->d_red) LSB_TRY(dev_alloc((void**)&c->d_red, (size_t)(64 + 3 * nzp + 64) * sizeof(double)));
+    LSB_HIP(hipStreamSynchronize(s));
+    return LSB_OK;
+}
+
+static int check_ctx(const lsb_ctx* c) {
+    if (!c) {
+        set_error("null context");
+        return LSB_E_ARG;
+    }
+    return LSB_OK;
+}
+
+}  // namespace lsb
+
+using namespace lsb;
+
+// ============================================================== context
+extern "C" const char* lsb_last_error(void) { return g_err.c_str(); }
+
+extern "C" int lsb_create(const lsb_config* cfg, lsb_ctx** out) {
+    if (!cfg || !out) {
+        set_error("lsb_create: null argument");
+        return LSB_E_ARG;
+    }
+    *out = nullptr;
+    const lsb_config& g = *cfg;
+    if (g.nx < 4 || g.nx % 2 || g.ny < 2 || g.ny % 2 || g.nz < 4) {
+        set_error("lsb_create: need even nx >= 4, even ny >= 2, nz >= 4");
+        return LSB_E_ARG;
+    }
+    if (g.forcing != 0) {
+        set_error("lsb_create: forcing (sponge layer) is outside this path");
+        return LSB_E_UNSUPPORTED;
+    }
+    if (g.dealiasflag < 0 || g.dealiasflag > 2) {
+        set_error("lsb_create: dealiasflag must be 0, 1 or 2");
+        return LSB_E_UNSUPPORTED;
+    }
+    if (g.bctype != 0 && g.bctype != 1 && g.bctype != 3) {
+        set_error("lsb_create: bctype must be 0, 1 or 3");
+        return LSB_E_UNSUPPORTED;
+    }
+    lsb_ctx* c = new lsb_ctx();
+    c->cfg = g;
+    c->nx = g.nx;
+    c->ny = g.ny;
+    c->nz = g.nz;
+    c->nzp = g.nz + 1;
+    c->nzm = g.nz - 1;
+    c->nxh = g.nx / 2;
+    c->nyh = g.ny / 2;
+    c->nxpp = g.nx + 2;
+    c->plane_s = (size_t)g.nx * g.ny;
+    c->plane_p = (size_t)c->nxpp * g.ny;
+    c->nspec = c->plane_s * c->nzp;
+    c->nphys = c->plane_p * c->nzp;
+    c->dt = c->dt2 = c->dt3 = g.dt;
+    c->device = g.device;
+    int rc = build_grid(c);
+    if (rc == LSB_OK) rc = build_operators(c);
+    if (rc != LSB_OK) {
+        delete c;
+        return rc;
+    }
+    c->ubar.assign(c->nzp, 0.0);
+    c->ubarp.assign(c->nzp, 0.0);
+    c->ubarpp.assign(c->nzp, 0.0);
+    c->wbar.assign(c->nzp, 0.0);
+    c->wbarpp.assign(c->nzp, 0.0);
+    hipError_t e = hipSetDevice(g.device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("lsb_create: ") + hipGetErrorString(e));
+        delete c;
+        return LSB_E_HIP;
+    }
+    rc = hor_plan_create(&c->hp, c->nx, c->ny);
+    if (rc == LSB_OK) rc = upload_context(c);
+    if (rc != LSB_OK) {
+        lsb_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LSB_OK;
+}
+
+extern "C" int lsb_destroy(lsb_ctx* c) {
+    if (!c) return LSB_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    step_free(c);
+    double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
+                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red};
+    for (double* p : ds)
+        if (p) (void)hipFree(p);
+    for (void* p : c->stage)
+        if (p) (void)hipFree(p);
+    if (c->st) (void)hipFree(c->st);
+    hor_plan_destroy(&c->hp);
+    hor_plan_destroy(&c->hp_lg);
+    for (auto& ev : c->prof) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LSB_OK;
+}
+
+extern "C" int lsb_set_grid(lsb_ctx* c, const double* wavx, const double* wavy, const double* zpts, const double* gf,
+                            const double* xpts, double dx, double dy, double xl) {
+    LSB_TRY(check_ctx(c));
+    if (wavx) c->wavx.assign(wavx, wavx + c->nx);
+    if (wavy) c->wavy.assign(wavy, wavy + c->ny);
+    if (zpts) c->zpts.assign(zpts, zpts + c->nzp);
+    if (gf) c->gf.assign(gf, gf + c->nzp);
+    if (xpts) c->xpts.assign(xpts, xpts + c->nx);
+    c->dx = dx;
+    c->dy = dy;
+    c->xl = xl;
+    for (int i = 0; i < c->nx; ++i) c->wavxx[i] = c->wavx[i] * c->wavx[i];
+    for (int j = 0; j < c->ny; ++j) c->wavyy[j] = c->wavy[j] * c->wavy[j];
+    LSB_TRY(build_derived(c));
+    return upload_context(c);
+}
+
+extern "C" int lsb_set_mats(lsb_ctx* c, const double* d, const double* d2, const double* diag, const double* smat,
+                            const double* simat) {
+    LSB_TRY(check_ctx(c));
+    const size_t n2 = (size_t)c->nzp * c->nzp, m2 = (size_t)c->nzm * c->nzm;
+    if (d) c->d.assign(d, d + n2);
+    if (d2) c->d2.assign(d2, d2 + n2);
+    if (diag) c->diag.assign(diag, diag + c->nzm);
+    if (smat) c->smat.assign(smat, smat + m2);
+    if (simat) c->simat.assign(simat, simat + m2);
+    LSB_TRY(build_derived(c));
+    return upload_context(c);
+}
+
+extern "C" int lsb_set_paral(lsb_ctx* c, const double* ubar, const double* ubarp, const double* ubarpp,
+                             const double* wbar, const double* wbarpp) {
+    LSB_TRY(check_ctx(c));
+    const int n = c->nzp;
+    if (ubar) c->ubar.assign(ubar, ubar + n);
+    if (ubarp) c->ubarp.assign(ubarp, ubarp + n);
+    if (ubarpp) c->ubarpp.assign(ubarpp, ubarpp + n);
+    if (wbar) c->wbar.assign(wbar, wbar + n);
+    if (wbarpp) c->wbarpp.assign(wbarpp, wbarpp + n);
+    lifting_functions(c);
+    return upload_context(c);
+}
+
+extern "C" int lsb_get_grid(const lsb_ctx* c, double* wavx, double* wavy, double* zpts, double* gf, double* xpts,
+                            double* dxdyxl) {
+    LSB_TRY(check_ctx(c));
+    if (wavx) std::memcpy(wavx, c->wavx.data(), c->nx * sizeof(double));
+    if (wavy) std::memcpy(wavy, c->wavy.data(), c->ny * sizeof(double));
+    if (zpts) std::memcpy(zpts, c->zpts.data(), c->nzp * sizeof(double));
+    if (gf) std::memcpy(gf, c->gf.data(), c->nzp * sizeof(double));
+    if (xpts) std::memcpy(xpts, c->xpts.data(), c->nx * sizeof(double));
+    if (dxdyxl) {
+        dxdyxl[0] = c->dx;
+        dxdyxl[1] = c->dy;
+        dxdyxl[2] = c->xl;
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_get_mats(const lsb_ctx* c, double* dor, double* d, double* d2, double* mdz, double* md2,
+                            double* diag, double* smat, double* simat) {
+    LSB_TRY(check_ctx(c));
+    const size_t n2 = (size_t)c->nzp * c->nzp * sizeof(double), m2 = (size_t)c->nzm * c->nzm * sizeof(double);
+    if (dor) std::memcpy(dor, c->dor.data(), n2);
+    if (d) std::memcpy(d, c->d.data(), n2);
+    if (d2) std::memcpy(d2, c->d2.data(), n2);
+    if (mdz) std::memcpy(mdz, c->mdz.data(), n2);
+    if (md2) std::memcpy(md2, c->md2.data(), n2);
+    if (diag) std::memcpy(diag, c->diag.data(), c->nzm * sizeof(double));
+    if (smat) std::memcpy(smat, c->smat.data(), m2);
+    if (simat) std::memcpy(simat, c->simat.data(), m2);
+    return LSB_OK;
+}
+
+// ================================================= host-only setup routines
+extern "C" int lsb_build_cheb_zder_matrix(int nzp, double xl, const double* gf, double* dor, double* d, double* d2) {
+    if (nzp < 3 || !gf || !dor || !d || !d2) {
+        set_error("build_cheb_zder_matrix: bad arguments");
+        return LSB_E_ARG;
+    }
+    cheb_matrix(nzp, xl, gf, dor, d, d2);
+    return LSB_OK;
+}
+
+extern "C" int lsb_dct_diff(int n1, int n2, int n, double xl, const double* gf, const double* y, double* dydx, int p) {
+    if (n < 3 || n1 < 1 || n2 < 1 || !gf || !y || !dydx || p < 1) {
+        set_error("dct_diff: bad arguments");
+        return LSB_E_ARG;
+    }
+    std::vector<double> m((size_t)n * n);
+    dct_diff_matrix(n, xl, gf, p, m.data());
+    const size_t P = (size_t)n1 * n2;
+    std::vector<long double> acc(n);
+    for (size_t col = 0; col < P; ++col) {
+        for (int k = 0; k < n; ++k) {
+            long double s = 0;
+            for (int j = 0; j < n; ++j) s += (long double)m[(size_t)j * n + k] * (long double)y[col + P * j];
+            acc[k] = s;
+        }
+        for (int k = 0; k < n; ++k) dydx[col + P * k] = (double)acc[k];
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_diagonalize_matrix(const double* D2, int n, double* lamda, double* e, double* ei) {
+    if (!D2 || n < 1 || !lamda || !e || !ei) {
+        set_error("diagonalize_matrix: bad arguments");
+        return LSB_E_ARG;
+    }
+    return eigen_decompose(D2, n, lamda, e, ei);
+}
+
+// ================================================================ tier A
+namespace {
+
+struct Stage {
+    lsb_ctx* c;
+    explicit Stage(lsb_ctx* cc) : c(cc) {}
+    int get(int slot, size_t nreal, double** p) { return stage_get(c, slot, nreal * sizeof(double), (void**)p); }
+    int in(double* d, const double* h, size_t n) {
+        LSB_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        return LSB_OK;
+    }
+    int out(double* h, const double* d, size_t n) {
+        LSB_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        return LSB_OK;
+    }
+    int sync() {
+        LSB_HIP(hipStreamSynchronize(c->stream));
+        return LSB_OK;
+    }
+};
+
+int set_device(lsb_ctx* c) {
+    LSB_HIP(hipSetDevice(c->device));
+    return LSB_OK;
+}
+
+}  // namespace
+
+#define LSB_ENTER(c)          \
+    LSB_TRY(check_ctx(c));    \
+    LSB_TRY(set_device(c));   \
+    Stage st(c)
+
+extern "C" int lsb_horfft(lsb_ctx* c, double* ain, double* ainf, int is, int nfields) {
+    LSB_ENTER(c);
+    if (!ain || !ainf || (is != 1 && is != -1) || nfields < 1) {
+        set_error("horfft: bad arguments");
+        return LSB_E_ARG;
+    }
+    double *dp, *ds;
+    LSB_TRY(st.get(0, c->nphys * nfields, &dp));
+    LSB_TRY(st.get(1, c->nspec * nfields, &ds));
+    for (int f0 = 0; f0 < nfields; f0 += kMaxFields) {
+        const int nb = std::min(kMaxFields, nfields - f0);
+        const double* pin[kMaxFields];
+        double* pout[kMaxFields];
+        for (int q = 0; q < nb; ++q) {
+            pin[q] = (is == -1) ? dp + (size_t)(f0 + q) * c->nphys : ds + (size_t)(f0 + q) * c->nspec;
+            pout[q] = (is == -1) ? ds + (size_t)(f0 + q) * c->nspec : dp + (size_t)(f0 + q) * c->nphys;
+        }
+        if (is == -1) {
+            LSB_TRY(st.in(dp + (size_t)f0 * c->nphys, ain + (size_t)f0 * c->nphys, c->nphys * nb));
+            LSB_TRY(horfft_fwd(c, c->hp, pin, pout, nb, c->nzp, c->stream));
+            LSB_TRY(st.out(ainf + (size_t)f0 * c->nspec, ds + (size_t)f0 * c->nspec, c->nspec * nb));
+        } else {
+            LSB_TRY(st.in(ds + (size_t)f0 * c->nspec, ainf + (size_t)f0 * c->nspec, c->nspec * nb));
+            LSB_TRY(horfft_inv(c, c->hp, pin, pout, nb, c->nzp, c->stream));
+            LSB_TRY(st.out(ain + (size_t)f0 * c->nphys, dp + (size_t)f0 * c->nphys, c->nphys * nb));
+        }
+    }
+    return st.sync();
+}
+
+extern "C" int lsb_horfft_dev(lsb_ctx* c, double* ain, double* ainf, int is, int nfields, void* stream) {
+    LSB_TRY(check_ctx(c));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    for (int f0 = 0; f0 < nfields; f0 += kMaxFields) {
+        const int nb = std::min(kMaxFields, nfields - f0);
+        const double* pin[kMaxFields];
+        double* pout[kMaxFields];
+        for (int q = 0; q < nb; ++q) {
+            if (is == -1) {
+                pin[q] = ain + (size_t)(f0 + q) * c->nphys;
+                pout[q] = ainf + (size_t)(f0 + q) * c->nspec;
+            } else {
+                pin[q] = ainf + (size_t)(f0 + q) * c->nspec;
+                pout[q] = ain + (size_t)(f0 + q) * c->nphys;
+            }
+        }
+        if (is == -1)
+            LSB_TRY(horfft_fwd(c, c->hp, pin, pout, nb, c->nzp, s));
+        else
+            LSB_TRY(horfft_inv(c, c->hp, pin, pout, nb, c->nzp, s));
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_norm(lsb_ctx* c, double* uf, double* vf, double* wf, double* tempf) {
+    LSB_ENTER(c);
+    double* hs[4] = {uf, vf, wf, tempf};
+    double* d;
+    LSB_TRY(st.get(0, c->nspec * 4, &d));
+    double* dl[4];
+    int n = 0;
+    for (int q = 0; q < 4; ++q)
+        if (hs[q]) {
+            dl[n] = d + (size_t)n * c->nspec;
+            LSB_TRY(st.in(dl[n], hs[q], c->nspec));
+            ++n;
+        }
+    LSB_TRY(sp_norm(c, dl, n, c->stream));
+    n = 0;
+    for (int q = 0; q < 4; ++q)
+        if (hs[q]) {
+            LSB_TRY(st.out(hs[q], dl[n], c->nspec));
+            ++n;
+        }
+    return st.sync();
+}
+
+// At one rank the transposes are pure reorders between the x-row layout and
+// the spectral layout (horfft.f90:303-315, 333-345, 401-413, 431-443).
+extern "C" int lsb_dataflip_xtoy(lsb_ctx* c, const double* x, double* xf) {
+    LSB_TRY(check_ctx(c));
+    for (int k = 0; k < c->nzp; ++k)
+        for (int j = 0; j < c->ny; ++j)
+            std::memcpy(xf + ((size_t)k * c->ny + j) * c->nx, x + ((size_t)k * c->ny + j) * c->nxpp,
+                        c->nx * sizeof(double));
+    return LSB_OK;
+}
+extern "C" int lsb_dataflip_ytox(lsb_ctx* c, double* x, const double* xf) {
+    LSB_TRY(check_ctx(c));
+    for (int k = 0; k < c->nzp; ++k)
+        for (int j = 0; j < c->ny; ++j)
+            std::memcpy(x + ((size_t)k * c->ny + j) * c->nxpp, xf + ((size_t)k * c->ny + j) * c->nx,
+                        c->nx * sizeof(double));
+    return LSB_OK;
+}
+
+// generic one-in one-out spectral op
+template <class F>
+static int spec_unary(lsb_ctx* c, const double* in, double* out, F fn) {
+    LSB_ENTER(c);
+    if (!in || !out) {
+        set_error("null array");
+        return LSB_E_ARG;
+    }
+    double *a, *b;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.in(a, in, c->nspec));
+    LSB_TRY(fn(a, b));
+    LSB_TRY(st.out(out, b, c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_dfdx(lsb_ctx* c, const double* uf, double* dudxf) {
+    return spec_unary(c, uf, dudxf, [&](double* a, double* b) { return sp_dfdx(c, a, b, c->stream); });
+}
+extern "C" int lsb_dfdy(lsb_ctx* c, const double* uf, double* dudyf) {
+    return spec_unary(c, uf, dudyf, [&](double* a, double* b) { return sp_dfdy(c, a, b, c->stream); });
+}
+extern "C" int lsb_dfdz(lsb_ctx* c, const double* uf, double* dudzf) {
+    return spec_unary(c, uf, dudzf, [&](double* a, double* b) {
+        const double* in[1] = {a};
+        double* out[1] = {b};
+        return zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream);
+    });
+}
+extern "C" int lsb_d2fdz2(lsb_ctx* c, const double* uf, double* d2f) {
+    return spec_unary(c, uf, d2f, [&](double* a, double* b) {
+        const double* in[1] = {a};
+        double* out[1] = {b};
+        return zcontract(c, c->d_md2, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream);
+    });
+}
+extern "C" int lsb_dfdz_dev(lsb_ctx* c, const double* uf, double* dudzf, int nfields, void* stream) {
+    LSB_TRY(check_ctx(c));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    for (int f0 = 0; f0 < nfields; f0 += kMaxFields) {
+        const int nb = std::min(kMaxFields, nfields - f0);
+        const double* in[kMaxFields];
+        double* out[kMaxFields];
+        for (int q = 0; q < nb; ++q) {
+            in[q] = uf + (size_t)(f0 + q) * c->nspec;
+            out[q] = dudzf + (size_t)(f0 + q) * c->nspec;
+        }
+        LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, nb, s));
+    }
+    return LSB_OK;
+}
+
+// derivatives.f90:135-158 with n = nzp: dct_diff (n > 31) or matmul(d, .) once (any p).
+extern "C" int lsb_dpdz(lsb_ctx* c, int n1, int n2, const double* u, double* dudz, int p) {
+    LSB_ENTER(c);
+    if (!u || !dudz || n1 < 1 || n2 < 1 || (p != 1 && p != 2)) {
+        set_error("dpdz: bad arguments");
+        return LSB_E_ARG;
+    }
+    const size_t P = (size_t)n1 * n2, n = P * c->nzp;
+    double *a, *b;
+    LSB_TRY(st.get(0, n, &a));
+    LSB_TRY(st.get(1, n, &b));
+    LSB_TRY(st.in(a, u, n));
+    const double* M = (c->nzp > 31) ? (p == 1 ? c->d_mdz : c->d_md2) : c->d_d;
+    // contraction over planes of P columns: reuse the kernel with P = n1*n2
+    const size_t save = c->plane_s;
+    c->plane_s = P;
+    const double* in[1] = {a};
+    double* out[1] = {b};
+    int rc = zcontract(c, M, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream);
+    c->plane_s = save;
+    LSB_TRY(rc);
+    LSB_TRY(st.out(dudz, b, n));
+    return st.sync();
+}
+
+extern "C" int lsb_divuv(lsb_ctx* c, const double* uf, const double* vf, double* divf) {
+    LSB_ENTER(c);
+    double *a, *b, *o;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.get(2, c->nspec, &o));
+    LSB_TRY(st.in(a, uf, c->nspec));
+    LSB_TRY(st.in(b, vf, c->nspec));
+    LSB_TRY(sp_divuv(c, a, b, o, c->stream));
+    LSB_TRY(st.out(divf, o, c->nspec));
+    return st.sync();
+}
+
+__global__ void k_add(const double* a, const double* b, double* o, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        o[t] = a[t] + b[t];
+}
+
+extern "C" int lsb_div(lsb_ctx* c, const double* uf, const double* vf, const double* wf, double* divf) {
+    LSB_ENTER(c);
+    double *a, *b, *w, *o, *t;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.get(2, c->nspec, &w));
+    LSB_TRY(st.get(3, c->nspec, &o));
+    LSB_TRY(st.get(4, c->nspec, &t));
+    LSB_TRY(st.in(a, uf, c->nspec));
+    LSB_TRY(st.in(b, vf, c->nspec));
+    LSB_TRY(st.in(w, wf, c->nspec));
+    const double* in[1] = {w};
+    double* out[1] = {o};
+    LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream));  // divf = dfdz(w)
+    LSB_TRY(sp_divuv(c, a, b, t, c->stream));                                          // dqf
+    const long n = (long)c->nspec;
+    hipLaunchKernelGGL(k_add, dim3((unsigned)std::min<long>((n + 255) / 256, 65535L * 8)), dim3(256), 0, c->stream, t,
+                       o, o, n);  // divf = dqf + divf
+    LSB_HIP(hipGetLastError());
+    LSB_TRY(st.out(divf, o, c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_lapl(lsb_ctx* c, const double* uf, double* lapluf) {
+    LSB_ENTER(c);
+    double *a, *b, *o;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.get(2, c->nspec, &o));
+    LSB_TRY(st.in(a, uf, c->nspec));
+    const double* in[1] = {a};
+    double* out[1] = {b};
+    LSB_TRY(zcontract(c, c->d_md2, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream));
+    LSB_TRY(sp_lapl_combine(c, b, a, o, c->stream));
+    LSB_TRY(st.out(lapluf, o, c->nspec));
+    return st.sync();
+}
+
+__global__ void k_abs_sum(const double* __restrict__ p, int nx, int nxpp, long nrows, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    const long n = nrows * nx;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const long r = t / nx;
+        const int i = (int)(t - r * nx);
+        s += fabs(p[r * nxpp + i]);
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+extern "C" int lsb_divcheck(lsb_ctx* c, const double* uf, const double* vf, const double* wf, double* diverr) {
+    LSB_ENTER(c);
+    if (!diverr) return LSB_E_ARG;
+    double *a, *b, *w, *t1, *t2, *ph;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.get(2, c->nspec, &w));
+    LSB_TRY(st.get(3, c->nspec, &t1));
+    LSB_TRY(st.get(4, c->nspec, &t2));
+    LSB_TRY(st.get(5, c->nphys, &ph));
+    LSB_TRY(st.in(a, uf, c->nspec));
+    LSB_TRY(st.in(b, vf, c->nspec));
+    LSB_TRY(st.in(w, wf, c->nspec));
+    LSB_TRY(sp_dfdx(c, a, t1, c->stream));
+    LSB_TRY(sp_dfdy(c, b, t2, c->stream));
+    const long n = (long)c->nspec;
+    const dim3 g((unsigned)std::min<long>((n + 255) / 256, 65535L * 8));
+    hipLaunchKernelGGL(k_add, g, dim3(256), 0, c->stream, t1, t2, t1, n);
+    const double* in[1] = {w};
+    double* out[1] = {t2};
+    LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 1, c->stream));
+    hipLaunchKernelGGL(k_add, g, dim3(256), 0, c->stream, t1, t2, t1, n);
+    const double* si[1] = {t1};
+    double* so[1] = {ph};
+    LSB_TRY(horfft_inv(c, c->hp, si, so, 1, c->nzp, c->stream));
+    const int nb = 256;
+    hipLaunchKernelGGL(k_abs_sum, dim3(nb), dim3(256), 0, c->stream, ph, c->nx, c->nxpp, (long)c->nzp * c->ny,
+                       c->d_red);
+    std::vector<double> part(nb);
+    LSB_HIP(hipMemcpyAsync(part.data(), c->d_red, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LSB_TRY(st.sync());
+    double s = 0.0;
+    for (double v : part) s += v;
+    *diverr = s / ((double)c->nx * c->ny * c->nzp);
+    return LSB_OK;
+}
+
+// ------------------------------------------------------------- helmholtz
+static int check_mats(lsb_ctx* c, int n, const double* diag, const double* smat, const double* simat) {
+    if (n != c->nzm) {
+        set_error("solve_helmholtz: n must be nzm");
+        return LSB_E_ARG;
+    }
+    const size_t m2 = (size_t)n * n;
+    bool same = (!diag || std::memcmp(diag, c->diag.data(), n * sizeof(double)) == 0) &&
+                (!smat || std::memcmp(smat, c->smat.data(), m2 * sizeof(double)) == 0) &&
+                (!simat || std::memcmp(simat, c->simat.data(), m2 * sizeof(double)) == 0);
+    if (!same) return lsb_set_mats(c, nullptr, nullptr, diag, smat, simat);
+    return LSB_OK;
+}
+
+extern "C" int lsb_solve_helmholtz(lsb_ctx* c, double* rhs, double dtnui, int n, const double* diag,
+                                   const double* smat, const double* simat) {
+    LSB_ENTER(c);
+    LSB_TRY(check_mats(c, n, diag, smat, simat));
+    double* a;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.in(a, rhs, c->nspec));
+    double* r[1] = {a};
+    LSB_TRY(helmholtz(c, r, nullptr, nullptr, 1, dtnui, c->stream));
+    LSB_TRY(st.out(rhs, a, c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_solve_helmholtz_dev(lsb_ctx* c, double* rhs, double dtnui, int nfields, void* stream) {
+    LSB_TRY(check_ctx(c));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    for (int f0 = 0; f0 < nfields; f0 += kMaxFields) {
+        const int nb = std::min(kMaxFields, nfields - f0);
+        double* r[kMaxFields];
+        for (int q = 0; q < nb; ++q) r[q] = rhs + (size_t)(f0 + q) * c->nspec;
+        LSB_TRY(helmholtz(c, r, nullptr, nullptr, nb, dtnui, s));
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_solve_nhom_helmholtz(lsb_ctx* c, double* rhs, const double* bc, double dtnui, const double* diag,
+                                        const double* smat, const double* simat) {
+    LSB_ENTER(c);
+    LSB_TRY(check_mats(c, c->nzm, diag, smat, simat));
+    double *a, *b;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.in(a, rhs, c->nspec));
+    LSB_TRY(st.in(b, bc, c->nspec));
+    double* r[1] = {a};
+    const double* bt[1] = {b};
+    const double* bb[1] = {b + (size_t)(c->nzp - 1) * c->plane_s};
+    LSB_TRY(helmholtz(c, r, bt, bb, 1, dtnui, c->stream));
+    LSB_TRY(st.out(rhs, a, c->nspec));
+    return st.sync();
+}
+
+// ------------------------------------------------------------ sgs_models
+
+static int tensor_in(Stage& st, lsb_ctx* c, int slot, const double* h, int nf, double** d) {
+    LSB_TRY(st.get(slot, c->nphys * nf, d));
+    if (h) LSB_TRY(st.in(*d, h, c->nphys * nf));
+    return LSB_OK;
+}
+
+extern "C" int lsb_sigma_model(lsb_ctx* c, const double* duidxj, double* tauij) {
+    LSB_ENTER(c);
+    double *g, *nlt, *u, *t;
+    LSB_TRY(tensor_in(st, c, 0, duidxj, 9, &g));
+    LSB_TRY(st.get(1, c->nphys * 3, &nlt));
+    LSB_TRY(st.get(2, c->nphys * 3, &u));
+    LSB_TRY(st.get(3, c->nphys * 9, &t));
+    LSB_HIP(hipMemsetAsync(u, 0, c->nphys * 3 * sizeof(double), c->stream));
+    // zero base flow and u: h = 1/2 * 0 + tau = tau
+    double* zero;
+    LSB_TRY(st.get(4, 3 * (size_t)c->nzp, &zero));
+    LSB_HIP(hipMemsetAsync(zero, 0, 3 * (size_t)c->nzp * sizeof(double), c->stream));
+    NlArgs a{};
+    for (int l = 0; l < 3; ++l) {
+        a.ui[l] = u + l * c->nphys;
+        a.nlt[l] = nlt + l * c->nphys;
+    }
+    for (int q = 0; q < 9; ++q) {
+        a.g[q] = g + q * c->nphys;
+        a.h[q] = t + q * c->nphys;
+    }
+    a.ubar = zero;
+    a.ubarp = zero;
+    a.delta = c->d_delta + 2 * c->nzp;
+    LSB_TRY(ph_nonlinear(c, NL_SIGMA, a, c->stream));
+    double* tl[9];
+    for (int q = 0; q < 9; ++q) tl[q] = t + q * c->nphys;
+    LSB_TRY(ph_zero_pads(c, tl, 9, c->stream));
+    LSB_TRY(st.out(tauij, t, c->nphys * 9));
+    return st.sync();
+}
+
+extern "C" int lsb_static_smagorinsky(lsb_ctx* c, double cs, const double* duidxj, double* tauij) {
+    LSB_ENTER(c);
+    double *g, *nlt, *u, *t, *zero;
+    LSB_TRY(tensor_in(st, c, 0, duidxj, 9, &g));
+    LSB_TRY(st.get(1, c->nphys * 3, &nlt));
+    LSB_TRY(st.get(2, c->nphys * 3, &u));
+    LSB_TRY(st.get(3, c->nphys * 9, &t));
+    LSB_TRY(st.get(4, 3 * (size_t)c->nzp, &zero));
+    LSB_HIP(hipMemsetAsync(u, 0, c->nphys * 3 * sizeof(double), c->stream));
+    LSB_HIP(hipMemsetAsync(zero, 0, 3 * (size_t)c->nzp * sizeof(double), c->stream));
+    LSB_TRY(ph_smag_factor(c, g + 6 * c->nphys, nullptr, cs, c->stream));
+    NlArgs a{};
+    for (int l = 0; l < 3; ++l) {
+        a.ui[l] = u + l * c->nphys;
+        a.nlt[l] = nlt + l * c->nphys;
+    }
+    for (int q = 0; q < 9; ++q) {
+        a.g[q] = g + q * c->nphys;
+        a.h[q] = t + q * c->nphys;
+    }
+    a.ubar = zero;
+    a.ubarp = zero;
+    a.smag = c->d_smag;
+    LSB_TRY(ph_nonlinear(c, NL_STATIC, a, c->stream));
+    double* tl[9];
+    for (int q = 0; q < 9; ++q) tl[q] = t + q * c->nphys;
+    LSB_TRY(ph_zero_pads(c, tl, 9, c->stream));
+    LSB_TRY(st.out(tauij, t, c->nphys * 9));
+    return st.sync();
+}
+
+extern "C" int lsb_dynamic_smagorinsky(lsb_ctx* c, const double* ui, const double* uiuj, const double* duidxj,
+                                       double* tauij) {
+    LSB_ENTER(c);
+    double *u, *uu, *g, *t, *w;
+    LSB_TRY(tensor_in(st, c, 0, ui, 3, &u));
+    LSB_TRY(tensor_in(st, c, 1, uiuj, 9, &uu));
+    LSB_TRY(tensor_in(st, c, 2, duidxj, 9, &g));
+    LSB_TRY(st.get(3, c->nphys * 9, &t));
+    LSB_TRY(st.get(4, c->nphys * (1 + 9 * 5 + 3 + 4 + 16), &w));
+    LSB_TRY(dyn_full(c, u, uu, g, t, w, c->stream));
+    double* tl[9];
+    for (int q = 0; q < 9; ++q) tl[q] = t + q * c->nphys;
+    LSB_TRY(ph_zero_pads(c, tl, 9, c->stream));
+    LSB_TRY(st.out(tauij, t, c->nphys * 9));
+    return st.sync();
+}
+
+extern "C" int lsb_interscale_model(lsb_ctx* c, const double* ui, double* tauij) {
+    LSB_ENTER(c);
+    double *u, *t, *w;
+    LSB_TRY(tensor_in(st, c, 0, ui, 3, &u));
+    LSB_TRY(st.get(3, c->nphys * 9, &t));
+    LSB_TRY(st.get(4, c->nphys * 40, &w));
+    LSB_TRY(isc_full(c, u, t, w, c->stream));
+    double* tl[9];
+    for (int q = 0; q < 9; ++q) tl[q] = t + q * c->nphys;
+    LSB_TRY(ph_zero_pads(c, tl, 9, c->stream));
+    LSB_TRY(st.out(tauij, t, c->nphys * 9));
+    return st.sync();
+}
+
+extern "C" int lsb_tau_sgs(lsb_ctx* c, const double* ui, const double* uiuj, const double* duidxj, double* tauij) {
+    LSB_TRY(check_ctx(c));
+    switch (c->cfg.sgsmodel) {
+        case 0: return lsb_static_smagorinsky(c, c->cfg.cs, duidxj, tauij);
+        case 1: return lsb_dynamic_smagorinsky(c, ui, uiuj, duidxj, tauij);
+        case 2: return lsb_sigma_model(c, duidxj, tauij);
+        case 3: return lsb_interscale_model(c, ui, tauij);
+        default: set_error("tau_sgs: sgsmodel must be 0..3"); return LSB_E_ARG;
+    }
+}
+
+// --------------------------------------------------------------- filters
+extern "C" int lsb_filter_q_3pt(lsb_ctx* c, const double* q, double* q_filt, int n4, int n5, double a, double b,
+                                int interpflag) {
+    LSB_ENTER(c);
+    const int nf = n4 * n5;
+    if (!q || !q_filt || nf < 1) {
+        set_error("filter_q_3pt: bad arguments");
+        return LSB_E_ARG;
+    }
+    double *din, *dout, *tmp;
+    LSB_TRY(tensor_in(st, c, 0, q, nf, &din));
+    LSB_TRY(st.get(1, c->nphys * nf, &dout));
+    LSB_TRY(st.get(2, c->nphys * std::min(nf, (int)kMaxFields), &tmp));
+    std::vector<const double*> pi(nf);
+    std::vector<double*> po(nf);
+    for (int f = 0; f < nf; ++f) {
+        pi[f] = din + (size_t)f * c->nphys;
+        po[f] = dout + (size_t)f * c->nphys;
+    }
+    LSB_TRY(ph_filter3(c, pi.data(), po.data(), nf, a, b, interpflag, tmp, c->stream));
+    LSB_TRY(ph_zero_pads(c, po.data(), nf, c->stream));
+    LSB_TRY(st.out(q_filt, dout, c->nphys * nf));
+    return st.sync();
+}
+
+extern "C" int lsb_specfilter_exp_xy(lsb_ctx* c, const double* uf, double* uff, double kc, double p, double amp,
+                                     int filtery) {
+    LSB_ENTER(c);
+    double *a, *b;
+    LSB_TRY(st.get(0, c->nspec, &a));
+    LSB_TRY(st.get(1, c->nspec, &b));
+    LSB_TRY(st.in(a, uf, c->nspec));
+    const double* in[1] = {a};
+    double* out[1] = {b};
+    LSB_TRY(sp_specfilter(c, in, out, 1, kc, p, amp, filtery, c->stream));
+    LSB_TRY(st.out(uff, b, c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_filterqspec(lsb_ctx* c, double* qf, int nq) {
+    LSB_ENTER(c);
+    if (!qf || nq < 1 || nq > kMaxFields) {
+        set_error("filterqspec: 1..16 fields");
+        return LSB_E_ARG;
+    }
+    double *a, *b;
+    LSB_TRY(st.get(0, c->nspec * nq, &a));
+    LSB_TRY(st.get(1, c->nspec * nq, &b));
+    LSB_TRY(st.in(a, qf, c->nspec * nq));
+    const double* in[kMaxFields];
+    double* out[kMaxFields];
+    for (int f = 0; f < nq; ++f) {
+        in[f] = a + (size_t)f * c->nspec;
+        out[f] = b + (size_t)f * c->nspec;
+    }
+    LSB_TRY(sp_specfilter(c, in, out, nq, c->cfg.kc, c->cfg.filt_order, c->cfg.filt_amp, 1, c->stream));
+    LSB_TRY(st.out(qf, b, c->nspec * nq));
+    return st.sync();
+}
+
+// --------------------------------------------------------------- dealias
+
+extern "C" int lsb_setup_dealiasing(lsb_ctx* c, int* nx_lg, int* ny_lg) {
+    LSB_ENTER(c);
+    LSB_TRY(dealias_setup(c));
+    if (nx_lg) *nx_lg = c->nx_lg;
+    if (ny_lg) *ny_lg = c->ny_lg;
+    return LSB_OK;
+}
+
+extern "C" int lsb_dealiasing_2d_fftw(lsb_ctx* c, const double* a, const double* b, double* ab) {
+    LSB_ENTER(c);
+    LSB_TRY(dealias_setup(c));
+    double *da, *db, *dab;
+    LSB_TRY(st.get(0, c->nphys, &da));
+    LSB_TRY(st.get(1, c->nphys, &db));
+    LSB_TRY(st.get(2, c->nphys, &dab));
+    LSB_TRY(st.in(da, a, c->nphys));
+    LSB_TRY(st.in(db, b, c->nphys));
+    LSB_TRY(dealias_product(c, da, db, dab, c->stream));
+    LSB_TRY(st.out(ab, dab, c->nphys));
+    return st.sync();
+}
+
+// ------------------------------------------------------------ step API
+extern "C" int lsb_nlterm_skewsym(lsb_ctx* c, const double* uif, double* nltermif) {
+    LSB_ENTER(c);
+    if (c->cfg.dealiasflag == 1) {
+        set_error("nlterm_skewsym: the 3/2-rule products are only available through dealiasing_2d_fftw");
+        return LSB_E_UNSUPPORTED;
+    }
+    double *u, *n;
+    LSB_TRY(st.get(0, c->nspec * 3, &u));
+    LSB_TRY(st.get(1, c->nspec * 3, &n));
+    LSB_TRY(st.in(u, uif, c->nspec * 3));
+    const double* ui[3] = {u, u + c->nspec, u + 2 * c->nspec};
+    double* no[3] = {n, n + c->nspec, n + 2 * c->nspec};
+    LSB_TRY(nlterm_dev(c, ui, no, c->stream));
+    LSB_TRY(st.out(nltermif, n, c->nspec * 3));
+    return st.sync();
+}
+
+extern "C" int lsb_state_set(lsb_ctx* c, const double* uf, const double* vf, const double* wf, const double* pf,
+                             const double* pnf, const double* unif, const double* unmif, const double* nlnif,
+                             const double* nlnmif, int istep, double t) {
+    LSB_ENTER(c);
+    if (c->cfg.dealiasflag == 1) {
+        set_error("timestep: dealiasflag 1 runs through tier A (dealiasing_2d_fftw) only");
+        return LSB_E_UNSUPPORTED;
+    }
+    LSB_TRY(step_alloc(c));
+    const size_t n = c->nspec;
+    auto put = [&](int which, const double* h) -> int {
+        double* d = step_field(c, which);
+        if (h)
+            LSB_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        else
+            LSB_HIP(hipMemsetAsync(d, 0, n * sizeof(double), c->stream));
+        return LSB_OK;
+    };
+    LSB_TRY(put(0, uf));
+    LSB_TRY(put(1, vf));
+    LSB_TRY(put(2, wf));
+    for (int l = 0; l < 3; ++l) {
+        LSB_TRY(put(3 + l, unif ? unif + l * n : nullptr));
+        LSB_TRY(put(6 + l, unmif ? unmif + l * n : nullptr));
+        LSB_TRY(put(9 + l, nlnif ? nlnif + l * n : nullptr));
+        LSB_TRY(put(12 + l, nlnmif ? nlnmif + l * n : nullptr));
+    }
+    LSB_TRY(put(15, pf));
+    LSB_TRY(put(16, pnf));
+    c->istep = istep;
+    c->t = t;
+    c->dt = c->dt2 = c->dt3 = c->cfg.dt;
+    LSB_TRY(step_prepare(c));
+    c->state_ready = true;
+    return st.sync();
+}
+
+extern "C" int lsb_state_get(lsb_ctx* c, double* uf, double* vf, double* wf, double* pf, double* pnf, double* unif,
+                             double* unmif, double* nlnif, double* nlnmif, int* istep, double* t) {
+    LSB_ENTER(c);
+    if (!c->state_ready) {
+        set_error("state_get: no state");
+        return LSB_E_STATE;
+    }
+    const size_t n = c->nspec;
+    auto get = [&](int which, double* h) -> int {
+        if (h) LSB_HIP(hipMemcpyAsync(h, step_field(c, which), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        return LSB_OK;
+    };
+    LSB_TRY(get(0, uf));
+    LSB_TRY(get(1, vf));
+    LSB_TRY(get(2, wf));
+    for (int l = 0; l < 3; ++l) {
+        LSB_TRY(get(3 + l, unif ? unif + l * n : nullptr));
+        LSB_TRY(get(6 + l, unmif ? unmif + l * n : nullptr));
+        LSB_TRY(get(9 + l, nlnif ? nlnif + l * n : nullptr));
+        LSB_TRY(get(12 + l, nlnmif ? nlnmif + l * n : nullptr));
+    }
+    LSB_TRY(get(15, pf));
+    LSB_TRY(get(16, pnf));
+    if (istep) *istep = c->istep;
+    if (t) *t = c->t;
+    return st.sync();
+}
+
+extern "C" int lsb_timestep(lsb_ctx* c, int nsteps) {
+    LSB_TRY(check_ctx(c));
+    LSB_TRY(set_device(c));
+    LSB_TRY(step_run(c, nsteps, true));
+    return LSB_OK;
+}
+
+extern "C" int lsb_navierstokes_update(lsb_ctx* c) {
+    LSB_TRY(check_ctx(c));
+    LSB_TRY(set_device(c));
+    return step_run(c, 1, false);
+}
+
+extern "C" int lsb_state_device_ptrs(lsb_ctx* c, void** uf, void** vf, void** wf, void** pf) {
+    LSB_TRY(check_ctx(c));
+    if (!c->state_ready) return LSB_E_STATE;
+    if (uf) *uf = step_field(c, 0);
+    if (vf) *vf = step_field(c, 1);
+    if (wf) *wf = step_field(c, 2);
+    if (pf) *pf = step_field(c, 15);
+    return LSB_OK;
+}
+
+extern "C" void* lsb_stream(lsb_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int lsb_profile_begin(lsb_ctx* c) {
+    LSB_TRY(check_ctx(c));
+    for (auto& ev : c->prof) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    c->prof.clear();
+    c->profiling = true;
+    return LSB_OK;
+}
+
+extern "C" int lsb_profile_end(lsb_ctx* c, double* ms_fft, double* ms_z, double* ms_h, double* ms_pw, int* n_fft,
+                               int* n_z, int* n_h, int* n_pw) {
+    LSB_TRY(check_ctx(c));
+    LSB_HIP(hipStreamSynchronize(c->stream));
+    double ms[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
+    for (auto& ev : c->prof) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, ev.a, ev.b) == hipSuccess) {
+            ms[ev.cls] += t;
+            cnt[ev.cls] += 1;
+        }
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    c->prof.clear();
+    c->profiling = false;
+    if (ms_fft) *ms_fft = ms[0];
+    if (ms_z) *ms_z = ms[1];
+    if (ms_h) *ms_h = ms[2];
+    if (ms_pw) *ms_pw = ms[3];
+    if (n_fft) *n_fft = cnt[0];
+    if (n_z) *n_z = cnt[1];
+    if (n_h) *n_h = cnt[2];
+    if (n_pw) *n_pw = cnt[3];
+    return LSB_OK;
+}

@@ -1,0 +1,443 @@
+// Horizontal 2-D FFTs (replaces horfft.f90:61-150 and the FFTW plans of
+// modfftw.f90:59-149).
+//
+// Layout (one rank): physical rows of nxpp = nx+2 reals (x fastest), spectral
+// rows of nx reals = nx/2 complex (kx = 0..nx/2-1; the kx-Nyquist mode is not
+// stored, horfft.f90:303-315), ny rows per z-plane, nzp planes per field.
+//
+// Forward (is = -1): X pass  physical row -> spectral row (r2c as a length
+//   nx/2 complex FFT + split), then Y pass in place over tiles of kx columns
+//   (c2c along ky, stride nx/2 complex) with the 1/(nx*ny) scale fused.
+// Inverse (is = +1): Y pass spectral -> physical buffer (complex rows of
+//   stride nxpp/2), then X pass in place (c2r; Im of the kx = 0 term and the
+//   missing Nyquist term are ignored, as FFTW's c2r does).  Pads are zeroed.
+//
+// Each workgroup keeps its lines in LDS and runs a mixed-radix Stockham
+// sequence; twiddles come from tables computed in long double on the host.
+
+#include <cmath>
+
+#include "lsb_internal.h"
+
+namespace lsb {
+
+// ---------------------------------------------------------------- plans
+bool make_plan(int L, FftPlan1D* plan) {
+    plan->L = L;
+    plan->nstage = 0;
+    int n = L;
+    // radix-8 first, then 4, then 2, 3, 5
+    const int order[5] = {8, 4, 2, 3, 5};
+    for (int oi = 0; oi < 5 && n > 1; ++oi) {
+        int r = order[oi];
+        while (n % r == 0 && n > 1) {
+            if (plan->nstage >= kMaxStages) return false;
+            plan->radix[plan->nstage++] = r;
+            n /= r;
+        }
+    }
+    return n == 1 && L >= 1;
+}
+
+template <class R>
+static int make_twiddles(void** dptr, int L, int denom, int count) {
+    // count entries exp(-2 pi i m / denom), m = 0..count-1
+    std::vector<R> h(2 * (size_t)count);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < count; ++m) {
+        long double a = -2.0L * pi * (long double)m / (long double)denom;
+        h[2 * m] = (R)cosl(a);
+        h[2 * m + 1] = (R)sinl(a);
+    }
+    (void)L;
+    LSB_TRY(dev_alloc(dptr, h.size() * sizeof(R)));
+    LSB_HIP(hipMemcpy(*dptr, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice));
+    return LSB_OK;
+}
+
+int hor_plan_create(HorPlan* hp, int nx, int ny) {
+    hp->nx = nx;
+    hp->ny = ny;
+    hp->lh = nx / 2;
+    if (nx % 2 != 0 || !make_plan(hp->lh, &hp->px) || !make_plan(ny, &hp->py)) {
+        set_error("horfft: nx must be even and nx/2, ny products of 2,3,5");
+        return LSB_E_ARG;
+    }
+    LSB_TRY(make_twiddles<double>((void**)&hp->tw_x, hp->lh, hp->lh, hp->lh));
+    LSB_TRY(make_twiddles<double>((void**)&hp->tw_y, ny, ny, ny));
+    LSB_TRY(make_twiddles<double>((void**)&hp->tw_r, nx, nx, hp->lh));
+    LSB_TRY(make_twiddles<float>((void**)&hp->tw_x32, hp->lh, hp->lh, hp->lh));
+    LSB_TRY(make_twiddles<float>((void**)&hp->tw_y32, ny, ny, ny));
+    LSB_TRY(make_twiddles<float>((void**)&hp->tw_r32, nx, nx, hp->lh));
+    return LSB_OK;
+}
+
+void hor_plan_destroy(HorPlan* hp) {
+    void* ps[6] = {hp->tw_x, hp->tw_y, hp->tw_r, hp->tw_x32, hp->tw_y32, hp->tw_r32};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    hp->tw_x = hp->tw_y = hp->tw_r = nullptr;
+    hp->tw_x32 = hp->tw_y32 = hp->tw_r32 = nullptr;
+}
+
+// ------------------------------------------------------- small DFT kernels
+template <class R, int S>
+__device__ __forceinline__ cpx<R> mul_si(cpx<R> a) {  // (S*i) * a
+    if (S < 0) return {a.im, -a.re};
+    return {-a.im, a.re};
+}
+
+template <class R, int S>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+    cpx<R> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <class R, int S>
+__device__ __forceinline__ void dft4(cpx<R>& a0, cpx<R>& a1, cpx<R>& a2, cpx<R>& a3) {
+    cpx<R> s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+    cpx<R> w = mul_si<R, S>(d13);
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, w);
+    a3 = csub(d02, w);
+}
+
+template <class R, int S>
+__device__ __forceinline__ void dft3(cpx<R>* v) {
+    const R h = (R)0.86602540378443864676372317075293618L;  // sqrt(3)/2
+    cpx<R> a0 = v[0], a1 = v[1], a2 = v[2];
+    cpx<R> t = cadd(a1, a2);
+    cpx<R> m = {a0.re - (R)0.5 * t.re, a0.im - (R)0.5 * t.im};
+    cpx<R> d = csub(a1, a2);
+    cpx<R> w = mul_si<R, S>(cpx<R>{h * d.re, h * d.im});
+    v[0] = cadd(a0, t);
+    v[1] = cadd(m, w);
+    v[2] = csub(m, w);
+}
+
+template <class R, int S>
+__device__ __forceinline__ void dft5(cpx<R>* v) {
+    const R c1 = (R)0.30901699437494742410229341718281906L;   // cos(2pi/5)
+    const R c2 = (R)-0.80901699437494742410229341718281906L;  // cos(4pi/5)
+    const R s1 = (R)0.95105651629515357211643933337938214L;   // sin(2pi/5)
+    const R s2 = (R)0.58778525229247312916870595463907277L;   // sin(4pi/5)
+    cpx<R> a0 = v[0];
+    cpx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    cpx<R> b1 = {a0.re + c1 * t1.re + c2 * t2.re, a0.im + c1 * t1.im + c2 * t2.im};
+    cpx<R> b2 = {a0.re + c2 * t1.re + c1 * t2.re, a0.im + c2 * t1.im + c1 * t2.im};
+    cpx<R> e1 = mul_si<R, S>(cpx<R>{s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im});
+    cpx<R> e2 = mul_si<R, S>(cpx<R>{s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im});
+    v[0] = cadd(a0, cadd(t1, t2));
+    v[1] = cadd(b1, e1);
+    v[4] = csub(b1, e1);
+    v[2] = cadd(b2, e2);
+    v[3] = csub(b2, e2);
+}
+
+template <class R, int S>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+    const R r = (R)0.70710678118654752440084436210484904L;
+    cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<R, S>(e0, e1, e2, e3);
+    dft4<R, S>(o0, o1, o2, o3);
+    // w^1 = r(1 + S i), w^2 = S i, w^3 = r(-1 + S i)
+    cpx<R> w1 = {r * (o1.re - (R)S * o1.im), r * (o1.im + (R)S * o1.re)};
+    cpx<R> w2 = mul_si<R, S>(o2);
+    cpx<R> w3 = {r * (-o3.re - (R)S * o3.im), r * (-o3.im + (R)S * o3.re)};
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1);
+    v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2);
+    v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3);
+    v[7] = csub(e3, w3);
+}
+
+// One mixed-radix Stockham pass over `nlines` lines of length L held in LDS.
+// Twiddle table tw[m] = exp(-2 pi i m / L); S = -1 forward, +1 backward.
+template <class R, int S>
+__device__ cpx<R>* fft_lines(cpx<R>* b0, cpx<R>* b1, const FftPlan1D& plan, int nlines, int lstride,
+                             const cpx<R>* __restrict__ tw) {
+    const int L = plan.L;
+    cpx<R>* in = b0;
+    cpx<R>* out = b1;
+    int Ns = 1;
+    for (int s = 0; s < plan.nstage; ++s) {
+        const int rad = plan.radix[s];
+        const int nb = L / rad;
+        const int twstep = L / (Ns * rad);
+        const int total = nlines * nb;
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int line = t / nb;
+            const int j = t - line * nb;
+            const int jm = j % Ns;
+            const cpx<R>* src = in + (size_t)line * lstride;
+            cpx<R>* dst = out + (size_t)line * lstride;
+            cpx<R> v[8];
+            for (int r = 0; r < rad; ++r) v[r] = src[j + r * nb];
+            if (Ns > 1) {
+                for (int r = 1; r < rad; ++r) {
+                    cpx<R> w = tw[jm * r * twstep];
+                    if (S > 0) w.im = -w.im;
+                    v[r] = cmul(v[r], w);
+                }
+            }
+            switch (rad) {
+                case 2: dft2<R, S>(v); break;
+                case 3: dft3<R, S>(v); break;
+                case 4: dft4<R, S>(v[0], v[1], v[2], v[3]); break;
+                case 5: dft5<R, S>(v); break;
+                default: dft8<R, S>(v); break;
+            }
+            const int base = (j - jm) * rad + jm;
+            for (int r = 0; r < rad; ++r) dst[base + r * Ns] = v[r];
+        }
+        __syncthreads();
+        cpx<R>* tmp = in;
+        in = out;
+        out = tmp;
+        Ns *= rad;
+    }
+    return in;
+}
+
+// ------------------------------------------------------------- X passes
+// Forward: rows of real input (stride nxpp) -> spectral rows (stride nx).
+// Row index r in [0, nf*nrows); nrows = nplanes*ny per field.
+template <class R>
+__global__ void __launch_bounds__(256) k_xfwd(FieldList<R> in, FieldList<R> out, int nrows, int nf, int nx, int nxpp,
+                                              FftPlan1D plan, const cpx<R>* __restrict__ tw,
+                                              const cpx<R>* __restrict__ twr, int lines) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lh = nx / 2;
+    cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
+    cpx<R>* b1 = b0 + (size_t)lines * lh;
+    const long row0 = (long)blockIdx.x * lines;
+    const long total_rows = (long)nrows * nf;
+    // load
+    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
+        const int l = t / nx, i = t - l * nx;
+        const long r = row0 + l;
+        R v = 0;
+        if (r < total_rows) {
+            const int f = (int)(r / nrows);
+            const long rr = r - (long)f * nrows;
+            v = in.p[f][rr * nxpp + i];
+        }
+        reinterpret_cast<R*>(b0)[(size_t)l * nx + i] = v;
+    }
+    __syncthreads();
+    cpx<R>* z = fft_lines<R, -1>(b0, b1, plan, lines, lh, tw);
+    // split: X_k = 1/2 (Z_k + conj Z_{lh-k}) - i/2 W^k (Z_k - conj Z_{lh-k})
+    for (int t = threadIdx.x; t < lines * lh; t += blockDim.x) {
+        const int l = t / lh, k = t - l * lh;
+        const long r = row0 + l;
+        if (r >= total_rows) continue;
+        const cpx<R> zk = z[(size_t)l * lh + k];
+        const cpx<R> zc = cconj(z[(size_t)l * lh + ((k == 0) ? 0 : lh - k)]);
+        const cpx<R> a = {(R)0.5 * (zk.re + zc.re), (R)0.5 * (zk.im + zc.im)};
+        const cpx<R> d = {(R)0.5 * (zk.re - zc.re), (R)0.5 * (zk.im - zc.im)};
+        const cpx<R> wd = cmul(twr[k], d);
+        // -i * wd = (wd.im, -wd.re)
+        const cpx<R> x = {a.re + wd.im, a.im - wd.re};
+        const int f = (int)(r / nrows);
+        const long rr = r - (long)f * nrows;
+        reinterpret_cast<cpx<R>*>(out.p[f] + rr * nx)[k] = x;
+    }
+}
+
+// Inverse: in place on physical rows (stride nxpp) holding nx/2 complex
+// coefficients; produces nx reals and zero pads.
+template <class R>
+__global__ void __launch_bounds__(256) k_xinv(FieldList<R> io, int nrows, int nf, int nx, int nxpp, FftPlan1D plan,
+                                              const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr,
+                                              int lines) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lh = nx / 2;
+    cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
+    cpx<R>* b1 = b0 + (size_t)lines * lh;
+    const long row0 = (long)blockIdx.x * lines;
+    const long total_rows = (long)nrows * nf;
+    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
+        const int l = t / nx, i = t - l * nx;
+        const long r = row0 + l;
+        R v = 0;
+        if (r < total_rows) {
+            const int f = (int)(r / nrows);
+            const long rr = r - (long)f * nrows;
+            v = io.p[f][rr * nxpp + i];
+        }
+        reinterpret_cast<R*>(b1)[(size_t)l * nx + i] = v;
+    }
+    __syncthreads();
+    // Z'_k = (X_k + conj X_{lh-k}) + i W^-k (X_k - conj X_{lh-k}); X_lh = 0, Im X_0 ignored
+    for (int t = threadIdx.x; t < lines * lh; t += blockDim.x) {
+        const int l = t / lh, k = t - l * lh;
+        cpx<R> xk = b1[(size_t)l * lh + k];
+        cpx<R> xc;
+        if (k == 0) {
+            xk.im = 0;
+            xc = {0, 0};
+        } else {
+            xc = cconj(b1[(size_t)l * lh + (lh - k)]);
+        }
+        const cpx<R> s = cadd(xk, xc);
+        const cpx<R> d = csub(xk, xc);
+        const cpx<R> wd = cmul(cconj(twr[k]), d);
+        // i * wd = (-wd.im, wd.re)
+        b0[(size_t)l * lh + k] = {s.re - wd.im, s.im + wd.re};
+    }
+    __syncthreads();
+    cpx<R>* y = fft_lines<R, 1>(b0, b1, plan, lines, lh, tw);
+    for (int t = threadIdx.x; t < lines * nxpp; t += blockDim.x) {
+        const int l = t / nxpp, i = t - l * nxpp;
+        const long r = row0 + l;
+        if (r >= total_rows) continue;
+        const int f = (int)(r / nrows);
+        const long rr = r - (long)f * nrows;
+        io.p[f][rr * nxpp + i] = (i < nx) ? reinterpret_cast<R*>(y)[(size_t)l * nx + i] : (R)0;
+    }
+}
+
+// ------------------------------------------------------------- Y passes
+// Forward, in place on spectral planes: tiles of `cols` kx columns x ny rows.
+template <class R>
+__global__ void __launch_bounds__(256) k_yfwd(FieldList<R> io, int nplanes, int nx, int ny, FftPlan1D plan,
+                                              const cpx<R>* __restrict__ tw, int cols, R scale) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lh = nx / 2;
+    cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
+    cpx<R>* b1 = b0 + (size_t)cols * ny;
+    const int ntile = (lh + cols - 1) / cols;
+    const int tile = blockIdx.x % ntile;
+    const long plane = blockIdx.x / ntile;  // over nf * nplanes
+    const int f = (int)(plane / nplanes);
+    const long pk = plane - (long)f * nplanes;
+    cpx<R>* base = reinterpret_cast<cpx<R>*>(io.p[f] + pk * (long)nx * ny);
+    const int m0 = tile * cols;
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int j = t / cols, cc = t - j * cols;
+        const int m = m0 + cc;
+        b0[(size_t)cc * ny + j] = (m < lh) ? base[(size_t)j * lh + m] : cpx<R>{0, 0};
+    }
+    __syncthreads();
+    cpx<R>* y = fft_lines<R, -1>(b0, b1, plan, cols, ny, tw);
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int j = t / cols, cc = t - j * cols;
+        const int m = m0 + cc;
+        if (m < lh) {
+            cpx<R> v = y[(size_t)cc * ny + j];
+            base[(size_t)j * lh + m] = {v.re * scale, v.im * scale};
+        }
+    }
+}
+
+// Inverse: spectral planes (read) -> physical buffer rows (complex stride nxpp/2).
+template <class R>
+__global__ void __launch_bounds__(256) k_yinv(FieldList<R> in, FieldList<R> out, int nplanes, int nx, int ny,
+                                              int nxpp, FftPlan1D plan, const cpx<R>* __restrict__ tw, int cols) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lh = nx / 2;
+    const int ph = nxpp / 2;
+    cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
+    cpx<R>* b1 = b0 + (size_t)cols * ny;
+    const int ntile = (lh + cols - 1) / cols;
+    const int tile = blockIdx.x % ntile;
+    const long plane = blockIdx.x / ntile;
+    const int f = (int)(plane / nplanes);
+    const long pk = plane - (long)f * nplanes;
+    const cpx<R>* src = reinterpret_cast<const cpx<R>*>(in.p[f] + pk * (long)nx * ny);
+    cpx<R>* dst = reinterpret_cast<cpx<R>*>(out.p[f] + pk * (long)nxpp * ny);
+    const int m0 = tile * cols;
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int j = t / cols, cc = t - j * cols;
+        const int m = m0 + cc;
+        b0[(size_t)cc * ny + j] = (m < lh) ? src[(size_t)j * lh + m] : cpx<R>{0, 0};
+    }
+    __syncthreads();
+    cpx<R>* y = fft_lines<R, 1>(b0, b1, plan, cols, ny, tw);
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int j = t / cols, cc = t - j * cols;
+        const int m = m0 + cc;
+        if (m < lh) dst[(size_t)j * ph + m] = y[(size_t)cc * ny + j];
+    }
+}
+
+// ------------------------------------------------------------- launchers
+static int lines_for(int len, size_t elem) {
+    int l = (int)(32768 / ((size_t)len * elem));
+    if (l < 1) l = 1;
+    if (l > 16) l = 16;
+    return l;
+}
+
+int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double* const* spec, int nf, int nplanes,
+               hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields) {
+        set_error("horfft: 1..16 fields per launch");
+        return LSB_E_ARG;
+    }
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    FieldList<double> in{}, out{};
+    for (int f = 0; f < nf; ++f) {
+        in.p[f] = const_cast<double*>(phys[f]);
+        out.p[f] = spec[f];
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_FFT, s, &tok);
+    const int nrows = nplanes * ny;
+    int lines = lines_for(lh, sizeof(cpx<double>));
+    size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
+    long nblk = ((long)nrows * nf + lines - 1) / lines;
+    hipLaunchKernelGGL(k_xfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nrows, nf, nx, nxpp, hp.px,
+                       (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
+    LSB_HIP(hipGetLastError());
+    int cols = lines_for(ny, sizeof(cpx<double>));
+    shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
+    const int ntile = (lh + cols - 1) / cols;
+    nblk = (long)ntile * nplanes * nf;
+    const double scale = 1.0 / (double)(nx * ny);
+    hipLaunchKernelGGL(k_yfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nplanes, nx, ny, hp.py,
+                       (const cpx<double>*)hp.tw_y, cols, scale);
+    LSB_HIP(hipGetLastError());
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+
+int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf, int nplanes,
+               hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields) {
+        set_error("horfft: 1..16 fields per launch");
+        return LSB_E_ARG;
+    }
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    FieldList<double> in{}, out{};
+    for (int f = 0; f < nf; ++f) {
+        in.p[f] = const_cast<double*>(spec[f]);
+        out.p[f] = phys[f];
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_FFT, s, &tok);
+    int cols = lines_for(ny, sizeof(cpx<double>));
+    size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
+    const int ntile = (lh + cols - 1) / cols;
+    long nblk = (long)ntile * nplanes * nf;
+    hipLaunchKernelGGL(k_yinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nplanes, nx, ny, nxpp, hp.py,
+                       (const cpx<double>*)hp.tw_y, cols);
+    LSB_HIP(hipGetLastError());
+    const int nrows = nplanes * ny;
+    int lines = lines_for(lh, sizeof(cpx<double>));
+    shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
+    nblk = ((long)nrows * nf + lines - 1) / lines;
+    hipLaunchKernelGGL(k_xinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nrows, nf, nx, nxpp, hp.px,
+                       (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
+    LSB_HIP(hipGetLastError());
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+
+}  // namespace lsb

@@ -1,0 +1,241 @@
+// Internal declarations of the lsbflow MI355X library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lsbflow.h"
+
+namespace lsb {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+
+#define LSB_HIP(call)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::lsb::set_error(std::string(#call) + ": " + hipGetErrorString(e_));    \
+            return LSB_E_HIP;                                                       \
+        }                                                                           \
+    } while (0)
+
+#define LSB_TRY(call)                \
+    do {                             \
+        int rc_ = (call);            \
+        if (rc_ != LSB_OK) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------ complex math
+template <class R>
+struct cpx {
+    R re, im;
+};
+
+template <class R>
+__host__ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R>
+__host__ __device__ inline cpx<R> cadd(cpx<R> a, cpx<R> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <class R>
+__host__ __device__ inline cpx<R> csub(cpx<R> a, cpx<R> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <class R>
+__host__ __device__ inline cpx<R> cconj(cpx<R> a) {
+    return {a.re, -a.im};
+}
+
+// ------------------------------------------------------------- FFT plans
+// A 1-D complex FFT of length L = prod(radix[0..nstage-1]) executed as a
+// Stockham autosort sequence in LDS.  Twiddles are exp(-2 pi i m / L),
+// m = 0..L-1, computed in long double on the host.
+constexpr int kMaxStages = 16;
+struct FftPlan1D {
+    int L = 0;
+    int nstage = 0;
+    int radix[kMaxStages] = {0};
+};
+
+bool make_plan(int L, FftPlan1D* plan);
+
+// Device-side description of one horizontal transform size.
+struct HorPlan {
+    int nx = 0, ny = 0;        // real length in x, complex length in y
+    int lh = 0;                // nx/2 complex points per x-row
+    FftPlan1D px, py;          // plans for lh and ny
+    double* tw_x = nullptr;    // lh twiddles exp(-2 pi i m/lh) (cpx<double>)
+    double* tw_y = nullptr;    // ny twiddles
+    double* tw_r = nullptr;    // lh post-process twiddles exp(-2 pi i k/nx)
+    float* tw_x32 = nullptr;
+    float* tw_y32 = nullptr;
+    float* tw_r32 = nullptr;
+};
+
+// A list of field base pointers passed by value to batched kernels.
+constexpr int kMaxFields = 16;
+template <class R>
+struct FieldList {
+    R* p[kMaxFields];
+};
+
+}  // namespace lsb
+
+// --------------------------------------------------------------- context
+struct lsb_ctx {
+    lsb_config cfg{};
+    int nx = 0, ny = 0, nz = 0, nzp = 0, nzm = 0, nxh = 0, nyh = 0, nxpp = 0;
+    size_t nspec = 0;   // nx*ny*nzp   reals per spectral field
+    size_t nphys = 0;   // nxpp*ny*nzp reals per physical field
+    size_t plane_s = 0; // nx*ny
+    size_t plane_p = 0; // nxpp*ny
+    int ldm_z = 0;      // padded leading dimension of the nzp x nzp device operators
+
+    // grid module (common.f90:112-123)
+    double xl = 0, dx = 0, dy = 0, alpha = 0, beta = 0, aa = 0, bb = 0;
+    std::vector<double> wavx, wavy, wavxx, wavyy, zpts, gf, xpts;
+    // mats module (common.f90:189-197), column-major like Fortran
+    std::vector<double> dor, d, d2;      // nzp*nzp
+    std::vector<double> mdz, md2;        // dfdz / d2fdz2 operators (nzp*nzp)
+    std::vector<double> diag, smat, simat;  // nzm, nzm*nzm
+    // paral module
+    std::vector<double> ubar, ubarp, ubarpp, wbar, wbarpp;
+    // lifting functions of solve_nhom_helmholtz (helmholtz.f90:240-307)
+    std::vector<double> hbot, htop, d2hbot, d2htop;
+    // static Smagorinsky / sigma filter widths (sgsterm.f90:95-103, 158-164, 367-376)
+    std::vector<double> delta_static, delta_dyn, delta_sigma;
+    // filter_q_3pt z weights for (a,b) and (1,1) (filters.f90:396-448)
+    std::vector<double> fz_ab, fz_11;    // 3*nzp each: az, bz, cz
+
+    // device copies
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* d_wavx = nullptr;   // nx
+    double* d_wavy = nullptr;   // ny
+    double* d_zpts = nullptr;   // nzp
+    double* d_mdz = nullptr;    // nzp x nzp column-major (ld = nzp)
+    double* d_md2 = nullptr;    // nzp x nzp
+    double* d_md2b = nullptr;   // rows {0, nzp-1} of md2, 2 x nzp column-major
+    double* d_d = nullptr;      // nzp x nzp (matmul path for nz <= 31)
+    double* d_smat = nullptr;   // nzm x nzm
+    double* d_simat = nullptr;  // nzm x nzm
+    double* d_diag = nullptr;   // nzm
+    double* d_lift = nullptr;   // 4*nzp: hbot, htop, d2hbot, d2htop
+    double* d_paral = nullptr;  // 3*nzp: ubar, ubarp, ubarpp
+    double* d_delta = nullptr;  // 3*nzp: static, dynamic, sigma
+    double* d_fz = nullptr;     // 6*nzp: (a,b) weights then (1,1) weights
+    double* d_smag = nullptr;   // nzp (static Smagorinsky -(cs delta fvd)^2) + scratch
+    double* d_red = nullptr;    // reduction scratch (1024 doubles)
+    float* d_mdz32 = nullptr;
+    float* d_smat32 = nullptr;
+    float* d_simat32 = nullptr;
+
+    lsb::HorPlan hp;            // nx, ny
+    lsb::HorPlan hp_lg;         // dealiasing sizes
+    int nx_lg = 0, ny_lg = 0;
+
+    // staging buffers for tier A (device)
+    std::vector<void*> stage;
+    std::vector<size_t> stage_bytes;
+
+    // tier B state
+    bool state_ready = false;
+    double* st = nullptr;       // contiguous block of all state + work fields
+    size_t st_bytes = 0;
+    int istep = 0;
+    double t = 0.0, dt = 0.0, dt2 = 0.0, dt3 = 0.0;
+    double abfac[3] = {0, 0, 0};
+    double bdf[3] = {0, 0, 0};
+    double bdfv = 0.0;
+    // rotating slots (indices into the state block)
+    int s_u[3], s_un[3], s_unm[3], s_nl[3], s_nlm[3], s_p, s_pn;
+    double* bc_spec = nullptr;  // spectral bound() field, planes 0 and nzp-1 (2*plane_s)
+
+    // profiling
+    bool profiling = false;
+    struct ProfEv {
+        hipEvent_t a, b;
+        int cls;
+    };
+    std::vector<ProfEv> prof;
+    double prof_ms[4] = {0, 0, 0, 0};
+    int prof_n[4] = {0, 0, 0, 0};
+};
+
+namespace lsb {
+
+// host setup (setup.cpp)
+int build_grid(lsb_ctx* c);
+int build_operators(lsb_ctx* c);
+int build_derived(lsb_ctx* c);
+void cheb_matrix(int nzp, double xl, const double* gf, double* dor, double* d, double* d2);
+void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m);
+int eigen_decompose(const double* a, int n, double* lam, double* e, double* ei);
+void lifting_functions(lsb_ctx* c);
+
+// device memory helpers (capi.cpp)
+int dev_alloc(void** p, size_t bytes);
+int upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p);
+int upload_context(lsb_ctx* c);
+
+// profiling helpers
+enum { PC_FFT = 0, PC_Z = 1, PC_HELM = 2, PC_PW = 3 };
+int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token);
+int prof_mark_end(lsb_ctx* c, int token, hipStream_t s);
+
+// FFT (fft.hip)
+int hor_plan_create(HorPlan* hp, int nx, int ny);
+void hor_plan_destroy(HorPlan* hp);
+// nf physical fields (row stride nxpp) -> spectral (row stride nx); batched.
+int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double* const* spec, int nf,
+               int nplanes, hipStream_t s);
+int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
+               int nplanes, hipStream_t s);
+
+// z-contractions (zops.hip)
+// out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).
+int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
+              double* const* out, int nf, hipStream_t s);
+// Helmholtz solves in place: rhs_f (nzp planes); bc_f may be null (homogeneous).
+// sigma = dtnui (constant over fields).
+int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top,
+              const double* const* bc_bot, int nf, double sigma, hipStream_t s);
+
+// spectral pointwise (spectral.hip)
+int sp_dfdx(lsb_ctx* c, const double* in, double* out, hipStream_t s);
+int sp_dfdy(lsb_ctx* c, const double* in, double* out, hipStream_t s);
+int sp_divuv(lsb_ctx* c, const double* u, const double* v, double* out, hipStream_t s);
+int sp_norm(lsb_ctx* c, double* const* f, int nf, hipStream_t s);
+int sp_lapl_combine(lsb_ctx* c, const double* d2u, const double* u, double* out, hipStream_t s);
+int sp_axpby(lsb_ctx* c, double* y, const double* x, double a, double b, hipStream_t s);
+
+// physical (physical.hip)
+int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, double a, double b,
+               int interpflag, double* tmp, hipStream_t s);
+int ph_sgs(lsb_ctx* c, int model, const double* ui, const double* uiuj, const double* g, double* tau,
+           hipStream_t s);
+
+// step (step.hip)
+int step_alloc(lsb_ctx* c);
+int step_run(lsb_ctx* c, int nsteps, bool with_norm);
+int step_prepare(lsb_ctx* c);
+void step_free(lsb_ctx* c);
+double* step_field(lsb_ctx* c, int which);
+int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStream_t s);
+// full 9-component tensors (tier A): work must hold 69 (dynamic) / 40 (interscale) physical fields
+int dyn_full(lsb_ctx* c, double* ui, double* uiuj, double* g, double* tau, double* work, hipStream_t s);
+int isc_full(lsb_ctx* c, double* ui, double* tau, double* work, hipStream_t s);
+
+// dealiasing (dealias.hip)
+int dealias_setup(lsb_ctx* c);
+int dealias_product(lsb_ctx* c, const double* a, const double* b, double* ab, hipStream_t s);
+
+}  // namespace lsb

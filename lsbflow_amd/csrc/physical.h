@@ -1,0 +1,52 @@
+// Argument blocks and launchers of the physical-space kernels.
+#pragma once
+
+#include "lsb_internal.h"
+
+namespace lsb {
+
+enum { NL_NONE = 0, NL_SIGMA = 1, NL_STATIC = 2, NL_EXTTAU = 3, NL_PREP = 4 };
+
+struct NlArgs {
+    const double* ui[3];   // velocity perturbation (physical)
+    const double* g[9];    // du_l/dx_m at l + 3m (physical)
+    const double* ubar;    // nzp
+    const double* ubarp;   // nzp
+    const double* delta;   // nzp (sigma model filter width)
+    const double* smag;    // nzp (static model factor)
+    const double* tau[9];  // NL_EXTTAU: precomputed tau (nullable entries = 0)
+    double* nlt[3];        // -1/2 u_j du_i/dx_j
+    double* h[9];          // h_lm = 1/2 u_l u_m + tau_lm (NL_PREP: u_l u_m); null entries skipped
+    double* uo[3];         // NL_PREP: velocity with ubar added (may alias ui)
+    double* go[9];         // NL_PREP: g with ubarp added at (0, 2) (may alias g)
+    int write_base;
+};
+
+struct DynArgs {
+    const double* g[9];
+    const double* delta;   // nzp (dynamic model widths)
+    double* snorm;
+    double* sij[9];
+    double* l1[9];
+    const double* sijhat[9];
+    const double* l1hat[9];
+    const double* uihat[3];
+    const double* uuhat[9];
+    double* num;
+    double* den;
+    const double* numf;
+    const double* denf;
+    double* tau[9];
+};
+
+int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s);
+int ph_nonlinear(lsb_ctx* c, int mode, const NlArgs& a, hipStream_t s);
+int ph_hcombine(lsb_ctx* c, const double* const* uu, const double* const* tau, double* const* h, int n,
+                hipStream_t s);
+int ph_smag_factor(lsb_ctx* c, const double* g02, const double* ubarp, double cs, hipStream_t s);
+int ph_dyn(lsb_ctx* c, int stage, const DynArgs& a, hipStream_t s);
+int ph_interscale(lsb_ctx* c, int stage, const FieldList<double>& p0, const FieldList<double>& p1,
+                  const FieldList<double>& p2, const FieldList<double>& p3, const FieldList<double>& p4,
+                  hipStream_t s);
+
+}  // namespace lsb

@@ -1,0 +1,530 @@
+// Physical-space kernels: skew-symmetric products, SGS models, 3-point filters.
+//
+// Physical layout: [nzp][ny][nxpp] reals; only i < nx is computed (pads are
+// written as zero).  Tensors are addressed through a list of 9 field pointers,
+// element (l, m) at index l + 3*m like the reference's (..., 3, 3) arrays; the
+// step passes aliased pointers for symmetric tensors (same values, fewer
+// fields).
+
+#include <cmath>
+
+#include "lsb_internal.h"
+#include "physical.h"
+
+namespace lsb {
+
+struct PhDims {
+    long npts;  // nzp*ny*nx
+    int nx, ny, nxpp, nzp;
+};
+
+static PhDims ph_dims(const lsb_ctx* c) {
+    PhDims d;
+    d.nx = c->nx;
+    d.ny = c->ny;
+    d.nxpp = c->nxpp;
+    d.nzp = c->nzp;
+    d.npts = (long)c->nzp * c->ny * c->nx;
+    return d;
+}
+
+static dim3 ph_grid(long n) {
+    long b = (n + 255) / 256;
+    if (b > 65535L * 8) b = 65535L * 8;
+    return dim3((unsigned)b);
+}
+
+#define PH_LOOP(d)                                                                                          \
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (d).npts; e += (long)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ void ph_index(const PhDims& d, long e, int& i, int& j, int& k, long& off) {
+    i = (int)(e % d.nx);
+    const long jk = e / d.nx;
+    j = (int)(jk % d.ny);
+    k = (int)(jk / d.ny);
+    off = ((long)k * d.ny + j) * d.nxpp + i;
+}
+
+// Zero the two pad reals of every x-row of nf fields.
+__global__ void k_zero_pads(FieldList<double> f, int nf, long nrows, int nx, int nxpp) {
+    const long n = nrows * nf * 2;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int q = (int)(t & 1);
+        const long r = (t >> 1) % nrows;
+        const int fi = (int)((t >> 1) / nrows);
+        f.p[fi][r * nxpp + nx + q] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------- SGS per point
+// sgsterm.f90:283-423 sigma model: eddy viscosity from the singular values of g.
+__device__ __forceinline__ double sigma_nu(const double g[3][3], double delta) {
+    const double pi = 3.141592653589793;
+    double G[3][3], G2[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) G[a][b] = g[0][a] * g[0][b] + g[1][a] * g[1][b] + g[2][a] * g[2][b];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) G2[a][b] = G[a][0] * G[0][b] + G[a][1] * G[1][b] + G[a][2] * G[2][b];
+    const double i1 = G[0][0] + G[1][1] + G[2][2];
+    const double i2 = 0.5 * (i1 * i1 - (G2[0][0] + G2[1][1] + G2[2][2]));
+    const double i3 = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
+                      G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                      G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    double a1 = i1 * i1 / 9. - i2 / 3.;
+    const double a2 = i1 * i1 * i1 / 27. - i1 * i2 / 6. + 0.5 * i3;
+    if (a1 < 0.0) a1 = fabs(a1);
+    const double sa = sqrt(a1);
+    const double pre = a2 / (sa * sa * sa);
+    double a3;
+    if (pre >= 1. || isnan(pre))
+        a3 = 0.0;
+    else if (pre <= -1.)
+        a3 = pi / 3.;
+    else
+        a3 = acos(pre) / 3.;
+    const double s1 = sqrt(fabs(i1 / 3. + 2. * sa * cos(a3)));
+    const double s2 = sqrt(fabs(i1 / 3. - 2. * sa * cos(pi / 3. + a3)));
+    const double s3 = sqrt(fabs(i1 / 3. - 2. * sa * cos(pi / 3. - a3)));
+    double ds = s3 * (s1 - s2) * (s2 - s3);
+    if (s1 != 0.) ds = ds / (s1 * s1);
+    if (ds < 0.0 || isnan(ds)) ds = 0.0;
+    const double cd = 1.35 * delta;
+    return -(cd * cd) * ds;
+}
+
+// ---------------------------------------------------------- nonlinear term
+// navierstokes.f90:725-769: base flow added, nlt_l = -1/2 sum_m u_m du_l/dx_m,
+// uu_lm = u_l u_m, tau from the SGS model, h_lm = 1/2 uu_lm + tau_lm.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgs a) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        double u[3], g[3][3];
+        u[0] = a.ui[0][off] + a.ubar[k];
+        u[1] = a.ui[1][off];
+        u[2] = a.ui[2][off];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) g[l][m] = a.g[l + 3 * m][off];
+        g[0][2] = g[0][2] + a.ubarp[k];
+        if (a.write_base) {
+            a.uo[0][off] = u[0];
+            a.uo[1][off] = u[1];
+            a.uo[2][off] = u[2];
+            a.go[0 + 3 * 2][off] = g[0][2];
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * g[l][m];
+            a.nlt[l][off] = s;
+        }
+        double tau[3][3];
+        if (MODE == NL_SIGMA) {
+            const double nu = sigma_nu(g, a.delta[k]);
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) tau[l][m] = 2.0 * nu * 0.5 * (g[l][m] + g[m][l]);
+        } else if (MODE == NL_STATIC) {
+            double sn = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii) {
+                    const double sij = 0.5 * (g[ii][jj] + g[jj][ii]);
+                    sn = sn + sij * sij;
+                }
+            sn = sqrt(2. * sn);
+            const double nu = a.smag[k] * sn;
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii) tau[ii][jj] = nu * 0.5 * (g[ii][jj] + g[jj][ii]);
+        } else if (MODE == NL_EXTTAU) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) tau[l][m] = a.tau[l + 3 * m] ? a.tau[l + 3 * m][off] : 0.0;
+        } else {
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) tau[l][m] = 0.0;
+        }
+        if (MODE == NL_PREP) {
+            // products only (dynamic model computes tau from them next)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = u[l] * u[m];
+        } else {
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = 0.5 * (u[l] * u[m]) + tau[l][m];
+        }
+    }
+}
+
+// h_lm = 1/2 uu_lm + tau_lm (after the dynamic model)
+__global__ void k_hcombine(PhDims d, FieldList<double> uu, FieldList<double> tau, FieldList<double> h, int n) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        for (int q = 0; q < n; ++q) h.p[q][off] = 0.5 * uu.p[q][off] + tau.p[q][off];
+    }
+}
+
+// sgsterm.f90:59-81 utau: sum over the wall planes of xnu*du/dz, then the
+// per-plane factor -(CS delta (1 - exp(-z+/25)))^2 (sgsterm.f90:84-108).
+__global__ void k_utau_reduce(const double* __restrict__ g02, const double* __restrict__ ubarp, int nx, int ny,
+                              int nxpp, const int* __restrict__ walls, int nwall, double xnu,
+                              double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    const long n = (long)nx * ny;
+    for (int w = 0; w < nwall; ++w) {
+        const int k = walls[w];
+        for (long t = threadIdx.x; t < n; t += blockDim.x) {
+            const int i = (int)(t % nx), j = (int)(t / nx);
+            const double v = g02[((long)k * ny + j) * nxpp + i] + (ubarp ? ubarp[k] : 0.0);
+            s += xnu * v;
+        }
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+__global__ void k_smag_fac(const double* __restrict__ sum, int nwall, long nxny, const double* __restrict__ zpts,
+                           const double* __restrict__ delta, int nzp, double xnu, double cs,
+                           double* __restrict__ fac) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nzp) return;
+    const double utau_avg = sqrt(fabs(sum[0] / nwall / (double)nxny));
+    const double zplus = zpts[k] * utau_avg / xnu;
+    const double v = cs * delta[k] * (1. - exp(-zplus / 25.));
+    fac[k] = -(v * v);
+}
+
+// -------------------------------------------------- dynamic Smagorinsky
+// stage 1 (sgsterm.f90:171-205): S_ij, |S|, L1_ij = delta^2 |S| S_ij
+__global__ void k_dyn1(PhDims d, DynArgs a) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        double g[3][3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) g[l][m] = a.g[l + 3 * m][off];
+        double sij[3][3];
+        double sn = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                sij[l][m] = 0.5 * (g[l][m] + g[m][l]);
+                sn = sn + 2. * sij[l][m] * sij[l][m];
+            }
+        sn = sqrt(sn);
+        a.snorm[off] = sn;
+        const double dl = a.delta[k];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                const int q = l + 3 * m;
+                a.sij[q][off] = sij[l][m];
+                a.l1[q][off] = dl * dl * sn * sij[l][m];
+            }
+    }
+}
+// stage 2 (sgsterm.f90:187-238): |S^|, M_ij, L_ij, numerator and denominator
+__global__ void k_dyn2(PhDims d, DynArgs a) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        double sh[3][3];
+        double snh = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                sh[l][m] = a.sijhat[l + 3 * m][off];
+                snh = snh + 2. * sh[l][m] * sh[l][m];
+            }
+        snh = sqrt(snh);
+        const double cd = 2.0 * a.delta[k];
+        double uh[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) uh[l] = a.uihat[l][off];
+        double num = 0.0, den = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                const int q = l + 3 * m;
+                const double shat = cd * cd * snh * sh[l][m];
+                const double mij = 2.0 * (a.l1hat[q][off] - shat);
+                const double lij = a.uuhat[q][off] - uh[l] * uh[m];
+                num = num + mij * lij;
+                den = den + mij * mij;
+            }
+        a.num[off] = num;
+        a.den[off] = den;
+    }
+}
+// stage 3 (sgsterm.f90:248-267): C = <ML>/<MM> clipped to [0, 0.04], tau = -2 C delta^2 |S| S_ij
+__global__ void k_dyn3(PhDims d, DynArgs a) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        double ct = a.numf[off] / a.denf[off];
+        if (ct < 0.0 || isnan(ct)) ct = 0.0;
+        if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
+        const double dl = a.delta[k];
+        const double nu = ct * dl * dl * a.snorm[off];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                const int q = l + 3 * m;
+                if (a.tau[q]) a.tau[q][off] = -2.0 * nu * a.sij[q][off];
+            }
+    }
+}
+
+// ----------------------------------------------------- interscale model
+// sgsterm.f90:448-462: uibuj_lm = [u_l] u_m and tau = [[u_l]u_m] - [[u_l]][[u_m]] - [u_l](u_m - [u_m])
+__global__ void k_isc1(PhDims d, FieldList<double> ub, FieldList<double> ui, FieldList<double> out) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) out.p[l + 3 * m][off] = ub.p[l][off] * ui.p[m][off];
+    }
+}
+__global__ void k_isc2(PhDims d, FieldList<double> ubb, FieldList<double> ub2, FieldList<double> ub,
+                       FieldList<double> ui, FieldList<double> tau) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+                tau.p[l + 3 * m][off] = ubb.p[l + 3 * m][off] - ub2.p[l][off] * ub2.p[m][off] -
+                                        ub.p[l][off] * (ui.p[m][off] - ub.p[m][off]);
+    }
+}
+
+// ------------------------------------------------------------- 3-pt filter
+// filters.f90:363-613 at one rank: x then y periodic (a,b,a)/(2a+b) passes
+// (fused, rounding after each pass as in the reference), then the z pass with
+// interpolating weights on interior planes (k = 0 and nzp-1 keep the xy value).
+__global__ void k_filt_xy(PhDims d, FieldList<double> in, FieldList<double> out, int nf, double a, double b) {
+    const long n = d.npts * nf;
+    const double s = 2. * a + b;
+    const double s2 = 2.00 * a + b;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(t / d.npts);
+        const long e = t - (long)f * d.npts;
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        const double* q = in.p[f];
+        const int im = (i == 0) ? d.nx - 1 : i - 1;
+        const int ip = (i == d.nx - 1) ? 0 : i + 1;
+        const int jm = (j == 0) ? d.ny - 1 : j - 1;
+        const int jp = (j == d.ny - 1) ? 0 : j + 1;
+        const long pk = (long)k * d.ny;
+        double X[3];
+        const int js[3] = {jm, j, jp};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double* row = q + (pk + js[r]) * d.nxpp;
+            X[r] = (a * row[im] + b * row[i] + a * row[ip]) / s;
+        }
+        out.p[f][off] = (a * X[0] + b * X[1] + a * X[2]) / s2;
+    }
+}
+__global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, int nf, const double* __restrict__ w,
+                         double a, double b) {
+    const long n = d.npts * nf;
+    const double s = 2.0 * a + b;
+    const long plane = (long)d.ny * d.nxpp;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(t / d.npts);
+        const long e = t - (long)f * d.npts;
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        const double* q = in.p[f];
+        double v;
+        if (k == 0 || k == d.nzp - 1) {
+            v = q[off];
+        } else {
+            v = w[k] * q[off - plane] + w[d.nzp + k] * q[off] + w[2 * d.nzp + k] * q[off + plane];
+            v = v / s;
+        }
+        out.p[f][off] = v;
+    }
+}
+
+// ------------------------------------------------------------- launchers
+int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s) {
+    FieldList<double> fl{};
+    for (int i = 0; i < nf; ++i) fl.p[i] = f[i];
+    const long nrows = (long)c->nzp * c->ny;
+    hipLaunchKernelGGL(k_zero_pads, ph_grid(nrows * nf * 2), dim3(256), 0, s, fl, nf, nrows, c->nx, c->nxpp);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int ph_nonlinear(lsb_ctx* c, int mode, const NlArgs& a, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    switch (mode) {
+        case NL_NONE: hipLaunchKernelGGL(k_nonlinear<NL_NONE>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case NL_SIGMA: hipLaunchKernelGGL(k_nonlinear<NL_SIGMA>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case NL_STATIC: hipLaunchKernelGGL(k_nonlinear<NL_STATIC>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case NL_EXTTAU: hipLaunchKernelGGL(k_nonlinear<NL_EXTTAU>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case NL_PREP: hipLaunchKernelGGL(k_nonlinear<NL_PREP>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        default: set_error("nonlinear: bad mode"); return LSB_E_ARG;
+    }
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int ph_hcombine(lsb_ctx* c, const double* const* uu, const double* const* tau, double* const* h, int n, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    FieldList<double> a{}, b{}, o{};
+    for (int q = 0; q < n; ++q) {
+        a.p[q] = const_cast<double*>(uu[q]);
+        b.p[q] = const_cast<double*>(tau[q]);
+        o.p[q] = h[q];
+    }
+    hipLaunchKernelGGL(k_hcombine, ph_grid(d.npts), dim3(256), 0, s, d, a, b, o, n);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int ph_smag_factor(lsb_ctx* c, const double* g02, const double* ubarp, double cs, hipStream_t s) {
+    std::vector<int> walls;
+    for (int k = 0; k < c->nzp; ++k)
+        if (c->zpts[k] == 0.0 || std::fabs(c->zpts[k]) == 0.5) walls.push_back(k);
+    if (walls.empty()) {
+        set_error("static_smagorinsky: no wall plane (zpts == 0 or |zpts| == 0.5)");
+        return LSB_E_ARG;
+    }
+    int* dw = reinterpret_cast<int*>(c->d_red + 16);
+    LSB_HIP(hipMemcpyAsync(dw, walls.data(), walls.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_utau_reduce, dim3(1), dim3(256), 0, s, g02, ubarp, c->nx, c->ny, c->nxpp, dw,
+                       (int)walls.size(), c->cfg.xnu, c->d_red);
+    LSB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_smag_fac, dim3((c->nzp + 63) / 64), dim3(64), 0, s, c->d_red, (int)walls.size(),
+                       (long)c->nx * c->ny, c->d_zpts, c->d_delta, c->nzp, c->cfg.xnu, cs, c->d_smag);
+    LSB_HIP(hipGetLastError());
+    // the kernel above wrote the per-plane factors; stage the wall list lifetime
+    LSB_HIP(hipStreamSynchronize(s));
+    return LSB_OK;
+}
+
+int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, double a, double b, int interpflag,
+               double* tmp, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    const double* w;
+    if (a == c->cfg.a && b == c->cfg.b)
+        w = c->d_fz;
+    else if (a == 1.0 && b == 1.0)
+        w = c->d_fz + 3 * c->nzp;
+    else
+        w = nullptr;
+    std::vector<double> hw;
+    double* wtmp = nullptr;
+    if (interpflag != 1 || w == nullptr) {
+        // general (a, b) or interpflag = 0: uniform weights a, b, a or interpolating with (a, b)
+        hw.assign(3 * c->nzp, 0.0);
+        if (interpflag != 1) {
+            for (int k = 0; k < c->nzp; ++k) {
+                hw[k] = a;
+                hw[c->nzp + k] = b;
+                hw[2 * c->nzp + k] = a;
+            }
+        } else {
+            set_error("filter_q_3pt: (a, b) must be the configured pair or (1, 1)");
+            return LSB_E_ARG;
+        }
+        wtmp = c->d_red + 64;
+        LSB_HIP(hipMemcpyAsync(wtmp, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        w = wtmp;
+    }
+    for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+        const int nb = (nf - f0 < kMaxFields) ? nf - f0 : kMaxFields;
+        FieldList<double> fi{}, ft{}, fo{};
+        for (int q = 0; q < nb; ++q) {
+            fi.p[q] = const_cast<double*>(in[f0 + q]);
+            ft.p[q] = tmp + (size_t)q * c->nphys;
+            fo.p[q] = out[f0 + q];
+        }
+        hipLaunchKernelGGL(k_filt_xy, ph_grid(d.npts * nb), dim3(256), 0, s, d, fi, ft, nb, a, b);
+        LSB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_filt_z, ph_grid(d.npts * nb), dim3(256), 0, s, d, ft, fo, nb, w, a, b);
+        LSB_HIP(hipGetLastError());
+    }
+    prof_mark_end(c, tok, s);
+    if (wtmp) LSB_HIP(hipStreamSynchronize(s));
+    return LSB_OK;
+}
+
+int ph_dyn(lsb_ctx* c, int stage, const DynArgs& a, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL(k_dyn1, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case 2: hipLaunchKernelGGL(k_dyn2, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        default: hipLaunchKernelGGL(k_dyn3, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int ph_interscale(lsb_ctx* c, int stage, const FieldList<double>& p0, const FieldList<double>& p1,
+                  const FieldList<double>& p2, const FieldList<double>& p3, const FieldList<double>& p4,
+                  hipStream_t s) {
+    PhDims d = ph_dims(c);
+    if (stage == 1)
+        hipLaunchKernelGGL(k_isc1, ph_grid(d.npts), dim3(256), 0, s, d, p0, p1, p2);
+    else
+        hipLaunchKernelGGL(k_isc2, ph_grid(d.npts), dim3(256), 0, s, d, p0, p1, p2, p3, p4);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+}  // namespace lsb

@@ -1,0 +1,47 @@
+// Argument blocks and launchers of the fused spectral step kernels.
+#pragma once
+
+#include "lsb_internal.h"
+
+namespace lsb {
+
+struct StepRhsArgs {
+    const double* nlt[3];  // forward FFT of the convective term
+    const double* h0[3];   // hij(l, x) spectral
+    const double* h1[3];   // hij(l, y) spectral
+    const double* hz[3];   // dfdz(hij(l, z))
+    double* nout[3];       // N_l (becomes nlnif)
+    const double* nl[3];   // nlnif
+    const double* nlm[3];  // nlnmif
+    const double* u[3];
+    const double* un[3];
+    const double* unm[3];
+    double* rhs[3];
+    const double* ubarpp;  // nzp (device)
+    double ab0, ab1, ab2, bdf0, bdf1, bdf2, dt, dtnui;
+};
+
+struct StepBcArgs {
+    const double* rhs0;    // plane 0 of rhs_u
+    const double* rhs1;    // plane 0 of rhs_v
+    const double* p_wall;  // plane nzp-1 of pf
+    const double* pn_wall; // plane nzp-1 of pnf
+    const double* d2w;     // 2 planes: rows 0 and nzp-1 of d2fdz2(w^n)
+    const double* bnd_top; // bound() spectral plane 0 (nullable)
+    const double* bnd_bot; // bound() spectral plane nzp-1 (nullable)
+    double* bct[3];
+    double* bcb[3];
+    double dtnui, bdfv, dt, xnu, pfac0, pfac1;
+    int nhom;
+};
+
+int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s);
+int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s);
+int sp_press_rhs(lsb_ctx* c, const double* dudz, const double* dvdz, const double* u, const double* v,
+                 const double* w, double* divf, double* rhs, hipStream_t s);
+int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, double* u, double* v, double dt,
+               double bdfv, hipStream_t s);
+int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,
+                  int filtery, hipStream_t s);
+
+}  // namespace lsb

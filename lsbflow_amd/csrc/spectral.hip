@@ -1,0 +1,382 @@
+// Pointwise kernels on spectral fields (one thread per complex (kx,ky,z) mode).
+//
+// Spectral layout: [nzp][ny][nx/2] complex; pair index pr -> m = pr % lh (kx),
+// j = (pr / lh) % ny (ky), k = pr / (lh*ny) (z).  wavx is stored per real
+// index like the reference (wavx(2m-1) = wavx(2m)), so kx(m) = wavx[2m].
+
+#include <cmath>
+
+#include "lsb_internal.h"
+#include "spectral.h"
+
+namespace lsb {
+
+struct SpDims {
+    long npairs;  // nzp*ny*lh
+    int lh, ny;
+    long plane_pairs;
+};
+
+static SpDims sp_dims(const lsb_ctx* c, int nplanes) {
+    SpDims d;
+    d.lh = c->nx / 2;
+    d.ny = c->ny;
+    d.plane_pairs = (long)d.lh * d.ny;
+    d.npairs = d.plane_pairs * nplanes;
+    return d;
+}
+
+static dim3 sp_grid(long n) {
+    long b = (n + 255) / 256;
+    if (b > 65535L * 8) b = 65535L * 8;
+    return dim3((unsigned)b);
+}
+
+#define SP_LOOP(n) for (long pr = (long)blockIdx.x * blockDim.x + threadIdx.x; pr < (n); pr += (long)gridDim.x * blockDim.x)
+
+typedef double2 c2;
+
+// derivatives.f90:261-303: d/dx -> (-kx im, kx re), d/dy -> (-ky im, ky re)
+__global__ void k_dfdx(SpDims d, const double* __restrict__ wavx, const c2* __restrict__ in, c2* __restrict__ out) {
+    SP_LOOP(d.npairs) {
+        const int m = (int)(pr % d.lh);
+        const double w = wavx[2 * m];
+        const c2 u = in[pr];
+        out[pr] = make_double2(-w * u.y, w * u.x);
+    }
+}
+__global__ void k_dfdy(SpDims d, const double* __restrict__ wavy, const c2* __restrict__ in, c2* __restrict__ out) {
+    SP_LOOP(d.npairs) {
+        const int j = (int)((pr / d.lh) % d.ny);
+        const double w = wavy[j];
+        const c2 u = in[pr];
+        out[pr] = make_double2(-w * u.y, w * u.x);
+    }
+}
+// derivatives.f90:394-409
+__global__ void k_divuv(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                        const c2* __restrict__ u, const c2* __restrict__ v, c2* __restrict__ out) {
+    SP_LOOP(d.npairs) {
+        const int m = (int)(pr % d.lh);
+        const int j = (int)((pr / d.lh) % d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const c2 a = u[pr], b = v[pr];
+        out[pr] = make_double2(-wx * a.y - wy * b.y, wx * a.x + wy * b.x);
+    }
+}
+// derivatives.f90:306-326 lapl: d2u - (kx^2 + ky^2) u
+__global__ void k_lapl(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                       const c2* __restrict__ d2u, const c2* __restrict__ u, c2* __restrict__ out) {
+    SP_LOOP(d.npairs) {
+        const int m = (int)(pr % d.lh);
+        const int j = (int)((pr / d.lh) % d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const double w2 = wx * wx + wy * wy;
+        const c2 a = d2u[pr], b = u[pr];
+        out[pr] = make_double2(a.x - w2 * b.x, a.y - w2 * b.y);
+    }
+}
+
+// filters.f90:281-360 specfilter_exp_xy: magnitude/phase form of the radial
+// exponential filter exp(-amp theta^(2p)) for |k| >= kc*kmax.
+__device__ __forceinline__ c2 specfilter_mode(c2 u, double ak, double akmax, double kc, double p, double amp) {
+    const double fr = u.x, fi = u.y;
+    const double fmag = sqrt(fr * fr + fi * fi);
+    if (fmag == 0.0) return make_double2(0.0, 0.0);
+    const double cosc = fr / fmag;
+    const double sinc = fi / fmag;
+    double phase = acos(cosc);
+    if (sinc < 0.0) phase = -phase;
+    const double akc = akmax * kc;
+    double filt = 1.0;
+    if (fabs(ak) >= akc) {
+        const double theta = (fabs(ak) - akc) / (akmax - akc);
+        filt = exp(-amp * pow(theta, 2.0 * p));
+    }
+    const double fmagf = fmag * filt;
+    return make_double2(fmagf * cos(phase), fmagf * sin(phase));
+}
+
+__global__ void k_specfilter(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                             FieldList<double> in, FieldList<double> out, int nf, double akmax, double kc, double p,
+                             double amp, int filtery) {
+    const long n = d.npairs * nf;
+    SP_LOOP(n) {
+        const int f = (int)(pr / d.npairs);
+        const long q = pr - (long)f * d.npairs;
+        const int m = (int)(q % d.lh);
+        const int j = (int)((q / d.lh) % d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const double ak = filtery == 1 ? sqrt(wx * wx + wy * wy) : wx;
+        const c2 u = reinterpret_cast<const c2*>(in.p[f])[q];
+        reinterpret_cast<c2*>(out.p[f])[q] = specfilter_mode(u, ak, akmax, kc, p, amp);
+    }
+}
+
+// horfft.f90:153-224 norm on the kx = 0 column and the ky-Nyquist row.
+// One thread per (field, z, j) with j in 0..ny-1 for the Nyquist row and the
+// symmetric pairs handled by the thread owning the lower index.
+__global__ void k_norm(FieldList<double> f, int nf, int nzp, int ny, int nx, int kx0_zero) {
+    const int nyh = ny / 2;
+    const long n = (long)nf * nzp * ny;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % ny);
+        const long kf = t / ny;
+        const int k = (int)(kf % nzp);
+        const int fi = (int)(kf / nzp);
+        double* pl = f.p[fi] + (long)k * nx * ny;
+        if (kx0_zero) {
+            if (j == 0) pl[1] = 0.0;
+            if (ny > 2 && j >= 1 && j <= nyh - 1) {
+                const int jp = ny - j;  // 0-based partner of j (ny + 2 - (j+1) - 1)
+                double* a = pl + (long)j * nx;
+                double* b = pl + (long)jp * nx;
+                const double re = 0.5 * (a[0] + b[0]);
+                const double im = 0.5 * (a[1] - b[1]);
+                a[0] = re;
+                a[1] = im;
+                b[0] = re;
+                b[1] = -im;
+            }
+        }
+    }
+}
+__global__ void k_norm_nyq(FieldList<double> f, int nf, int nzp, int ny, int nx) {
+    const int nyh = ny / 2;
+    const long n = (long)nf * nzp * nx;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const long kf = t / nx;
+        const int k = (int)(kf % nzp);
+        const int fi = (int)(kf / nzp);
+        f.p[fi][((long)k * ny + nyh) * nx + i] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+int sp_dfdx(lsb_ctx* c, const double* in, double* out, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    hipLaunchKernelGGL(k_dfdx, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, (const c2*)in, (c2*)out);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_dfdy(lsb_ctx* c, const double* in, double* out, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    hipLaunchKernelGGL(k_dfdy, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavy, (const c2*)in, (c2*)out);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_divuv(lsb_ctx* c, const double* u, const double* v, double* out, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    hipLaunchKernelGGL(k_divuv, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)u,
+                       (const c2*)v, (c2*)out);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_lapl_combine(lsb_ctx* c, const double* d2u, const double* u, double* out, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    hipLaunchKernelGGL(k_lapl, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)d2u,
+                       (const c2*)u, (c2*)out);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,
+                  int filtery, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    const double wavxmax = c->nxh * c->alpha;
+    const double wavymax = c->nyh * c->beta;
+    double akmax = wavxmax;
+    if (filtery == 1) akmax = sqrt(wavxmax * wavxmax + wavymax * wavymax);
+    FieldList<double> fi{}, fo{};
+    for (int f = 0; f < nf; ++f) {
+        fi.p[f] = const_cast<double*>(in[f]);
+        fo.p[f] = out[f];
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    hipLaunchKernelGGL(k_specfilter, sp_grid(d.npairs * nf), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, fi, fo, nf,
+                       akmax, kc, p, amp, filtery);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_norm(lsb_ctx* c, double* const* f, int nf, hipStream_t s) {
+    FieldList<double> fl{};
+    for (int i = 0; i < nf; ++i) fl.p[i] = f[i];
+    const long n1 = (long)nf * c->nzp * c->ny;
+    const int kx0 = (c->wavx[0] == 0.0) ? 1 : 0;
+    hipLaunchKernelGGL(k_norm, sp_grid(n1), dim3(256), 0, s, fl, nf, c->nzp, c->ny, c->nx, kx0);
+    LSB_HIP(hipGetLastError());
+    if (c->ny > 2) {
+        const long n2 = (long)nf * c->nzp * c->nx;
+        hipLaunchKernelGGL(k_norm_nyq, sp_grid(n2), dim3(256), 0, s, fl, nf, c->nzp, c->ny, c->nx);
+        LSB_HIP(hipGetLastError());
+    }
+    return LSB_OK;
+}
+
+// ------------------------------------------------------- step kernels
+// N_l = nlt_l - dfdx(h_l0) - dfdy(h_l1) - dhdz_l   (navierstokes.f90:781-795),
+// then EX3/BDF3 right-hand side (navierstokes.f90:1236-1296):
+//   rhs_l = -dtnui*( dt*(ab0 N + ab1 Nn + ab2 Nnm) + bdf0 u + bdf1 un + bdf2 unm )
+//   rhs_0(kx=0,ky=0,z) -= ubarpp(z)
+// N_l is also written (it becomes the next step's nlnif).
+__global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy, StepRhsArgs a) {
+    const long n = d.npairs * 3;
+    SP_LOOP(n) {
+        const int l = (int)(pr / d.npairs);
+        const long q = pr - (long)l * d.npairs;
+        const int m = (int)(q % d.lh);
+        const long jk = q / d.lh;
+        const int j = (int)(jk % d.ny);
+        const int k = (int)(jk / d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const c2 nl = reinterpret_cast<const c2*>(a.nlt[l])[q];
+        const c2 h0 = reinterpret_cast<const c2*>(a.h0[l])[q];
+        const c2 h1 = reinterpret_cast<const c2*>(a.h1[l])[q];
+        const c2 hz = reinterpret_cast<const c2*>(a.hz[l])[q];
+        double nr = nl.x, ni = nl.y;
+        nr = nr - (-wx * h0.y);
+        ni = ni - (wx * h0.x);
+        nr = nr - (-wy * h1.y);
+        ni = ni - (wy * h1.x);
+        nr = nr - hz.x;
+        ni = ni - hz.y;
+        reinterpret_cast<c2*>(a.nout[l])[q] = make_double2(nr, ni);
+        const c2 n1 = reinterpret_cast<const c2*>(a.nl[l])[q];
+        const c2 n2 = reinterpret_cast<const c2*>(a.nlm[l])[q];
+        const c2 u0 = reinterpret_cast<const c2*>(a.u[l])[q];
+        const c2 u1 = reinterpret_cast<const c2*>(a.un[l])[q];
+        const c2 u2 = reinterpret_cast<const c2*>(a.unm[l])[q];
+        const double cr = a.ab0 * nr + a.ab1 * n1.x + a.ab2 * n2.x;
+        const double ci = a.ab0 * ni + a.ab1 * n1.y + a.ab2 * n2.y;
+        double rr = -a.dtnui * (a.dt * cr + a.bdf0 * u0.x + a.bdf1 * u1.x + a.bdf2 * u2.x);
+        double ri = -a.dtnui * (a.dt * ci + a.bdf0 * u0.y + a.bdf1 * u1.y + a.bdf2 * u2.y);
+        if (l == 0 && m == 0 && j == 0) rr = rr - a.ubarpp[k];
+        reinterpret_cast<c2*>(a.rhs[l])[q] = make_double2(rr, ri);
+    }
+}
+
+// Boundary planes of the viscous step (navierstokes.f90:1321-1373), one thread
+// per (kx,ky) mode.  Writes bc_top[l], bc_bot[l] (one plane each).
+__global__ void k_bc_vel(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy, StepBcArgs a) {
+    SP_LOOP(d.plane_pairs) {
+        const int m = (int)(pr % d.lh);
+        const int j = (int)(pr / d.lh);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const double wav2 = wx * wx + wy * wy;
+        const double den = wav2 + a.dtnui / a.bdfv;
+        // top (plane 0): -rhs/(wav2 + dtnui/bdfv) for u and v
+        const c2 r0 = reinterpret_cast<const c2*>(a.rhs0)[pr];
+        const c2 r1 = reinterpret_cast<const c2*>(a.rhs1)[pr];
+        reinterpret_cast<c2*>(a.bct[0])[pr] = make_double2(-r0.x / den, -r0.y / den);
+        reinterpret_cast<c2*>(a.bct[1])[pr] = make_double2(-r1.x / den, -r1.y / den);
+        // wall (plane nzp-1): dt*bdfv*d(p*)/dx, dt*bdfv*d(p*)/dy, p* = pfac0 p + pfac1 pn
+        const c2 p0 = reinterpret_cast<const c2*>(a.p_wall)[pr];
+        const c2 p1 = reinterpret_cast<const c2*>(a.pn_wall)[pr];
+        const double psr = a.pfac0 * p0.x + a.pfac1 * p1.x;
+        const double psi = a.pfac0 * p0.y + a.pfac1 * p1.y;
+        const double dtb = a.dt * a.bdfv;
+        reinterpret_cast<c2*>(a.bcb[0])[pr] = make_double2(dtb * (-wx * psi), dtb * (wx * psr));
+        reinterpret_cast<c2*>(a.bcb[1])[pr] = make_double2(dtb * (-wy * psi), dtb * (wy * psr));
+        // w: imposed field (bound) plus the Zang-Hussaini correction xnu*dt*bdfv*d2w/dz2
+        const c2 d2t = reinterpret_cast<const c2*>(a.d2w)[pr];                // row 0
+        const c2 d2b = reinterpret_cast<const c2*>(a.d2w)[d.plane_pairs + pr];  // row nzp-1
+        const double zh = a.xnu * a.dt * a.bdfv;
+        if (a.nhom) {
+            const c2 gt = a.bnd_top ? reinterpret_cast<const c2*>(a.bnd_top)[pr] : make_double2(0.0, 0.0);
+            const c2 gb = a.bnd_bot ? reinterpret_cast<const c2*>(a.bnd_bot)[pr] : make_double2(0.0, 0.0);
+            reinterpret_cast<c2*>(a.bct[2])[pr] = make_double2(gt.x + zh * d2t.x, gt.y + zh * d2t.y);
+            reinterpret_cast<c2*>(a.bcb[2])[pr] = gb;
+        } else {
+            reinterpret_cast<c2*>(a.bct[2])[pr] = make_double2(zh * d2t.x, zh * d2t.y);
+            reinterpret_cast<c2*>(a.bcb[2])[pr] = make_double2(zh * d2b.x, zh * d2b.y);
+        }
+    }
+}
+
+// Pressure-step right-hand side (navierstokes.f90:1411-1422):
+//   divdz = divuv(dudz, dvdz); divf = divuv(u, v); rhs3 = -divdz - wav2 w
+__global__ void k_press_rhs(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                            const c2* __restrict__ dudz, const c2* __restrict__ dvdz, const c2* __restrict__ u,
+                            const c2* __restrict__ v, const c2* __restrict__ w, c2* __restrict__ divf,
+                            c2* __restrict__ rhs) {
+    SP_LOOP(d.npairs) {
+        const int m = (int)(pr % d.lh);
+        const int j = (int)((pr / d.lh) % d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const c2 a = dudz[pr], b = dvdz[pr];
+        const double ddr = -wx * a.y - wy * b.y;
+        const double ddi = wx * a.x + wy * b.x;
+        const c2 uu = u[pr], vv = v[pr];
+        divf[pr] = make_double2(-wx * uu.y - wy * vv.y, wx * uu.x + wy * vv.x);
+        const double wav2 = wx * wx + wy * wy;
+        const c2 ww = w[pr];
+        rhs[pr] = make_double2(-ddr - wav2 * ww.x, -ddi - wav2 * ww.y);
+    }
+}
+
+// Projection (navierstokes.f90:1466-1488):
+//   p = -(dwdz + divf)/(wav2 dt bdfv) (0 where wav2 = 0); u -= dt bdfv dp/dx; v -= dt bdfv dp/dy
+__global__ void k_project(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                          const c2* __restrict__ dwdz, const c2* __restrict__ divf, c2* __restrict__ p,
+                          c2* __restrict__ u, c2* __restrict__ v, double dt, double bdfv) {
+    SP_LOOP(d.npairs) {
+        const int m = (int)(pr % d.lh);
+        const int j = (int)((pr / d.lh) % d.ny);
+        const double wx = wavx[2 * m], wy = wavy[j];
+        const double wav2 = wx * wx + wy * wy;
+        double pr_ = 0.0, pi_ = 0.0;
+        if (wav2 != 0.0) {
+            const c2 a = dwdz[pr], b = divf[pr];
+            const double den = wav2 * dt * bdfv;
+            pr_ = -(a.x + b.x) / den;
+            pi_ = -(a.y + b.y) / den;
+        }
+        p[pr] = make_double2(pr_, pi_);
+        const double dtb = dt * bdfv;
+        const c2 uu = u[pr], vv = v[pr];
+        u[pr] = make_double2(uu.x - dtb * (-wx * pi_), uu.y - dtb * (wx * pr_));
+        v[pr] = make_double2(vv.x - dtb * (-wy * pi_), vv.y - dtb * (wy * pr_));
+    }
+}
+
+int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    hipLaunchKernelGGL(k_rhs_vel, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s) {
+    SpDims d = sp_dims(c, 1);
+    hipLaunchKernelGGL(k_bc_vel, sp_grid(d.plane_pairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_press_rhs(lsb_ctx* c, const double* dudz, const double* dvdz, const double* u, const double* v,
+                 const double* w, double* divf, double* rhs, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    hipLaunchKernelGGL(k_press_rhs, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)dudz,
+                       (const c2*)dvdz, (const c2*)u, (const c2*)v, (const c2*)w, (c2*)divf, (c2*)rhs);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, double* u, double* v, double dt,
+               double bdfv, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok);
+    hipLaunchKernelGGL(k_project, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)dwdz,
+                       (const c2*)divf, (c2*)p, (c2*)u, (c2*)v, dt, bdfv);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+}  // namespace lsb

@@ -1,0 +1,1126 @@
+"""FP64 numpy restatement of lsbflow's per-time-step hot path (TEST ORACLE ONLY).
+
+Layout convention (identical to the reference's Fortran memory layout):
+
+* spectral field  Fortran ``qf(nxpl, ny, nzp)``  ->  numpy ``(nzp, ny, nx)``;
+  real index ``2m`` / ``2m+1`` hold Re/Im of kx = ``wavx[2m]``.
+* physical field  Fortran ``q(nxpp, nypl, nzp)``  ->  numpy ``(nzp, ny, nx+2)``.
+* rank-1 vector   Fortran ``(..., 3)``  ->  numpy ``(3, nzp, ...)`` indexed ``[l]``.
+* rank-2 tensor   Fortran ``(..., 3, 3)`` element ``(l, m)``  ->  numpy
+  ``(3, 3, nzp, ...)`` indexed ``[l][m]`` (the C-ABI buffers of the product are
+  Fortran-ordered, i.e. ``[m][l]`` in memory; tests convert).
+
+One MPI rank (P = 1): nxpl = nx, nypl = ny, nzpl = nzp.
+
+Pinning (tests/test_oracle_*.py):
+  setup_grid           -- run/grid.dat, bit for bit
+  build_cheb_zder_matrix + diagonalize_matrix -- run/output.txt:53-84 eigenvalues
+  blasfl + dpdz/dct_diff -- run/paral.dat
+  horfft/norm/dfdx/dfdy -- single-mode closed forms, Nyquist drop, round trip
+  dct_diff             -- exactness on low Chebyshev polynomials, Q2 top mode
+  solve_helmholtz/nhom -- residual of (D2-k2-s)u = rhs, E L E^-1 = D2
+  sigma_model          -- singular values of g from numpy SVD
+  static_smagorinsky / filter_q_3pt / specfilter_exp_xy / dealiasing -- brute
+                          force and closed forms on tiny grids
+  navierstokes_update  -- divergence-free projection, Stokes (0,0)-mode decay,
+                          linear-operator invariants; parity unpinned against
+                          reference-produced fields (the reference cannot run here)
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+import scipy.fft as sfft
+
+PI = 4.0 * math.atan(1.0)
+
+
+@dataclass
+class OracleConfig:
+    """The namelist values the hot path reads (common.f90:20,65,78,92,107)."""
+
+    nx: int = 64
+    ny: int = 32
+    nz: int = 64
+    xlen: float = 2.0 * PI
+    ylen: float = PI
+    zlen: float = 2.0
+    x0: float = 0.0
+    u0: float = 1.0
+    xnu: float = 1.0e-3
+    prtl: float = 0.71
+    coex: float = 0.0
+    gx: float = 0.0
+    gy: float = 0.0
+    gz: float = 0.0
+    mapping: int = 3
+    blfac: float = 8.0
+    bctype: int = 3
+    samp: float = 0.0
+    istartbound: float = 0.0
+    forcing: int = 0
+    sgsflag: int = 0
+    sgsmodel: int = 0
+    a: float = 1.0
+    b: float = 2.0
+    cs: float = 0.1
+    kc: float = 0.6
+    filt_order: float = 1.0
+    filt_amp: float = 20.0
+    dealiasflag: int = 2
+    dealias_ref_compat: int = 1
+    ifrz: int = 0
+    nstart: int = 0
+    dt: float = 1.0e-3
+
+    def with_(self, **kw) -> "OracleConfig":
+        return replace(self, **kw)
+
+
+class Oracle:
+    """Module state (dim, grid, mats, paral, time, core) + the hot-path routines."""
+
+    def __init__(self, cfg: OracleConfig, build_mats: bool = True):
+        self.cfg = cfg
+        c = cfg
+        # init_dim, common.f90:23-54 (P = 1)
+        self.nx, self.ny, self.nz = c.nx, c.ny, c.nz
+        self.nzp = c.nz + 1
+        self.nzm = c.nz - 1
+        self.nxh = c.nx // 2
+        self.nyh = c.ny // 2
+        self.nxpp = c.nx + 2
+        self.setup_grid()
+        if build_mats:
+            self.setup_zder()
+        self.ubar = np.zeros(self.nzp)
+        self.ubarp = np.zeros(self.nzp)
+        self.ubarpp = np.zeros(self.nzp)
+        self.wbar = np.zeros(self.nzp)
+        self.wbarp = np.zeros(self.nzp)
+        self.wbarpp = np.zeros(self.nzp)
+        self.tbar = np.zeros(self.nzp)
+        self.tbarp = np.zeros(self.nzp)
+        self.tbarpp = np.zeros(self.nzp)
+
+    # ------------------------------------------------------------------ setup
+    def setup_grid(self):
+        """init.f90:130-273 (linstab = 0)."""
+        c = self.cfg
+        nx, ny, nz, nzp = self.nx, self.ny, self.nz, self.nzp
+        twopi = 8.0 * math.atan(1.0)
+        self.dx = c.xlen / (nx - 1.0)                       # Q4: xlen/(nx-1)
+        self.xpts = np.array([c.x0 + i * self.dx for i in range(nx)])
+        self.dy = c.ylen / (ny - 1.0)
+        self.ypts = np.array([j * self.dy for j in range(ny)])
+        self.alpha = twopi / c.xlen
+        wavx = np.zeros(nx)
+        for i in range(nx // 2):
+            wavx[2 * i] = (i + 0.0) * self.alpha            # (myid*nxhpl + i - 1.)*alpha
+            wavx[2 * i + 1] = wavx[2 * i]
+        self.wavx = wavx
+        self.wavxx = wavx * wavx
+        self.beta = twopi / c.ylen
+        wavy = np.zeros(ny)
+        for j in range(1, ny // 2 + 1):
+            wavy[ny - j] = -j * self.beta
+            wavy[j - 1] = (j - 1.0) * self.beta
+        self.wavy = wavy
+        self.wavyy = wavy * wavy
+        xi = np.zeros(nzp)
+        zpts = np.zeros(nzp)
+        gf = np.zeros(nzp)
+        self.aa = 0.0
+        self.bb = 0.0
+        m = c.mapping
+        if m == 0:
+            self.xl = c.zlen
+            for k in range(nzp):
+                xi[k] = self.xl / 2.0 * math.cos(PI * (k + 0.0) / nz)
+                zpts[k] = xi[k]
+                gf[k] = 1.0
+        elif m == 1:
+            self.xl = 1.0
+            self.aa = 1.0
+            self.bb = 1.875 * math.sqrt(c.xnu * c.x0 / c.u0)
+            for k in range(1, nzp):
+                xi[k] = self.xl / 2.0 * (1.0 + math.cos(PI * (k + 0.0) / nz))
+                zpts[k] = self.bb * xi[k] / (self.aa - xi[k])
+                gf[k] = (self.aa - xi[k]) ** 2 / (self.bb * self.aa)
+            xi[0] = 1.0
+            zpts[0] = 1.0e20
+            gf[0] = 0.0
+        elif m == 2:
+            self.xl = 1.0
+            self.bb = 1.875 * math.sqrt(c.xnu * (c.x0 + c.blfac * c.xlen) / c.u0)
+            self.aa = 1.0 + self.bb / c.zlen
+            for k in range(nzp):
+                xi[k] = self.xl / 2.0 * (1.0 + math.cos(PI * (k + 0.0) / float(nz)))
+                zpts[k] = self.bb * xi[k] / (self.aa - xi[k])
+                gf[k] = (self.aa - xi[k]) ** 2 / (self.aa * self.bb)
+        elif m == 3:
+            self.xl = c.zlen
+            for k in range(nzp):
+                xi[k] = self.xl / 2.0 * math.cos(PI * (k + 0.0) / float(nz))
+                zpts[k] = self.xl / 2.0 + xi[k]
+                gf[k] = 1.0
+        else:
+            raise ValueError("mapping must be 0..3")
+        self.xi, self.zpts, self.gf = xi, zpts, gf
+
+    # chebyshev.f90:51-159 -------------------------------------------------
+    @staticmethod
+    def _ff(i, nr):
+        if i == 0:
+            return 0.0
+        v = nr * 0.5 / math.tan(PI * i / (2.0 * nr))
+        if i % 2 == 0:
+            v = -v
+        return v
+
+    @classmethod
+    def _xnum(cls, kr, jr, nr):
+        if kr == 0:
+            if jr == 0:
+                return (1.0 / 3.0) * nr * nr + 1.0 / 6.0
+            if jr == nr:
+                v = 0.5
+                if nr % 2 == 1:
+                    v = -v
+                return v
+            t = 1.0 / math.tan(PI * jr / (2.0 * nr))
+            s = math.sin(PI * jr / (2.0 * nr))
+            return (0.5 * (nr + 0.5) * ((nr + 0.5) + t ** 2) + 1.0 / 8.0
+                    - 0.25 / (s ** 2.0) - 0.5 * nr * nr)
+        return cls._ff(kr + jr, nr) + cls._ff(kr - jr, nr)
+
+    @staticmethod
+    def _den(kr, jr, nr):
+        if kr == 0:
+            d = 0.5 * nr
+            if jr % 2 == 1:
+                d = -d
+            if jr == 0 or jr == nr:
+                d = 1.0
+            return d
+        d = nr * math.sin(PI * kr / nr)
+        if jr == 0 or jr == nr:
+            d = d * 2.0
+        return d
+
+    def _del(self, k, j, n):
+        """chebyshev.f90:51-77 (1-based k, j)."""
+        kr, jr, nr = k - 1, j - 1, n - 1
+        fac = 1.0
+        if kr == nr:
+            kr = 0
+            jr = nr - jr
+            fac = -1.0
+        d = fac * self._xnum(kr, jr, nr) / self._den(kr, jr, nr)
+        return d * (2.0 / self.xl)
+
+    def build_cheb_zder_matrix(self):
+        """chebyshev.f90:9-48: dor, d = diag(gf) dor, d2 = d d."""
+        n = self.nzp
+        dor = np.zeros((n, n))
+        for j in range(1, n + 1):
+            for i in range(1, n + 1):
+                dor[i - 1, j - 1] = self._del(i, j, n)
+        d = dor * self.gf[:, None]
+        d2 = d @ d
+        return dor, d, d2
+
+    @staticmethod
+    def diagonalize_matrix(D2):
+        """helmholtz.f90:13-126: D2 = E diag(lamda) E^-1 (dggev with B = I; dgetrf/dgetri)."""
+        w, v = np.linalg.eig(D2)
+        if np.max(np.abs(w.imag)) > 1e-8 * np.max(np.abs(w.real)):
+            raise RuntimeError("complex eigenvalues in D2 interior block")
+        lam = w.real.copy()
+        e = v.real.copy()
+        ei = np.linalg.inv(e)
+        return lam, e, ei
+
+    def setup_zder(self):
+        """init.f90:276-293."""
+        self.dor, self.d, self.d2 = self.build_cheb_zder_matrix()
+        nz = self.nz
+        self.diag, self.smat, self.simat = self.diagonalize_matrix(self.d2[1:nz, 1:nz].copy())
+
+    # ---------------------------------------------------------- blasfl/setup_ic
+    def blasfl(self, eps, iters, alph, zpts, dor, d, d2):
+        """init.f90:497-586 (Newton iteration with dgesv)."""
+        nzp = zpts.shape[0]
+        nzpp = nzp + 1
+        f = (-1.0 + np.exp(-zpts * alph)) / alph
+        d3 = d2 @ d
+        rat = 1.0 / (1.0 + zpts)
+        it_done = iters
+        for ii in range(1, iters + 1):
+            rhs = d2 @ f
+            A = np.zeros((nzpp, nzpp))
+            A[:nzp, :nzp] = (0.5 * rat * f)[:, None] * d2 + rat[:, None] * d3 + (0.5 * (zpts * rat))[:, None] * d2
+            A[np.arange(nzp), np.arange(nzp)] += 0.5 * rat * rhs
+            A[:nzp, nzpp - 1] = d[:nzp, nzp - 1]
+            A[0, :nzp] = dor[0, :nzp]
+            A[nzp - 1, :nzp] = 0.0
+            A[nzpp - 1, :nzp] = d[nzp - 1, :nzp]
+            A[0, nzpp - 1] = 0.0
+            A[nzp - 1, nzpp - 1] = 0.0
+            A[nzpp - 1, nzpp - 1] = 0.0
+            A[nzp - 1, nzp - 1] = 1.0
+            t1 = d3 @ f
+            t2 = d2 @ f
+            df = np.zeros(nzpp)
+            df[:nzp] = -0.5 * rat * f * t2 - 0.5 * zpts * rat * t2 - rat * t1
+            df[0] = 0.0
+            df[nzp - 1] = 0.0
+            df[nzpp - 1] = 0.0
+            df = np.linalg.solve(A, df)
+            f = f + df[:nzp]
+            err = np.sum(np.abs(df[:nzp]))
+            if err <= eps:
+                it_done = ii
+                break
+        ubar = 1.0 + d @ f
+        wbar = 0.5 * (zpts * ubar - (f + zpts))
+        return ubar, wbar, it_done
+
+    def setup_blasius(self):
+        """The ic=0 base flow of setup_ic, init.f90:353-424."""
+        c = self.cfg
+        okr = math.sqrt(c.x0 * c.xnu / c.u0)
+        bz = self.zpts / okr
+        bd = self.d * okr
+        bd2 = bd @ bd
+        ubar, wbar, it = self.blasfl(1.0e-11, 100, 0.33, bz, self.dor, bd, bd2)
+        ubar = c.u0 * ubar
+        wbar = wbar * math.sqrt(c.xnu * c.u0 / c.x0)
+        wbar[0] = wbar[1]
+        wbar[-1] = 0.0
+        self.ubar = ubar
+        self.wbar = wbar
+        self.ubarp = self.dpdz(ubar[:, None], 1)[:, 0]
+        self.ubarpp = self.dpdz(ubar[:, None], 2)[:, 0]
+        self.wbarp = self.dpdz(wbar[:, None], 1)[:, 0]
+        self.wbarpp = self.dpdz(wbar[:, None], 2)[:, 0]
+        self.blasfl_iters = it
+
+    # ------------------------------------------------------------------ horfft
+    def horfft_fwd(self, a):
+        """horfft(ain, ainf, -1), horfft.f90:94-119: r2c x (Nyquist dropped :303-315),
+        c2c y forward, scale 1/(nx*ny).  ``a`` (..., nzp, ny, nxpp) -> (..., nzp, ny, nx)."""
+        nx, ny = self.nx, self.ny
+        X = np.fft.rfft(a[..., :nx], axis=-1)[..., : nx // 2]
+        X = np.fft.fft(X, axis=-2)
+        X = X * (1.0 / float(nx * ny))
+        X = np.ascontiguousarray(X)
+        return X.view(np.float64).reshape(a.shape[:-1] + (nx,))
+
+    def horfft_inv(self, af):
+        """horfft(ain, ainf, +1), horfft.f90:121-144: c2c y backward, c2r x (unscaled).
+        The kx-Nyquist slot is zero (scratch zeroed at :90); pads are returned as 0."""
+        nx, ny = self.nx, self.ny
+        af = np.ascontiguousarray(af)
+        X = af.view(np.complex128)
+        X = np.fft.ifft(X, axis=-2, norm="forward")
+        Xf = np.zeros(X.shape[:-1] + (nx // 2 + 1,), dtype=np.complex128)
+        Xf[..., : nx // 2] = X
+        x = np.fft.irfft(Xf, n=nx, axis=-1, norm="forward")
+        out = np.zeros(af.shape[:-1] + (nx + 2,))
+        out[..., :nx] = x
+        return out
+
+    def norm(self, *fields):
+        """horfft.f90:153-224 (in place on spectral fields)."""
+        ny, nyh = self.ny, self.nyh
+        for f in fields:
+            if self.wavx[0] == 0:
+                f[:, 0, 1] = 0.0
+        if ny > 2:
+            for f in fields:
+                if self.wavx[0] == 0:
+                    for j in range(2, nyh + 1):
+                        jp = ny + 2 - j
+                        f[:, j - 1, 0] = 0.5 * (f[:, j - 1, 0] + f[:, jp - 1, 0])
+                        f[:, j - 1, 1] = 0.5 * (f[:, j - 1, 1] - f[:, jp - 1, 1])
+                    for j in range(2, nyh + 1):
+                        jp = ny + 2 - j
+                        f[:, jp - 1, 0] = f[:, j - 1, 0]
+                        f[:, jp - 1, 1] = -f[:, j - 1, 1]
+                f[:, nyh, :] = 0.0
+
+    # ------------------------------------------------------------- derivatives
+    def dfdx(self, uf):
+        """derivatives.f90:261-280."""
+        out = np.empty_like(uf)
+        w = self.wavx[0::2]
+        out[..., 0::2] = -w * uf[..., 1::2]
+        out[..., 1::2] = w * uf[..., 0::2]
+        return out
+
+    def dfdy(self, uf):
+        """derivatives.f90:283-303."""
+        out = np.empty_like(uf)
+        w = self.wavy[:, None]
+        out[..., 0::2] = -w * uf[..., 1::2]
+        out[..., 1::2] = w * uf[..., 0::2]
+        return out
+
+    def divuv(self, uf, vf):
+        """derivatives.f90:394-409."""
+        out = np.empty_like(uf)
+        wx = self.wavx[0::2]
+        wy = self.wavy[:, None]
+        out[..., 0::2] = -wx * uf[..., 1::2] - wy * vf[..., 1::2]
+        out[..., 1::2] = wx * uf[..., 0::2] + wy * vf[..., 0::2]
+        return out
+
+    def dct_diff(self, y, p):
+        """chebyshev.f90:204-270 along axis 0 of y (n, ...): DCT-I, recurrence, DCT-I, *gf/2(n-1)."""
+        n = y.shape[0]
+        half = 0.5 * self.xl
+        nlog = 2 * (n - 1)
+        beta = np.array(y, dtype=np.float64, copy=True)
+        gf = self.gf.reshape((n,) + (1,) * (y.ndim - 1))
+        for _ in range(p):
+            alpha = sfft.dct(beta, type=1, axis=0)
+            alpha[1 : n - 1] = alpha[1 : n - 1] / half
+            beta = np.zeros_like(alpha)
+            beta[n - 2] = 2.0 * (n - 1.0) * alpha[n - 1]
+            for k in range(n - 1, 1, -1):               # 1-based k = n-1 .. 2
+                beta[k - 2] = beta[k] + 2.0 * (k - 1.0) * alpha[k - 1]
+            alpha = sfft.dct(beta, type=1, axis=0)
+            beta = alpha * gf / nlog
+        return beta
+
+    def _zapply(self, mat, u):
+        n = u.shape[0]
+        return (mat @ u.reshape(n, -1)).reshape(u.shape)
+
+    def dfdz(self, uf):
+        """derivatives.f90:112-132 (dct_diff if nz > 31 else matmul(d, .))."""
+        if self.nz > 31:
+            return self.dct_diff(uf, 1)
+        return self._zapply(self.d, uf)
+
+    def d2fdz2(self, uf):
+        """derivatives.f90:161-188."""
+        if self.nz > 31:
+            return self.dct_diff(uf, 2)
+        return self._zapply(self.d2, uf)
+
+    def dpdz(self, u, p):
+        """derivatives.f90:135-158 on arrays (n, ...); n <= 31 applies d once for any p."""
+        if u.shape[0] > 31:
+            return self.dct_diff(u, p)
+        return self._zapply(self.d, u)
+
+    # ----------------------------------------------------------------- helmholtz
+    def solve_helmholtz(self, rhs, dtnui, diag=None, smat=None, simat=None):
+        """helmholtz.f90:130-208: interior rows 2..nz, E^-1, divide, E, zero end rows."""
+        diag = self.diag if diag is None else diag
+        smat = self.smat if smat is None else smat
+        simat = self.simat if simat is None else simat
+        nz = self.nz
+        n = diag.shape[0]
+        out = np.array(rhs, copy=True)
+        q = out[1 : n + 1].reshape(n, -1)
+        t = simat @ q
+        dfac = (-self.wavxx[None, :] - self.wavyy[:, None] - dtnui).reshape(-1)
+        t = t / (diag[:, None] + dfac[None, :])
+        q = smat @ t
+        out[1 : n + 1] = q.reshape((n,) + out.shape[1:])
+        out[0] = 0.0
+        out[self.nzp - 1] = 0.0
+        del nz
+        return out
+
+    def lifting_functions(self):
+        """helmholtz.f90:240-307: hbot, htop, d2hbot, d2htop per mapping."""
+        z = self.zpts
+        nzp = self.nzp
+        m = self.cfg.mapping
+        if m == 0:
+            width = z[0] - z[nzp - 1]
+            htop = (z - z[nzp - 1]) / width
+            hbot = -(z - z[0]) / width
+            d2htop = np.zeros(nzp)
+            d2hbot = np.zeros(nzp)
+        elif m == 1:
+            bbot = 50.0
+            htop = np.zeros(nzp)
+            d2htop = np.zeros(nzp)
+            hbot = np.exp(-bbot * z)
+            hbot[0] = 0.0
+            d2hbot = bbot ** 2.0 * hbot
+        elif m == 2:
+            h = z[0] - z[nzp - 1]
+            bbot = 5.0
+            hbot = (np.exp(-bbot * z) - np.exp(bbot * (z - 2.0 * h)))
+            hbot = hbot / (1.0 - math.exp(-2.0 * bbot * h))
+            d2hbot = bbot ** 2 * hbot
+            htop = self.wbar / self.wbar[0]
+            htop[nzp - 1] = 0.0
+            d2htop = self.wbarpp / self.wbar[0]
+        else:
+            btop = 50.0
+            bbot = 50.0
+            h = z[0] - z[nzp - 1]
+            hbot = (np.exp(-bbot * z) - np.exp(bbot * (z - 2.0 * h)))
+            hbot = hbot / (1.0 - math.exp(-2.0 * bbot * h))
+            d2hbot = bbot ** 2.0 * hbot
+            htop = np.exp(btop * (z - h)) * (1.0 - np.exp(-2.0 * btop * z))
+            htop = htop / (1.0 - math.exp(-2.0 * btop * h))
+            d2htop = btop ** 2.0 * htop
+        return hbot, htop, d2hbot, d2htop
+
+    def solve_nhom_helmholtz(self, rhs, bc, dtnui):
+        """helmholtz.f90:211-358 (Green's-function lifting, only bc planes 1 and nzp used)."""
+        hbot, htop, d2hbot, d2htop = self.lifting_functions()
+        nzp = self.nzp
+        dfac = -self.wavxx[None, :] - self.wavyy[:, None] - dtnui
+        r = np.array(rhs, copy=True)
+        bcb = bc[nzp - 1]
+        bct = bc[0]
+        for k in range(nzp):
+            abot = d2hbot[k] + dfac * hbot[k]
+            r[k] = r[k] - abot * bcb
+            atop = d2htop[k] + dfac * htop[k]
+            r[k] = r[k] - atop * bct
+        r = self.solve_helmholtz(r, dtnui)
+        for k in range(nzp):
+            r[k] = r[k] + hbot[k] * bcb + htop[k] * bct
+        return r
+
+    # ------------------------------------------------------------------ filters
+    def specfilter_fourier2d(self, ak, akmax, kc, p, amp):
+        """filters.f90:338-360 (vectorised over ak)."""
+        akc = akmax * kc
+        theta = (np.abs(ak) - akc) / (akmax - akc)
+        return np.where(np.abs(ak) >= akc, np.exp(-amp * theta ** (2 * p)), 1.0)
+
+    def specfilter_exp_xy(self, uf, kc, p, amp, filtery=1):
+        """filters.f90:281-335: magnitude/phase exponential filter."""
+        wavxmax = self.nxh * self.alpha
+        wavymax = self.nyh * self.beta
+        akmax = wavxmax
+        if filtery == 1:
+            akmax = math.sqrt(wavxmax ** 2 + wavymax ** 2)
+        fr = uf[..., 0::2]
+        fi = uf[..., 1::2]
+        fmag = np.sqrt(fr * fr + fi * fi)
+        nzero = fmag != 0.0
+        with np.errstate(invalid="ignore", divide="ignore"):
+            cosc = np.where(nzero, fr / np.where(nzero, fmag, 1.0), 1.0)
+            sinc = np.where(nzero, fi / np.where(nzero, fmag, 1.0), 0.0)
+        ac = np.arccos(cosc)
+        phase = np.where(sinc >= 0.0, ac, -ac)
+        if filtery == 1:
+            ak = np.sqrt(self.wavxx[0::2][None, :] + self.wavyy[:, None])
+        else:
+            ak = np.broadcast_to(self.wavx[0::2][None, :], (self.ny, self.nxh))
+        filt = self.specfilter_fourier2d(ak, akmax, kc, p, amp)
+        fmagf = fmag * filt
+        out = np.empty_like(uf)
+        out[..., 0::2] = np.where(nzero, fmagf * np.cos(phase), 0.0)
+        out[..., 1::2] = np.where(nzero, fmagf * np.sin(phase), 0.0)
+        return out
+
+    def filterqspec(self, qf):
+        """filters.f90:164-192 (qf: (..., nzp, ny, nx))."""
+        c = self.cfg
+        return self.specfilter_exp_xy(qf, c.kc, c.filt_order, c.filt_amp, 1)
+
+    def filter_z_weights(self, a, b, interpflag=1):
+        """filters.f90:396-448: az, bz, cz for the z pass."""
+        z = self.zpts
+        nzp = self.nzp
+        az = np.full(nzp, a)
+        bz = np.full(nzp, b)
+        cz = np.full(nzp, a)
+        if interpflag == 1:
+            m = self.cfg.mapping
+            if m in (1, 2):
+                for k in range(2, nzp):                     # 1-based k = 2..nzp-1
+                    dltm = z[k - 1] - z[k - 2]
+                    dltp = z[k] - z[k - 1]
+                    az[k - 1] = a * 2.0 * dltm / (dltp + dltm)
+                    bz[k - 1] = a * 2.0 * (1.0 - dltm / dltp) + b
+                    cz[k - 1] = a * 2.0 * dltm * dltm / (dltp * (dltp + dltm))
+                if m == 1:
+                    az[1] = 0.0
+                    bz[1] = 2.0 * a + b
+                    cz[1] = 0.0
+            else:
+                for k in range(2, (nzp + 1) // 2):         # 2 .. (nzp+1)/2-1
+                    dltm = z[k - 1] - z[k - 2]
+                    dltp = z[k] - z[k - 1]
+                    az[k - 1] = a * 2.0 * dltm / (dltp + dltm)
+                    bz[k - 1] = a * 2.0 * (1.0 - dltm / dltp) + b
+                    cz[k - 1] = a * 2.0 * dltm * dltm / (dltp * (dltp + dltm))
+                for k in range((nzp + 1) // 2 + 1, nzp):   # (nzp+1)/2+1 .. nzp-1
+                    dltm = z[k - 1] - z[k - 2]
+                    dltp = z[k] - z[k - 1]
+                    az[k - 1] = a * 2.0 * dltp * dltp / (dltm * (dltm + dltp))
+                    bz[k - 1] = a * 2.0 * (1.0 - dltp / dltm) + b
+                    cz[k - 1] = a * 2.0 * dltp / (dltm + dltp)
+                mid = (nzp + 1) // 2 - 1
+                az[mid] = a
+                bz[mid] = b
+                cz[mid] = a
+            az[0] = 0.0
+            bz[0] = 2.0 * a + b
+            cz[0] = 0.0
+            az[nzp - 1] = 0.0
+            bz[nzp - 1] = 2.0 * a + b
+            cz[nzp - 1] = 0.0
+        return az, bz, cz
+
+    def filter_q_3pt(self, q, a, b, interpflag=1):
+        """filters.f90:363-458 + filtery_q_3pt :461-613 at P = 1.
+        q: (..., nzp, ny, nxpp) physical; returns the filtered field, pads = 0.
+        x: periodic (a,b,a)/(2a+b); y: periodic; z: interpolating weights on
+        interior rows only (k = 1 and nzp keep the xy-filtered value, Q7)."""
+        nx = self.nx
+        s = 2.0 * a + b
+        qx = q[..., :nx]
+        qh = (a * np.roll(qx, 1, axis=-1) + b * qx + a * np.roll(qx, -1, axis=-1)) / s
+        qy = (a * np.roll(qh, 1, axis=-2) + b * qh + a * np.roll(qh, -1, axis=-2)) / s
+        az, bz, cz = self.filter_z_weights(a, b, interpflag)
+        out = np.array(qy, copy=True)
+        nzp = self.nzp
+        sh = (nzp - 2,) + (1, 1)
+        out[..., 1 : nzp - 1, :, :] = (az[1 : nzp - 1].reshape(sh) * qy[..., 0 : nzp - 2, :, :]
+                                       + bz[1 : nzp - 1].reshape(sh) * qy[..., 1 : nzp - 1, :, :]
+                                       + cz[1 : nzp - 1].reshape(sh) * qy[..., 2:nzp, :, :]) / s
+        res = np.zeros(q.shape)
+        res[..., :nx] = out
+        return res
+
+    # ---------------------------------------------------------------------- SGS
+    def sgs_delta_static(self):
+        """sgsterm.f90:95-103 (Q8: delta(nzp) is never set; the Van Driest factor is 0 there)."""
+        z = self.zpts
+        nzp, nz = self.nzp, self.nz
+        dx, dy = self.dx, self.dy
+        delta = np.zeros(nzp)
+        dz = abs(z[0] - z[1])
+        delta[0] = (dx * dy * dz) ** (1.0 / 3.0)
+        for k in range(2, nz + 1):
+            dz = (z[k - 2] - z[k]) / 2.0
+            delta[k - 1] = (dx * dy * dz) ** (1.0 / 3.0)
+        dz = abs(z[nzp - 1] - z[nzp - 2])
+        delta[0] = (dx * dy * dz) ** (1.0 / 3.0)
+        delta[nzp - 1] = 0.0
+        return delta
+
+    def static_smagorinsky(self, cs, duidxj):
+        """sgsterm.f90:39-120.  duidxj[l][m] physical (nzp, ny, nxpp)."""
+        c = self.cfg
+        z = self.zpts
+        nx, ny = self.nx, self.ny
+        utau = np.zeros(duidxj.shape[2:][1:])
+        wallcount = 0
+        for k in range(self.nzp):
+            if z[k] == 0.0 or abs(z[k]) == 0.5:
+                utau = utau + c.xnu * duidxj[0, 2, k]
+                wallcount += 1
+        utau = utau / wallcount
+        utau_avg = np.sum(utau[:, :nx]) / (nx * ny)
+        utau_avg = math.sqrt(abs(utau_avg))
+        zplus = z * utau_avg / c.xnu
+        snorm = np.zeros(duidxj.shape[2:])
+        for j in range(3):
+            for i in range(3):
+                snorm = snorm + (0.5 * (duidxj[i, j] + duidxj[j, i])) ** 2
+        snorm = np.sqrt(2.0 * snorm)
+        delta = self.sgs_delta_static()
+        fac = -((cs * delta * (1.0 - np.exp(-zplus / 25.0))) ** 2)
+        nu = fac[:, None, None] * snorm
+        tau = np.empty_like(duidxj)
+        for j in range(3):
+            for i in range(3):
+                tau[i, j] = nu * 0.5 * (duidxj[i, j] + duidxj[j, i])
+        return tau
+
+    def dynamic_smagorinsky(self, ui, uiuj, duidxj):
+        """sgsterm.f90:123-280 (local 3-pt average of MijLij / MijMij, Q9)."""
+        c = self.cfg
+        a, b = c.a, c.b
+        z = self.zpts
+        nzp, nx = self.nzp, self.nx
+        dx, dy = self.dx, self.dy
+        dz1 = np.zeros(nzp)
+        dz1[0] = (dx * dy * (z[0] - z[1])) ** (1.0 / 3.0)
+        for k in range(2, nzp):
+            dzk = 0.5 * (z[k - 2] - z[k])
+            dz1[k - 1] = (dx * dy * dzk) ** (1.0 / 3.0)
+        dz1[nzp - 1] = (dx * dy * (z[nzp - 2] - z[nzp - 1])) ** (1.0 / 3.0)
+        delta = dz1[:, None, None]
+        cd = 2.0 * delta
+        sij = np.empty_like(duidxj)
+        snorm = np.zeros(duidxj.shape[2:])
+        for m in range(3):
+            for l in range(3):
+                sij[l, m] = 0.5 * (duidxj[l, m] + duidxj[m, l])
+                snorm = snorm + 2.0 * sij[l, m] * sij[l, m]
+        snorm = np.sqrt(snorm)
+        sijhat = self.filter_q_3pt(sij, a, b, 1)
+        snormhat = np.zeros(duidxj.shape[2:])
+        for m in range(3):
+            for l in range(3):
+                snormhat = snormhat + 2.0 * sijhat[l, m] * sijhat[l, m]
+        snormhat = np.sqrt(snormhat)
+        lij = np.empty_like(duidxj)
+        for m in range(3):
+            for l in range(3):
+                lij[l, m] = delta * delta * snorm * sij[l, m]
+                sijhat[l, m] = cd * cd * snormhat * sijhat[l, m]
+        mij = self.filter_q_3pt(lij, a, b, 1)
+        mij = 2.0 * (mij - sijhat)
+        uihat = self.filter_q_3pt(ui, a, b, 1)
+        lij = self.filter_q_3pt(uiuj, a, b, 1)
+        for m in range(3):
+            for l in range(3):
+                lij[l, m] = lij[l, m] - uihat[l] * uihat[m]
+        numc = np.zeros(duidxj.shape[2:])
+        denomc = np.zeros(duidxj.shape[2:])
+        for m in range(3):
+            for l in range(3):
+                numc = numc + mij[l, m] * lij[l, m]
+                denomc = denomc + mij[l, m] * mij[l, m]
+        ctot = self.filter_q_3pt(numc, 1.0, 1.0, 1)
+        cdd = self.filter_q_3pt(denomc, 1.0, 1.0, 1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            cc = ctot[..., :nx] / cdd[..., :nx]
+        cc = np.where((cc < 0.0) | np.isnan(cc), 0.0, cc)
+        cc = np.where(cc > 0.2 ** 2, 0.2 ** 2, cc)
+        ctot[..., :nx] = cc
+        nu = ctot * delta * delta * snorm
+        tau = np.empty_like(duidxj)
+        for m in range(3):
+            for l in range(3):
+                tau[l, m] = -2.0 * nu * sij[l, m]
+        return tau
+
+    def sigma_model(self, duidxj):
+        """sgsterm.f90:283-423 (per-point invariants of G = g^T g, acos angles)."""
+        nx, nzp, nz = self.nx, self.nzp, self.nz
+        z = self.zpts
+        g = duidxj[..., :nx]                                # g[l][m] = du_l/dx_m
+        G = [[g[0, a] * g[0, b] + g[1, a] * g[1, b] + g[2, a] * g[2, b] for b in range(3)] for a in range(3)]
+        G2 = [[G[a][0] * G[0][b] + G[a][1] * G[1][b] + G[a][2] * G[2][b] for b in range(3)] for a in range(3)]
+        i1 = G[0][0] + G[1][1] + G[2][2]
+        i2 = 0.5 * (i1 ** 2 - (G2[0][0] + G2[1][1] + G2[2][2]))
+        i3 = (G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1])
+              - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0])
+              + G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]))
+        a1 = i1 ** 2 / 9.0 - i2 / 3.0
+        a2 = i1 ** 3 / 27.0 - i1 * i2 / 6.0 + 0.5 * i3
+        a1 = np.where(a1 < 0.0, np.abs(a1), a1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            pre = a2 / np.sqrt(a1) ** 3.0
+            a3 = np.where((pre >= 1.0) | np.isnan(pre), 0.0,
+                          np.where(pre <= -1.0, PI / 3.0, np.arccos(np.clip(pre, -1.0, 1.0)) / 3.0))
+        sa = np.sqrt(a1)
+        s1 = np.sqrt(np.abs(i1 / 3.0 + 2.0 * sa * np.cos(a3)))
+        s2 = np.sqrt(np.abs(i1 / 3.0 - 2.0 * sa * np.cos(PI / 3.0 + a3)))
+        s3 = np.sqrt(np.abs(i1 / 3.0 - 2.0 * sa * np.cos(PI / 3.0 - a3)))
+        ds = s3 * (s1 - s2) * (s2 - s3)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            ds = np.where(s1 != 0.0, ds / np.where(s1 != 0.0, s1, 1.0) ** 2, ds)
+        ds = np.where((ds < 0.0) | np.isnan(ds), 0.0, ds)
+        dz = np.zeros(nzp)
+        dz[0] = z[0] - z[1]
+        dz[nzp - 1] = z[nz - 1] - z[nzp - 1]
+        for k in range(1, nzp - 1):
+            dz[k] = 0.5 * (z[k - 1] - z[k + 1])
+        delta = (self.dx * self.dy * dz) ** (1.0 / 3.0)
+        nu = np.zeros(duidxj.shape[2:])
+        nu[..., :nx] = -((1.35 * delta) ** 2)[:, None, None] * ds
+        tau = np.empty_like(duidxj)
+        for m in range(3):
+            for l in range(3):
+                tau[l, m] = 2.0 * nu * 0.5 * (duidxj[l, m] + duidxj[m, l])
+        return tau
+
+    def interscale_model(self, ui):
+        """sgsterm.f90:426-467."""
+        c = self.cfg
+        ub = self.filter_q_3pt(ui, c.a, c.b, 1)
+        ub2 = self.filter_q_3pt(ub, c.a, c.b, 1)
+        uibuj = np.empty((3, 3) + ui.shape[1:])
+        for m in range(3):
+            for l in range(3):
+                uibuj[l, m] = ub[l] * ui[m]
+        ubb = self.filter_q_3pt(uibuj, c.a, c.b, 1)
+        tau = np.empty_like(uibuj)
+        for m in range(3):
+            for l in range(3):
+                tau[l, m] = ubb[l, m] - ub2[l] * ub2[m] - ub[l] * (ui[m] - ub[m])
+        return tau
+
+    def tau_sgs(self, ui, uiuj, duidxj):
+        """sgsterm.f90:10-37."""
+        c = self.cfg
+        if c.sgsmodel == 0:
+            return self.static_smagorinsky(c.cs, duidxj)
+        if c.sgsmodel == 1:
+            return self.dynamic_smagorinsky(ui, uiuj, duidxj)
+        if c.sgsmodel == 2:
+            return self.sigma_model(duidxj)
+        if c.sgsmodel == 3:
+            return self.interscale_model(ui)
+        raise ValueError("sgsmodel")
+
+    # --------------------------------------------------------------- dealiasing
+    def dealias_sizes(self):
+        """dealiasing.f90:133-154 at P = 1."""
+        nx, ny = self.nx, self.ny
+        if nx % 2 == 0 and nx % 4 == 0:
+            nxl = 3 * nx // 2
+        elif nx % 3 == 0:
+            nxl = 5 * nx // 3
+        elif nx % 5 == 0:
+            nxl = 8 * nx // 5
+        else:
+            nxl = 2 * nx
+        if ny % 2 == 0:
+            nyl = 3 * ny // 2
+        elif ny % 3 == 0:
+            nyl = 5 * ny // 3
+        elif ny % 5 == 0:
+            nyl = 8 * ny // 5
+        else:
+            nyl = 2 * ny
+        return nxl, nyl
+
+    def dealiasing_2d_fftw(self, a, b, ref_compat=True):
+        """dealiasing.f90:262-447.  ref_compat reproduces the reference's ky placement
+        (ky slots copied to the first ny of ny_lg, Q1); otherwise negative ky go to the
+        top of the padded spectrum (textbook 3/2 rule)."""
+        nx, ny = self.nx, self.ny
+        nxl, nyl = self.dealias_sizes()
+        af = self.horfft_fwd(a).view(np.complex128)
+        bf = self.horfft_fwd(b).view(np.complex128)
+
+        def pad(xf):
+            big = np.zeros(xf.shape[:-2] + (nyl, nxl // 2 + 1), dtype=np.complex128)
+            if ref_compat:
+                big[..., :ny, : nx // 2] = xf
+            else:
+                h = ny // 2
+                big[..., :h, : nx // 2] = xf[..., :h, :]
+                big[..., nyl - (ny - h):, : nx // 2] = xf[..., h:, :]
+            big = np.fft.ifft(big, axis=-2, norm="forward")
+            return np.fft.irfft(big, n=nxl, axis=-1, norm="forward")
+
+        prod = pad(af) * pad(bf)
+        P = np.fft.rfft(prod, axis=-1)
+        P = np.fft.fft(P, axis=-2)
+        P = P * (1.0 / (float(nxl) * float(nyl)))
+        out = np.zeros(P.shape[:-2] + (ny, nx // 2), dtype=np.complex128)
+        if ref_compat:
+            out[...] = P[..., :ny, : nx // 2]
+        else:
+            h = ny // 2
+            out[..., :h, :] = P[..., :h, : nx // 2]
+            out[..., h:, :] = P[..., nyl - (ny - h):, : nx // 2]
+        return self.horfft_inv(np.ascontiguousarray(out).view(np.float64))
+
+    # ---------------------------------------------------------------- boundary
+    def bound(self, t):
+        """navierstokes.f90:214-355 -> spectral bc field (nzp, ny, nx).
+        bctype 1: Spalart ceiling suction (+ fringe blowing rescaled for mass balance);
+        bctype 3: steady ceiling profile V(x) = samp*(-G(0.4 Lx) + G(0.7 Lx)) with
+        Gaussian width 0.05 Lx, no-slip wall (the configs' driver patch)."""
+        c = self.cfg
+        nx, nzp = self.nx, self.nzp
+        bc = np.zeros((nzp, self.ny, self.nxpp))
+        x = self.xpts
+        if c.bctype == 1:
+            pi = PI
+            h = self.zpts[0]
+            xloc = 1.0e5 * c.xnu / c.u0
+            if x[nx // 2 - 1] < xloc:
+                xloc = c.x0 + 3.0 / 7.0 * c.xlen
+            xwidth = 0.24 * h
+            amp = c.samp * c.u0 / (0.24 * math.sqrt(pi))
+            bamp = c.samp / 2.25
+            ref_time = c.xlen / c.u0
+            if t - c.istartbound <= ref_time:
+                s = math.sin(0.5 * pi * (t - c.istartbound) / ref_time) ** 2.0
+                amp, bamp = amp * s, bamp * s
+            bwidth = xwidth * 2.25
+            x1 = x[nx - 1] - 2.0 * bwidth
+            eta = (x - xloc) / xwidth
+            eta0 = (x - x1) / bwidth
+            bc[0, :, :nx] = (amp * np.exp(-eta ** 2.0))[None, :]
+            bc[nzp - 1, :, :nx] = (bamp * np.exp(-eta0 ** 4.0))[None, :]
+        elif c.bctype == 3:
+            L = c.xlen
+            xr = x - c.x0
+            g1 = np.exp(-(((xr - 0.4 * L) / (0.05 * L)) ** 2))
+            g2 = np.exp(-(((xr - 0.7 * L) / (0.05 * L)) ** 2))
+            bc[0, :, :nx] = (c.samp * (g2 - g1))[None, :]
+        bcf = self.horfft_fwd(bc)
+        if c.bctype == 1:
+            ampf = bcf[0, 0, 0]
+            bampf = bcf[nzp - 1, 0, 0]
+            bfac = ampf / bampf if bampf != 0 else 0.0
+            bcf[nzp - 1] = bfac * bcf[nzp - 1]
+        return bcf
+
+    # ------------------------------------------------------------ time coeffs
+    @staticmethod
+    def abcoef(nstart, istep):
+        """navierstokes.f90:1500-1542 (EX3 3,-3,1 with Euler/SS2 start, Q13)."""
+        if nstart == 0:
+            if istep == 1:
+                return np.array([1.0, 0.0, 0.0])
+            if istep == 2:
+                return np.array([2.0, -1.0, 0.0])
+        return np.array([3.0, -3.0, 1.0])
+
+    @staticmethod
+    def bdfcoef(nstart, istep, dt1, dt2, dt3):
+        """navierstokes.f90:1545-1590 + bdfvar* :1623-1662."""
+        def var():
+            dt12 = dt1 + dt2
+            dt23 = dt2 + dt3
+            dt123 = dt1 + dt2 + dt3
+            v = 1.0 + dt1 / dt12 + dt1 / dt123
+            b1 = -(-1.0 - dt1 / dt2 - dt1 / dt123 - dt1 * dt1 / (dt123 * dt2) - dt1 * dt1 * dt12 / (dt123 * dt23 * dt2))
+            b2 = -(dt1 * dt1 / (dt12 * dt2) + dt1 * dt1 / (dt2 * dt123) + dt1 * dt1 * dt12 / (dt2 * dt23 * dt123)
+                   + dt1 * dt1 * dt12 / (dt3 * dt23 * dt123))
+            b3 = -(-dt1 * dt1 * dt12 / (dt3 * dt23 * dt123))
+            return np.array([b1, b2, b3]), 1.0 / v
+        if nstart == 0:
+            if istep == 1:
+                return np.array([1.0, 0.0, 0.0]), 1.0
+            if istep == 2:
+                return np.array([2.0, -0.5, 0.0]), 2.0 / 3.0
+            if dt1 != dt2 or dt2 != dt3:
+                return var()
+            return np.array([3.0, -1.5, 1.0 / 3.0]), 6.0 / 11.0
+        if dt1 != dt2 or dt2 != dt3:
+            return var()
+        return np.array([3.0, -1.5, 1.0 / 3.0]), 6.0 / 11.0
+
+    # ------------------------------------------------------------------- state
+    def new_state(self, uf, vf, wf, tempf=None):
+        """Core arrays after setup_ic (init.f90:473-486): histories and pressure zero."""
+        sh = uf.shape
+        z = lambda: np.zeros(sh)  # noqa: E731
+        st = dict(uf=np.array(uf, copy=True), vf=np.array(vf, copy=True), wf=np.array(wf, copy=True),
+                  tempf=z() if tempf is None else np.array(tempf, copy=True),
+                  pf=z(), pnf=z(), tnf=z(), tnmf=z(), nltnf=z(), nltnmf=z(),
+                  unif=np.zeros((3,) + sh), unmif=np.zeros((3,) + sh),
+                  nlnif=np.zeros((3,) + sh), nlnmif=np.zeros((3,) + sh),
+                  istep=0, t=0.0, dt=self.cfg.dt, dt2=self.cfg.dt, dt3=self.cfg.dt,
+                  abfac=np.zeros(3), bdf=np.zeros(3), bdfv=0.0)
+        return st
+
+    # ----------------------------------------------------------- nonlinear term
+    def nlterm_skewsym(self, uif, tempf, sgsflag):
+        """navierstokes.f90:658-855.  uif (3, nzp, ny, nx) spectral -> (nltermif, nltempf)."""
+        c = self.cfg
+        nzp = self.nzp
+        duidxjf = np.empty((3, 3) + uif.shape[1:])
+        for l in range(3):
+            duidxjf[l, 0] = self.dfdx(uif[l])
+            duidxjf[l, 1] = self.dfdy(uif[l])
+            duidxjf[l, 2] = self.dfdz(uif[l])
+        duidxj = self.horfft_inv(duidxjf)
+        ui = self.horfft_inv(uif)
+        ubk = self.ubar.reshape(nzp, 1, 1)
+        ui[0] = ui[0] + ubk
+        duidxj[0, 2] = duidxj[0, 2] + self.ubarp.reshape(nzp, 1, 1)
+        nlt = np.zeros(ui.shape)
+        uiuj = np.empty((3, 3) + ui.shape[1:])
+        if c.dealiasflag == 1:
+            rc = bool(c.dealias_ref_compat)
+            for l in range(3):
+                for m in range(3):
+                    nlt[l] = nlt[l] - 0.5 * self.dealiasing_2d_fftw(ui[m], duidxj[l, m], rc)
+                    uiuj[m, l] = self.dealiasing_2d_fftw(ui[m], ui[l], rc)
+        else:
+            for l in range(3):
+                for m in range(3):
+                    nlt[l] = nlt[l] - 0.5 * ui[m] * duidxj[l, m]
+                    uiuj[m, l] = ui[m] * ui[l]
+        if c.forcing == 1:
+            raise NotImplementedError("sponge forcing is outside the restated path")
+        tau = np.zeros_like(uiuj)
+        if sgsflag == 1:
+            tau = self.tau_sgs(ui, uiuj, duidxj)
+        h = 0.5 * uiuj + tau
+        nltermif = self.horfft_fwd(nlt)
+        hijf = self.horfft_fwd(h)
+        for l in range(3):
+            nltermif[l] = nltermif[l] - self.dfdx(hijf[l, 0])
+            nltermif[l] = nltermif[l] - self.dfdy(hijf[l, 1])
+            nltermif[l] = nltermif[l] - self.dfdz(hijf[l, 2])
+        # temperature (navierstokes.f90:800-845); exact zero when T == 0 and tbar == 0
+        if np.any(np.abs(tempf) > 0.0) or np.any(self.tbar != 0.0):
+            dtdxif = np.stack([self.dfdx(tempf), self.dfdy(tempf), self.dfdz(tempf)])
+            if np.any(np.abs(tempf) > 0.0):
+                dtdxi = self.horfft_inv(dtdxif)
+                temp = self.horfft_inv(tempf)
+            else:
+                dtdxi = np.zeros(ui.shape)
+                temp = np.zeros(ui.shape[1:])
+            temp = temp + self.tbar.reshape(nzp, 1, 1)
+            dtdxi[2] = dtdxi[2] + np.sum(self.tbarp)
+            nltemp = np.zeros(temp.shape)
+            uitemp = np.empty(ui.shape)
+            for l in range(3):
+                nltemp = nltemp - 0.5 * ui[l] * dtdxi[l]
+                uitemp[l] = ui[l] * temp
+            nltempf = self.horfft_fwd(nltemp)
+            hf = self.horfft_fwd(uitemp)
+            nltempf = nltempf - 0.5 * self.dfdx(hf[0])
+            nltempf = nltempf - 0.5 * self.dfdy(hf[1])
+            nltempf = nltempf - 0.5 * self.dfdz(hf[2])
+        else:
+            nltempf = np.zeros(tempf.shape)
+        return nltermif, nltempf
+
+    # -------------------------------------------------------- step (a20 rows)
+    def navierstokes_update(self, st):
+        """navierstokes.f90:1157-1496 (bctype > 0 path uses solve_nhom_helmholtz)."""
+        c = self.cfg
+        nzp = self.nzp
+        uf, vf, wf = st["uf"], st["vf"], st["wf"]
+        istep = st["istep"]
+        dt, dt2, dt3 = st["dt"], st["dt2"], st["dt3"]
+        uif = np.stack([uf, vf, wf])
+        tempsf = np.array(st["tempf"], copy=True)
+        uif_nl = uif
+        if c.dealiasflag >= 2:
+            uif_nl = self.filterqspec(uif)
+            if np.any(np.abs(tempsf) > 1.0e-10):
+                tempsf = self.filterqspec(tempsf)
+        nltermif, nltempf = self.nlterm_skewsym(uif_nl, tempsf, c.sgsflag)
+        if c.dealiasflag != 2:
+            uif = uif_nl
+        tmp = nltermif.copy()
+        tempsf = nltempf.copy()
+        if istep <= 3 or dt != dt2 or dt2 != dt3:
+            st["abfac"] = self.abcoef(c.nstart, istep)
+        ab = st["abfac"]
+        nltermif = ab[0] * nltermif + ab[1] * st["nlnif"] + ab[2] * st["nlnmif"]
+        nltempf = ab[0] * nltempf + ab[1] * st["nltnf"] + ab[2] * st["nltnmf"]
+        if istep <= 3 or dt != dt2 or dt2 != dt3:
+            st["bdf"], st["bdfv"] = self.bdfcoef(c.nstart, istep, dt, dt2, dt3)
+        bdf, bdfv = st["bdf"], st["bdfv"]
+        # temperature (navierstokes.f90:1251-1275)
+        xkap = c.xnu / c.prtl
+        dtkapi = 1.0 / (xkap * dt)
+        rhs_t = -dtkapi * (dt * nltempf + bdf[0] * st["tempf"] + bdf[1] * st["tnf"] + bdf[2] * st["tnmf"])
+        if np.sum(np.abs(self.tbarpp)) > 0.0:
+            du0 = np.zeros((nzp, self.ny, self.nxpp))
+            du0[:, :, : self.nx] = self.tbarpp.reshape(nzp, 1, 1)
+            rhs_t = rhs_t - self.horfft_fwd(du0)
+        rhs_t = self.solve_helmholtz(rhs_t, dtkapi / bdfv)
+        st["nltnmf"] = st["nltnf"]
+        st["tnmf"] = st["tnf"]
+        st["nltnf"] = tempsf
+        st["tnf"] = st["tempf"]
+        st["tempf"] = rhs_t
+        tempf = rhs_t
+        # velocity rhs (navierstokes.f90:1282-1303)
+        dtnui = 1.0 / (c.xnu * dt)
+        rhsf = -dtnui * (dt * nltermif + bdf[0] * uif + bdf[1] * st["unif"] + bdf[2] * st["unmif"])
+        if np.sum(np.abs(self.ubarpp)) > 0.0:
+            du0 = np.zeros((nzp, self.ny, self.nxpp))
+            du0[:, :, : self.nx] = self.ubarpp.reshape(nzp, 1, 1)
+            rhsf[0] = rhsf[0] - self.horfft_fwd(du0)
+        if np.any(np.abs(tempf) > 1.0e-10):
+            rhsf[0] = rhsf[0] - c.coex * c.gx * tempf / c.xnu
+            rhsf[1] = rhsf[1] - c.coex * c.gy * tempf / c.xnu
+            rhsf[2] = rhsf[2] - c.coex * c.gz * tempf / c.xnu
+        st["nlnmif"] = st["nlnif"]
+        st["unmif"] = st["unif"]
+        st["nlnif"] = tmp
+        st["unif"] = uif.copy()
+        uif = uif.copy()
+        # BCs (navierstokes.f90:1321-1373)
+        if istep == 1:
+            pfac = (0.0, 0.0)
+        elif istep == 2:
+            pfac = (1.0, 0.0)
+        else:
+            pfac = (2.0, -1.0)
+        psf = pfac[0] * st["pf"] + pfac[1] * st["pnf"]
+        st["pnf"] = st["pf"]
+        dpf = self.dfdx(psf)
+        uif[0, nzp - 1] = dt * bdfv * dpf[nzp - 1]
+        wav2 = self.wavxx[None, :] + self.wavyy[:, None]
+        uif[0, 0] = -rhsf[0, 0] / (wav2 + dtnui / bdfv)
+        uif[1, 0] = -rhsf[1, 0] / (wav2 + dtnui / bdfv)
+        dwdzf = self.d2fdz2(uif[2])
+        nhom = (c.bctype > 0 and st["t"] >= c.istartbound) or c.forcing == 1
+        if c.bctype > 0 and st["t"] >= c.istartbound:
+            uif[2] = self.bound(st["t"])
+            uif[2, 0] = uif[2, 0] + c.xnu * dt * bdfv * dwdzf[0]
+        else:
+            uif[2, 0] = c.xnu * dt * bdfv * dwdzf[0]
+            uif[2, nzp - 1] = c.xnu * dt * bdfv * dwdzf[nzp - 1]
+        dpf = self.dfdy(psf)
+        uif[1, nzp - 1] = dt * bdfv * dpf[nzp - 1]
+        for i in range(3):
+            if nhom:
+                rhsf[i] = self.solve_nhom_helmholtz(rhsf[i], uif[i], dtnui / bdfv)
+            else:
+                rhsf[i] = self.solve_helmholtz(rhsf[i], dtnui / bdfv)
+        uf = rhsf[0].copy()
+        vf = rhsf[1].copy()
+        wf = rhsf[2].copy()
+        # pressure step (navierstokes.f90:1411-1488)
+        dpf = self.dfdz(uf)
+        dwdzf = self.dfdz(vf)
+        divdzf = self.divuv(dpf, dwdzf)
+        divf = self.divuv(uf, vf)
+        rhs3 = -divdzf - wav2[None] * wf
+        if c.bctype > 0 and st["t"] >= c.istartbound:
+            wf = self.bound(st["t"])
+        else:
+            wf[0] = 0.0
+            wf[nzp - 1] = 0.0
+        wf = self.solve_nhom_helmholtz(rhs3, wf, 0.0)
+        dwdzf = self.dfdz(wf)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            pf = np.where(wav2[None] == 0.0, 0.0, -(dwdzf + divf) / (wav2[None] * dt * bdfv))
+        dpf = self.dfdx(pf)
+        uf = uf - dt * bdfv * dpf
+        dpf = self.dfdy(pf)
+        vf = vf - dt * bdfv * dpf
+        st["uf"], st["vf"], st["wf"], st["pf"] = uf, vf, wf, pf
+
+    def timestep(self, st):
+        """navierstokes.f90:10-90 with ifilt = 0, dtadapt = 0 (the per-step part)."""
+        c = self.cfg
+        st["istep"] += 1
+        self.navierstokes_update(st)
+        self.norm(st["uf"], st["vf"], st["wf"], st["tempf"])
+        if c.ifrz == 1 and self.wavx[0] == 0:
+            for k in ("uf", "vf", "wf"):
+                st[k][:, 0, 0:2] = 0.0
+        st["t"] = st["t"] + st["dt"]
+        st["dt3"] = st["dt2"]
+        st["dt2"] = st["dt"]
+        return st
+
+    def divcheck(self, uf, vf, wf):
+        """derivatives.f90:348-392: mean |div u| in physical space (pads are 0 here)."""
+        divf = self.dfdx(uf) + self.dfdy(vf) + self.dfdz(wf)
+        divp = self.horfft_inv(divf)
+        return float(np.sum(np.abs(divp)) / (self.nx * self.ny * self.nzp))
+
+
+__all__ = ["Oracle", "OracleConfig", "PI", "field"]
