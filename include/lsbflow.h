@@ -206,6 +206,8 @@ int lsb_navierstokes_update(lsb_ctx* ctx);
 int lsb_state_device_ptrs(lsb_ctx* ctx, void** uf, void** vf, void** wf, void** pf);
 /* Stream the context launches on (hipStream_t). */
 void* lsb_stream(lsb_ctx* ctx);
+/* Wait for all work queued on the context stream (lsb_timestep only enqueues). */
+int lsb_synchronize(lsb_ctx* ctx);
 
 /* ------------------------------------- device-pointer kernel entry points */
 /* Same semantics as the tier-A routines, on device memory, asynchronous on
@@ -214,12 +216,13 @@ int lsb_horfft_dev(lsb_ctx* ctx, double* ain, double* ainf, int is, int nfields,
 int lsb_dfdz_dev(lsb_ctx* ctx, const double* uf, double* dudzf, int nfields, void* stream);
 int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields, void* stream);
 
-/* Kernel timing for roofline accounting: average device time (ms) of the
- * last lsb_profile_* window per kernel class, and the launch counts. */
+/* Kernel timing for roofline accounting.  Between lsb_profile_begin and
+ * lsb_profile_end every launch group is bracketed by HIP events on the launch
+ * stream and tagged with its algorithmic work.  Per class c (0 FFT, 1 Chebyshev
+ * contraction, 2 Helmholtz, 3 pointwise): total device ms, launch count,
+ * algorithmic flops and algorithmic HBM bytes (arrays of 4). */
 int lsb_profile_begin(lsb_ctx* ctx);
-int lsb_profile_end(lsb_ctx* ctx, double* ms_fft, double* ms_zcontract, double* ms_helmholtz,
-                    double* ms_pointwise, int* n_fft, int* n_zcontract, int* n_helmholtz,
-                    int* n_pointwise);
+int lsb_profile_end(lsb_ctx* ctx, double* ms, int* count, double* flops, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -397,11 +397,18 @@ class Context:
         check(self._L.lsb_profile_begin(self.h), "profile_begin")
 
     def profile_end(self):
-        ms = [C.c_double() for _ in range(4)]
-        n = [C.c_int() for _ in range(4)]
-        check(self._L.lsb_profile_end(self.h, *[C.byref(x) for x in ms], *[C.byref(x) for x in n]), "profile_end")
+        """{class: dict(ms=total device ms, n=launch groups, flops=..., bytes=...)}."""
+        ms = np.zeros(4)
+        n = np.zeros(4, dtype=np.int32)
+        fl = np.zeros(4)
+        by = np.zeros(4)
+        check(self._L.lsb_profile_end(self.h, _p(ms), n.ctypes.data_as(C.POINTER(C.c_int)), _p(fl), _p(by)),
+              "profile_end")
         keys = ("fft", "zcontract", "helmholtz", "pointwise")
-        return {k: (ms[i].value, n[i].value) for i, k in enumerate(keys)}
+        return {k: dict(ms=float(ms[i]), n=int(n[i]), flops=float(fl[i]), bytes=float(by[i])) for i, k in enumerate(keys)}
+
+    def synchronize(self):
+        check(self._L.lsb_synchronize(self.h), "synchronize")
 
 
 def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:

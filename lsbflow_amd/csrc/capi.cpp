@@ -56,11 +56,13 @@ int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p) {
     return LSB_OK;
 }
 
-int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token) {
+int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token, double flops, double bytes) {
     *token = -1;
     if (!c->profiling) return LSB_OK;
     lsb_ctx::ProfEv ev;
     ev.cls = cls;
+    ev.flops = flops;
+    ev.bytes = bytes;
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return LSB_OK;
     (void)hipEventRecord(ev.a, s);
     c->prof.push_back(ev);
@@ -100,10 +102,10 @@ int upload_context(lsb_ctx* c) {
     pad_matrix(c->d, nzp, nzp, c->ldm_z, t);
     LSB_TRY(put(&c->d_d, t));
     {   // rows 0 and nzp-1 of md2, ld 64
-        std::vector<double> b((size_t)64 * nzp, 0.0);
+        std::vector<double> b((size_t)kLdmBnd * nzp, 0.0);
         for (int j = 0; j < nzp; ++j) {
-            b[(size_t)j * 64 + 0] = c->md2[(size_t)j * nzp + 0];
-            b[(size_t)j * 64 + 1] = c->md2[(size_t)j * nzp + nzp - 1];
+            b[(size_t)j * kLdmBnd + 0] = c->md2[(size_t)j * nzp + 0];
+            b[(size_t)j * kLdmBnd + 1] = c->md2[(size_t)j * nzp + nzp - 1];
         }
         LSB_TRY(put(&c->d_md2b, b));
     }
@@ -134,7 +136,11 @@ int upload_context(lsb_ctx* c) {
     fz.insert(fz.end(), c->fz_11.begin(), c->fz_11.end());
     LSB_TRY(put(&c->d_fz, fz));
     if (!c->d_smag) LSB_TRY(dev_alloc((void**)&c->d_smag, (size_t)nzp * sizeof(double)));
-    if (!c->d_red) LSB_TRY(dev_alloc((void**)&c->d_red, (size_t)(64 + 3 * nzp + 64) * sizeof(double)));
+    if (3 * nzp + 128 > kRedDoubles) {
+        set_error("nz too large for the small-table scratch");
+        return LSB_E_ARG;
+    }
+    if (!c->d_red) LSB_TRY(dev_alloc((void**)&c->d_red, (size_t)kRedDoubles * sizeof(double)));
     LSB_HIP(hipStreamSynchronize(s));
     return LSB_OK;
 }
@@ -1046,30 +1052,35 @@ extern "C" int lsb_profile_begin(lsb_ctx* c) {
     return LSB_OK;
 }
 
-extern "C" int lsb_profile_end(lsb_ctx* c, double* ms_fft, double* ms_z, double* ms_h, double* ms_pw, int* n_fft,
-                               int* n_z, int* n_h, int* n_pw) {
+extern "C" int lsb_synchronize(lsb_ctx* c) {
     LSB_TRY(check_ctx(c));
     LSB_HIP(hipStreamSynchronize(c->stream));
-    double ms[4] = {0, 0, 0, 0};
+    return LSB_OK;
+}
+
+extern "C" int lsb_profile_end(lsb_ctx* c, double* ms, int* count, double* flops, double* bytes) {
+    LSB_TRY(check_ctx(c));
+    LSB_HIP(hipStreamSynchronize(c->stream));
+    double t_ms[4] = {0, 0, 0, 0}, fl[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
     int cnt[4] = {0, 0, 0, 0};
     for (auto& ev : c->prof) {
         float t = 0.f;
         if (hipEventElapsedTime(&t, ev.a, ev.b) == hipSuccess) {
-            ms[ev.cls] += t;
+            t_ms[ev.cls] += t;
             cnt[ev.cls] += 1;
+            fl[ev.cls] += ev.flops;
+            by[ev.cls] += ev.bytes;
         }
         (void)hipEventDestroy(ev.a);
         (void)hipEventDestroy(ev.b);
     }
     c->prof.clear();
     c->profiling = false;
-    if (ms_fft) *ms_fft = ms[0];
-    if (ms_z) *ms_z = ms[1];
-    if (ms_h) *ms_h = ms[2];
-    if (ms_pw) *ms_pw = ms[3];
-    if (n_fft) *n_fft = cnt[0];
-    if (n_z) *n_z = cnt[1];
-    if (n_h) *n_h = cnt[2];
-    if (n_pw) *n_pw = cnt[3];
+    for (int i = 0; i < 4; ++i) {
+        if (ms) ms[i] = t_ms[i];
+        if (count) count[i] = cnt[i];
+        if (flops) flops[i] = fl[i];
+        if (bytes) bytes[i] = by[i];
+    }
     return LSB_OK;
 }

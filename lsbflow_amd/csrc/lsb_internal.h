@@ -81,6 +81,10 @@ struct HorPlan {
 
 // A list of field base pointers passed by value to batched kernels.
 constexpr int kMaxFields = 16;
+// leading dimension of the two-row boundary operator d_md2b (rows 0 and nzp-1 of d2fdz2)
+constexpr int kLdmBnd = 64;
+// doubles in the reduction / small-table scratch d_red
+constexpr int kRedDoubles = 8192;
 template <class R>
 struct FieldList {
     R* p[kMaxFields];
@@ -163,6 +167,7 @@ struct lsb_ctx {
     struct ProfEv {
         hipEvent_t a, b;
         int cls;
+        double flops, bytes;
     };
     std::vector<ProfEv> prof;
     double prof_ms[4] = {0, 0, 0, 0};
@@ -188,7 +193,7 @@ int upload_context(lsb_ctx* c);
 
 // profiling helpers
 enum { PC_FFT = 0, PC_Z = 1, PC_HELM = 2, PC_PW = 3 };
-int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token);
+int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token, double flops = 0.0, double bytes = 0.0);
 int prof_mark_end(lsb_ctx* c, int token, hipStream_t s);
 
 // FFT (fft.hip)

@@ -692,7 +692,7 @@ static int update_once(lsb_ctx* c) {
     {
         const double* in[1] = {L.spec[S->u[2]]};
         double* out[1] = {S->d2w};
-        LSB_TRY(zcontract(c, c->d_md2b, 2, 2, c->nzp, in, out, 1, s));
+        LSB_TRY(zcontract(c, c->d_md2b, kLdmBnd, 2, c->nzp, in, out, 1, s));
     }
     const double ramp = bound_ramp(c, c->t);
     double st = ramp, sb = 0.0;

