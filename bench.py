@@ -1,0 +1,150 @@
+"""Throughput of the device-resident lsbflow time step on one MI355X.
+
+A "step" is one timestep() of the hot path (navierstokes_update + norm) over the
+whole 512 x 193 x 256 (nx x nzp x ny) LES domain resident in HBM, started from
+the seeded tanh-plus-perturbation initial condition with the ceiling
+suction/blowing profile (lsbflow_amd.cases).  W untimed warm-up steps, then K
+timed steps bracketed by device synchronisation; value = nx*nzp*ny*K / seconds.
+
+Besides the metric the JSON line carries
+  roofline      the dominant kernel class measured live with HIP events on the
+                launch stream during the timed steps (algorithmic work per launch /
+                device time), against the MI355X peak that bounds it;
+  cpu_baseline  the CPU oracle (numpy/scipy FP64 restatement, 1 thread) timed on a
+                bounded sample of the same workload (same nx, nzp and SGS model,
+                spanwise extent reduced), in the same unit.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
+PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 vector = FP64 matrix
+PEAK_FP32_TFLOPS = 157.3
+
+
+def gpu_run(case, steps, warmup, precision):
+    import numpy as np
+    import lsbflow_amd as L
+    from lsbflow_amd.core import setup_case
+
+    ctx = setup_case(case, precision=precision)
+    u, v, w = L.cases.initial_condition(case)
+    ctx.state_set(u, v, w)
+    del u, v, w
+    if warmup > 0:
+        ctx.timestep(warmup)
+    ctx.synchronize()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    ctx.timestep(steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_end()
+    g = ctx.state_get()
+    finite = all(bool(np.all(np.isfinite(g[k]))) for k in ("uf", "vf", "wf", "pf"))
+    ctx.close()
+    return t1 - t0, prof, finite
+
+
+def roofline(prof, precision):
+    """Dominant class by device time -> achieved vs peak (algorithmic work / measured time)."""
+    cls = max(prof, key=lambda k: prof[k]["ms"])
+    p = prof[cls]
+    sec = p["ms"] * 1e-3
+    if cls in ("zcontract", "helmholtz"):
+        peak = PEAK_FP64_TFLOPS if precision == 0 else PEAK_FP32_TFLOPS
+        ach = p["flops"] / sec / 1e12 if sec > 0 else 0.0
+        return dict(bound="mfma", kernel=cls, achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(ach / peak, 4), traffic=None, launches=p["n"], device_ms=round(p["ms"], 3))
+    ach = p["bytes"] / sec / 1e9 if sec > 0 else 0.0
+    return dict(bound="hbm", kernel=cls, achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(ach / PEAK_HBM_GBS, 4), traffic=None, launches=p["n"], device_ms=round(p["ms"], 3))
+
+
+def cpu_baseline(case, sample_ny, sample_steps):
+    """The CPU oracle on a spanwise-reduced sample of the same workload, one thread."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    import lsbflow_amd as L
+    from oracle import Oracle, OracleConfig
+
+    sc = case.with_(ny=sample_ny)
+    keys = OracleConfig.__dataclass_fields__
+    with threadpool_limits(limits=1):
+        o = Oracle(OracleConfig(**{k: v for k, v in sc.as_dict().items() if k in keys}))
+        o.ubar, o.ubarp, o.ubarpp = L.cases.base_flow(sc)
+        u, v, w = L.cases.initial_condition(sc)
+        st = o.new_state(u, v, w)
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            o.timestep(st)
+        t1 = time.perf_counter()
+    pts = sc.nx * sc.nzp * sc.ny
+    val = pts * sample_steps / (t1 - t0)
+    ok = bool(np.all(np.isfinite(st["uf"])))
+    return dict(value=float(f"{val:.4g}"), unit="grid-point-steps/s", cores=1, kind="port",
+                sample=f"{sample_steps} oracle steps of {sc.nx}x{sc.nzp}x{sc.ny} (nx x nzp x ny; spanwise "
+                       f"extent {sample_ny}/{case.ny} of the workload), {t1 - t0:.1f} s, finite={ok}")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--case", default="c4_sigma_les")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-ny", type=int, default=16)
+    ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
+              "(use --gpus 1)", file=sys.stderr)
+        return 2
+    import lsbflow_amd as L
+    case = L.cases.CASES[args.case]
+    precision = 0 if args.precision == "fp64" else 1
+    secs, prof, finite = gpu_run(case, args.steps, args.warmup, precision)
+    pts = case.points
+    value = pts * args.steps / secs
+    out = {
+        "metric": "grid-point-steps/s @512x193x256 LES, mixed vs FP64 GPU at rel-L2<=1e-5",
+        "value": float(f"{value:.6g}"),
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * secs / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "mixed(f32/f64)",
+        "data": "synthetic: seeded tanh base flow + curl of a random vector potential (rms 1e-3), "
+                "ceiling suction/blowing profile",
+        "config": {"workload": f"{case.name}: {case.nx}x{case.nzp}x{case.ny} (nx x nzp x ny), sgsmodel="
+                               f"{case.sgsmodel if case.sgsflag else 'off'}, dt={case.dt}, xnu={case.xnu:.3g}",
+                   "nx": case.nx, "nzp": case.nzp, "ny": case.ny, "precision": args.precision,
+                   "finite_after_run": finite},
+        "roofline": roofline(prof, precision),
+        "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_ny, args.cpu_sample_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -180,35 +180,37 @@ def initial_condition(case: Case):
     ncoef = 3 * (mx + 1) * len(jys) * (nc + 1) * 2
     g = normal_stream(case.seed, ncoef).reshape(3, mx + 1, len(jys), nc + 1, 2)
     coef = g[..., 0] + 1j * g[..., 1]
-    A = np.zeros((3, nzp, ny, nx // 2), dtype=np.complex128)
-    dA = np.zeros_like(A)
-    for c in range(3):
-        for m in range(mx + 1):
-            for jj, jy in enumerate(jys):
-                cc = coef[c, m, jj]
-                p = npcheb.chebval(xi, cc)
-                dp = npcheb.chebval(xi, npcheb.chebder(cc)) * (2.0 / xl)
-                A[c, :, jy % ny, m] += env * p
-                dA[c, :, jy % ny, m] += denv * p + env * dp
-    # Hermitian symmetry of the kx = 0 column (real field)
+    # Chebyshev Vandermonde matrices: T[k, n] = T_n(xi_k), dT[k, n] = T_n'(xi_k) d(xi)/dz
+    T = npcheb.chebvander(xi, nc)
+    dT = np.stack([npcheb.chebval(xi, npcheb.chebder(np.eye(nc + 1)[n])) for n in range(nc + 1)], axis=1)
+    dT = dT * (2.0 / xl)
+    P = np.einsum("kn,cmjn->ckjm", T, coef)          # (3, nzp, 2my+1, mx+1), rows jy = -my..my
+    dP = np.einsum("kn,cmjn->ckjm", dT, coef)
+    A = env[None, :, None, None] * P
+    dA = denv[None, :, None, None] * P + env[None, :, None, None] * dP
+    # Hermitian symmetry of the kx = 0 column (real field): A(0, -jy) = conj A(0, jy)
     for arr in (A, dA):
-        for jy in range(1, my + 1):
-            arr[:, :, (-jy) % ny, 0] = np.conj(arr[:, :, jy, 0])
-        arr[:, :, 0, 0] = arr[:, :, 0, 0].real
-    kx = (np.arange(nx // 2) * alpha)[None, None, :]
-    ky = np.array([(j if j < ny // 2 else j - ny) * beta for j in range(ny)])[None, :, None]
+        for j in range(1, my + 1):
+            arr[:, :, my - j, 0] = np.conj(arr[:, :, my + j, 0])
+        arr[:, :, my, 0] = arr[:, :, my, 0].real
+    kx = (np.arange(mx + 1) * alpha)[None, None, :]
+    ky = (np.arange(-my, my + 1) * beta)[None, :, None]
     ax, ay, az = A
     dax, day, daz = dA
-    uh = 1j * ky * az - day
-    vh = dax - 1j * kx * az
-    wh = 1j * kx * ay - 1j * ky * ax
-    phys = [_spectral_to_physical(f, nx) for f in (uh, vh, wh)]
-    rms = math.sqrt(sum(float(np.mean(p * p)) for p in phys) / 3.0)
+    band = (1j * ky * az - day, dax - 1j * kx * az, 1j * kx * ay - 1j * ky * ax)
+    # rms over the physical grid by Parseval: mean(u^2) = sum_kx=0 |u|^2 + 2 sum_kx>0 |u|^2 per plane
+    ms = 0.0
+    for f in band:
+        e = np.abs(f) ** 2
+        ms += (float(np.sum(e[..., 0])) + 2.0 * float(np.sum(e[..., 1:]))) / nzp
+    rms = math.sqrt(ms / 3.0)
     s = case.pert_rms / rms
+    rows = np.array([jy % ny for jy in jys])
     out = []
-    for f in (uh, vh, wh):
-        f = np.ascontiguousarray(f * s)
-        out.append(f.view(np.float64).reshape(nzp, ny, nx).copy())
+    for f in band:
+        full = np.zeros((nzp, ny, nx // 2), dtype=np.complex128)
+        full[:, rows, : mx + 1] = f * s
+        out.append(full.view(np.float64).reshape(nzp, ny, nx))
     return tuple(out)
 
 

@@ -157,8 +157,50 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
     v[7] = csub(e3, w3);
 }
 
-// One mixed-radix Stockham pass over `nlines` lines of length L held in LDS.
+template <class R, int S, int RAD>
+__device__ __forceinline__ void dft_small(cpx<R>* v) {
+    if (RAD == 2) dft2<R, S>(v);
+    else if (RAD == 3) dft3<R, S>(v);
+    else if (RAD == 4) dft4<R, S>(v[0], v[1], v[2], v[3]);
+    else if (RAD == 5) dft5<R, S>(v);
+    else dft8<R, S>(v);
+}
+
+// One Stockham stage of compile-time radix RAD over `nlines` lines of length L
+// held in LDS (in -> out).  The butterfly operands stay in registers.
+template <class R, int S, int RAD>
+__device__ __forceinline__ void stockham_stage(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, int L, int Ns,
+                                               int nlines, int lstride, const cpx<R>* __restrict__ tw) {
+    const int nb = L / RAD;
+    const int twstep = L / (Ns * RAD);
+    const int total = nlines * nb;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int line = t / nb;
+        const int j = t - line * nb;
+        const int jm = j % Ns;
+        const cpx<R>* src = in + line * lstride;
+        cpx<R>* dst = out + line * lstride;
+        cpx<R> v[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) v[r] = src[j + r * nb];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                cpx<R> w = tw[jm * r * twstep];
+                if (S > 0) w.im = -w.im;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small<R, S, RAD>(v);
+        const int base = (j - jm) * RAD + jm;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) dst[base + r * Ns] = v[r];
+    }
+}
+
+// Mixed-radix Stockham sequence over `nlines` lines of length L held in LDS.
 // Twiddle table tw[m] = exp(-2 pi i m / L); S = -1 forward, +1 backward.
+// Returns the buffer holding the result (b0 or b1).
 template <class R, int S>
 __device__ cpx<R>* fft_lines(cpx<R>* b0, cpx<R>* b1, const FftPlan1D& plan, int nlines, int lstride,
                              const cpx<R>* __restrict__ tw) {
@@ -168,33 +210,12 @@ __device__ cpx<R>* fft_lines(cpx<R>* b0, cpx<R>* b1, const FftPlan1D& plan, int 
     int Ns = 1;
     for (int s = 0; s < plan.nstage; ++s) {
         const int rad = plan.radix[s];
-        const int nb = L / rad;
-        const int twstep = L / (Ns * rad);
-        const int total = nlines * nb;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int line = t / nb;
-            const int j = t - line * nb;
-            const int jm = j % Ns;
-            const cpx<R>* src = in + (size_t)line * lstride;
-            cpx<R>* dst = out + (size_t)line * lstride;
-            cpx<R> v[8];
-            for (int r = 0; r < rad; ++r) v[r] = src[j + r * nb];
-            if (Ns > 1) {
-                for (int r = 1; r < rad; ++r) {
-                    cpx<R> w = tw[jm * r * twstep];
-                    if (S > 0) w.im = -w.im;
-                    v[r] = cmul(v[r], w);
-                }
-            }
-            switch (rad) {
-                case 2: dft2<R, S>(v); break;
-                case 3: dft3<R, S>(v); break;
-                case 4: dft4<R, S>(v[0], v[1], v[2], v[3]); break;
-                case 5: dft5<R, S>(v); break;
-                default: dft8<R, S>(v); break;
-            }
-            const int base = (j - jm) * rad + jm;
-            for (int r = 0; r < rad; ++r) dst[base + r * Ns] = v[r];
+        switch (rad) {
+            case 2: stockham_stage<R, S, 2>(in, out, L, Ns, nlines, lstride, tw); break;
+            case 3: stockham_stage<R, S, 3>(in, out, L, Ns, nlines, lstride, tw); break;
+            case 4: stockham_stage<R, S, 4>(in, out, L, Ns, nlines, lstride, tw); break;
+            case 5: stockham_stage<R, S, 5>(in, out, L, Ns, nlines, lstride, tw); break;
+            default: stockham_stage<R, S, 8>(in, out, L, Ns, nlines, lstride, tw); break;
         }
         __syncthreads();
         cpx<R>* tmp = in;
@@ -216,37 +237,44 @@ __global__ void __launch_bounds__(256) k_xfwd(FieldList<R> in, FieldList<R> out,
     const int lh = nx / 2;
     cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
     cpx<R>* b1 = b0 + (size_t)lines * lh;
+    __shared__ const R* rin[16];
+    __shared__ R* rout[16];
     const long row0 = (long)blockIdx.x * lines;
     const long total_rows = (long)nrows * nf;
-    // load
-    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
-        const int l = t / nx, i = t - l * nx;
-        const long r = row0 + l;
-        R v = 0;
+    if ((int)threadIdx.x < lines) {
+        const long r = row0 + threadIdx.x;
         if (r < total_rows) {
             const int f = (int)(r / nrows);
             const long rr = r - (long)f * nrows;
-            v = in.p[f][rr * nxpp + i];
+            rin[threadIdx.x] = in.p[f] + rr * nxpp;
+            rout[threadIdx.x] = out.p[f] + rr * nx;
+        } else {
+            rin[threadIdx.x] = nullptr;
+            rout[threadIdx.x] = nullptr;
         }
-        reinterpret_cast<R*>(b0)[(size_t)l * nx + i] = v;
+    }
+    __syncthreads();
+    // load
+    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
+        const int l = t / nx, i = t - l * nx;
+        const R* src = rin[l];
+        reinterpret_cast<R*>(b0)[l * nx + i] = src ? src[i] : (R)0;
     }
     __syncthreads();
     cpx<R>* z = fft_lines<R, -1>(b0, b1, plan, lines, lh, tw);
     // split: X_k = 1/2 (Z_k + conj Z_{lh-k}) - i/2 W^k (Z_k - conj Z_{lh-k})
     for (int t = threadIdx.x; t < lines * lh; t += blockDim.x) {
         const int l = t / lh, k = t - l * lh;
-        const long r = row0 + l;
-        if (r >= total_rows) continue;
-        const cpx<R> zk = z[(size_t)l * lh + k];
-        const cpx<R> zc = cconj(z[(size_t)l * lh + ((k == 0) ? 0 : lh - k)]);
+        R* dstrow = rout[l];
+        if (!dstrow) continue;
+        const cpx<R> zk = z[l * lh + k];
+        const cpx<R> zc = cconj(z[l * lh + ((k == 0) ? 0 : lh - k)]);
         const cpx<R> a = {(R)0.5 * (zk.re + zc.re), (R)0.5 * (zk.im + zc.im)};
         const cpx<R> d = {(R)0.5 * (zk.re - zc.re), (R)0.5 * (zk.im - zc.im)};
         const cpx<R> wd = cmul(twr[k], d);
         // -i * wd = (wd.im, -wd.re)
         const cpx<R> x = {a.re + wd.im, a.im - wd.re};
-        const int f = (int)(r / nrows);
-        const long rr = r - (long)f * nrows;
-        reinterpret_cast<cpx<R>*>(out.p[f] + rr * nx)[k] = x;
+        reinterpret_cast<cpx<R>*>(dstrow)[k] = x;
     }
 }
 
@@ -260,18 +288,23 @@ __global__ void __launch_bounds__(256) k_xinv(FieldList<R> io, int nrows, int nf
     const int lh = nx / 2;
     cpx<R>* b0 = reinterpret_cast<cpx<R>*>(smem_raw);
     cpx<R>* b1 = b0 + (size_t)lines * lh;
+    __shared__ R* rio[16];
     const long row0 = (long)blockIdx.x * lines;
     const long total_rows = (long)nrows * nf;
-    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
-        const int l = t / nx, i = t - l * nx;
-        const long r = row0 + l;
-        R v = 0;
+    if ((int)threadIdx.x < lines) {
+        const long r = row0 + threadIdx.x;
         if (r < total_rows) {
             const int f = (int)(r / nrows);
-            const long rr = r - (long)f * nrows;
-            v = io.p[f][rr * nxpp + i];
+            rio[threadIdx.x] = io.p[f] + (r - (long)f * nrows) * nxpp;
+        } else {
+            rio[threadIdx.x] = nullptr;
         }
-        reinterpret_cast<R*>(b1)[(size_t)l * nx + i] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < lines * nx; t += blockDim.x) {
+        const int l = t / nx, i = t - l * nx;
+        const R* src = rio[l];
+        reinterpret_cast<R*>(b1)[l * nx + i] = src ? src[i] : (R)0;
     }
     __syncthreads();
     // Z'_k = (X_k + conj X_{lh-k}) + i W^-k (X_k - conj X_{lh-k}); X_lh = 0, Im X_0 ignored
@@ -295,11 +328,9 @@ __global__ void __launch_bounds__(256) k_xinv(FieldList<R> io, int nrows, int nf
     cpx<R>* y = fft_lines<R, 1>(b0, b1, plan, lines, lh, tw);
     for (int t = threadIdx.x; t < lines * nxpp; t += blockDim.x) {
         const int l = t / nxpp, i = t - l * nxpp;
-        const long r = row0 + l;
-        if (r >= total_rows) continue;
-        const int f = (int)(r / nrows);
-        const long rr = r - (long)f * nrows;
-        io.p[f][rr * nxpp + i] = (i < nx) ? reinterpret_cast<R*>(y)[(size_t)l * nx + i] : (R)0;
+        R* dst = rio[l];
+        if (!dst) continue;
+        dst[i] = (i < nx) ? reinterpret_cast<R*>(y)[l * nx + i] : (R)0;
     }
 }
 
@@ -388,7 +419,12 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
         out.p[f] = spec[f];
     }
     int tok = -1;
-    prof_mark_begin(c, PC_FFT, s, &tok);
+    {   // algorithmic work: one read of the physical and one write of the spectral layout per
+        // field-plane; 2.5 N log2 N flops per real 2-D transform of N = nx*ny points
+        const double N = (double)nx * ny, planes = (double)nf * nplanes;
+        prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
+                        planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
+    }
     const int nrows = nplanes * ny;
     int lines = lines_for(lh, sizeof(cpx<double>));
     size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
@@ -421,7 +457,11 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
         out.p[f] = phys[f];
     }
     int tok = -1;
-    prof_mark_begin(c, PC_FFT, s, &tok);
+    {
+        const double N = (double)nx * ny, planes = (double)nf * nplanes;
+        prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
+                        planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
+    }
     int cols = lines_for(ny, sizeof(cpx<double>));
     size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
     const int ntile = (lh + cols - 1) / cols;

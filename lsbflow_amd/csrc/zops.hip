@@ -109,9 +109,10 @@ __global__ void __launch_bounds__(1024) k_zcontract(ZArgs a, FieldList<double> i
 }
 
 static void pick_rows(int nout, int* rows, int* nwaves) {
-    static const int cand[7] = {64, 56, 48, 40, 32, 24, 16};
+    // row blocks above 48 exceed the 128 VGPRs a 1024-thread block allows and spill
+    static const int cand[5] = {48, 40, 32, 24, 16};
     int best = -1, bw = 0, bwaste = 1 << 30;
-    for (int ci = 0; ci < 7; ++ci) {
+    for (int ci = 0; ci < 5; ++ci) {
         int r = cand[ci];
         int nw = (nout + r - 1) / r;
         if (nw > 16) continue;
@@ -123,7 +124,7 @@ static void pick_rows(int nout, int* rows, int* nwaves) {
         }
     }
     if (best < 0) {
-        best = 64;
+        best = 48;
         bw = 16;
     }
     *rows = best;
@@ -145,8 +146,6 @@ static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldLis
         hipLaunchKernelGGL((k_zcontract<R, MODE>), grid, block, 0, s, a, in, out, bt, bb); \
         break;
     switch (rows) {
-        LSB_ZL(64)
-        LSB_ZL(56)
         LSB_ZL(48)
         LSB_ZL(40)
         LSB_ZL(32)
@@ -183,7 +182,8 @@ int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const dou
         z.p[f] = nullptr;
     }
     int tok = -1;
-    prof_mark_begin(c, PC_Z, s, &tok);
+    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
+                    (double)(nin + nout) * a.P * nf * sizeof(double));
     int rc = launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
     prof_mark_end(c, tok, s);
     return rc;
@@ -222,7 +222,9 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
     a.lam = c->d_diag;
     a.lift = c->d_lift;
     int tok = -1;
-    prof_mark_begin(c, PC_HELM, s, &tok);
+    // two nzm x nzm projections per pencil; read rhs and write the solution once each
+    prof_mark_begin(c, PC_HELM, s, &tok, 4.0 * nzm * (double)nzm * a.P * nf,
+                    2.0 * c->nzp * (double)a.P * nf * sizeof(double));
     a.M = c->d_simat;
     a.in_row0 = 1;
     a.out_row0 = 0;
