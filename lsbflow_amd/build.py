@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblsbflow_amd.so")
-SOURCES = ["setup.cpp", "capi.cpp", "fft.hip", "zops.hip", "spectral.hip", "physical.hip", "step.hip",
+SOURCES = ["setup.cpp", "capi.cpp", "fft.hip", "fft_fast.hip", "zops.hip", "spectral.hip", "physical.hip", "step.hip",
            "dealias.hip"]
 ARCH = "gfx950"
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",

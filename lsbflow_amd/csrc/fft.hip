@@ -426,20 +426,29 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
                         planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
     }
     const int nrows = nplanes * ny;
-    int lines = lines_for(lh, sizeof(cpx<double>));
-    size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
-    long nblk = ((long)nrows * nf + lines - 1) / lines;
-    hipLaunchKernelGGL(k_xfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nrows, nf, nx, nxpp, hp.px,
-                       (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
-    LSB_HIP(hipGetLastError());
-    int cols = lines_for(ny, sizeof(cpx<double>));
-    shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
-    const int ntile = (lh + cols - 1) / cols;
-    nblk = (long)ntile * nplanes * nf;
+    if (fast_fft_len(lh)) {
+        LSB_TRY(fast_xfwd<double>(in, out, nrows, nf, lh, nxpp, (const cpx<double>*)hp.tw_x,
+                                  (const cpx<double>*)hp.tw_r, s));
+    } else {
+        int lines = lines_for(lh, sizeof(cpx<double>));
+        size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
+        long nblk = ((long)nrows * nf + lines - 1) / lines;
+        hipLaunchKernelGGL(k_xfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nrows, nf, nx, nxpp,
+                           hp.px, (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
+        LSB_HIP(hipGetLastError());
+    }
     const double scale = 1.0 / (double)(nx * ny);
-    hipLaunchKernelGGL(k_yfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nplanes, nx, ny, hp.py,
-                       (const cpx<double>*)hp.tw_y, cols, scale);
-    LSB_HIP(hipGetLastError());
+    if (fast_fft_len(ny)) {
+        LSB_TRY(fast_yfwd<double>(out, nplanes, nf, lh, ny, (const cpx<double>*)hp.tw_y, scale, s));
+    } else {
+        int cols = lines_for(ny, sizeof(cpx<double>));
+        size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
+        const int ntile = (lh + cols - 1) / cols;
+        long nblk = (long)ntile * nplanes * nf;
+        hipLaunchKernelGGL(k_yfwd<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nplanes, nx, ny, hp.py,
+                           (const cpx<double>*)hp.tw_y, cols, scale);
+        LSB_HIP(hipGetLastError());
+    }
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }
@@ -462,20 +471,29 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
         prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
                         planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
     }
-    int cols = lines_for(ny, sizeof(cpx<double>));
-    size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
-    const int ntile = (lh + cols - 1) / cols;
-    long nblk = (long)ntile * nplanes * nf;
-    hipLaunchKernelGGL(k_yinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nplanes, nx, ny, nxpp, hp.py,
-                       (const cpx<double>*)hp.tw_y, cols);
-    LSB_HIP(hipGetLastError());
+    if (fast_fft_len(ny)) {
+        LSB_TRY(fast_yinv<double>(in, out, nplanes, nf, lh, ny, nxpp / 2, (const cpx<double>*)hp.tw_y, s));
+    } else {
+        int cols = lines_for(ny, sizeof(cpx<double>));
+        size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
+        const int ntile = (lh + cols - 1) / cols;
+        long nblk = (long)ntile * nplanes * nf;
+        hipLaunchKernelGGL(k_yinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, in, out, nplanes, nx, ny, nxpp,
+                           hp.py, (const cpx<double>*)hp.tw_y, cols);
+        LSB_HIP(hipGetLastError());
+    }
     const int nrows = nplanes * ny;
-    int lines = lines_for(lh, sizeof(cpx<double>));
-    shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
-    nblk = ((long)nrows * nf + lines - 1) / lines;
-    hipLaunchKernelGGL(k_xinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nrows, nf, nx, nxpp, hp.px,
-                       (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
-    LSB_HIP(hipGetLastError());
+    if (fast_fft_len(lh)) {
+        LSB_TRY(fast_xinv<double>(out, nrows, nf, lh, nxpp, (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r,
+                                  s));
+    } else {
+        int lines = lines_for(lh, sizeof(cpx<double>));
+        size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
+        long nblk = ((long)nrows * nf + lines - 1) / lines;
+        hipLaunchKernelGGL(k_xinv<double>, dim3((unsigned)nblk), dim3(256), shm, s, out, nrows, nf, nx, nxpp, hp.px,
+                           (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r, lines);
+        LSB_HIP(hipGetLastError());
+    }
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }

@@ -1,0 +1,419 @@
+// Register-resident horizontal FFT passes for power-of-two lengths
+// L = N1 * N2 in {16, 32, 64, 128, 256} (the general mixed-radix path in
+// fft.hip covers every other length).
+//
+// Index split n = N2*n1 + n2, k = k1 + N1*k2:
+//   stage 1  each thread (one n2) loads its N1 points straight from HBM and does
+//            a length-N1 DFT in registers, then multiplies by W_L^(n2*k1);
+//   exchange one write/read through a padded LDS tile (the N1 x N2 transpose);
+//   stage 2  length-N2 DFTs in registers, results stored with unit-stride lanes.
+// Loads and stores are coalesced (consecutive lanes touch consecutive complex
+// words of one row, or of one row of a tile of kx columns).  The r2c split /
+// c2r merge of the x pass use the natural-order spectrum staged once more in
+// the same LDS tile.  Semantics are those of fft.hip (scaling, Nyquist drop,
+// zero pads).
+
+#include "lsb_internal.h"
+
+namespace lsb {
+
+// ------------------------------------------------------------- small DFTs
+template <class R, int S>
+__device__ __forceinline__ cpx<R> f_mul_si(cpx<R> a) {
+    if (S < 0) return {a.im, -a.re};
+    return {-a.im, a.re};
+}
+
+template <class R, int S>
+__device__ __forceinline__ void f_dft4(cpx<R>& a0, cpx<R>& a1, cpx<R>& a2, cpx<R>& a3) {
+    const cpx<R> s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+    const cpx<R> w = f_mul_si<R, S>(d13);
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, w);
+    a3 = csub(d02, w);
+}
+
+// exp(S*2*pi*i*m/16) for m = 0..15
+template <class R, int S>
+__device__ __forceinline__ cpx<R> w16(int m) {
+    const R c[16] = {(R)1.0L,
+                     (R)0.92387953251128675612818318939678829L,
+                     (R)0.70710678118654752440084436210484904L,
+                     (R)0.38268343236508977172845998403039887L,
+                     (R)0.0L,
+                     (R)-0.38268343236508977172845998403039887L,
+                     (R)-0.70710678118654752440084436210484904L,
+                     (R)-0.92387953251128675612818318939678829L,
+                     (R)-1.0L,
+                     (R)-0.92387953251128675612818318939678829L,
+                     (R)-0.70710678118654752440084436210484904L,
+                     (R)-0.38268343236508977172845998403039887L,
+                     (R)0.0L,
+                     (R)0.38268343236508977172845998403039887L,
+                     (R)0.70710678118654752440084436210484904L,
+                     (R)0.92387953251128675612818318939678829L};
+    const int ms = (m + 12) & 15;  // sin(x) = cos(x - pi/2)
+    return {c[m & 15], (R)S * c[ms]};
+}
+
+template <class R, int S, int N>
+__device__ __forceinline__ void f_dft(cpx<R>* x) {
+    if constexpr (N == 2) {
+        const cpx<R> a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (N == 4) {
+        f_dft4<R, S>(x[0], x[1], x[2], x[3]);
+    } else if constexpr (N == 8) {
+        cpx<R> e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+        cpx<R> o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+        f_dft4<R, S>(e0, e1, e2, e3);
+        f_dft4<R, S>(o0, o1, o2, o3);
+        const cpx<R> w1 = cmul(o1, w16<R, S>(2));
+        const cpx<R> w2 = cmul(o2, w16<R, S>(4));
+        const cpx<R> w3 = cmul(o3, w16<R, S>(6));
+        x[0] = cadd(e0, o0);
+        x[4] = csub(e0, o0);
+        x[1] = cadd(e1, w1);
+        x[5] = csub(e1, w1);
+        x[2] = cadd(e2, w2);
+        x[6] = csub(e2, w2);
+        x[3] = cadd(e3, w3);
+        x[7] = csub(e3, w3);
+    } else {  // 16 = 4 x 4
+        cpx<R> t[16];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            cpx<R> a0 = x[b], a1 = x[4 + b], a2 = x[8 + b], a3 = x[12 + b];
+            f_dft4<R, S>(a0, a1, a2, a3);
+            t[4 * b + 0] = a0;
+            t[4 * b + 1] = a1;
+            t[4 * b + 2] = a2;
+            t[4 * b + 3] = a3;
+        }
+#pragma unroll
+        for (int b = 1; b < 4; ++b)
+#pragma unroll
+            for (int c = 1; c < 4; ++c) t[4 * b + c] = cmul(t[4 * b + c], w16<R, S>(b * c));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            cpx<R> a0 = t[c], a1 = t[4 + c], a2 = t[8 + c], a3 = t[12 + c];
+            f_dft4<R, S>(a0, a1, a2, a3);
+            x[c] = a0;
+            x[c + 4] = a1;
+            x[c + 8] = a2;
+            x[c + 12] = a3;
+        }
+    }
+}
+
+// stage 1 twiddle: v[k1] *= W_L^(n2 k1) with W_L = exp(S 2 pi i / L), table tw[m] = exp(-2 pi i m/L)
+template <class R, int S, int N1>
+__device__ __forceinline__ void f_twiddle(cpx<R>* v, int n2, const cpx<R>* __restrict__ tw) {
+    if (n2 == 0) return;
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        cpx<R> w = tw[n2 * k1];
+        if (S > 0) w.im = -w.im;
+        v[k1] = cmul(v[k1], w);
+    }
+}
+
+// --------------------------------------------------------------- X passes
+// Forward r2c of rows (row r of nf*nrows; physical stride nxpp, spectral stride nx).
+template <class R, int N1, int N2>
+__global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<R> in, FieldList<R> out, int nrows, int nf, int nxpp,
+                                                   const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr) {
+    constexpr int L = N1 * N2;
+    constexpr int LPB = 128 / N2;         // lines per block
+    constexpr int TS = N1 * (N2 + 1);      // padded transpose tile per line (>= L)
+    __shared__ cpx<R> T[LPB * TS];
+    const int tid = threadIdx.x;
+    const int l = tid / N2, n2 = tid % N2;
+    const long r = (long)blockIdx.x * LPB + l;
+    const bool valid = r < (long)nrows * nf;
+    const cpx<R>* src = nullptr;
+    cpx<R>* dst = nullptr;
+    if (valid) {
+        const int f = (int)(r / nrows);
+        const long rr = r - (long)f * nrows;
+        src = reinterpret_cast<const cpx<R>*>(in.p[f] + rr * nxpp);
+        dst = reinterpret_cast<cpx<R>*>(out.p[f] + rr * (2 * L));
+    }
+    cpx<R> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? src[N2 * n1 + n2] : cpx<R>{0, 0};
+    f_dft<R, -1, N1>(v);
+    f_twiddle<R, -1, N1>(v, n2, tw);
+    cpx<R>* Tl = T + l * TS;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) Tl[k1 * (N2 + 1) + n2] = v[k1];
+    __syncthreads();
+    cpx<R> u[N1 / N2][N2];
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        const int k1 = n2 + N2 * q;
+#pragma unroll
+        for (int j = 0; j < N2; ++j) u[q][j] = Tl[k1 * (N2 + 1) + j];
+        f_dft<R, -1, N2>(u[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) Tl[(n2 + N2 * q) + N1 * k2] = u[q][k2];
+    __syncthreads();
+    if (!valid) return;
+    // X_k = 1/2 (Z_k + conj Z_{L-k}) - i/2 W^k (Z_k - conj Z_{L-k})
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int k = n2 + N2 * j;
+        const cpx<R> zk = Tl[k];
+        const cpx<R> zc = cconj(Tl[(L - k) & (L - 1)]);
+        const cpx<R> a = {(R)0.5 * (zk.re + zc.re), (R)0.5 * (zk.im + zc.im)};
+        const cpx<R> d = {(R)0.5 * (zk.re - zc.re), (R)0.5 * (zk.im - zc.im)};
+        const cpx<R> wd = cmul(twr[k], d);
+        dst[k] = {a.re + wd.im, a.im - wd.re};
+    }
+}
+
+// Inverse c2r in place on physical rows (stride nxpp) holding L complex coefficients.
+template <class R, int N1, int N2>
+__global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> io, int nrows, int nf, int nxpp,
+                                                   const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr) {
+    constexpr int L = N1 * N2;
+    constexpr int LPB = 128 / N2;
+    constexpr int TS = N1 * (N2 + 1);
+    __shared__ cpx<R> T[LPB * TS];
+    const int tid = threadIdx.x;
+    const int l = tid / N2, n2 = tid % N2;
+    const long r = (long)blockIdx.x * LPB + l;
+    const bool valid = r < (long)nrows * nf;
+    cpx<R>* row = nullptr;
+    if (valid) {
+        const int f = (int)(r / nrows);
+        row = reinterpret_cast<cpx<R>*>(io.p[f] + (r - (long)f * nrows) * nxpp);
+    }
+    cpx<R>* Tl = T + l * TS;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int k = n2 + N2 * j;
+        Tl[k] = valid ? row[k] : cpx<R>{0, 0};
+    }
+    __syncthreads();
+    // Z'_n = (X_n + conj X_{L-n}) + i W^-n (X_n - conj X_{L-n}); X_L = 0, Im X_0 ignored
+    cpx<R> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int n = N2 * n1 + n2;
+        cpx<R> xk = Tl[n];
+        cpx<R> xc;
+        if (n == 0) {
+            xk.im = 0;
+            xc = {0, 0};
+        } else {
+            xc = cconj(Tl[L - n]);
+        }
+        const cpx<R> s = cadd(xk, xc);
+        const cpx<R> d = csub(xk, xc);
+        const cpx<R> wd = cmul(cconj(twr[n]), d);
+        v[n1] = {s.re - wd.im, s.im + wd.re};
+    }
+    f_dft<R, 1, N1>(v);
+    f_twiddle<R, 1, N1>(v, n2, tw);
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) Tl[k1 * (N2 + 1) + n2] = v[k1];
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        const int k1 = n2 + N2 * q;
+        cpx<R> u[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) u[j] = Tl[k1 * (N2 + 1) + j];
+        f_dft<R, 1, N2>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) row[k1 + N1 * k2] = u[k2];
+    }
+    if (n2 == 0) row[L] = {0, 0};  // the two pad reals
+}
+
+// --------------------------------------------------------------- Y passes
+// Tiles of C kx-columns x L ky-rows of one plane; lane index c fastest.
+template <class R, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<R> io, int nplanes, int lh,
+                                                      const cpx<R>* __restrict__ tw, R scale) {
+    constexpr int TS = N1 * (N2 + 1) + 1;
+    __shared__ cpx<R> T[C * TS];
+    const int tid = threadIdx.x;
+    const int c = tid % C, n2 = tid / C;
+    const int ntile = (lh + C - 1) / C;
+    const int tile = blockIdx.x % ntile;
+    const long plane = blockIdx.x / ntile;
+    const int f = (int)(plane / nplanes);
+    const long pk = plane - (long)f * nplanes;
+    cpx<R>* base = reinterpret_cast<cpx<R>*>(io.p[f] + pk * (long)(N1 * N2) * (2 * lh));
+    const int m = tile * C + c;
+    const bool valid = m < lh;
+    cpx<R> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? base[(long)(N2 * n1 + n2) * lh + m] : cpx<R>{0, 0};
+    f_dft<R, -1, N1>(v);
+    f_twiddle<R, -1, N1>(v, n2, tw);
+    cpx<R>* Tc = T + c * TS;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) Tc[k1 * (N2 + 1) + n2] = v[k1];
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        const int k1 = n2 + N2 * q;
+        cpx<R> u[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) u[j] = Tc[k1 * (N2 + 1) + j];
+        f_dft<R, -1, N2>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            base[(long)(k1 + N1 * k2) * lh + m] = {u[k2].re * scale, u[k2].im * scale};
+    }
+}
+
+template <class R, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<R> in, FieldList<R> out, int nplanes, int lh,
+                                                      int ph, const cpx<R>* __restrict__ tw) {
+    constexpr int TS = N1 * (N2 + 1) + 1;
+    __shared__ cpx<R> T[C * TS];
+    const int tid = threadIdx.x;
+    const int c = tid % C, n2 = tid / C;
+    const int ntile = (lh + C - 1) / C;
+    const int tile = blockIdx.x % ntile;
+    const long plane = blockIdx.x / ntile;
+    const int f = (int)(plane / nplanes);
+    const long pk = plane - (long)f * nplanes;
+    const cpx<R>* src = reinterpret_cast<const cpx<R>*>(in.p[f] + pk * (long)(N1 * N2) * (2 * lh));
+    cpx<R>* dst = reinterpret_cast<cpx<R>*>(out.p[f] + pk * (long)(N1 * N2) * (2 * ph));
+    const int m = tile * C + c;
+    const bool valid = m < lh;
+    cpx<R> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? src[(long)(N2 * n1 + n2) * lh + m] : cpx<R>{0, 0};
+    f_dft<R, 1, N1>(v);
+    f_twiddle<R, 1, N1>(v, n2, tw);
+    cpx<R>* Tc = T + c * TS;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) Tc[k1 * (N2 + 1) + n2] = v[k1];
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < N1 / N2; ++q) {
+        const int k1 = n2 + N2 * q;
+        cpx<R> u[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) u[j] = Tc[k1 * (N2 + 1) + j];
+        f_dft<R, 1, N2>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) dst[(long)(k1 + N1 * k2) * ph + m] = u[k2];
+    }
+}
+
+// ------------------------------------------------------------- dispatch
+bool fast_fft_len(int L) { return L == 16 || L == 32 || L == 64 || L == 128 || L == 256; }
+
+#define LSB_FAST_CASES(X) \
+    X(16, 4, 4)           \
+    X(32, 8, 4)           \
+    X(64, 8, 8)           \
+    X(128, 16, 8)         \
+    X(256, 16, 16)
+
+template <class R>
+int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int nxpp,
+              const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
+    const long rows = (long)nrows * nf;
+    switch (lh) {
+#define X(LL, A, B)                                                                                              \
+    case LL:                                                                                                     \
+        hipLaunchKernelGGL((k_xfwd_fast<R, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), \
+                           0, s, in, out, nrows, nf, nxpp, tw, twr);                                             \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class R>
+int fast_xinv(const FieldList<R>& io, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw, const cpx<R>* twr,
+              hipStream_t s) {
+    const long rows = (long)nrows * nf;
+    switch (lh) {
+#define X(LL, A, B)                                                                                              \
+    case LL:                                                                                                     \
+        hipLaunchKernelGGL((k_xinv_fast<R, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), \
+                           0, s, io, nrows, nf, nxpp, tw, twr);                                                  \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class R>
+int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const cpx<R>* tw, R scale, hipStream_t s) {
+    constexpr int C = 8;
+    const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
+    switch (ny) {
+#define X(LL, A, B)                                                                                      \
+    case LL:                                                                                             \
+        hipLaunchKernelGGL((k_yfwd_fast<R, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, io, nplanes, lh, tw, \
+                           scale);                                                                       \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class R>
+int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int nf, int lh, int ny, int ph,
+              const cpx<R>* tw, hipStream_t s) {
+    constexpr int C = 8;
+    const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
+    switch (ny) {
+#define X(LL, A, B)                                                                                             \
+    case LL:                                                                                                    \
+        hipLaunchKernelGGL((k_yinv_fast<R, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, nplanes, lh, \
+                           ph, tw);                                                                             \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template int fast_xfwd<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int,
+                               const cpx<double>*, const cpx<double>*, hipStream_t);
+template int fast_xinv<double>(const FieldList<double>&, int, int, int, int, const cpx<double>*, const cpx<double>*,
+                               hipStream_t);
+template int fast_yfwd<double>(const FieldList<double>&, int, int, int, int, const cpx<double>*, double, hipStream_t);
+template int fast_yinv<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int, int,
+                               const cpx<double>*, hipStream_t);
+template int fast_xfwd<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, const cpx<float>*,
+                              const cpx<float>*, hipStream_t);
+template int fast_xinv<float>(const FieldList<float>&, int, int, int, int, const cpx<float>*, const cpx<float>*,
+                              hipStream_t);
+template int fast_yfwd<float>(const FieldList<float>&, int, int, int, int, const cpx<float>*, float, hipStream_t);
+template int fast_yinv<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, int,
+                              const cpx<float>*, hipStream_t);
+
+}  // namespace lsb

@@ -205,6 +205,20 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
 int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
                int nplanes, hipStream_t s);
 
+// register-resident power-of-two passes (fft_fast.hip)
+bool fast_fft_len(int L);
+template <class R>
+int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw,
+              const cpx<R>* twr, hipStream_t s);
+template <class R>
+int fast_xinv(const FieldList<R>& io, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw, const cpx<R>* twr,
+              hipStream_t s);
+template <class R>
+int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const cpx<R>* tw, R scale, hipStream_t s);
+template <class R>
+int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int nf, int lh, int ny, int ph,
+              const cpx<R>* tw, hipStream_t s);
+
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
