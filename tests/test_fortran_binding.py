@@ -1,0 +1,73 @@
+"""The Fortran side of the boundary: lsbflow_amd/fortran/lsb_capi.f90 (iso_c_binding
+interfaces of every entry point) compiles with amdflang, links against
+liblsbflow_amd.so, and a Fortran program calling the host-side setup routines
+through it gets the oracle's results (no GPU calls)."""
+
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import Oracle, OracleConfig
+from lsbflow_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FC = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+
+pytestmark = pytest.mark.skipif(FC is None, reason="amdflang not available")
+
+
+def test_interface_module_declares_every_entry_point():
+    src = open(os.path.join(ROOT, "lsbflow_amd", "fortran", "lsb_capi.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(lsb_[a-z0-9_]+)"\)', src))
+    hdr = open(os.path.join(ROOT, "include", "lsbflow.h")).read()
+    declared = set(re.findall(r"\b(lsb_[a-z0-9_]+)\s*\(", hdr))
+    assert bound == declared
+
+
+def test_fortran_program_through_the_binding(tmp_path):
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    mod = os.path.join(ROOT, "lsbflow_amd", "fortran", "lsb_capi.f90")
+    prog = os.path.join(ROOT, "tests", "fortran", "binding_check.f90")
+    exe = str(tmp_path / "binding_check")
+    subprocess.run([FC, "-c", mod, "-o", str(tmp_path / "lsb_capi.o"), "-module-dir", str(tmp_path)],
+                   check=True, cwd=tmp_path)
+    subprocess.run([FC, prog, str(tmp_path / "lsb_capi.o"), "-I", str(tmp_path), "-o", exe,
+                    "-L", libdir, "-llsbflow_amd", f"-Wl,-rpath,{libdir}"], check=True, cwd=tmp_path)
+
+    o = Oracle(OracleConfig(nx=8, ny=4, nz=40, mapping=3, zlen=2.0))
+    nzp, n1, n2 = o.nzp, 3, 2
+    y = np.random.default_rng(5).standard_normal((nzp, n2, n1))     # Fortran y(n1, n2, nzp)
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as fh:
+        np.array([nzp, n1, n2], dtype=np.int32).tofile(fh)
+        np.array([o.xl], dtype=np.float64).tofile(fh)
+        o.gf.astype(np.float64).tofile(fh)
+        y.astype(np.float64).tofile(fh)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    st = np.frombuffer(raw[:20], dtype=np.int32)
+    assert list(st[:3]) == [0, 0, 0]
+    assert st[3] == -1 and "nx" in r.stdout             # LSB_E_ARG with the library's message
+    assert st[4] == 14 * 4 + 17 * 8                       # sizeof(lsb_config) seen from Fortran
+    vals = np.frombuffer(raw[20:], dtype=np.float64)
+    nzm = nzp - 2
+    sizes = [nzp * nzp] * 3 + [n1 * n2 * nzp, nzm, nzm * nzm, nzm * nzm]
+    parts = np.split(vals, np.cumsum(sizes)[:-1])
+    dor, d, d2 = (p.reshape(nzp, nzp, order="F") for p in parts[:3])
+    assert np.array_equal(dor, o.dor)
+    assert np.max(np.abs(d2 - o.d2)) < 1e-14 * np.max(np.abs(o.d2))
+    dydx = parts[3].reshape(nzp, n2, n1)
+    ref = o.dct_diff(y, 1)
+    assert np.max(np.abs(dydx - ref)) < 1e-12 * np.max(np.abs(ref))
+    lam = parts[4]
+    e = parts[5].reshape(nzm, nzm, order="F")
+    ei = parts[6].reshape(nzm, nzm, order="F")
+    D = o.d2[1:o.nz, 1:o.nz]
+    assert np.max(np.abs(e @ np.diag(lam) @ ei - D)) < 1e-13 * np.max(np.abs(D))
+    assert np.max(np.abs(np.sort(lam) - np.sort(o.diag)) / np.abs(np.sort(o.diag))) < 1e-10
