@@ -44,7 +44,8 @@ int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p) {
     }
     if (c->stage_bytes[slot] < bytes) {
         if (c->stage[slot]) {
-            LSB_HIP(hipStreamSynchronize(c->stream));
+            // the old buffer may still be in use on a caller's stream
+            LSB_HIP(hipDeviceSynchronize());
             LSB_HIP(hipFree(c->stage[slot]));
         }
         c->stage[slot] = nullptr;
@@ -75,8 +76,10 @@ int prof_mark_end(lsb_ctx* c, int token, hipStream_t s) {
     return LSB_OK;
 }
 
+// Column-major copy with leading dimension ldm and zero columns up to a multiple of 4
+// (the matrix-core contraction consumes k in steps of 4).
 static void pad_matrix(const std::vector<double>& src, int nrow, int ncol, int ldm, std::vector<double>& dst) {
-    dst.assign((size_t)ldm * ncol, 0.0);
+    dst.assign((size_t)ldm * ((ncol + 3) / 4 * 4), 0.0);
     for (int j = 0; j < ncol; ++j)
         for (int i = 0; i < nrow; ++i) dst[(size_t)j * ldm + i] = src[(size_t)j * nrow + i];
 }
@@ -102,7 +105,7 @@ int upload_context(lsb_ctx* c) {
     pad_matrix(c->d, nzp, nzp, c->ldm_z, t);
     LSB_TRY(put(&c->d_d, t));
     {   // rows 0 and nzp-1 of md2, ld 64
-        std::vector<double> b((size_t)kLdmBnd * nzp, 0.0);
+        std::vector<double> b((size_t)kLdmBnd * ((nzp + 3) / 4 * 4), 0.0);
         for (int j = 0; j < nzp; ++j) {
             b[(size_t)j * kLdmBnd + 0] = c->md2[(size_t)j * nzp + 0];
             b[(size_t)j * kLdmBnd + 1] = c->md2[(size_t)j * nzp + nzp - 1];
@@ -120,6 +123,17 @@ int upload_context(lsb_ctx* c) {
     lift.insert(lift.end(), c->htop.begin(), c->htop.end());
     lift.insert(lift.end(), c->d2hbot.begin(), c->d2hbot.end());
     lift.insert(lift.end(), c->d2htop.begin(), c->d2htop.end());
+    // Einv applied to the interior values of each lifting function (rows 1..nz-1)
+    std::vector<double> elift((size_t)4 * nzm);
+    for (int q = 0; q < 4; ++q) {
+        const double* h = lift.data() + (size_t)q * nzp;
+        for (int r = 0; r < nzm; ++r) {
+            long double acc = 0.0L;
+            for (int m = 0; m < nzm; ++m) acc += (long double)c->simat[(size_t)m * nzm + r] * h[m + 1];
+            elift[(size_t)q * nzm + r] = (double)acc;
+        }
+    }
+    lift.insert(lift.end(), elift.begin(), elift.end());
     LSB_TRY(put(&c->d_lift, lift));
     std::vector<double> par;
     par.insert(par.end(), c->ubar.begin(), c->ubar.end());
@@ -233,7 +247,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     step_free(c);
     double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
-                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red};
+                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab};
     for (double* p : ds)
         if (p) (void)hipFree(p);
     for (void* p : c->stage)

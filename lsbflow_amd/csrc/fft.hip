@@ -471,6 +471,21 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
         prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
                         planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
     }
+    const int nrows = nplanes * ny;
+    if (fast_fft_len(ny) && fast_fft_len(lh)) {
+        // y pass into a compact scratch (rows of lh complex, 128-byte aligned), then the
+        // x pass out of place into the padded physical rows
+        void* scr = nullptr;
+        const size_t per = (size_t)nplanes * ny * nx;
+        LSB_TRY(stage_get(c, 8, per * nf * sizeof(double), &scr));
+        FieldList<double> mid{};
+        for (int f = 0; f < nf; ++f) mid.p[f] = (double*)scr + per * f;
+        LSB_TRY(fast_yinv<double>(in, mid, nplanes, nf, lh, ny, lh, (const cpx<double>*)hp.tw_y, s));
+        LSB_TRY(fast_xinv<double>(mid, out, nrows, nf, lh, nx, nxpp, (const cpx<double>*)hp.tw_x,
+                                  (const cpx<double>*)hp.tw_r, s));
+        prof_mark_end(c, tok, s);
+        return LSB_OK;
+    }
     if (fast_fft_len(ny)) {
         LSB_TRY(fast_yinv<double>(in, out, nplanes, nf, lh, ny, nxpp / 2, (const cpx<double>*)hp.tw_y, s));
     } else {
@@ -482,10 +497,9 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
                            hp.py, (const cpx<double>*)hp.tw_y, cols);
         LSB_HIP(hipGetLastError());
     }
-    const int nrows = nplanes * ny;
     if (fast_fft_len(lh)) {
-        LSB_TRY(fast_xinv<double>(out, nrows, nf, lh, nxpp, (const cpx<double>*)hp.tw_x, (const cpx<double>*)hp.tw_r,
-                                  s));
+        LSB_TRY(fast_xinv<double>(out, out, nrows, nf, lh, nxpp, nxpp, (const cpx<double>*)hp.tw_x,
+                                  (const cpx<double>*)hp.tw_r, s));
     } else {
         int lines = lines_for(lh, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);

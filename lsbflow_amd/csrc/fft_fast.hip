@@ -178,9 +178,11 @@ __global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<R> in, FieldList<R>
     }
 }
 
-// Inverse c2r in place on physical rows (stride nxpp) holding L complex coefficients.
+// Inverse c2r of rows holding L complex coefficients (input row stride istride reals)
+// into physical rows (stride nxpp); in place when in == out and istride == nxpp.
 template <class R, int N1, int N2>
-__global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> io, int nrows, int nf, int nxpp,
+__global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> in, FieldList<R> out, int nrows, int nf,
+                                                   int istride, int nxpp,
                                                    const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr) {
     constexpr int L = N1 * N2;
     constexpr int LPB = 128 / N2;
@@ -190,16 +192,19 @@ __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> io, int nrows, i
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
     const bool valid = r < (long)nrows * nf;
+    const cpx<R>* rin = nullptr;
     cpx<R>* row = nullptr;
     if (valid) {
         const int f = (int)(r / nrows);
-        row = reinterpret_cast<cpx<R>*>(io.p[f] + (r - (long)f * nrows) * nxpp);
+        const long rr = r - (long)f * nrows;
+        rin = reinterpret_cast<const cpx<R>*>(in.p[f] + rr * istride);
+        row = reinterpret_cast<cpx<R>*>(out.p[f] + rr * nxpp);
     }
     cpx<R>* Tl = T + l * TS;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
         const int k = n2 + N2 * j;
-        Tl[k] = valid ? row[k] : cpx<R>{0, 0};
+        Tl[k] = valid ? rin[k] : cpx<R>{0, 0};
     }
     __syncthreads();
     // Z'_n = (X_n + conj X_{L-n}) + i W^-n (X_n - conj X_{L-n}); X_L = 0, Im X_0 ignored
@@ -347,14 +352,14 @@ int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf
 }
 
 template <class R>
-int fast_xinv(const FieldList<R>& io, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw, const cpx<R>* twr,
-              hipStream_t s) {
+int fast_xinv(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int istride, int nxpp,
+              const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
     const long rows = (long)nrows * nf;
     switch (lh) {
 #define X(LL, A, B)                                                                                              \
     case LL:                                                                                                     \
         hipLaunchKernelGGL((k_xinv_fast<R, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), \
-                           0, s, io, nrows, nf, nxpp, tw, twr);                                                  \
+                           0, s, in, out, nrows, nf, istride, nxpp, tw, twr);                                 \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -403,15 +408,15 @@ int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int 
 
 template int fast_xfwd<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int,
                                const cpx<double>*, const cpx<double>*, hipStream_t);
-template int fast_xinv<double>(const FieldList<double>&, int, int, int, int, const cpx<double>*, const cpx<double>*,
-                               hipStream_t);
+template int fast_xinv<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int, int,
+                               const cpx<double>*, const cpx<double>*, hipStream_t);
 template int fast_yfwd<double>(const FieldList<double>&, int, int, int, int, const cpx<double>*, double, hipStream_t);
 template int fast_yinv<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int, int,
                                const cpx<double>*, hipStream_t);
 template int fast_xfwd<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, const cpx<float>*,
                               const cpx<float>*, hipStream_t);
-template int fast_xinv<float>(const FieldList<float>&, int, int, int, int, const cpx<float>*, const cpx<float>*,
-                              hipStream_t);
+template int fast_xinv<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, int,
+                              const cpx<float>*, const cpx<float>*, hipStream_t);
 template int fast_yfwd<float>(const FieldList<float>&, int, int, int, int, const cpx<float>*, float, hipStream_t);
 template int fast_yinv<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, int,
                               const cpx<float>*, hipStream_t);

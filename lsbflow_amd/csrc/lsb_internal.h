@@ -131,12 +131,13 @@ struct lsb_ctx {
     double* d_smat = nullptr;   // nzm x nzm
     double* d_simat = nullptr;  // nzm x nzm
     double* d_diag = nullptr;   // nzm
-    double* d_lift = nullptr;   // 4*nzp: hbot, htop, d2hbot, d2htop
+    double* d_lift = nullptr;   // 4*nzp: hbot, htop, d2hbot, d2htop; then 4*nzm: Einv of each
     double* d_paral = nullptr;  // 3*nzp: ubar, ubarp, ubarpp
     double* d_delta = nullptr;  // 3*nzp: static, dynamic, sigma
     double* d_fz = nullptr;     // 6*nzp: (a,b) weights then (1,1) weights
     double* d_smag = nullptr;   // nzp (static Smagorinsky -(cs delta fvd)^2) + scratch
-    double* d_red = nullptr;    // reduction scratch (1024 doubles)
+    double* d_red = nullptr;    // reduction scratch (kRedDoubles doubles)
+    double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
     float* d_mdz32 = nullptr;
     float* d_smat32 = nullptr;
     float* d_simat32 = nullptr;
@@ -211,8 +212,8 @@ template <class R>
 int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw,
               const cpx<R>* twr, hipStream_t s);
 template <class R>
-int fast_xinv(const FieldList<R>& io, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw, const cpx<R>* twr,
-              hipStream_t s);
+int fast_xinv(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int istride, int nxpp,
+              const cpx<R>* tw, const cpx<R>* twr, hipStream_t s);
 template <class R>
 int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const cpx<R>* tw, R scale, hipStream_t s);
 template <class R>

@@ -77,40 +77,47 @@ __global__ void k_lapl(SpDims d, const double* __restrict__ wavx, const double* 
     }
 }
 
-// filters.f90:281-360 specfilter_exp_xy: magnitude/phase form of the radial
-// exponential filter exp(-amp theta^(2p)) for |k| >= kc*kmax.
-__device__ __forceinline__ c2 specfilter_mode(c2 u, double ak, double akmax, double kc, double p, double amp) {
-    const double fr = u.x, fi = u.y;
-    const double fmag = sqrt(fr * fr + fi * fi);
-    if (fmag == 0.0) return make_double2(0.0, 0.0);
-    const double cosc = fr / fmag;
-    const double sinc = fi / fmag;
-    double phase = acos(cosc);
-    if (sinc < 0.0) phase = -phase;
+// filters.f90:338-360 specfilter_fourier2d: exp(-amp theta^(2p)) for |k| >= kc*kmax,
+// tabulated once per call for the (kx, ky) modes of one plane.
+__global__ void k_filt_table(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
+                             double* __restrict__ tab, double akmax, double kc, double p, double amp, int filtery) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)d.plane_pairs) return;
+    const int m = t % d.lh, j = t / d.lh;
+    const double wx = wavx[2 * m], wy = wavy[j];
+    const double ak = filtery == 1 ? sqrt(wx * wx + wy * wy) : wx;
     const double akc = akmax * kc;
     double filt = 1.0;
     if (fabs(ak) >= akc) {
         const double theta = (fabs(ak) - akc) / (akmax - akc);
         filt = exp(-amp * pow(theta, 2.0 * p));
     }
-    const double fmagf = fmag * filt;
-    return make_double2(fmagf * cos(phase), fmagf * sin(phase));
+    tab[t] = filt;
 }
 
-__global__ void k_specfilter(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
-                             FieldList<double> in, FieldList<double> out, int nf, double akmax, double kc, double p,
-                             double amp, int filtery) {
-    const long n = d.npairs * nf;
-    SP_LOOP(n) {
-        const int f = (int)(pr / d.npairs);
-        const long q = pr - (long)f * d.npairs;
-        const int m = (int)(q % d.lh);
-        const int j = (int)((q / d.lh) % d.ny);
-        const double wx = wavx[2 * m], wy = wavy[j];
-        const double ak = filtery == 1 ? sqrt(wx * wx + wy * wy) : wx;
-        const c2 u = reinterpret_cast<const c2*>(in.p[f])[q];
-        reinterpret_cast<c2*>(out.p[f])[q] = specfilter_mode(u, ak, akmax, kc, p, amp);
+// filters.f90:281-335 specfilter_exp_xy: the mode is rebuilt from its magnitude and
+// phase, fmag*filt*(cos(phase), sin(phase)) with cos(phase) = fr/fmag and phase
+// signed by fi.  cos(acos(c)) = c and |sin(acos(c))| = sqrt((1-c)(1+c)) are the
+// same functions of the rounded c = fr/fmag, so the form keeps the reference's
+// loss of Im parts that are tiny next to Re (c rounds to +-1) without the
+// transcendental round trip.  One plane per blockIdx.y, no 64-bit index division.
+__global__ void k_specfilter(SpDims d, const double* __restrict__ tab, FieldList<double> in, FieldList<double> out,
+                             int nzp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)d.plane_pairs) return;
+    const int f = blockIdx.y / nzp;
+    const long q = (long)(blockIdx.y - f * nzp) * d.plane_pairs + t;
+    const c2 u = reinterpret_cast<const c2*>(in.p[f])[q];
+    const double fmag = sqrt(u.x * u.x + u.y * u.y);
+    c2 r = make_double2(0.0, 0.0);
+    if (fmag != 0.0) {
+        const double cosc = u.x / fmag;
+        double sn = sqrt(fmax((1.0 - cosc) * (1.0 + cosc), 0.0));
+        if (u.y < 0.0) sn = -sn;
+        const double fmagf = fmag * tab[t];
+        r = make_double2(fmagf * cosc, fmagf * sn);
     }
+    reinterpret_cast<c2*>(out.p[f])[q] = r;
 }
 
 // horfft.f90:153-224 norm on the kx = 0 column and the ky-Nyquist row.
@@ -192,10 +199,15 @@ int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int n
         fi.p[f] = const_cast<double*>(in[f]);
         fo.p[f] = out[f];
     }
+    if (!c->d_ftab) LSB_TRY(dev_alloc((void**)&c->d_ftab, (size_t)d.plane_pairs * sizeof(double)));
+    const unsigned nb = (unsigned)((d.plane_pairs + 255) / 256);
     int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
-    hipLaunchKernelGGL(k_specfilter, sp_grid(d.npairs * nf), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, fi, fo, nf,
-                       akmax, kc, p, amp, filtery);
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, 2.0 * (double)d.npairs * nf * 2 * sizeof(double));
+    hipLaunchKernelGGL(k_filt_table, dim3(nb), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, c->d_ftab, akmax, kc, p, amp,
+                       filtery);
+    LSB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_specfilter, dim3(nb, (unsigned)(c->nzp * nf)), dim3(256), 0, s, d, c->d_ftab, fi, fo,
+                       c->nzp);
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;

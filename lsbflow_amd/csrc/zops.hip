@@ -5,16 +5,23 @@
 //   solve_helmholtz       (helmholtz.f90:130-208), two contractions:
 //       t[r][p]   = (sum_m Einv[r][m] q[m+1][p]) / (lam_r - kx^2 - ky^2 - sigma)
 //       q[r+1][p] = sum_m E[r][m] t[m][p],   q[0][p] = q[nz][p] = 0
-//   solve_nhom_helmholtz  (helmholtz.f90:211-358): the lifting subtraction is
-//       fused into the stage-1 load and the h*bc addition into the stage-2 store.
+//   solve_nhom_helmholtz  (helmholtz.f90:211-358): the lifting subtraction
+//       q - L(h_bot) bc_bot - L(h_top) bc_top, L(h) = h'' - (kx^2+ky^2+sigma) h, is
+//       applied after the stage-1 projection as Einv q - (Einv L(h)) bc, with Einv h
+//       and Einv h'' precomputed; the h*bc addition is fused into the stage-2 store.
 //
 // A spectral field is a row-major [nzp][P] matrix with P = nx*ny reals, so every
 // contraction is M (nout x nin) times a tall skinny panel; each (Re or Im) column
-// is one (kx,ky) pencil.  FP64 path: one wave per ROWS output rows and 64
-// columns, the matrix column M[:, m] read through the scalar cache (wave-uniform
-// address) and the panel element broadcast into ROWS FMAs.  On gfx950 the FP64
-// vector FMA rate equals the FP64 MFMA rate (78.6 TFLOP/s), so this form runs at
-// the same roofline as v_mfma_f64 without the operand shuffles.
+// is one (kx,ky) pencil.  FP64 path on the matrix cores: v_mfma_f64_16x16x4f64
+// tiles.  A block holds up to 8 waves that share 64 pencils (4 column blocks of
+// 16); wave w owns at most 3 of the ceil(nout/16) row blocks, so each lane keeps
+// 3 x 4 accumulator tiles (48 doubles).  Fragments come straight from memory:
+// A[i][k] = M[r0+i][k0+k] on lane i + 16k (16 consecutive matrix rows, one
+// 128-byte segment per k), B[k][j] = X[k0+k][p0+j] on lane j + 16k (16 consecutive
+// pencils); the panel rows of k-step s+2 are in flight while step s is multiplied.
+// Output lane layout (measured): lane l, element v holds D[l/16 + 4v][l%16].
+// On gfx950 the FP64 matrix rate equals the FP64 vector rate (78.6 TFLOP/s); the
+// matrix form wins by its 2-D register blocking (7 fragment loads per 12 MFMAs).
 
 #include "lsb_internal.h"
 
@@ -22,140 +29,175 @@ namespace lsb {
 
 enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2 };
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 struct ZArgs {
-    const double* M;    // column-major, leading dim ldm (zero padded rows)
+    const double* M;    // column-major, leading dim ldm; zero rows to ldm, zero columns to round_up(nin, 4)
     int ldm, nout, nin;
     int in_row0;        // first input plane
     int out_row0;       // first output plane
     long P;             // columns per plane (nx*ny)
     int nx, ny;
-    int nwaves;
+    int gblocks;        // row blocks per blockIdx.y group
     double sigma;
     const double* wavx;   // nx
     const double* wavy;   // ny
     const double* lam;    // nzm (stage 1)
     const double* lift;   // hbot, htop, d2hbot, d2htop (4*nzp)
+    const double* elift;  // Einv applied to interior hbot, htop, d2hbot, d2htop (4*nzm)
     int nzp;
 };
 
-template <int ROWS, int MODE>
-__global__ void __launch_bounds__(1024) k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out,
-                                                    FieldList<double> bct, FieldList<double> bcb) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+constexpr int kZRB = 3;   // row blocks (of 16) per wave
+constexpr int kZCB = 4;   // column blocks (of 16) per block
+constexpr int kZPD = 2;   // k-steps of fragments in flight
+constexpr int kZMaxWaves = 8;
+
+template <int MODE>
+__global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out,
+                                                                FieldList<double> bct, FieldList<double> bcb) {
+    constexpr int RB = kZRB, CB = kZCB, PD = kZPD;
     const int lane = threadIdx.x & 63;
-    const long col = (long)blockIdx.x * 64 + lane;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int f = blockIdx.z;
-    const int r0 = (blockIdx.y * a.nwaves + wave) * ROWS;
-    if (r0 >= a.nout) return;
-    const bool valid = col < a.P;
-    const long cc = valid ? col : 0;
-    const double* __restrict__ src = in.p[f] + (long)a.in_row0 * a.P;
-    double dfac = 0.0;
-    double vbt = 0.0, vbb = 0.0;
-    const bool nhom = (MODE != ZM_PLAIN) && (bct.p[f] != nullptr);
-    if (MODE != ZM_PLAIN) {
-        const int i = (int)(cc % a.nx);
-        const int j = (int)((cc / a.nx) % a.ny);
-        const double wx = a.wavx[i], wy = a.wavy[j];
-        dfac = -(wx * wx) - wy * wy - a.sigma;
-        if (nhom) {
-            vbt = bct.p[f][cc];
-            vbb = bcb.p[f][cc];
-        }
+    const int nrb_all = (a.nout + 15) >> 4;
+    const int g0 = blockIdx.y * a.gblocks;
+    const int nrb = min(a.gblocks, nrb_all - g0);
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    if (myn <= 0) return;
+    const int rb0 = g0 + wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * (16 * CB);
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    const double* __restrict__ X = in.p[f] + (long)a.in_row0 * P;
+    const double* __restrict__ Ma = a.M + (long)(rb0 * 16 + li) + (long)lk * a.ldm;
+    const int nin = a.nin;
+    const int nks = (nin + 3) >> 2;
+    v4d acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double fa[PD][RB], fb[PD][CB];
+#pragma unroll
+    for (int s = 0; s < PD; ++s) {
+        const int ks = min(s, nks - 1);
+        const long kr = min(4 * ks + lk, nin - 1);
+#pragma unroll
+        for (int i = 0; i < RB; ++i) fa[s][i] = Ma[(long)(4 * ks) * a.ldm + 16 * i];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) fb[s][j] = X[kr * P + cj[j]];
     }
-    double acc[ROWS];
+    for (int k0 = 0; k0 < nks; k0 += PD) {
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) acc[r] = 0.0;
-    const double* __restrict__ M = a.M + r0;
-    for (int m = 0; m < a.nin; ++m) {
-        double x = src[(long)m * a.P + cc];
-        if (MODE == ZM_HELM1 && nhom) {
-            const int k = m + 1;  // physical plane of interior row m
-            const double abot = a.lift[2 * a.nzp + k] + dfac * a.lift[k];
-            x = x - abot * vbb;
-            const double atop = a.lift[3 * a.nzp + k] + dfac * a.lift[a.nzp + k];
-            x = x - atop * vbt;
-        }
-        const double* __restrict__ Mc = M + (long)m * a.ldm;
+        for (int s = 0; s < PD; ++s) {
+            const int ks = k0 + s;
+            if (ks < nks) {
+                double ca[RB], cb[CB];
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) acc[r] = fma(Mc[r], x, acc[r]);
-    }
-    if (!valid) return;
-    double* __restrict__ dst = out.p[f] + (long)a.out_row0 * a.P;
+                for (int i = 0; i < RB; ++i) ca[i] = fa[s][i];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const int row = r0 + r;
-        if (row < a.nout) {
-            double v = acc[r];
-            if (MODE == ZM_HELM1) v = v / (a.lam[row] + dfac);
-            if (MODE == ZM_HELM2 && nhom) {
-                const int k = row + 1;
-                v = v + a.lift[k] * vbb + a.lift[a.nzp + k] * vbt;
+                for (int j = 0; j < CB; ++j) cb[j] = fb[s][j];
+                const int kn = min(ks + PD, nks - 1);
+                const long kr = min(4 * kn + lk, nin - 1);
+#pragma unroll
+                for (int i = 0; i < RB; ++i) fa[s][i] = Ma[(long)(4 * kn) * a.ldm + 16 * i];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) fb[s][j] = X[kr * P + cj[j]];
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    if (i < myn)
+#pragma unroll
+                        for (int j = 0; j < CB; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i], cb[j], acc[i][j], 0, 0, 0);
             }
-            dst[(long)row * a.P + col] = v;
         }
     }
-    if (MODE == ZM_HELM2 && r0 == 0) {
+    // per-pencil epilogue data
+    double dfac[CB], vbt[CB], vbb[CB];
+    const bool nhom = (MODE != ZM_PLAIN) && (bct.p[f] != nullptr);
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        dfac[j] = 0.0;
+        vbt[j] = 0.0;
+        vbb[j] = 0.0;
+        if (MODE != ZM_PLAIN) {
+            const int c = (int)cj[j];
+            const double wx = a.wavx[c % a.nx], wy = a.wavy[c / a.nx];
+            dfac[j] = -(wx * wx) - wy * wy - a.sigma;
+            if (nhom) {
+                vbt[j] = bct.p[f][c];
+                vbb[j] = bcb.p[f][c];
+            }
+        }
+    }
+    double* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = (rb0 + i) * 16 + lk + 4 * v;
+                if (row < a.nout) {
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        const long col = p0 + 16 * j + li;
+                        double x = acc[i][j][v];
+                        if (MODE == ZM_HELM1) {
+                            if (nhom) {
+                                // Einv (q - L(h_bot) bc_bot - L(h_top) bc_top), L(h) = h'' + dfac h,
+                                // with Einv h and Einv h'' precomputed per interior row
+                                const double* e = a.elift;
+                                const int n = a.nout;
+                                x = x - vbb[j] * (e[2 * n + row] + dfac[j] * e[row]);
+                                x = x - vbt[j] * (e[3 * n + row] + dfac[j] * e[n + row]);
+                            }
+                            x = x / (a.lam[row] + dfac[j]);
+                        }
+                        if (MODE == ZM_HELM2 && nhom) {
+                            const int k = row + 1;
+                            x = x + a.lift[k] * vbb[j] + a.lift[a.nzp + k] * vbt[j];
+                        }
+                        if (col < P) dst[(long)row * P + col] = x;
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == ZM_HELM2 && g0 == 0 && wave == 0 && lk == 0) {
         // boundary planes 0 and nz of the full field (out_row0 == 1)
         double* full = out.p[f];
-        double v0 = 0.0, vn = 0.0;
-        if (nhom) {
-            v0 = 0.0 + a.lift[0] * vbb + a.lift[a.nzp + 0] * vbt;
-            vn = 0.0 + a.lift[a.nzp - 1] * vbb + a.lift[a.nzp + a.nzp - 1] * vbt;
-        }
-        full[col] = v0;
-        full[(long)(a.nzp - 1) * a.P + col] = vn;
-    }
-}
-
-static void pick_rows(int nout, int* rows, int* nwaves) {
-    // row blocks above 48 exceed the 128 VGPRs a 1024-thread block allows and spill
-    static const int cand[5] = {48, 40, 32, 24, 16};
-    int best = -1, bw = 0, bwaste = 1 << 30;
-    for (int ci = 0; ci < 5; ++ci) {
-        int r = cand[ci];
-        int nw = (nout + r - 1) / r;
-        if (nw > 16) continue;
-        int waste = nw * r - nout;
-        if (waste < bwaste) {
-            bwaste = waste;
-            best = r;
-            bw = nw;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const long col = p0 + 16 * j + li;
+            double v0 = 0.0, vn = 0.0;
+            if (nhom) {
+                v0 = 0.0 + a.lift[0] * vbb[j] + a.lift[a.nzp + 0] * vbt[j];
+                vn = 0.0 + a.lift[a.nzp - 1] * vbb[j] + a.lift[a.nzp + a.nzp - 1] * vbt[j];
+            }
+            if (col < P) {
+                full[col] = v0;
+                full[(long)(a.nzp - 1) * P + col] = vn;
+            }
         }
     }
-    if (best < 0) {
-        best = 48;
-        bw = 16;
-    }
-    *rows = best;
-    *nwaves = bw;
 }
 
 template <int MODE>
 static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldList<double>& out,
                     const FieldList<double>& bt, const FieldList<double>& bb, int nf, hipStream_t s) {
     ZArgs a = a0;
-    int rows, nw;
-    pick_rows(a.nout, &rows, &nw);
-    const int gy = (a.nout + rows * nw - 1) / (rows * nw);
-    a.nwaves = nw;
-    dim3 grid((unsigned)((a.P + 63) / 64), (unsigned)gy, (unsigned)nf);
-    dim3 block(64 * nw);
-#define LSB_ZL(R)                                                                                  \
-    case R:                                                                                        \
-        hipLaunchKernelGGL((k_zcontract<R, MODE>), grid, block, 0, s, a, in, out, bt, bb); \
-        break;
-    switch (rows) {
-        LSB_ZL(48)
-        LSB_ZL(40)
-        LSB_ZL(32)
-        LSB_ZL(24)
-        LSB_ZL(16)
-        default:
-            set_error("zcontract: bad row block");
-            return LSB_E_ARG;
-    }
-#undef LSB_ZL
+    const int nrb = (a.nout + 15) / 16;
+    a.gblocks = std::min(nrb, kZRB * kZMaxWaves);
+    const int gy = (nrb + a.gblocks - 1) / a.gblocks;
+    const int nw = (a.gblocks + kZRB - 1) / kZRB;
+    dim3 grid((unsigned)((a.P + 16 * kZCB - 1) / (16 * kZCB)), (unsigned)gy, (unsigned)nf);
+    hipLaunchKernelGGL((k_zcontract<MODE>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
@@ -221,6 +263,7 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
     a.wavy = c->d_wavy;
     a.lam = c->d_diag;
     a.lift = c->d_lift;
+    a.elift = c->d_lift + 4 * c->nzp;
     int tok = -1;
     // two nzm x nzm projections per pencil; read rhs and write the solution once each
     prof_mark_begin(c, PC_HELM, s, &tok, 4.0 * nzm * (double)nzm * a.P * nf,
