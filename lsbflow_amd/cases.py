@@ -124,7 +124,11 @@ def normal_stream(seed: int, n: int) -> np.ndarray:
 
 # ------------------------------------------------------------------ grid helpers
 def zpts_mapping3(case: Case) -> np.ndarray:
-    """init.f90:231-237 (mapping 3): z = zlen/2 + zlen/2 cos(pi k/nz), k = 0..nz."""
+    """init.f90:231-237 (mapping 3): z = zlen/2 + zlen/2 cos(pi k/nz), k = 0..nz.
+
+    This is also the distance from the bottom wall for mapping 0 (init.f90:188-196,
+    z = zlen/2 cos(pi k/nz), the same channel shifted by zlen/2), so the base flow
+    and the seeded perturbation are the same arrays for both channel mappings."""
     xl = case.zlen
     nz = case.nz
     return np.array([xl / 2.0 + xl / 2.0 * math.cos(PI * (k + 0.0) / float(nz)) for k in range(nz + 1)])
@@ -164,8 +168,8 @@ def _spectral_to_physical(fh: np.ndarray, nx: int) -> np.ndarray:
 
 def initial_condition(case: Case):
     """Seeded divergence-free perturbation -> (uf, vf, wf) spectral (nzp, ny, nx) float64."""
-    if case.mapping != 3:
-        raise ValueError("the seeded perturbation is defined for the channel mapping 3")
+    if case.mapping not in (0, 3):
+        raise ValueError("the seeded perturbation is defined for the channel mappings 0 and 3")
     nx, ny, nz = case.nx, case.ny, case.nz
     nzp = nz + 1
     z = zpts_mapping3(case)

@@ -155,6 +155,16 @@ int upload_context(lsb_ctx* c) {
         return LSB_E_ARG;
     }
     if (!c->d_red) LSB_TRY(dev_alloc((void**)&c->d_red, (size_t)kRedDoubles * sizeof(double)));
+    {   // wall planes of the static model (sgsterm.f90:59-81)
+        std::vector<int> walls;
+        for (int k = 0; k < nzp; ++k)
+            if (c->zpts[k] == 0.0 || std::fabs(c->zpts[k]) == 0.5) walls.push_back(k);
+        c->nwalls = (int)walls.size();
+        if (!c->d_walls) LSB_TRY(dev_alloc((void**)&c->d_walls, (size_t)nzp * sizeof(int)));
+        if (!walls.empty())
+            LSB_HIP(hipMemcpyAsync(c->d_walls, walls.data(), walls.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        LSB_HIP(hipStreamSynchronize(s));
+    }
     LSB_HIP(hipStreamSynchronize(s));
     return LSB_OK;
 }
@@ -250,6 +260,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
                     c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab};
     for (double* p : ds)
         if (p) (void)hipFree(p);
+    if (c->d_walls) (void)hipFree(c->d_walls);
     for (void* p : c->stage)
         if (p) (void)hipFree(p);
     if (c->st) (void)hipFree(c->st);

@@ -406,6 +406,16 @@ static int lines_for(int len, size_t elem) {
     return l;
 }
 
+static double fft_flops(const HorPlan& hp, int nf, int nplanes) {
+    const double N = (double)hp.nx * hp.ny;
+    return (double)nf * nplanes * 2.5 * N * std::log2(N);
+}
+// algorithmic bytes of nf plane-batched 2-D transforms: read one layout, write the other
+static double fft_bytes(const HorPlan& hp, int nf, int nplanes, size_t phys_elem, size_t spec_elem) {
+    const double planes = (double)nf * nplanes;
+    return planes * ((double)(hp.nx + 2) * hp.ny * phys_elem + (double)hp.nx * hp.ny * spec_elem);
+}
+
 int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double* const* spec, int nf, int nplanes,
                hipStream_t s) {
     if (nf < 1 || nf > kMaxFields) {
@@ -419,16 +429,11 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
         out.p[f] = spec[f];
     }
     int tok = -1;
-    {   // algorithmic work: one read of the physical and one write of the spectral layout per
-        // field-plane; 2.5 N log2 N flops per real 2-D transform of N = nx*ny points
-        const double N = (double)nx * ny, planes = (double)nf * nplanes;
-        prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
-                        planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
-    }
+    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 8, 8));
     const int nrows = nplanes * ny;
     if (fast_fft_len(lh)) {
-        LSB_TRY(fast_xfwd<double>(in, out, nrows, nf, lh, nxpp, (const cpx<double>*)hp.tw_x,
-                                  (const cpx<double>*)hp.tw_r, s));
+        LSB_TRY((fast_xfwd<double, double, double>(in, out, nrows, nf, lh, nxpp, nx, (const cpx<double>*)hp.tw_x,
+                                                   (const cpx<double>*)hp.tw_r, s)));
     } else {
         int lines = lines_for(lh, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
@@ -439,7 +444,8 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
     }
     const double scale = 1.0 / (double)(nx * ny);
     if (fast_fft_len(ny)) {
-        LSB_TRY(fast_yfwd<double>(out, nplanes, nf, lh, ny, (const cpx<double>*)hp.tw_y, scale, s));
+        LSB_TRY((fast_yfwd<double, double, double>(out, out, nplanes, nf, lh, ny, (const cpx<double>*)hp.tw_y, scale,
+                                                   s)));
     } else {
         int cols = lines_for(ny, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
@@ -453,26 +459,32 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
     return LSB_OK;
 }
 
-int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf, int nplanes,
-               hipStream_t s) {
+bool horfft_has_ops(const HorPlan& hp) { return fast_fft_len(hp.ny) && fast_fft_len(hp.lh); }
+
+int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
+                   int nplanes, const int* ops, hipStream_t s) {
     if (nf < 1 || nf > kMaxFields) {
         set_error("horfft: 1..16 fields per launch");
         return LSB_E_ARG;
     }
     const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
     FieldList<double> in{}, out{};
+    FftOps fo{};
+    bool any_op = false;
     for (int f = 0; f < nf; ++f) {
         in.p[f] = const_cast<double*>(spec[f]);
         out.p[f] = phys[f];
+        fo.op[f] = ops ? ops[f] : FOP_NONE;
+        any_op = any_op || fo.op[f] != FOP_NONE;
+    }
+    if (any_op && !horfft_has_ops(hp)) {
+        set_error("horfft: derivative-on-load needs power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
     }
     int tok = -1;
-    {
-        const double N = (double)nx * ny, planes = (double)nf * nplanes;
-        prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * N * std::log2(N),
-                        planes * ((double)(nx + 2) * ny + (double)nx * ny) * sizeof(double));
-    }
+    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 8, 8));
     const int nrows = nplanes * ny;
-    if (fast_fft_len(ny) && fast_fft_len(lh)) {
+    if (horfft_has_ops(hp)) {
         // y pass into a compact scratch (rows of lh complex, 128-byte aligned), then the
         // x pass out of place into the padded physical rows
         void* scr = nullptr;
@@ -480,14 +492,16 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
         LSB_TRY(stage_get(c, 8, per * nf * sizeof(double), &scr));
         FieldList<double> mid{};
         for (int f = 0; f < nf; ++f) mid.p[f] = (double*)scr + per * f;
-        LSB_TRY(fast_yinv<double>(in, mid, nplanes, nf, lh, ny, lh, (const cpx<double>*)hp.tw_y, s));
-        LSB_TRY(fast_xinv<double>(mid, out, nrows, nf, lh, nx, nxpp, (const cpx<double>*)hp.tw_x,
-                                  (const cpx<double>*)hp.tw_r, s));
+        LSB_TRY((fast_yinv<double, double, double>(in, mid, nplanes, nf, lh, ny, lh, (const cpx<double>*)hp.tw_y, fo,
+                                                   c->d_wavx, c->d_wavy, s)));
+        LSB_TRY((fast_xinv<double, double, double>(mid, out, nrows, nf, lh, nx, nxpp, (const cpx<double>*)hp.tw_x,
+                                                   (const cpx<double>*)hp.tw_r, s)));
         prof_mark_end(c, tok, s);
         return LSB_OK;
     }
     if (fast_fft_len(ny)) {
-        LSB_TRY(fast_yinv<double>(in, out, nplanes, nf, lh, ny, nxpp / 2, (const cpx<double>*)hp.tw_y, s));
+        LSB_TRY((fast_yinv<double, double, double>(in, out, nplanes, nf, lh, ny, nxpp / 2, (const cpx<double>*)hp.tw_y,
+                                                   fo, c->d_wavx, c->d_wavy, s)));
     } else {
         int cols = lines_for(ny, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
@@ -498,8 +512,8 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
         LSB_HIP(hipGetLastError());
     }
     if (fast_fft_len(lh)) {
-        LSB_TRY(fast_xinv<double>(out, out, nrows, nf, lh, nxpp, nxpp, (const cpx<double>*)hp.tw_x,
-                                  (const cpx<double>*)hp.tw_r, s));
+        LSB_TRY((fast_xinv<double, double, double>(out, out, nrows, nf, lh, nxpp, nxpp, (const cpx<double>*)hp.tw_x,
+                                                   (const cpx<double>*)hp.tw_r, s)));
     } else {
         int lines = lines_for(lh, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<double>);
@@ -511,5 +525,74 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }
+
+int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf, int nplanes,
+               hipStream_t s) {
+    return horfft_inv_ops(c, hp, spec, phys, nf, nplanes, nullptr, s);
+}
+
+// Mixed form: FP32 physical fields, FP64 spectra, FP32 arithmetic.
+int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, double* const* spec, int nf,
+                     int nplanes, hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields || !horfft_has_ops(hp)) {
+        set_error("horfft (mixed): 1..16 fields, power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
+    }
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    void* scr = nullptr;
+    const size_t per = (size_t)nplanes * ny * nx;
+    LSB_TRY(stage_get(c, 8, per * nf * sizeof(float), &scr));
+    FieldList<float> in{}, mid{};
+    FieldList<double> out{};
+    for (int f = 0; f < nf; ++f) {
+        in.p[f] = const_cast<float*>(phys[f]);
+        mid.p[f] = (float*)scr + per * f;
+        out.p[f] = spec[f];
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 4, 8));
+    const int nrows = nplanes * ny;
+    LSB_TRY((fast_xfwd<float, float, float>(in, mid, nrows, nf, lh, nxpp, nx, (const cpx<float>*)hp.tw_x32,
+                                            (const cpx<float>*)hp.tw_r32, s)));
+    LSB_TRY((fast_yfwd<float, float, double>(mid, out, nplanes, nf, lh, ny, (const cpx<float>*)hp.tw_y32,
+                                             1.0f / (float)(nx * ny), s)));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+
+template <class TI>
+int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,
+                     const int* ops, hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields || !horfft_has_ops(hp)) {
+        set_error("horfft (mixed): 1..16 fields, power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
+    }
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    void* scr = nullptr;
+    const size_t per = (size_t)nplanes * ny * nx;
+    LSB_TRY(stage_get(c, 8, per * nf * sizeof(float), &scr));
+    FieldList<TI> in{};
+    FieldList<float> mid{}, out{};
+    FftOps fo{};
+    for (int f = 0; f < nf; ++f) {
+        in.p[f] = const_cast<TI*>(spec[f]);
+        mid.p[f] = (float*)scr + per * f;
+        out.p[f] = phys[f];
+        fo.op[f] = ops ? ops[f] : FOP_NONE;
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 4, sizeof(TI)));
+    const int nrows = nplanes * ny;
+    LSB_TRY((fast_yinv<float, TI, float>(in, mid, nplanes, nf, lh, ny, lh, (const cpx<float>*)hp.tw_y32, fo,
+                                         c->d_wavx, c->d_wavy, s)));
+    LSB_TRY((fast_xinv<float, float, float>(mid, out, nrows, nf, lh, nx, nxpp, (const cpx<float>*)hp.tw_x32,
+                                            (const cpx<float>*)hp.tw_r32, s)));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+template int horfft_inv_mixed<double>(lsb_ctx*, const HorPlan&, const double* const*, float* const*, int, int,
+                                      const int*, hipStream_t);
+template int horfft_inv_mixed<float>(lsb_ctx*, const HorPlan&, const float* const*, float* const*, int, int,
+                                     const int*, hipStream_t);
 
 }  // namespace lsb

@@ -12,6 +12,13 @@
 // c2r merge of the x pass use the natural-order spectrum staged once more in
 // the same LDS tile.  Semantics are those of fft.hip (scaling, Nyquist drop,
 // zero pads).
+//
+// Every pass is templated on the arithmetic type R and on the element types of
+// its input (TI) and output (TO): the FP64 form uses double throughout, the
+// mixed form reads FP64 spectra and writes FP32 physical fields (and back), the
+// conversion happening in registers at the load/store.  The inverse y pass can
+// apply d/dx or d/dy (x i*kx, x i*ky, derivatives.f90:261-303) to a field as it
+// loads it, so spectral derivative fields never round-trip through HBM.
 
 #include "lsb_internal.h"
 
@@ -121,10 +128,40 @@ __device__ __forceinline__ void f_twiddle(cpx<R>* v, int n2, const cpx<R>* __res
 }
 
 // --------------------------------------------------------------- X passes
-// Forward r2c of rows (row r of nf*nrows; physical stride nxpp, spectral stride nx).
-template <class R, int N1, int N2>
-__global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<R> in, FieldList<R> out, int nrows, int nf, int nxpp,
-                                                   const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr) {
+template <class R, class T>
+__device__ __forceinline__ cpx<R> ld_c(const T* p, long i) {
+    const T* q = p + 2 * i;
+    return {(R)q[0], (R)q[1]};
+}
+template <class R, class T>
+__device__ __forceinline__ void st_c(T* p, long i, cpx<R> v) {
+    T* q = p + 2 * i;
+    q[0] = (T)v.re;
+    q[1] = (T)v.im;
+}
+template <>
+__device__ __forceinline__ cpx<double> ld_c<double, double>(const double* p, long i) {
+    return reinterpret_cast<const cpx<double>*>(p)[i];
+}
+template <>
+__device__ __forceinline__ void st_c<double, double>(double* p, long i, cpx<double> v) {
+    reinterpret_cast<cpx<double>*>(p)[i] = v;
+}
+template <>
+__device__ __forceinline__ cpx<float> ld_c<float, float>(const float* p, long i) {
+    return reinterpret_cast<const cpx<float>*>(p)[i];
+}
+template <>
+__device__ __forceinline__ void st_c<float, float>(float* p, long i, cpx<float> v) {
+    reinterpret_cast<cpx<float>*>(p)[i] = v;
+}
+
+// Forward r2c of rows: row r of nf*nrows, input stride istride reals (physical
+// nxpp), output stride ostride reals (spectral nx).
+template <class R, class TI, class TO, int N1, int N2>
+__global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<TI> in, FieldList<TO> out, int nrows, int nf,
+                                                   int istride, int ostride, const cpx<R>* __restrict__ tw,
+                                                   const cpx<R>* __restrict__ twr) {
     constexpr int L = N1 * N2;
     constexpr int LPB = 128 / N2;         // lines per block
     constexpr int TS = N1 * (N2 + 1);      // padded transpose tile per line (>= L)
@@ -133,17 +170,17 @@ __global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<R> in, FieldList<R>
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
     const bool valid = r < (long)nrows * nf;
-    const cpx<R>* src = nullptr;
-    cpx<R>* dst = nullptr;
+    const TI* src = nullptr;
+    TO* dst = nullptr;
     if (valid) {
         const int f = (int)(r / nrows);
         const long rr = r - (long)f * nrows;
-        src = reinterpret_cast<const cpx<R>*>(in.p[f] + rr * nxpp);
-        dst = reinterpret_cast<cpx<R>*>(out.p[f] + rr * (2 * L));
+        src = in.p[f] + rr * istride;
+        dst = out.p[f] + rr * ostride;
     }
     cpx<R> v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? src[N2 * n1 + n2] : cpx<R>{0, 0};
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? ld_c<R>(src, N2 * n1 + n2) : cpx<R>{0, 0};
     f_dft<R, -1, N1>(v);
     f_twiddle<R, -1, N1>(v, n2, tw);
     cpx<R>* Tl = T + l * TS;
@@ -174,16 +211,17 @@ __global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<R> in, FieldList<R>
         const cpx<R> a = {(R)0.5 * (zk.re + zc.re), (R)0.5 * (zk.im + zc.im)};
         const cpx<R> d = {(R)0.5 * (zk.re - zc.re), (R)0.5 * (zk.im - zc.im)};
         const cpx<R> wd = cmul(twr[k], d);
-        dst[k] = {a.re + wd.im, a.im - wd.re};
+        st_c<R>(dst, k, cpx<R>{a.re + wd.im, a.im - wd.re});
     }
 }
 
-// Inverse c2r of rows holding L complex coefficients (input row stride istride reals)
-// into physical rows (stride nxpp); in place when in == out and istride == nxpp.
-template <class R, int N1, int N2>
-__global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> in, FieldList<R> out, int nrows, int nf,
-                                                   int istride, int nxpp,
-                                                   const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ twr) {
+// Inverse c2r of rows holding L complex coefficients (input row stride istride
+// reals) into physical rows (stride nxpp, the two pad reals written as zero);
+// in place when in == out and istride == nxpp.
+template <class R, class TI, class TO, int N1, int N2>
+__global__ void __launch_bounds__(128) k_xinv_fast(FieldList<TI> in, FieldList<TO> out, int nrows, int nf,
+                                                   int istride, int nxpp, const cpx<R>* __restrict__ tw,
+                                                   const cpx<R>* __restrict__ twr) {
     constexpr int L = N1 * N2;
     constexpr int LPB = 128 / N2;
     constexpr int TS = N1 * (N2 + 1);
@@ -192,19 +230,19 @@ __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> in, FieldList<R>
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
     const bool valid = r < (long)nrows * nf;
-    const cpx<R>* rin = nullptr;
-    cpx<R>* row = nullptr;
+    const TI* rin = nullptr;
+    TO* row = nullptr;
     if (valid) {
         const int f = (int)(r / nrows);
         const long rr = r - (long)f * nrows;
-        rin = reinterpret_cast<const cpx<R>*>(in.p[f] + rr * istride);
-        row = reinterpret_cast<cpx<R>*>(out.p[f] + rr * nxpp);
+        rin = in.p[f] + rr * istride;
+        row = out.p[f] + rr * nxpp;
     }
     cpx<R>* Tl = T + l * TS;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
         const int k = n2 + N2 * j;
-        Tl[k] = valid ? rin[k] : cpx<R>{0, 0};
+        Tl[k] = valid ? ld_c<R>(rin, k) : cpx<R>{0, 0};
     }
     __syncthreads();
     // Z'_n = (X_n + conj X_{L-n}) + i W^-n (X_n - conj X_{L-n}); X_L = 0, Im X_0 ignored
@@ -240,15 +278,17 @@ __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<R> in, FieldList<R>
         for (int j = 0; j < N2; ++j) u[j] = Tl[k1 * (N2 + 1) + j];
         f_dft<R, 1, N2>(u);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) row[k1 + N1 * k2] = u[k2];
+        for (int k2 = 0; k2 < N2; ++k2) st_c<R>(row, k1 + N1 * k2, u[k2]);
     }
-    if (n2 == 0) row[L] = {0, 0};  // the two pad reals
+    if (n2 == 0) st_c<R>(row, L, cpx<R>{0, 0});  // the two pad reals
 }
 
 // --------------------------------------------------------------- Y passes
 // Tiles of C kx-columns x L ky-rows of one plane; lane index c fastest.
-template <class R, int N1, int N2, int C>
-__global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<R> io, int nplanes, int lh,
+// Forward: input rows of lh complex (stride lh), output the same layout, scaled;
+// in place when in == out.
+template <class R, class TI, class TO, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<TI> in, FieldList<TO> out, int nplanes, int lh,
                                                       const cpx<R>* __restrict__ tw, R scale) {
     constexpr int TS = N1 * (N2 + 1) + 1;
     __shared__ cpx<R> T[C * TS];
@@ -259,12 +299,13 @@ __global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<R> io, int nplan
     const long plane = blockIdx.x / ntile;
     const int f = (int)(plane / nplanes);
     const long pk = plane - (long)f * nplanes;
-    cpx<R>* base = reinterpret_cast<cpx<R>*>(io.p[f] + pk * (long)(N1 * N2) * (2 * lh));
+    const TI* src = in.p[f] + pk * (long)(N1 * N2) * (2 * lh);
+    TO* dst = out.p[f] + pk * (long)(N1 * N2) * (2 * lh);
     const int m = tile * C + c;
     const bool valid = m < lh;
     cpx<R> v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? base[(long)(N2 * n1 + n2) * lh + m] : cpx<R>{0, 0};
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? ld_c<R>(src, (long)(N2 * n1 + n2) * lh + m) : cpx<R>{0, 0};
     f_dft<R, -1, N1>(v);
     f_twiddle<R, -1, N1>(v, n2, tw);
     cpx<R>* Tc = T + c * TS;
@@ -281,13 +322,18 @@ __global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<R> io, int nplan
         f_dft<R, -1, N2>(u);
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2)
-            base[(long)(k1 + N1 * k2) * lh + m] = {u[k2].re * scale, u[k2].im * scale};
+            st_c<R>(dst, (long)(k1 + N1 * k2) * lh + m, cpx<R>{u[k2].re * scale, u[k2].im * scale});
     }
 }
 
-template <class R, int N1, int N2, int C>
-__global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<R> in, FieldList<R> out, int nplanes, int lh,
-                                                      int ph, const cpx<R>* __restrict__ tw) {
+// Inverse: spectral rows (stride lh) in, rows of stride ph complex out.  ops.op[f]
+// selects d/dx (x i*wavx) or d/dy (x i*wavy) applied to field f as it is loaded,
+// in double before the conversion to R: (re, im) -> (-k im, k re).
+template <class R, class TI, class TO, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<TI> in, FieldList<TO> out, int nplanes, int lh,
+                                                      int ph, const cpx<R>* __restrict__ tw, FftOps ops,
+                                                      const double* __restrict__ wavx,
+                                                      const double* __restrict__ wavy) {
     constexpr int TS = N1 * (N2 + 1) + 1;
     __shared__ cpx<R> T[C * TS];
     const int tid = threadIdx.x;
@@ -297,13 +343,27 @@ __global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<R> in, FieldList
     const long plane = blockIdx.x / ntile;
     const int f = (int)(plane / nplanes);
     const long pk = plane - (long)f * nplanes;
-    const cpx<R>* src = reinterpret_cast<const cpx<R>*>(in.p[f] + pk * (long)(N1 * N2) * (2 * lh));
-    cpx<R>* dst = reinterpret_cast<cpx<R>*>(out.p[f] + pk * (long)(N1 * N2) * (2 * ph));
+    const TI* src = in.p[f] + pk * (long)(N1 * N2) * (2 * lh);
+    TO* dst = out.p[f] + pk * (long)(N1 * N2) * (2 * ph);
     const int m = tile * C + c;
     const bool valid = m < lh;
+    const int op = ops.op[f];
+    const double kx = (op == FOP_DX && valid) ? wavx[2 * m] : 0.0;
     cpx<R> v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? src[(long)(N2 * n1 + n2) * lh + m] : cpx<R>{0, 0};
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int n = N2 * n1 + n2;
+        if (!valid) {
+            v[n1] = {0, 0};
+        } else if (op == FOP_NONE) {
+            v[n1] = ld_c<R>(src, (long)n * lh + m);
+        } else {
+            const TI* q = src + 2 * ((long)n * lh + m);
+            const double k = (op == FOP_DX) ? kx : wavy[n];
+            const double re = (double)q[0], im = (double)q[1];
+            v[n1] = {(R)(-k * im), (R)(k * re)};
+        }
+    }
     f_dft<R, 1, N1>(v);
     f_twiddle<R, 1, N1>(v, n2, tw);
     cpx<R>* Tc = T + c * TS;
@@ -319,7 +379,7 @@ __global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<R> in, FieldList
         for (int j = 0; j < N2; ++j) u[j] = Tc[k1 * (N2 + 1) + j];
         f_dft<R, 1, N2>(u);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) dst[(long)(k1 + N1 * k2) * ph + m] = u[k2];
+        for (int k2 = 0; k2 < N2; ++k2) st_c<R>(dst, (long)(k1 + N1 * k2) * ph + m, u[k2]);
     }
 }
 
@@ -333,15 +393,33 @@ bool fast_fft_len(int L) { return L == 16 || L == 32 || L == 64 || L == 128 || L
     X(128, 16, 8)         \
     X(256, 16, 16)
 
-template <class R>
-int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int nxpp,
+template <class R, class TI, class TO>
+int fast_xfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride,
+              int ostride, const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
+    const long rows = (long)nrows * nf;
+    switch (lh) {
+#define X(LL, A, B)                                                                                                 \
+    case LL:                                                                                                        \
+        hipLaunchKernelGGL((k_xfwd_fast<R, TI, TO, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))),       \
+                           dim3(128), 0, s, in, out, nrows, nf, istride, ostride, tw, twr);                         \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class R, class TI, class TO>
+int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride, int nxpp,
               const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
     const long rows = (long)nrows * nf;
     switch (lh) {
-#define X(LL, A, B)                                                                                              \
-    case LL:                                                                                                     \
-        hipLaunchKernelGGL((k_xfwd_fast<R, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), \
-                           0, s, in, out, nrows, nf, nxpp, tw, twr);                                             \
+#define X(LL, A, B)                                                                                                 \
+    case LL:                                                                                                        \
+        hipLaunchKernelGGL((k_xinv_fast<R, TI, TO, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))),       \
+                           dim3(128), 0, s, in, out, nrows, nf, istride, nxpp, tw, twr);                            \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -351,33 +429,16 @@ int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf
     return LSB_OK;
 }
 
-template <class R>
-int fast_xinv(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int istride, int nxpp,
-              const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
-    const long rows = (long)nrows * nf;
-    switch (lh) {
-#define X(LL, A, B)                                                                                              \
-    case LL:                                                                                                     \
-        hipLaunchKernelGGL((k_xinv_fast<R, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), \
-                           0, s, in, out, nrows, nf, istride, nxpp, tw, twr);                                 \
-        break;
-        LSB_FAST_CASES(X)
-#undef X
-        default: return LSB_E_ARG;
-    }
-    LSB_HIP(hipGetLastError());
-    return LSB_OK;
-}
-
-template <class R>
-int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const cpx<R>* tw, R scale, hipStream_t s) {
+template <class R, class TI, class TO>
+int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
+              const cpx<R>* tw, R scale, hipStream_t s) {
     constexpr int C = 8;
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
     switch (ny) {
-#define X(LL, A, B)                                                                                      \
-    case LL:                                                                                             \
-        hipLaunchKernelGGL((k_yfwd_fast<R, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, io, nplanes, lh, tw, \
-                           scale);                                                                       \
+#define X(LL, A, B)                                                                                          \
+    case LL:                                                                                                 \
+        hipLaunchKernelGGL((k_yfwd_fast<R, TI, TO, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, \
+                           nplanes, lh, tw, scale);                                                          \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -387,16 +448,16 @@ int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const
     return LSB_OK;
 }
 
-template <class R>
-int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int nf, int lh, int ny, int ph,
-              const cpx<R>* tw, hipStream_t s) {
+template <class R, class TI, class TO>
+int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
+              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s) {
     constexpr int C = 8;
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
     switch (ny) {
-#define X(LL, A, B)                                                                                             \
-    case LL:                                                                                                    \
-        hipLaunchKernelGGL((k_yinv_fast<R, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, nplanes, lh, \
-                           ph, tw);                                                                             \
+#define X(LL, A, B)                                                                                          \
+    case LL:                                                                                                 \
+        hipLaunchKernelGGL((k_yinv_fast<R, TI, TO, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, \
+                           nplanes, lh, ph, tw, ops, wavx, wavy);                                            \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -406,19 +467,19 @@ int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int 
     return LSB_OK;
 }
 
-template int fast_xfwd<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int,
-                               const cpx<double>*, const cpx<double>*, hipStream_t);
-template int fast_xinv<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int, int,
-                               const cpx<double>*, const cpx<double>*, hipStream_t);
-template int fast_yfwd<double>(const FieldList<double>&, int, int, int, int, const cpx<double>*, double, hipStream_t);
-template int fast_yinv<double>(const FieldList<double>&, const FieldList<double>&, int, int, int, int, int,
-                               const cpx<double>*, hipStream_t);
-template int fast_xfwd<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, const cpx<float>*,
-                              const cpx<float>*, hipStream_t);
-template int fast_xinv<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, int,
-                              const cpx<float>*, const cpx<float>*, hipStream_t);
-template int fast_yfwd<float>(const FieldList<float>&, int, int, int, int, const cpx<float>*, float, hipStream_t);
-template int fast_yinv<float>(const FieldList<float>&, const FieldList<float>&, int, int, int, int, int,
-                              const cpx<float>*, hipStream_t);
+#define LSB_INST_FAST(R, TI, TO)                                                                                  \
+    template int fast_xfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
+                                      const cpx<R>*, const cpx<R>*, hipStream_t);                                 \
+    template int fast_xinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
+                                      const cpx<R>*, const cpx<R>*, hipStream_t);                                 \
+    template int fast_yfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int,             \
+                                      const cpx<R>*, R, hipStream_t);                                             \
+    template int fast_yinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
+                                      const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t);
+
+LSB_INST_FAST(double, double, double)
+LSB_INST_FAST(float, float, float)    // mixed: x passes (physical <-> compact staging)
+LSB_INST_FAST(float, double, float)   // mixed: inverse y pass (FP64 spectrum -> FP32 staging)
+LSB_INST_FAST(float, float, double)   // mixed: forward y pass (FP32 staging -> FP64 spectrum)
 
 }  // namespace lsb

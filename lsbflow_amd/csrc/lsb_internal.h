@@ -138,6 +138,8 @@ struct lsb_ctx {
     double* d_smag = nullptr;   // nzp (static Smagorinsky -(cs delta fvd)^2) + scratch
     double* d_red = nullptr;    // reduction scratch (kRedDoubles doubles)
     double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
+    int* d_walls = nullptr;     // wall planes of the static model's utau average
+    int nwalls = 0;
     float* d_mdz32 = nullptr;
     float* d_smat32 = nullptr;
     float* d_simat32 = nullptr;
@@ -205,25 +207,45 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
                int nplanes, hipStream_t s);
 int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
                int nplanes, hipStream_t s);
+// inverse with per-field d/dx, d/dy applied on load (FOP_*); fast lengths only
+bool horfft_has_ops(const HorPlan& hp);
+int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
+                   int nplanes, const int* ops, hipStream_t s);
+// mixed form: FP32 physical fields, FP64 spectra
+int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, double* const* spec, int nf,
+                     int nplanes, hipStream_t s);
+template <class TI>
+int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,
+                     const int* ops, hipStream_t s);
 
-// register-resident power-of-two passes (fft_fast.hip)
+// register-resident power-of-two passes (fft_fast.hip); R arithmetic, TI/TO
+// element types of input and output
+enum { FOP_NONE = 0, FOP_DX = 1, FOP_DY = 2 };
+struct FftOps {
+    int op[kMaxFields];
+};
 bool fast_fft_len(int L);
-template <class R>
-int fast_xfwd(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int nxpp, const cpx<R>* tw,
-              const cpx<R>* twr, hipStream_t s);
-template <class R>
-int fast_xinv(const FieldList<R>& in, const FieldList<R>& out, int nrows, int nf, int lh, int istride, int nxpp,
+template <class R, class TI, class TO>
+int fast_xfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride,
+              int ostride, const cpx<R>* tw, const cpx<R>* twr, hipStream_t s);
+template <class R, class TI, class TO>
+int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride, int nxpp,
               const cpx<R>* tw, const cpx<R>* twr, hipStream_t s);
-template <class R>
-int fast_yfwd(const FieldList<R>& io, int nplanes, int nf, int lh, int ny, const cpx<R>* tw, R scale, hipStream_t s);
-template <class R>
-int fast_yinv(const FieldList<R>& in, const FieldList<R>& out, int nplanes, int nf, int lh, int ny, int ph,
-              const cpx<R>* tw, hipStream_t s);
+template <class R, class TI, class TO>
+int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
+              const cpx<R>* tw, R scale, hipStream_t s);
+template <class R, class TI, class TO>
+int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
+              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s);
 
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
               double* const* out, int nf, hipStream_t s);
+// mixed form: FP32 matrix-core contraction, panel TI, output TO (FP32 products and sums)
+template <class TI, class TO>
+int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,
+                  int nf, hipStream_t s);
 // Helmholtz solves in place: rhs_f (nzp planes); bc_f may be null (homogeneous).
 // sigma = dtnui (constant over fields).
 int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top,

@@ -99,33 +99,35 @@ __device__ __forceinline__ double sigma_nu(const double g[3][3], double delta) {
 // ---------------------------------------------------------- nonlinear term
 // navierstokes.f90:725-769: base flow added, nlt_l = -1/2 sum_m u_m du_l/dx_m,
 // uu_lm = u_l u_m, tau from the SGS model, h_lm = 1/2 uu_lm + tau_lm.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgs a) {
+// Arithmetic in FP64 whatever the storage type T.
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
     PH_LOOP(d) {
         int i, j, k;
         long off;
         ph_index(d, e, i, j, k, off);
         double u[3], g[3][3];
-        u[0] = a.ui[0][off] + a.ubar[k];
-        u[1] = a.ui[1][off];
-        u[2] = a.ui[2][off];
+        const double up0 = (double)a.ui[0][off];
+        u[0] = up0 + a.ubar[k];
+        u[1] = (double)a.ui[1][off];
+        u[2] = (double)a.ui[2][off];
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) g[l][m] = a.g[l + 3 * m][off];
+            for (int m = 0; m < 3; ++m) g[l][m] = (double)a.g[l + 3 * m][off];
         g[0][2] = g[0][2] + a.ubarp[k];
         if (a.write_base) {
-            a.uo[0][off] = u[0];
-            a.uo[1][off] = u[1];
-            a.uo[2][off] = u[2];
-            a.go[0 + 3 * 2][off] = g[0][2];
+            a.uo[0][off] = (T)u[0];
+            a.uo[1][off] = (T)u[1];
+            a.uo[2][off] = (T)u[2];
+            a.go[0 + 3 * 2][off] = (T)g[0][2];
         }
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             double s = 0.0;
 #pragma unroll
             for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * g[l][m];
-            a.nlt[l][off] = s;
+            a.nlt[l][off] = (T)s;
         }
         double tau[3][3];
         if (MODE == NL_SIGMA) {
@@ -153,7 +155,7 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgs a) {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
 #pragma unroll
-                for (int m = 0; m < 3; ++m) tau[l][m] = a.tau[l + 3 * m] ? a.tau[l + 3 * m][off] : 0.0;
+                for (int m = 0; m < 3; ++m) tau[l][m] = a.tau[l + 3 * m] ? (double)a.tau[l + 3 * m][off] : 0.0;
         } else {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
@@ -166,13 +168,18 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgs a) {
             for (int l = 0; l < 3; ++l)
 #pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = u[l] * u[m];
+                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = (T)(u[l] * u[m]);
         } else {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
 #pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = 0.5 * (u[l] * u[m]) + tau[l][m];
+                    if (a.h[l + 3 * m]) {
+                        double hv = 0.5 * (u[l] * u[m]) + tau[l][m];
+                        // u0^2 - ubar^2 = up0 (u0 + ubar): the perturbation part without cancellation
+                        if (l == 0 && m == 0 && a.sub_base) hv = 0.5 * (up0 * (u[0] + a.ubar[k])) + tau[0][0];
+                        a.h[l + 3 * m][off] = (T)hv;
+                    }
         }
     }
 }
@@ -189,7 +196,8 @@ __global__ void k_hcombine(PhDims d, FieldList<double> uu, FieldList<double> tau
 
 // sgsterm.f90:59-81 utau: sum over the wall planes of xnu*du/dz, then the
 // per-plane factor -(CS delta (1 - exp(-z+/25)))^2 (sgsterm.f90:84-108).
-__global__ void k_utau_reduce(const double* __restrict__ g02, const double* __restrict__ ubarp, int nx, int ny,
+template <class T>
+__global__ void k_utau_reduce(const T* __restrict__ g02, const double* __restrict__ ubarp, int nx, int ny,
                               int nxpp, const int* __restrict__ walls, int nwall, double xnu,
                               double* __restrict__ out) {
     __shared__ double sh[256];
@@ -199,7 +207,7 @@ __global__ void k_utau_reduce(const double* __restrict__ g02, const double* __re
         const int k = walls[w];
         for (long t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = (int)(t % nx), j = (int)(t / nx);
-            const double v = g02[((long)k * ny + j) * nxpp + i] + (ubarp ? ubarp[k] : 0.0);
+            const double v = (double)g02[((long)k * ny + j) * nxpp + i] + (ubarp ? ubarp[k] : 0.0);
             s += xnu * v;
         }
     }
@@ -404,21 +412,30 @@ int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s) {
     return LSB_OK;
 }
 
-int ph_nonlinear(lsb_ctx* c, int mode, const NlArgs& a, hipStream_t s) {
+template <class T>
+static int launch_nonlinear(lsb_ctx* c, int mode, const NlArgsT<T>& a, hipStream_t s) {
     PhDims d = ph_dims(c);
     int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
+    // 12 fields in, 3 + up to 9 out
+    int nh = 0;
+    for (int q = 0; q < 9; ++q) nh += a.h[q] ? 1 : 0;
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, (double)(15 + nh) * d.npts * sizeof(T));
+    const dim3 g = ph_grid(d.npts);
     switch (mode) {
-        case NL_NONE: hipLaunchKernelGGL(k_nonlinear<NL_NONE>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
-        case NL_SIGMA: hipLaunchKernelGGL(k_nonlinear<NL_SIGMA>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
-        case NL_STATIC: hipLaunchKernelGGL(k_nonlinear<NL_STATIC>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
-        case NL_EXTTAU: hipLaunchKernelGGL(k_nonlinear<NL_EXTTAU>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
-        case NL_PREP: hipLaunchKernelGGL(k_nonlinear<NL_PREP>, ph_grid(d.npts), dim3(256), 0, s, d, a); break;
+        case NL_NONE: hipLaunchKernelGGL((k_nonlinear<T, NL_NONE>), g, dim3(256), 0, s, d, a); break;
+        case NL_SIGMA: hipLaunchKernelGGL((k_nonlinear<T, NL_SIGMA>), g, dim3(256), 0, s, d, a); break;
+        case NL_STATIC: hipLaunchKernelGGL((k_nonlinear<T, NL_STATIC>), g, dim3(256), 0, s, d, a); break;
+        case NL_EXTTAU: hipLaunchKernelGGL((k_nonlinear<T, NL_EXTTAU>), g, dim3(256), 0, s, d, a); break;
+        case NL_PREP: hipLaunchKernelGGL((k_nonlinear<T, NL_PREP>), g, dim3(256), 0, s, d, a); break;
         default: set_error("nonlinear: bad mode"); return LSB_E_ARG;
     }
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
+}
+int ph_nonlinear(lsb_ctx* c, int mode, const NlArgs& a, hipStream_t s) { return launch_nonlinear<double>(c, mode, a, s); }
+int ph_nonlinear_f32(lsb_ctx* c, int mode, const NlArgsT<float>& a, hipStream_t s) {
+    return launch_nonlinear<float>(c, mode, a, s);
 }
 
 int ph_hcombine(lsb_ctx* c, const double* const* uu, const double* const* tau, double* const* h, int n, hipStream_t s) {
@@ -434,25 +451,27 @@ int ph_hcombine(lsb_ctx* c, const double* const* uu, const double* const* tau, d
     return LSB_OK;
 }
 
-int ph_smag_factor(lsb_ctx* c, const double* g02, const double* ubarp, double cs, hipStream_t s) {
-    std::vector<int> walls;
-    for (int k = 0; k < c->nzp; ++k)
-        if (c->zpts[k] == 0.0 || std::fabs(c->zpts[k]) == 0.5) walls.push_back(k);
-    if (walls.empty()) {
+// The wall planes (zpts == 0 or |zpts| == 0.5, sgsterm.f90:59-81) are uploaded once
+// with the context (d_walls), so this only enqueues work.
+template <class T>
+static int smag_factor(lsb_ctx* c, const T* g02, const double* ubarp, double cs, hipStream_t s) {
+    if (c->nwalls <= 0) {
         set_error("static_smagorinsky: no wall plane (zpts == 0 or |zpts| == 0.5)");
         return LSB_E_ARG;
     }
-    int* dw = reinterpret_cast<int*>(c->d_red + 16);
-    LSB_HIP(hipMemcpyAsync(dw, walls.data(), walls.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_utau_reduce, dim3(1), dim3(256), 0, s, g02, ubarp, c->nx, c->ny, c->nxpp, dw,
-                       (int)walls.size(), c->cfg.xnu, c->d_red);
+    hipLaunchKernelGGL(k_utau_reduce<T>, dim3(1), dim3(256), 0, s, g02, ubarp, c->nx, c->ny, c->nxpp, c->d_walls,
+                       c->nwalls, c->cfg.xnu, c->d_red + kRedDoubles - 1);
     LSB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_smag_fac, dim3((c->nzp + 63) / 64), dim3(64), 0, s, c->d_red, (int)walls.size(),
+    hipLaunchKernelGGL(k_smag_fac, dim3((c->nzp + 63) / 64), dim3(64), 0, s, c->d_red + kRedDoubles - 1, c->nwalls,
                        (long)c->nx * c->ny, c->d_zpts, c->d_delta, c->nzp, c->cfg.xnu, cs, c->d_smag);
     LSB_HIP(hipGetLastError());
-    // the kernel above wrote the per-plane factors; stage the wall list lifetime
-    LSB_HIP(hipStreamSynchronize(s));
     return LSB_OK;
+}
+int ph_smag_factor(lsb_ctx* c, const double* g02, const double* ubarp, double cs, hipStream_t s) {
+    return smag_factor<double>(c, g02, ubarp, cs, s);
+}
+int ph_smag_factor_f32(lsb_ctx* c, const float* g02, const double* ubarp, double cs, hipStream_t s) {
+    return smag_factor<float>(c, g02, ubarp, cs, s);
 }
 
 int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, double a, double b, int interpflag,

@@ -412,33 +412,33 @@ int isc_full(lsb_ctx* c, double* ui, double* tau, double* work, hipStream_t s) {
 // ------------------------------------------------------- nonlinear term
 // navierstokes.f90:658-795 for T = 0: uif (3 spectral, already filtered) ->
 // N_l written to nlt (3 spectral).  Work comes from the step layout.
-static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
-                       double* hz[3], hipStream_t s) {
+// FP64 physical chain: inverse transforms, products + SGS model, forward transforms.
+static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
+                       bool ops_on_load, hipStream_t s) {
     StepLayout& L = S->L;
-    // derivatives
-    for (int l = 0; l < 3; ++l) {
-        LSB_TRY(sp_dfdx(c, uif[l], L.spec[S_DX0 + l], s));
-        LSB_TRY(sp_dfdy(c, uif[l], L.spec[S_DY0 + l], s));
-    }
-    {
-        double* dz[3] = {L.spec[S_DZ0], L.spec[S_DZ0 + 1], L.spec[S_DZ0 + 2]};
-        LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s));
-    }
     // inverse FFTs: ui (3) + g (9)
     {
         const double* in[12];
         double* out[12];
+        int ops[12];
         for (int l = 0; l < 3; ++l) {
             in[l] = uif[l];
             out[l] = L.phys[P_UI0 + l];
+            ops[l] = FOP_NONE;
         }
         for (int m = 0; m < 3; ++m)
             for (int l = 0; l < 3; ++l) {
                 const int slot = (m == 0 ? S_DX0 : (m == 1 ? S_DY0 : S_DZ0)) + l;
-                in[3 + l + 3 * m] = L.spec[slot];
-                out[3 + l + 3 * m] = L.phys[P_G0 + l + 3 * m];
+                const int q = 3 + l + 3 * m;
+                in[q] = L.spec[slot];
+                out[q] = L.phys[P_G0 + l + 3 * m];
+                ops[q] = FOP_NONE;
+                if (ops_on_load && m < 2) {
+                    in[q] = uif[l];
+                    ops[q] = (m == 0) ? FOP_DX : FOP_DY;
+                }
             }
-        LSB_TRY(horfft_inv(c, c->hp, in, out, 12, c->nzp, s));
+        LSB_TRY(horfft_inv_ops(c, c->hp, in, out, 12, c->nzp, ops, s));
     }
     const bool sgs = c->cfg.sgsflag == 1;
     const int model = c->cfg.sgsmodel;
@@ -533,10 +533,140 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         }
         LSB_TRY(horfft_fwd(c, c->hp, in, out, n, c->nzp, s));
     }
+    return LSB_OK;
+}
+
+// Mixed form: restore the plane-constant base part ubar^2/2 of h_00, removed before
+// the FP32 rounding, on the (kx, ky) = (0, 0) mode of every plane.
+__global__ void k_add_base00(double* __restrict__ hf00, const double* __restrict__ ubar, int nzp, long plane_s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nzp) return;
+    hf00[(long)k * plane_s] += 0.5 * (ubar[k] * ubar[k]);
+}
+
+static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED; }
+
+// FP32 physical chain of the mixed form (sigma, static or no SGS model): 12 inverse
+// transforms with d/dx, d/dy applied on load, the products and the SGS model in one
+// pass (FP64 arithmetic on FP32 storage), 3 + 6 forward transforms back to FP64 spectra.
+static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3],
+                             double* hf[9], hipStream_t s) {
+    StepLayout& L = S->L;
+    const bool sgs = c->cfg.sgsflag == 1;
+    const int model = c->cfg.sgsmodel;
+    if (sgs && model != 0 && model != 2) {
+        set_error("mixed precision: the dynamic and interscale models run in the FP64 form only");
+        return LSB_E_UNSUPPORTED;
+    }
+    auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
+    {
+        // u, v, w and their x/y derivatives (applied on load) from the FP64 spectra
+        const double* in[9];
+        float* out[9];
+        int ops[9];
+        for (int l = 0; l < 3; ++l) {
+            in[l] = uif[l];
+            out[l] = ph(P_UI0 + l);
+            ops[l] = FOP_NONE;
+            in[3 + l] = uif[l];
+            out[3 + l] = ph(P_G0 + l);          // (l, x)
+            ops[3 + l] = FOP_DX;
+            in[6 + l] = uif[l];
+            out[6 + l] = ph(P_G0 + l + 3);      // (l, y)
+            ops[6 + l] = FOP_DY;
+        }
+        LSB_TRY(horfft_inv_mixed<double>(c, c->hp, in, out, 9, c->nzp, ops, s));
+        // the z derivatives from their FP32 spectra
+        const float* inz[3];
+        float* outz[3];
+        for (int l = 0; l < 3; ++l) {
+            inz[l] = reinterpret_cast<const float*>(L.spec[S_DZ0 + l]);
+            outz[l] = ph(P_G0 + l + 6);         // (l, z)
+        }
+        LSB_TRY(horfft_inv_mixed<float>(c, c->hp, inz, outz, 3, c->nzp, nullptr, s));
+    }
+    NlArgsT<float> a{};
+    for (int l = 0; l < 3; ++l) {
+        a.ui[l] = ph(P_UI0 + l);
+        a.nlt[l] = ph(P_NLT0 + l);
+        a.uo[l] = ph(P_UI0 + l);
+    }
+    for (int q = 0; q < 9; ++q) {
+        a.g[q] = ph(P_G0 + q);
+        a.go[q] = ph(P_G0 + q);
+    }
+    a.ubar = c->d_paral;
+    a.ubarp = c->d_paral + c->nzp;
+    a.delta = c->d_delta + 2 * c->nzp;
+    a.smag = c->d_smag;
+    a.sub_base = 1;
+    const int map[9] = {0, 1, 2, -1, 3, 4, -1, -1, 5};
+    for (int q = 0; q < 9; ++q) a.h[q] = map[q] >= 0 ? ph(P_H0 + map[q]) : nullptr;
+    if (!sgs) {
+        LSB_TRY(ph_nonlinear_f32(c, NL_NONE, a, s));
+    } else if (model == 2) {
+        LSB_TRY(ph_nonlinear_f32(c, NL_SIGMA, a, s));
+    } else {
+        LSB_TRY(ph_smag_factor_f32(c, ph(P_G0 + 6), c->d_paral + c->nzp, c->cfg.cs, s));
+        LSB_TRY(ph_nonlinear_f32(c, NL_STATIC, a, s));
+    }
+    {
+        const float* in[9];
+        double* out[9];
+        int n = 0;
+        for (int l = 0; l < 3; ++l) {
+            in[n] = ph(P_NLT0 + l);
+            out[n++] = nltf_out[l];
+        }
+        const int uq[6] = {0, 1, 2, 4, 5, 8};
+        for (int q = 0; q < 6; ++q) {
+            in[n] = ph(P_H0 + q);
+            out[n++] = hf[uq[q]];
+        }
+        LSB_TRY(horfft_fwd_mixed(c, c->hp, in, out, n, c->nzp, s));
+    }
+    hipLaunchKernelGGL(k_add_base00, dim3((c->nzp + 63) / 64), dim3(64), 0, s, hf[0], c->d_paral, c->nzp,
+                       (long)c->plane_s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+// ------------------------------------------------------- nonlinear term
+// navierstokes.f90:658-795 for T = 0: uif (3 spectral, already filtered) ->
+// N_l written to nlt (3 spectral).  Work comes from the step layout.
+static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
+                       double* hz[3], hipStream_t s) {
+    StepLayout& L = S->L;
+    // d/dx and d/dy are applied as the inverse transform loads uif where the
+    // register-resident FFT serves the sizes; otherwise they are separate passes
+    const bool ops_on_load = horfft_has_ops(c->hp);
+    if (!ops_on_load) {
+        for (int l = 0; l < 3; ++l) {
+            LSB_TRY(sp_dfdx(c, uif[l], L.spec[S_DX0 + l], s));
+            LSB_TRY(sp_dfdy(c, uif[l], L.spec[S_DY0 + l], s));
+        }
+    }
+    if (mixed_form(c)) {
+        float* dz[3];
+        for (int l = 0; l < 3; ++l) dz[l] = reinterpret_cast<float*>(L.spec[S_DZ0 + l]);
+        LSB_TRY((zcontract_f32<double, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s)));
+    } else {
+        double* dz[3] = {L.spec[S_DZ0], L.spec[S_DZ0 + 1], L.spec[S_DZ0 + 2]};
+        LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s));
+    }
+    if (mixed_form(c)) {
+        LSB_TRY(nlterm_phys_mixed(c, S, uif, nltf_out, hf, s));
+    } else {
+        LSB_TRY(nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, s));
+    }
     // d/dz of h(l, z)
     {
         const double* in[3] = {hf[0 + 3 * 2], hf[1 + 3 * 2], hf[2 + 3 * 2]};
-        LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s));
+        if (mixed_form(c)) {
+            LSB_TRY((zcontract_f32<double, double>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s)));
+        } else {
+            LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s));
+        }
     }
     return LSB_OK;
 }

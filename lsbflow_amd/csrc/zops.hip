@@ -202,6 +202,136 @@ static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldLis
     return LSB_OK;
 }
 
+// ------------------------------------------------------------------ FP32
+// Mixed form: the same contraction on v_mfma_f32_16x16x4f32 (FP32 products, FP32
+// accumulation; twice the FP64 rate).  Matrix and panel fragments are loaded in
+// their stored precision and rounded to FP32 in registers; the output is stored
+// as TO.  Output lane layout of the FP32 instruction: lane l, element v holds
+// D[Z32_ROW(l/16, v)][l%16].
+#define Z32_ROW(lk, v) (4 * (lk) + (v))
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kZRB32 = 4;
+
+template <class TI, class TO>
+__global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, FieldList<TI> in, FieldList<TO> out) {
+    constexpr int RB = kZRB32, CB = kZCB, PD = kZPD;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int f = blockIdx.z;
+    const int nrb_all = (a.nout + 15) >> 4;
+    const int g0 = blockIdx.y * a.gblocks;
+    const int nrb = min(a.gblocks, nrb_all - g0);
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    if (myn <= 0) return;
+    const int rb0 = g0 + wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * (16 * CB);
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    const TI* __restrict__ X = in.p[f] + (long)a.in_row0 * P;
+    const double* __restrict__ Ma = a.M + (long)(rb0 * 16 + li) + (long)lk * a.ldm;
+    const int nin = a.nin;
+    const int nks = (nin + 3) >> 2;
+    v4f acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+    float fa[PD][RB], fb[PD][CB];
+#pragma unroll
+    for (int s = 0; s < PD; ++s) {
+        const int ks = min(s, nks - 1);
+        const long kr = min(4 * ks + lk, nin - 1);
+#pragma unroll
+        for (int i = 0; i < RB; ++i) fa[s][i] = (float)Ma[(long)(4 * ks) * a.ldm + 16 * i];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) fb[s][j] = (float)X[kr * P + cj[j]];
+    }
+    for (int k0 = 0; k0 < nks; k0 += PD) {
+#pragma unroll
+        for (int s = 0; s < PD; ++s) {
+            const int ks = k0 + s;
+            if (ks < nks) {
+                float ca[RB], cb[CB];
+#pragma unroll
+                for (int i = 0; i < RB; ++i) ca[i] = fa[s][i];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) cb[j] = fb[s][j];
+                const int kn = min(ks + PD, nks - 1);
+                const long kr = min(4 * kn + lk, nin - 1);
+#pragma unroll
+                for (int i = 0; i < RB; ++i) fa[s][i] = (float)Ma[(long)(4 * kn) * a.ldm + 16 * i];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) fb[s][j] = (float)X[kr * P + cj[j]];
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    if (i < myn)
+#pragma unroll
+                        for (int j = 0; j < CB; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[i], cb[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    TO* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = (rb0 + i) * 16 + Z32_ROW(lk, v);
+                if (row < a.nout) {
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        const long col = p0 + 16 * j + li;
+                        if (col < P) dst[(long)row * P + col] = (TO)acc[i][j][v];
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class TI, class TO>
+int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,
+                  int nf, hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields) {
+        set_error("zcontract: 1..16 fields per launch");
+        return LSB_E_ARG;
+    }
+    ZArgs a{};
+    a.M = M;
+    a.ldm = ldm;
+    a.nout = nout;
+    a.nin = nin;
+    a.P = (long)c->plane_s;
+    FieldList<TI> fi{};
+    FieldList<TO> fo{};
+    for (int f = 0; f < nf; ++f) {
+        fi.p[f] = const_cast<TI*>(in[f]);
+        fo.p[f] = out[f];
+    }
+    const int nrb = (nout + 15) / 16;
+    a.gblocks = std::min(nrb, kZRB32 * kZMaxWaves);
+    const int gy = (nrb + a.gblocks - 1) / a.gblocks;
+    const int nw = (a.gblocks + kZRB32 - 1) / kZRB32;
+    dim3 grid((unsigned)((a.P + 16 * kZCB - 1) / (16 * kZCB)), (unsigned)gy, (unsigned)nf);
+    int tok = -1;
+    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
+                    (double)a.P * nf * (nin * sizeof(TI) + nout * sizeof(TO)));
+    hipLaunchKernelGGL((k_zcontract_f32<TI, TO>), grid, dim3(64 * nw), 0, s, a, fi, fo);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+template int zcontract_f32<double, float>(lsb_ctx*, const double*, int, int, int, const double* const*,
+                                          float* const*, int, hipStream_t);
+template int zcontract_f32<double, double>(lsb_ctx*, const double*, int, int, int, const double* const*,
+                                           double* const*, int, hipStream_t);
+
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
               double* const* out, int nf, hipStream_t s) {
     if (nf < 1 || nf > kMaxFields) {
