@@ -242,6 +242,8 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
               double* const* out, int nf, hipStream_t s);
+// pressure right-hand side -d(div_h)/dz - k^2 w in one contraction
+int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs, hipStream_t s);
 // mixed form: FP32 matrix-core contraction, panel TI, output TO (FP32 products and sums)
 template <class TI, class TO>
 int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,

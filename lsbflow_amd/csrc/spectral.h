@@ -37,8 +37,6 @@ struct StepBcArgs {
 
 int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s);
 int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s);
-int sp_press_rhs(lsb_ctx* c, const double* dudz, const double* dvdz, const double* u, const double* v,
-                 const double* w, double* divf, double* rhs, hipStream_t s);
 int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, double* u, double* v, double dt,
                double bdfv, hipStream_t s);
 int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,

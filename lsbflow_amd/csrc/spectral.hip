@@ -307,27 +307,6 @@ __global__ void k_bc_vel(SpDims d, const double* __restrict__ wavx, const double
     }
 }
 
-// Pressure-step right-hand side (navierstokes.f90:1411-1422):
-//   divdz = divuv(dudz, dvdz); divf = divuv(u, v); rhs3 = -divdz - wav2 w
-__global__ void k_press_rhs(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
-                            const c2* __restrict__ dudz, const c2* __restrict__ dvdz, const c2* __restrict__ u,
-                            const c2* __restrict__ v, const c2* __restrict__ w, c2* __restrict__ divf,
-                            c2* __restrict__ rhs) {
-    SP_LOOP(d.npairs) {
-        const int m = (int)(pr % d.lh);
-        const int j = (int)((pr / d.lh) % d.ny);
-        const double wx = wavx[2 * m], wy = wavy[j];
-        const c2 a = dudz[pr], b = dvdz[pr];
-        const double ddr = -wx * a.y - wy * b.y;
-        const double ddi = wx * a.x + wy * b.x;
-        const c2 uu = u[pr], vv = v[pr];
-        divf[pr] = make_double2(-wx * uu.y - wy * vv.y, wx * uu.x + wy * vv.x);
-        const double wav2 = wx * wx + wy * wy;
-        const c2 ww = w[pr];
-        rhs[pr] = make_double2(-ddr - wav2 * ww.x, -ddi - wav2 * ww.y);
-    }
-}
-
 // Projection (navierstokes.f90:1466-1488):
 //   p = -(dwdz + divf)/(wav2 dt bdfv) (0 where wav2 = 0); u -= dt bdfv dp/dx; v -= dt bdfv dp/dy
 __global__ void k_project(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy,
@@ -365,17 +344,6 @@ int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
 int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s) {
     SpDims d = sp_dims(c, 1);
     hipLaunchKernelGGL(k_bc_vel, sp_grid(d.plane_pairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
-    LSB_HIP(hipGetLastError());
-    return LSB_OK;
-}
-int sp_press_rhs(lsb_ctx* c, const double* dudz, const double* dvdz, const double* u, const double* v,
-                 const double* w, double* divf, double* rhs, hipStream_t s) {
-    SpDims d = sp_dims(c, c->nzp);
-    int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
-    hipLaunchKernelGGL(k_press_rhs, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)dudz,
-                       (const c2*)dvdz, (const c2*)u, (const c2*)v, (const c2*)w, (c2*)divf, (c2*)rhs);
-    prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }

@@ -886,12 +886,9 @@ static int update_once(lsb_ctx* c) {
     double* vN = L.spec[S->rhs[1]];
     double* wS = L.spec[S->rhs[2]];
     double* r3 = L.spec[S->rhs[3]];
-    {
-        const double* in[2] = {uN, vN};
-        double* out[2] = {L.spec[S_DUDZ], L.spec[S_DVDZ]};
-        LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 2, s));
-    }
-    LSB_TRY(sp_press_rhs(c, L.spec[S_DUDZ], L.spec[S_DVDZ], uN, vN, wS, L.spec[S_DIVF], r3, s));
+    // divf = ikx u + iky v; rhs3 = -d(divf)/dz - k^2 w (navierstokes.f90:1411-1418)
+    LSB_TRY(sp_divuv(c, uN, vN, L.spec[S_DIVF], s));
+    LSB_TRY(zcontract_press(c, L.spec[S_DIVF], wS, r3, s));
     {
         double* rh[1] = {r3};
         const double* bt[1] = {sc_top};

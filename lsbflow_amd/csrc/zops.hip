@@ -27,7 +27,8 @@
 
 namespace lsb {
 
-enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2 };
+// ZM_PRESS: out = -acc - (kx^2 + ky^2) aux, aux a field at the output positions
+enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2, ZM_PRESS = 3 };
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldLis
     }
     // per-pencil epilogue data
     double dfac[CB], vbt[CB], vbb[CB];
-    const bool nhom = (MODE != ZM_PLAIN) && (bct.p[f] != nullptr);
+    const bool nhom = (MODE == ZM_HELM1 || MODE == ZM_HELM2) && (bct.p[f] != nullptr);
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
         dfac[j] = 0.0;
@@ -163,6 +164,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldLis
                             const int k = row + 1;
                             x = x + a.lift[k] * vbb[j] + a.lift[a.nzp + k] * vbt[j];
                         }
+                        if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)row * P + cj[j]];
                         if (col < P) dst[(long)row * P + col] = x;
                     }
                 }
@@ -357,6 +359,35 @@ int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const dou
     prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
                     (double)(nin + nout) * a.P * nf * sizeof(double));
     int rc = launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
+    prof_mark_end(c, tok, s);
+    return rc;
+}
+
+// Pressure right-hand side (navierstokes.f90:1411-1418) from div_h = ikx u + iky v:
+// rhs = -d(div_h)/dz - (kx^2 + ky^2) w.  d/dz commutes with the horizontal
+// divergence, so this is the reference's divuv(dfdz(u), dfdz(v)) with one
+// contraction instead of two.
+int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs, hipStream_t s) {
+    ZArgs a{};
+    a.M = c->d_mdz;
+    a.ldm = c->ldm_z;
+    a.nout = c->nzp;
+    a.nin = c->nzp;
+    a.P = (long)c->plane_s;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nzp = c->nzp;
+    a.sigma = 0.0;
+    a.wavx = c->d_wavx;
+    a.wavy = c->d_wavy;
+    FieldList<double> fi{}, fo{}, aux{}, z{};
+    fi.p[0] = const_cast<double*>(divh);
+    fo.p[0] = rhs;
+    aux.p[0] = const_cast<double*>(w);
+    z.p[0] = nullptr;
+    int tok = -1;
+    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * c->nzp * (double)c->nzp * a.P, 3.0 * c->nzp * (double)a.P * sizeof(double));
+    int rc = launch_z<ZM_PRESS>(a, fi, fo, aux, z, 1, s);
     prof_mark_end(c, tok, s);
     return rc;
 }
