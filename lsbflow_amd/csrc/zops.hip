@@ -40,6 +40,7 @@ struct ZArgs {
     long P;             // columns per plane (nx*ny)
     int nx, ny;
     int gblocks;        // row blocks per blockIdx.y group
+    int xrow;           // a last row past the 16-row blocks (nout % 16 == 1), done on the VALU; -1 if none
     double sigma;
     const double* wavx;   // nx
     const double* wavy;   // ny
@@ -54,16 +55,17 @@ constexpr int kZCB = 4;   // column blocks (of 16) per block
 constexpr int kZPD = 2;   // k-steps of fragments in flight
 constexpr int kZMaxWaves = 8;
 
+// three waves per SIMD (<= 168 VGPRs) keep enough loads in flight to feed the matrix cores
 template <int MODE>
-__global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out,
-                                                                FieldList<double> bct, FieldList<double> bcb) {
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(3)))
+k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double> bct, FieldList<double> bcb) {
     constexpr int RB = kZRB, CB = kZCB, PD = kZPD;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int f = blockIdx.z;
-    const int nrb_all = (a.nout + 15) >> 4;
+    const int nrb_all = (a.xrow >= 0) ? (a.nout >> 4) : ((a.nout + 15) >> 4);
     const int g0 = blockIdx.y * a.gblocks;
     const int nrb = min(a.gblocks, nrb_all - g0);
     const int base = nrb / nw, extra = nrb % nw;
@@ -84,6 +86,12 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldLis
     for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    // the extra row: wave 0 of the first group accumulates M[xrow][k] X[k][col] on the
+    // VALU from the B fragments it holds (lane quarter lk owns k = 4 ks + lk)
+    const bool dox = (a.xrow >= 0) && (g0 == 0) && (wave == 0);
+    double xacc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) xacc[j] = 0.0;
     double fa[PD][RB], fb[PD][CB];
 #pragma unroll
     for (int s = 0; s < PD; ++s) {
@@ -116,6 +124,14 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldLis
 #pragma unroll
                         for (int j = 0; j < CB; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i], cb[j], acc[i][j], 0, 0, 0);
+                if (dox) {
+                    // M[xrow][4 ks + lk]: four wave-uniform loads, one selected per lane quarter
+                    const double* mc = a.M + a.xrow + (long)(4 * ks) * a.ldm;
+                    const double m0 = mc[0], m1 = mc[a.ldm], m2 = mc[2 * a.ldm], m3 = mc[3 * a.ldm];
+                    const double cx = lk == 0 ? m0 : (lk == 1 ? m1 : (lk == 2 ? m2 : m3));
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) xacc[j] = fma(cx, cb[j], xacc[j]);
+                }
             }
         }
     }
@@ -171,6 +187,17 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract(ZArgs a, FieldLis
             }
         }
     }
+    if (dox && (MODE == ZM_PLAIN || MODE == ZM_PRESS)) {
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            double x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            const long col = p0 + 16 * j + li;
+            if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)a.xrow * P + cj[j]];
+            if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = x;
+        }
+    }
     if (MODE == ZM_HELM2 && g0 == 0 && wave == 0 && lk == 0) {
         // boundary planes 0 and nz of the full field (out_row0 == 1)
         double* full = out.p[f];
@@ -194,7 +221,8 @@ template <int MODE>
 static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldList<double>& out,
                     const FieldList<double>& bt, const FieldList<double>& bb, int nf, hipStream_t s) {
     ZArgs a = a0;
-    const int nrb = (a.nout + 15) / 16;
+    a.xrow = ((MODE == ZM_PLAIN || MODE == ZM_PRESS) && a.nout > 16 && a.nout % 16 == 1) ? a.nout - 1 : -1;
+    const int nrb = (a.xrow >= 0) ? a.nout / 16 : (a.nout + 15) / 16;
     a.gblocks = std::min(nrb, kZRB * kZMaxWaves);
     const int gy = (nrb + a.gblocks - 1) / a.gblocks;
     const int nw = (a.gblocks + kZRB - 1) / kZRB;
@@ -222,7 +250,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
     const int nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int f = blockIdx.z;
-    const int nrb_all = (a.nout + 15) >> 4;
+    const int nrb_all = (a.xrow >= 0) ? (a.nout >> 4) : ((a.nout + 15) >> 4);
     const int g0 = blockIdx.y * a.gblocks;
     const int nrb = min(a.gblocks, nrb_all - g0);
     const int base = nrb / nw, extra = nrb % nw;
@@ -243,7 +271,12 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
     for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
-    float fa[PD][RB], fb[PD][CB];
+    const bool dox = (a.xrow >= 0) && (g0 == 0) && (wave == 0);
+    const double* __restrict__ Mx = a.M + (dox ? a.xrow : 0) + (long)lk * a.ldm;
+    float xacc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) xacc[j] = 0.f;
+    float fa[PD][RB], fb[PD][CB], fx[PD];
 #pragma unroll
     for (int s = 0; s < PD; ++s) {
         const int ks = min(s, nks - 1);
@@ -252,6 +285,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
         for (int i = 0; i < RB; ++i) fa[s][i] = (float)Ma[(long)(4 * ks) * a.ldm + 16 * i];
 #pragma unroll
         for (int j = 0; j < CB; ++j) fb[s][j] = (float)X[kr * P + cj[j]];
+        fx[s] = dox ? (float)Mx[(long)(4 * ks) * a.ldm] : 0.f;
     }
     for (int k0 = 0; k0 < nks; k0 += PD) {
 #pragma unroll
@@ -263,18 +297,23 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
                 for (int i = 0; i < RB; ++i) ca[i] = fa[s][i];
 #pragma unroll
                 for (int j = 0; j < CB; ++j) cb[j] = fb[s][j];
+                const float cx = fx[s];
                 const int kn = min(ks + PD, nks - 1);
                 const long kr = min(4 * kn + lk, nin - 1);
 #pragma unroll
                 for (int i = 0; i < RB; ++i) fa[s][i] = (float)Ma[(long)(4 * kn) * a.ldm + 16 * i];
 #pragma unroll
                 for (int j = 0; j < CB; ++j) fb[s][j] = (float)X[kr * P + cj[j]];
+                fx[s] = dox ? (float)Mx[(long)(4 * kn) * a.ldm] : 0.f;
 #pragma unroll
                 for (int i = 0; i < RB; ++i)
                     if (i < myn)
 #pragma unroll
                         for (int j = 0; j < CB; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[i], cb[j], acc[i][j], 0, 0, 0);
+                if (dox)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) xacc[j] = fmaf(cx, cb[j], xacc[j]);
             }
         }
     }
@@ -293,6 +332,16 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
                     }
                 }
             }
+        }
+    }
+    if (dox) {
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            float x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            const long col = p0 + 16 * j + li;
+            if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = (TO)x;
         }
     }
 }
@@ -316,7 +365,8 @@ int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const
         fi.p[f] = const_cast<TI*>(in[f]);
         fo.p[f] = out[f];
     }
-    const int nrb = (nout + 15) / 16;
+    a.xrow = (nout > 16 && nout % 16 == 1) ? nout - 1 : -1;
+    const int nrb = (a.xrow >= 0) ? nout / 16 : (nout + 15) / 16;
     a.gblocks = std::min(nrb, kZRB32 * kZMaxWaves);
     const int gy = (nrb + a.gblocks - 1) / a.gblocks;
     const int nw = (a.gblocks + kZRB32 - 1) / kZRB32;
