@@ -81,7 +81,46 @@ def accuracy_leg(case, steps, ic):
     return errs, finite
 
 
-def roofline(prof, precision):
+COPY_BYTES = 1 << 30
+
+
+def pmc_probe(case):
+    """Known-size copy, then one step of each form (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    import lsbflow_amd as L
+    from lsbflow_amd.core import setup_case
+
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    ic = L.cases.initial_condition(case)
+    for prec in (0, 1):
+        ctx = setup_case(case, precision=prec)
+        ctx.state_set(*ic)
+        ctx.timestep(2)
+        ctx.synchronize()
+        ctx.close()
+    print(json.dumps({"pmc_probe": case.name, "copy_bytes": COPY_BYTES}), flush=True)
+    return 0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c4_sigma_les_512x193x256_hbm_traffic.json")
+
+
+def measured_traffic(case, cls, precision):
+    """HBM bytes per launch group of class `cls`, from the committed rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), when they cover this workload."""
+    if cls != "fft" or case.name != "c4_sigma_les" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        t = json.load(fh)
+    key = "fft_mixed" if precision == 1 else "fft_fp64"
+    return float(f"{t[key]['bytes_per_launch']:.4g}") if key in t else None
+
+
+def roofline(prof, precision, case=None):
     """Dominant class by device time -> achieved vs peak (algorithmic work / measured time)."""
     cls = max(prof, key=lambda k: prof[k]["ms"])
     p = prof[cls]
@@ -92,8 +131,11 @@ def roofline(prof, precision):
         return dict(bound="mfma", kernel=cls, achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
                     frac=round(ach / peak, 4), traffic=None, launches=p["n"], device_ms=round(p["ms"], 3))
     ach = p["bytes"] / sec / 1e9 if sec > 0 else 0.0
+    traffic = measured_traffic(case, cls, precision) if case is not None else None
     return dict(bound="hbm", kernel=cls, achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=round(ach / PEAK_HBM_GBS, 4), traffic=None, launches=p["n"], device_ms=round(p["ms"], 3))
+                frac=round(ach / PEAK_HBM_GBS, 4), traffic=traffic,
+                algorithmic_bytes_per_launch=float(f"{p['bytes'] / max(p['n'], 1):.4g}"),
+                launches=p["n"], device_ms=round(p["ms"], 3))
 
 
 def cpu_baseline(case, sample_ny, sample_steps):
@@ -131,6 +173,9 @@ def main(argv=None):
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="the workload of the HBM-traffic counter runs: a 1 GiB device copy (calibration), "
+                         "then one step of each form after one warm-up step")
     ap.add_argument("--cpu-sample-ny", type=int, default=16)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     args = ap.parse_args(argv)
@@ -141,6 +186,8 @@ def main(argv=None):
     import numpy as np
     import lsbflow_amd as L
     case = L.cases.CASES[args.case]
+    if args.pmc_probe:
+        return pmc_probe(case)
     ic = L.cases.initial_condition(case)
     pts = case.points
     secs64, prof64, finite64, st64 = gpu_run(case, args.steps, args.warmup, 0, ic)
@@ -184,7 +231,7 @@ def main(argv=None):
         "fp64": fp64,
         "speedup_vs_fp64": round(secs64 / secs, 3),
         "mixed_vs_fp64": accuracy,
-        "roofline": roofline(prof, precision),
+        "roofline": roofline(prof, precision, case),
         "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
     }
     if not args.no_cpu_baseline:
