@@ -531,9 +531,53 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
     return horfft_inv_ops(c, hp, spec, phys, nf, nplanes, nullptr, s);
 }
 
+// u, du/dx, du/dy of 3 velocity spectra in one pass over each spectrum: the inverse
+// y pass produces the three transforms from one load (fast_yinv3), the x pass
+// writes the padded physical rows.  RT is the physical storage type and the
+// arithmetic type (double: FP64 form; float: mixed form).
+template <class RT>
+int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
+                    RT* const* dudy, int nplanes, hipStream_t s) {
+    if (!horfft_has_ops(hp)) {
+        set_error("horfft: the fused gradient transform needs power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
+    }
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    void* scr = nullptr;
+    const size_t per = (size_t)nplanes * ny * nx;
+    LSB_TRY(stage_get(c, 8, per * 9 * sizeof(RT), &scr));
+    FieldList<double> in{};
+    FieldList<RT> mid{}, out{};
+    for (int l = 0; l < 3; ++l) {
+        in.p[l] = const_cast<double*>(uif[l]);
+        out.p[3 * l] = u[l];
+        out.p[3 * l + 1] = dudx[l];
+        out.p[3 * l + 2] = dudy[l];
+    }
+    for (int f = 0; f < 9; ++f) mid.p[f] = (RT*)scr + per * f;
+    int tok = -1;
+    {
+        const double planes = (double)nplanes;
+        const double bytes = planes * (3.0 * hp.nx * hp.ny * 8 + 9.0 * (hp.nx + 2) * hp.ny * sizeof(RT));
+        prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, 9, nplanes), bytes);
+    }
+    const cpx<RT>* twy = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_y : (void*)hp.tw_y32);
+    const cpx<RT>* twx = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_x : (void*)hp.tw_x32);
+    const cpx<RT>* twr = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_r : (void*)hp.tw_r32);
+    LSB_TRY((fast_yinv3<RT, double, RT>(in, mid, nplanes, 3, lh, ny, lh, twy, c->d_wavx, c->d_wavy, s)));
+    LSB_TRY((fast_xinv<RT, RT, RT>(mid, out, nplanes * ny, 9, lh, nx, nxpp, twx, twr, s)));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+template int horfft_inv_grad<double>(lsb_ctx*, const HorPlan&, const double* const*, double* const*, double* const*,
+                                     double* const*, int, hipStream_t);
+template int horfft_inv_grad<float>(lsb_ctx*, const HorPlan&, const double* const*, float* const*, float* const*,
+                                    float* const*, int, hipStream_t);
+
 // Mixed form: FP32 physical fields, FP64 spectra, FP32 arithmetic.
-int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, double* const* spec, int nf,
-                     int nplanes, hipStream_t s) {
+template <class TO>
+int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO* const* spec, int nf, int nplanes,
+                     hipStream_t s) {
     if (nf < 1 || nf > kMaxFields || !horfft_has_ops(hp)) {
         set_error("horfft (mixed): 1..16 fields, power-of-two nx/2 and ny in 16..256");
         return LSB_E_UNSUPPORTED;
@@ -543,22 +587,26 @@ int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, do
     const size_t per = (size_t)nplanes * ny * nx;
     LSB_TRY(stage_get(c, 8, per * nf * sizeof(float), &scr));
     FieldList<float> in{}, mid{};
-    FieldList<double> out{};
+    FieldList<TO> out{};
     for (int f = 0; f < nf; ++f) {
         in.p[f] = const_cast<float*>(phys[f]);
         mid.p[f] = (float*)scr + per * f;
         out.p[f] = spec[f];
     }
     int tok = -1;
-    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 4, 8));
+    prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 4, sizeof(TO)));
     const int nrows = nplanes * ny;
     LSB_TRY((fast_xfwd<float, float, float>(in, mid, nrows, nf, lh, nxpp, nx, (const cpx<float>*)hp.tw_x32,
                                             (const cpx<float>*)hp.tw_r32, s)));
-    LSB_TRY((fast_yfwd<float, float, double>(mid, out, nplanes, nf, lh, ny, (const cpx<float>*)hp.tw_y32,
+    LSB_TRY((fast_yfwd<float, float, TO>(mid, out, nplanes, nf, lh, ny, (const cpx<float>*)hp.tw_y32,
                                              1.0f / (float)(nx * ny), s)));
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }
+template int horfft_fwd_mixed<double>(lsb_ctx*, const HorPlan&, const float* const*, double* const*, int, int,
+                                      hipStream_t);
+template int horfft_fwd_mixed<float>(lsb_ctx*, const HorPlan&, const float* const*, float* const*, int, int,
+                                     hipStream_t);
 
 template <class TI>
 int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,

@@ -383,6 +383,85 @@ __global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<TI> in, FieldLis
     }
 }
 
+// Inverse y pass of a velocity spectrum and its x and y derivatives from one load:
+// out.p[3 f] = F^-1_y(u), out.p[3 f + 1] = i kx F^-1_y(u) (kx is constant along a
+// y column, so the x derivative is applied at the store), out.p[3 f + 2] =
+// F^-1_y(i ky u) (ky is applied at the load).  Outputs are rows of stride ph.
+template <class R, class TI, class TO, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yinv3_fast(FieldList<TI> in, FieldList<TO> out, int nplanes, int lh,
+                                                       int ph, const cpx<R>* __restrict__ tw,
+                                                       const double* __restrict__ wavx,
+                                                       const double* __restrict__ wavy) {
+    constexpr int TS = N1 * (N2 + 1) + 1;
+    __shared__ cpx<R> T[C * TS];
+    const int tid = threadIdx.x;
+    const int c = tid % C, n2 = tid / C;
+    const int ntile = (lh + C - 1) / C;
+    const int tile = blockIdx.x % ntile;
+    const long plane = blockIdx.x / ntile;
+    const int f = (int)(plane / nplanes);
+    const long pk = plane - (long)f * nplanes;
+    const TI* src = in.p[f] + pk * (long)(N1 * N2) * (2 * lh);
+    const long ooff = pk * (long)(N1 * N2) * (2 * ph);
+    TO* du = out.p[3 * f] + ooff;
+    TO* dx = out.p[3 * f + 1] + ooff;
+    TO* dy = out.p[3 * f + 2] + ooff;
+    const int m = tile * C + c;
+    const bool valid = m < lh;
+    const double kx = valid ? wavx[2 * m] : 0.0;
+    double re[N1], im[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        re[n1] = 0.0;
+        im[n1] = 0.0;
+        if (valid) {
+            const TI* q = src + 2 * ((long)(N2 * n1 + n2) * lh + m);
+            re[n1] = (double)q[0];
+            im[n1] = (double)q[1];
+        }
+    }
+    cpx<R>* Tc = T + c * TS;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        cpx<R> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            if (pass == 0) {
+                v[n1] = {(R)re[n1], (R)im[n1]};
+            } else {
+                const double k = wavy[N2 * n1 + n2];
+                v[n1] = {(R)(-k * im[n1]), (R)(k * re[n1])};
+            }
+        }
+        f_dft<R, 1, N1>(v);
+        f_twiddle<R, 1, N1>(v, n2, tw);
+        if (pass == 1) __syncthreads();  // the first pass has read its tile
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) Tc[k1 * (N2 + 1) + n2] = v[k1];
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < N1 / N2; ++q) {
+                const int k1 = n2 + N2 * q;
+                cpx<R> u[N2];
+#pragma unroll
+                for (int j = 0; j < N2; ++j) u[j] = Tc[k1 * (N2 + 1) + j];
+                f_dft<R, 1, N2>(u);
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) {
+                    const long o = (long)(k1 + N1 * k2) * ph + m;
+                    if (pass == 0) {
+                        st_c<R>(du, o, u[k2]);
+                        st_c<R>(dx, o, cpx<R>{(R)(-kx) * u[k2].im, (R)kx * u[k2].re});
+                    } else {
+                        st_c<R>(dy, o, u[k2]);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- dispatch
 bool fast_fft_len(int L) { return L == 16 || L == 32 || L == 64 || L == 128 || L == 256; }
 
@@ -429,10 +508,16 @@ int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int 
     return LSB_OK;
 }
 
+// kx columns per y-pass tile: 128-byte row segments of the arithmetic type
+template <class R>
+constexpr int ytile_cols() {
+    return 128 / (int)sizeof(cpx<R>);
+}
+
 template <class R, class TI, class TO>
 int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
               const cpx<R>* tw, R scale, hipStream_t s) {
-    constexpr int C = 8;
+    constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
     switch (ny) {
 #define X(LL, A, B)                                                                                          \
@@ -451,13 +536,32 @@ int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 template <class R, class TI, class TO>
 int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
               const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s) {
-    constexpr int C = 8;
+    constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
     switch (ny) {
 #define X(LL, A, B)                                                                                          \
     case LL:                                                                                                 \
         hipLaunchKernelGGL((k_yinv_fast<R, TI, TO, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, \
                            nplanes, lh, ph, tw, ops, wavx, wavy);                                            \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class R, class TI, class TO>
+int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
+               const cpx<R>* tw, const double* wavx, const double* wavy, hipStream_t s) {
+    constexpr int C = ytile_cols<R>();
+    const long nblk = (long)((lh + C - 1) / C) * nplanes * nsrc;
+    switch (ny) {
+#define X(LL, A, B)                                                                                           \
+    case LL:                                                                                                  \
+        hipLaunchKernelGGL((k_yinv3_fast<R, TI, TO, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, \
+                           nplanes, lh, ph, tw, wavx, wavy);                                                  \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -475,7 +579,9 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
     template int fast_yfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int,             \
                                       const cpx<R>*, R, hipStream_t);                                             \
     template int fast_yinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
-                                      const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t);
+                                      const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t); \
+    template int fast_yinv3<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,       \
+                                       const cpx<R>*, const double*, const double*, hipStream_t);
 
 LSB_INST_FAST(double, double, double)
 LSB_INST_FAST(float, float, float)    // mixed: x passes (physical <-> compact staging)

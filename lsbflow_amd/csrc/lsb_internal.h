@@ -211,9 +211,14 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
 bool horfft_has_ops(const HorPlan& hp);
 int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
                    int nplanes, const int* ops, hipStream_t s);
+// u, du/dx, du/dy of 3 spectra (one load each); RT = physical storage and arithmetic type
+template <class RT>
+int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
+                    RT* const* dudy, int nplanes, hipStream_t s);
 // mixed form: FP32 physical fields, FP64 spectra
-int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, double* const* spec, int nf,
-                     int nplanes, hipStream_t s);
+template <class TO>
+int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO* const* spec, int nf, int nplanes,
+                     hipStream_t s);
 template <class TI>
 int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,
                      const int* ops, hipStream_t s);
@@ -237,6 +242,10 @@ int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 template <class R, class TI, class TO>
 int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
               const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s);
+// nsrc spectra -> 3 nsrc outputs (u, du/dx, du/dy per source), one load per source
+template <class R, class TI, class TO>
+int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
+               const cpx<R>* tw, const double* wavx, const double* wavy, hipStream_t s);
 
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).

@@ -6,10 +6,12 @@
 namespace lsb {
 
 struct StepRhsArgs {
-    const double* nlt[3];  // forward FFT of the convective term
-    const double* h0[3];   // hij(l, x) spectral
-    const double* h1[3];   // hij(l, y) spectral
-    const double* hz[3];   // dfdz(hij(l, z))
+    // the four pieces of this step's N (double, or float when n32 = 1: mixed form)
+    const void* nlt[3];    // forward FFT of the convective term
+    const void* h0[3];     // hij(l, x) spectral
+    const void* h1[3];     // hij(l, y) spectral
+    const void* hz[3];     // dfdz(hij(l, z))
+    int n32;
     double* nout[3];       // N_l (becomes nlnif)
     const double* nl[3];   // nlnif
     const double* nlm[3];  // nlnmif

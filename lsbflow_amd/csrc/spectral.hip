@@ -233,6 +233,13 @@ int sp_norm(lsb_ctx* c, double* const* f, int nf, hipStream_t s) {
 //   rhs_l = -dtnui*( dt*(ab0 N + ab1 Nn + ab2 Nnm) + bdf0 u + bdf1 un + bdf2 unm )
 //   rhs_0(kx=0,ky=0,z) -= ubarpp(z)
 // N_l is also written (it becomes the next step's nlnif).
+template <class TN>
+__device__ __forceinline__ c2 ld_n(const void* p, long q) {
+    const TN* t = reinterpret_cast<const TN*>(p) + 2 * q;
+    return make_double2((double)t[0], (double)t[1]);
+}
+
+template <class TN>
 __global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy, StepRhsArgs a) {
     const long n = d.npairs * 3;
     SP_LOOP(n) {
@@ -243,10 +250,10 @@ __global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const doubl
         const int j = (int)(jk % d.ny);
         const int k = (int)(jk / d.ny);
         const double wx = wavx[2 * m], wy = wavy[j];
-        const c2 nl = reinterpret_cast<const c2*>(a.nlt[l])[q];
-        const c2 h0 = reinterpret_cast<const c2*>(a.h0[l])[q];
-        const c2 h1 = reinterpret_cast<const c2*>(a.h1[l])[q];
-        const c2 hz = reinterpret_cast<const c2*>(a.hz[l])[q];
+        const c2 nl = ld_n<TN>(a.nlt[l], q);
+        const c2 h0 = ld_n<TN>(a.h0[l], q);
+        const c2 h1 = ld_n<TN>(a.h1[l], q);
+        const c2 hz = ld_n<TN>(a.hz[l], q);
         double nr = nl.x, ni = nl.y;
         nr = nr - (-wx * h0.y);
         ni = ni - (wx * h0.x);
@@ -336,7 +343,10 @@ int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
     int tok = -1;
     prof_mark_begin(c, PC_PW, s, &tok);
-    hipLaunchKernelGGL(k_rhs_vel, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
+    if (a.n32)
+        hipLaunchKernelGGL(k_rhs_vel<float>, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
+    else
+        hipLaunchKernelGGL(k_rhs_vel<double>, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;

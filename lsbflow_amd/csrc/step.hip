@@ -417,28 +417,33 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
                        bool ops_on_load, hipStream_t s) {
     StepLayout& L = S->L;
     // inverse FFTs: ui (3) + g (9)
-    {
+    if (ops_on_load) {
+        double *u[3], *gx[3], *gy[3];
+        const double* dz[3];
+        double* gz[3];
+        for (int l = 0; l < 3; ++l) {
+            u[l] = L.phys[P_UI0 + l];
+            gx[l] = L.phys[P_G0 + l];
+            gy[l] = L.phys[P_G0 + l + 3];
+            dz[l] = L.spec[S_DZ0 + l];
+            gz[l] = L.phys[P_G0 + l + 6];
+        }
+        LSB_TRY(horfft_inv_grad<double>(c, c->hp, uif, u, gx, gy, c->nzp, s));
+        LSB_TRY(horfft_inv(c, c->hp, dz, gz, 3, c->nzp, s));
+    } else {
         const double* in[12];
         double* out[12];
-        int ops[12];
         for (int l = 0; l < 3; ++l) {
             in[l] = uif[l];
             out[l] = L.phys[P_UI0 + l];
-            ops[l] = FOP_NONE;
         }
         for (int m = 0; m < 3; ++m)
             for (int l = 0; l < 3; ++l) {
                 const int slot = (m == 0 ? S_DX0 : (m == 1 ? S_DY0 : S_DZ0)) + l;
-                const int q = 3 + l + 3 * m;
-                in[q] = L.spec[slot];
-                out[q] = L.phys[P_G0 + l + 3 * m];
-                ops[q] = FOP_NONE;
-                if (ops_on_load && m < 2) {
-                    in[q] = uif[l];
-                    ops[q] = (m == 0) ? FOP_DX : FOP_DY;
-                }
+                in[3 + l + 3 * m] = L.spec[slot];
+                out[3 + l + 3 * m] = L.phys[P_G0 + l + 3 * m];
             }
-        LSB_TRY(horfft_inv_ops(c, c->hp, in, out, 12, c->nzp, ops, s));
+        LSB_TRY(horfft_inv(c, c->hp, in, out, 12, c->nzp, s));
     }
     const bool sgs = c->cfg.sgsflag == 1;
     const int model = c->cfg.sgsmodel;
@@ -536,19 +541,14 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
     return LSB_OK;
 }
 
-// Mixed form: restore the plane-constant base part ubar^2/2 of h_00, removed before
-// the FP32 rounding, on the (kx, ky) = (0, 0) mode of every plane.
-__global__ void k_add_base00(double* __restrict__ hf00, const double* __restrict__ ubar, int nzp, long plane_s) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nzp) return;
-    hf00[(long)k * plane_s] += 0.5 * (ubar[k] * ubar[k]);
-}
-
 static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED; }
 
-// FP32 physical chain of the mixed form (sigma, static or no SGS model): 12 inverse
-// transforms with d/dx, d/dy applied on load, the products and the SGS model in one
-// pass (FP64 arithmetic on FP32 storage), 3 + 6 forward transforms back to FP64 spectra.
+// FP32 chain of the mixed form (sigma, static or no SGS model): the 9 + 3 inverse
+// transforms, the products and the SGS model in one pass (FP64 arithmetic on FP32
+// storage), 3 + 6 forward transforms to FP32 spectra (nltf_out and hf hold FP32).
+// h_00 is stored without its plane-constant base part ubar^2/2 (removed before the
+// rounding); that part lives only on the (kx, ky) = (0, 0) mode, where h_00 enters
+// N only through i kx h_00 = 0, so it is not restored.
 static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3],
                              double* hf[9], hipStream_t s) {
     StepLayout& L = S->L;
@@ -560,22 +560,14 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
     }
     auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
     {
-        // u, v, w and their x/y derivatives (applied on load) from the FP64 spectra
-        const double* in[9];
-        float* out[9];
-        int ops[9];
+        // u, v, w and their x/y derivatives from one load of each FP64 spectrum
+        float *u[3], *gx[3], *gy[3];
         for (int l = 0; l < 3; ++l) {
-            in[l] = uif[l];
-            out[l] = ph(P_UI0 + l);
-            ops[l] = FOP_NONE;
-            in[3 + l] = uif[l];
-            out[3 + l] = ph(P_G0 + l);          // (l, x)
-            ops[3 + l] = FOP_DX;
-            in[6 + l] = uif[l];
-            out[6 + l] = ph(P_G0 + l + 3);      // (l, y)
-            ops[6 + l] = FOP_DY;
+            u[l] = ph(P_UI0 + l);
+            gx[l] = ph(P_G0 + l);          // (l, x)
+            gy[l] = ph(P_G0 + l + 3);      // (l, y)
         }
-        LSB_TRY(horfft_inv_mixed<double>(c, c->hp, in, out, 9, c->nzp, ops, s));
+        LSB_TRY(horfft_inv_grad<float>(c, c->hp, uif, u, gx, gy, c->nzp, s));
         // the z derivatives from their FP32 spectra
         const float* inz[3];
         float* outz[3];
@@ -612,22 +604,19 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
     }
     {
         const float* in[9];
-        double* out[9];
+        float* out[9];
         int n = 0;
         for (int l = 0; l < 3; ++l) {
             in[n] = ph(P_NLT0 + l);
-            out[n++] = nltf_out[l];
+            out[n++] = reinterpret_cast<float*>(nltf_out[l]);
         }
         const int uq[6] = {0, 1, 2, 4, 5, 8};
         for (int q = 0; q < 6; ++q) {
             in[n] = ph(P_H0 + q);
-            out[n++] = hf[uq[q]];
+            out[n++] = reinterpret_cast<float*>(hf[uq[q]]);
         }
-        LSB_TRY(horfft_fwd_mixed(c, c->hp, in, out, n, c->nzp, s));
+        LSB_TRY(horfft_fwd_mixed<float>(c, c->hp, in, out, n, c->nzp, s));
     }
-    hipLaunchKernelGGL(k_add_base00, dim3((c->nzp + 63) / 64), dim3(64), 0, s, hf[0], c->d_paral, c->nzp,
-                       (long)c->plane_s);
-    LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
 
@@ -663,7 +652,13 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
     {
         const double* in[3] = {hf[0 + 3 * 2], hf[1 + 3 * 2], hf[2 + 3 * 2]};
         if (mixed_form(c)) {
-            LSB_TRY((zcontract_f32<double, double>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s)));
+            const float* in32[3];
+            float* out32[3];
+            for (int l = 0; l < 3; ++l) {
+                in32[l] = reinterpret_cast<const float*>(in[l]);
+                out32[l] = reinterpret_cast<float*>(hz[l]);
+            }
+            LSB_TRY((zcontract_f32<float, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in32, out32, 3, s)));
         } else {
             LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s));
         }
@@ -691,7 +686,9 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
     double* nltf[3] = {S->L.spec[S_NLTF0], S->L.spec[S_NLTF0 + 1], S->L.spec[S_NLTF0 + 2]};
     double* hz[3] = {S->L.spec[S_HZ0], S->L.spec[S_HZ0 + 1], S->L.spec[S_HZ0 + 2]};
     LSB_TRY(nlterm_core(c, S, uif, nltf, hf, hz, s));
-    // N_l = nltf - dfdx(h_l0) - dfdy(h_l1) - hz_l: reuse the rhs kernel with zero history
+    // N_l = nltf - dfdx(h_l0) - dfdy(h_l1) - hz_l: reuse the rhs kernel with zero-weighted
+    // history (finite FP64 fields) and its right-hand side written to the x-derivative
+    // slots, which the inverse transform has consumed (the step's rotating slots stay intact)
     StepRhsArgs r{};
     for (int l = 0; l < 3; ++l) {
         r.nlt[l] = nltf[l];
@@ -699,13 +696,14 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
         r.h1[l] = hf[l + 3];
         r.hz[l] = hz[l];
         r.nout[l] = nlt[l];
-        r.nl[l] = nltf[l];
-        r.nlm[l] = nltf[l];
-        r.u[l] = nltf[l];
-        r.un[l] = nltf[l];
-        r.unm[l] = nltf[l];
-        r.rhs[l] = S->L.spec[S_RHS0 + l];  // scratch
+        r.nl[l] = uif[l];
+        r.nlm[l] = uif[l];
+        r.u[l] = uif[l];
+        r.un[l] = uif[l];
+        r.unm[l] = uif[l];
+        r.rhs[l] = S->L.spec[S_DX0 + l];
     }
+    r.n32 = mixed_form(c) ? 1 : 0;
     r.ubarpp = c->d_paral + 2 * c->nzp;
     r.ab0 = 1.0;
     r.dtnui = 0.0;
@@ -807,6 +805,7 @@ static int update_once(lsb_ctx* c) {
         r.unm[l] = L.spec[S->unm[l]];
         r.rhs[l] = L.spec[S->rhs[l]];
     }
+    r.n32 = mixed_form(c) ? 1 : 0;
     r.ubarpp = c->d_paral + 2 * c->nzp;
     r.ab0 = c->abfac[0];
     r.ab1 = c->abfac[1];

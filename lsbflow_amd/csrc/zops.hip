@@ -383,6 +383,8 @@ template int zcontract_f32<double, float>(lsb_ctx*, const double*, int, int, int
                                           float* const*, int, hipStream_t);
 template int zcontract_f32<double, double>(lsb_ctx*, const double*, int, int, int, const double* const*,
                                            double* const*, int, hipStream_t);
+template int zcontract_f32<float, float>(lsb_ctx*, const double*, int, int, int, const float* const*,
+                                         float* const*, int, hipStream_t);
 
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
               double* const* out, int nf, hipStream_t s) {
