@@ -254,9 +254,12 @@ int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const dou
 // pressure right-hand side -d(div_h)/dz - k^2 w in one contraction
 int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs, hipStream_t s);
 // mixed form: FP32 matrix-core contraction, panel TI, output TO (FP32 products and sums)
+// P columns per plane (0: spectral nx*ny); with aux, out = aux - M in (out may alias aux)
 template <class TI, class TO>
 int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,
-                  int nf, hipStream_t s);
+                  int nf, hipStream_t s, long P = 0, const TO* const* aux = nullptr);
+int zcontract_ex(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
+                 double* const* out, int nf, long P, const double* const* aux, hipStream_t s);
 // Helmholtz solves in place: rhs_f (nzp planes); bc_f may be null (homogeneous).
 // sigma = dtnui (constant over fields).
 int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top,

@@ -10,7 +10,7 @@ struct StepRhsArgs {
     const void* nlt[3];    // forward FFT of the convective term
     const void* h0[3];     // hij(l, x) spectral
     const void* h1[3];     // hij(l, y) spectral
-    const void* hz[3];     // dfdz(hij(l, z))
+    const void* hz[3];     // dfdz(hij(l, z)); null when already folded into nlt
     int n32;
     double* nout[3];       // N_l (becomes nlnif)
     const double* nl[3];   // nlnif

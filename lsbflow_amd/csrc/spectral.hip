@@ -253,7 +253,7 @@ __global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const doubl
         const c2 nl = ld_n<TN>(a.nlt[l], q);
         const c2 h0 = ld_n<TN>(a.h0[l], q);
         const c2 h1 = ld_n<TN>(a.h1[l], q);
-        const c2 hz = ld_n<TN>(a.hz[l], q);
+        const c2 hz = a.hz[l] ? ld_n<TN>(a.hz[l], q) : make_double2(0.0, 0.0);
         double nr = nl.x, ni = nl.y;
         nr = nr - (-wx * h0.y);
         ni = ni - (wx * h0.x);

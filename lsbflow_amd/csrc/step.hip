@@ -416,20 +416,21 @@ int isc_full(lsb_ctx* c, double* ui, double* tau, double* work, hipStream_t s) {
 static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
                        bool ops_on_load, hipStream_t s) {
     StepLayout& L = S->L;
-    // inverse FFTs: ui (3) + g (9)
+    // inverse FFTs: ui (3) + g (9).  d/dz acts along pencils and commutes with the
+    // horizontal transform, so with the register-resident FFT the z derivatives are
+    // taken of the physical velocity (3 transforms fewer)
+    const long pphys = (long)c->nxpp * c->ny;
     if (ops_on_load) {
-        double *u[3], *gx[3], *gy[3];
-        const double* dz[3];
-        double* gz[3];
+        double *u[3], *gx[3], *gy[3], *gz[3];
         for (int l = 0; l < 3; ++l) {
             u[l] = L.phys[P_UI0 + l];
             gx[l] = L.phys[P_G0 + l];
             gy[l] = L.phys[P_G0 + l + 3];
-            dz[l] = L.spec[S_DZ0 + l];
             gz[l] = L.phys[P_G0 + l + 6];
         }
         LSB_TRY(horfft_inv_grad<double>(c, c->hp, uif, u, gx, gy, c->nzp, s));
-        LSB_TRY(horfft_inv(c, c->hp, dz, gz, 3, c->nzp, s));
+        const double* uc[3] = {u[0], u[1], u[2]};
+        LSB_TRY(zcontract_ex(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uc, gz, 3, pphys, nullptr, s));
     } else {
         const double* in[12];
         double* out[12];
@@ -515,26 +516,32 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         set_error("tau_sgs: sgsmodel must be 0..3");
         return LSB_E_ARG;
     }
-    // forward FFTs of nlt (3) and the distinct h
+    // N'_l = nlt_l - d h_(l,z)/dz in physical space (navierstokes.f90:781-795 with the
+    // z derivative taken before the transform), then forward FFTs of N' (3) and of the
+    // distinct h_(l,x), h_(l,y) (5 symmetric, 6 otherwise)
     {
-        const double* in[12];
-        double* out[12];
+        const double* hz_src[3];
+        double* nl[3];
+        const double* nlc[3];
+        for (int l = 0; l < 3; ++l) {
+            hz_src[l] = sym ? hph[(l == 0) ? 2 : (l == 1 ? 5 : 8)] : hph[l + 6];
+            nl[l] = L.phys[P_NLT0 + l];
+            nlc[l] = nl[l];
+        }
+        LSB_TRY(zcontract_ex(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, hz_src, nl, 3, pphys, nlc, s));
+    }
+    {
+        const double* in[9];
+        double* out[9];
         int n = 0;
         for (int l = 0; l < 3; ++l) {
             in[n] = L.phys[P_NLT0 + l];
             out[n++] = nltf_out[l];
         }
-        if (sym) {
-            const int uq[6] = {0, 1, 2, 4, 5, 8};
-            for (int q = 0; q < 6; ++q) {
-                in[n] = hph[uq[q]];
-                out[n++] = hf[uq[q]];
-            }
-        } else {
-            for (int q = 0; q < 9; ++q) {
-                in[n] = hph[q];
-                out[n++] = hf[q];
-            }
+        for (int q = 0; q < 6; ++q) {
+            if (!hph[q]) continue;  // symmetric alias
+            in[n] = hph[q];
+            out[n++] = hf[q];
         }
         LSB_TRY(horfft_fwd(c, c->hp, in, out, n, c->nzp, s));
     }
@@ -559,6 +566,7 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
         return LSB_E_UNSUPPORTED;
     }
     auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
+    const long pphys = (long)c->nxpp * c->ny;
     {
         // u, v, w and their x/y derivatives from one load of each FP64 spectrum
         float *u[3], *gx[3], *gy[3];
@@ -568,14 +576,10 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
             gy[l] = ph(P_G0 + l + 3);      // (l, y)
         }
         LSB_TRY(horfft_inv_grad<float>(c, c->hp, uif, u, gx, gy, c->nzp, s));
-        // the z derivatives from their FP32 spectra
-        const float* inz[3];
-        float* outz[3];
-        for (int l = 0; l < 3; ++l) {
-            inz[l] = reinterpret_cast<const float*>(L.spec[S_DZ0 + l]);
-            outz[l] = ph(P_G0 + l + 6);         // (l, z)
-        }
-        LSB_TRY(horfft_inv_mixed<float>(c, c->hp, inz, outz, 3, c->nzp, nullptr, s));
+        // d/dz of the physical velocity (commutes with the horizontal transform)
+        const float* uc[3] = {u[0], u[1], u[2]};
+        float* gz[3] = {ph(P_G0 + 6), ph(P_G0 + 7), ph(P_G0 + 8)};
+        LSB_TRY((zcontract_f32<float, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uc, gz, 3, s, pphys)));
     }
     NlArgsT<float> a{};
     for (int l = 0; l < 3; ++l) {
@@ -603,16 +607,23 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
         LSB_TRY(ph_nonlinear_f32(c, NL_STATIC, a, s));
     }
     {
-        const float* in[9];
-        float* out[9];
+        // N'_l = nlt_l - d h_(l,z)/dz in physical space
+        const float* hz_src[3] = {ph(P_H0 + 2), ph(P_H0 + 4), ph(P_H0 + 5)};  // (0,2)=(2,0), (1,2)=(2,1), (2,2)
+        float* nl[3] = {ph(P_NLT0), ph(P_NLT0 + 1), ph(P_NLT0 + 2)};
+        const float* nlc[3] = {nl[0], nl[1], nl[2]};
+        LSB_TRY((zcontract_f32<float, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, hz_src, nl, 3, s, pphys, nlc)));
+    }
+    {
+        const float* in[8];
+        float* out[8];
         int n = 0;
         for (int l = 0; l < 3; ++l) {
             in[n] = ph(P_NLT0 + l);
             out[n++] = reinterpret_cast<float*>(nltf_out[l]);
         }
-        const int uq[6] = {0, 1, 2, 4, 5, 8};
-        for (int q = 0; q < 6; ++q) {
-            in[n] = ph(P_H0 + q);
+        const int uq[5] = {0, 1, 2, 4, 5};  // h_(l,x), h_(l,y): (0,0) (1,0) (2,0) (1,1) (2,1)
+        for (int q = 0; q < 5; ++q) {
+            in[n] = ph(P_H0 + map[uq[q]]);
             out[n++] = reinterpret_cast<float*>(hf[uq[q]]);
         }
         LSB_TRY(horfft_fwd_mixed<float>(c, c->hp, in, out, n, c->nzp, s));
@@ -624,46 +635,25 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
 // navierstokes.f90:658-795 for T = 0: uif (3 spectral, already filtered) ->
 // N_l written to nlt (3 spectral).  Work comes from the step layout.
 static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
-                       double* hz[3], hipStream_t s) {
+                       hipStream_t s) {
     StepLayout& L = S->L;
-    // d/dx and d/dy are applied as the inverse transform loads uif where the
-    // register-resident FFT serves the sizes; otherwise they are separate passes
+    // With the register-resident FFT, d/dx and d/dy are applied as the inverse transform
+    // loads uif and d/dz is taken in physical space; otherwise all three are spectral passes
     const bool ops_on_load = horfft_has_ops(c->hp);
     if (!ops_on_load) {
+        if (mixed_form(c)) {
+            set_error("mixed precision: needs power-of-two nx/2 and ny in 16..256");
+            return LSB_E_UNSUPPORTED;
+        }
         for (int l = 0; l < 3; ++l) {
             LSB_TRY(sp_dfdx(c, uif[l], L.spec[S_DX0 + l], s));
             LSB_TRY(sp_dfdy(c, uif[l], L.spec[S_DY0 + l], s));
         }
-    }
-    if (mixed_form(c)) {
-        float* dz[3];
-        for (int l = 0; l < 3; ++l) dz[l] = reinterpret_cast<float*>(L.spec[S_DZ0 + l]);
-        LSB_TRY((zcontract_f32<double, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s)));
-    } else {
         double* dz[3] = {L.spec[S_DZ0], L.spec[S_DZ0 + 1], L.spec[S_DZ0 + 2]};
         LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s));
     }
-    if (mixed_form(c)) {
-        LSB_TRY(nlterm_phys_mixed(c, S, uif, nltf_out, hf, s));
-    } else {
-        LSB_TRY(nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, s));
-    }
-    // d/dz of h(l, z)
-    {
-        const double* in[3] = {hf[0 + 3 * 2], hf[1 + 3 * 2], hf[2 + 3 * 2]};
-        if (mixed_form(c)) {
-            const float* in32[3];
-            float* out32[3];
-            for (int l = 0; l < 3; ++l) {
-                in32[l] = reinterpret_cast<const float*>(in[l]);
-                out32[l] = reinterpret_cast<float*>(hz[l]);
-            }
-            LSB_TRY((zcontract_f32<float, float>(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in32, out32, 3, s)));
-        } else {
-            LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, hz, 3, s));
-        }
-    }
-    return LSB_OK;
+    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, s);
+    return nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, s);
 }
 
 static void hf_view(StepState* S, bool sym, double* hf[9]) {
@@ -684,9 +674,8 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
     double* hf[9];
     hf_view(S, sym, hf);
     double* nltf[3] = {S->L.spec[S_NLTF0], S->L.spec[S_NLTF0 + 1], S->L.spec[S_NLTF0 + 2]};
-    double* hz[3] = {S->L.spec[S_HZ0], S->L.spec[S_HZ0 + 1], S->L.spec[S_HZ0 + 2]};
-    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, hz, s));
-    // N_l = nltf - dfdx(h_l0) - dfdy(h_l1) - hz_l: reuse the rhs kernel with zero-weighted
+    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, s));
+    // N_l = nltf - dfdx(h_l0) - dfdy(h_l1) (nltf holds N' = nlt - dh_lz/dz): reuse the rhs kernel with zero-weighted
     // history (finite FP64 fields) and its right-hand side written to the x-derivative
     // slots, which the inverse transform has consumed (the step's rotating slots stay intact)
     StepRhsArgs r{};
@@ -694,7 +683,7 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
         r.nlt[l] = nltf[l];
         r.h0[l] = hf[l + 0];
         r.h1[l] = hf[l + 3];
-        r.hz[l] = hz[l];
+        r.hz[l] = nullptr;  // d h_(l,z)/dz is already in nltf (N')
         r.nout[l] = nlt[l];
         r.nl[l] = uif[l];
         r.nlm[l] = uif[l];
@@ -780,8 +769,7 @@ static int update_once(lsb_ctx* c) {
     double* hf[9];
     hf_view(S, sym, hf);
     double* nltf[3] = {L.spec[S_NLTF0], L.spec[S_NLTF0 + 1], L.spec[S_NLTF0 + 2]};
-    double* hz[3] = {L.spec[S_HZ0], L.spec[S_HZ0 + 1], L.spec[S_HZ0 + 2]};
-    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, hz, s));
+    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, s));
     // the unfiltered velocity is u^n for dealiasflag 0..2 (navierstokes.f90:1225-1229)
     const double* un_[3];
     for (int l = 0; l < 3; ++l) un_[l] = (g.dealiasflag > 2) ? uif[l] : L.spec[S->u[l]];
@@ -796,7 +784,7 @@ static int update_once(lsb_ctx* c) {
         r.nlt[l] = nltf[l];
         r.h0[l] = hf[l + 0];
         r.h1[l] = hf[l + 3];
-        r.hz[l] = hz[l];
+        r.hz[l] = nullptr;  // d h_(l,z)/dz is already in nltf (N')
         r.nout[l] = L.spec[S->nnew[l]];
         r.nl[l] = L.spec[S->nl[l]];
         r.nlm[l] = L.spec[S->nlm[l]];

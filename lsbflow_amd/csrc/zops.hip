@@ -27,8 +27,9 @@
 
 namespace lsb {
 
-// ZM_PRESS: out = -acc - (kx^2 + ky^2) aux, aux a field at the output positions
-enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2, ZM_PRESS = 3 };
+// ZM_PRESS: out = -acc - (kx^2 + ky^2) aux; ZM_AUXSUB: out = aux - acc (aux a field at the
+// output positions, passed in the bct slot; out may alias aux)
+enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2, ZM_PRESS = 3, ZM_AUXSUB = 4 };
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -181,13 +182,14 @@ k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<doub
                             x = x + a.lift[k] * vbb[j] + a.lift[a.nzp + k] * vbt[j];
                         }
                         if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)row * P + cj[j]];
+                        if (MODE == ZM_AUXSUB) x = bct.p[f][(long)row * P + cj[j]] - x;
                         if (col < P) dst[(long)row * P + col] = x;
                     }
                 }
             }
         }
     }
-    if (dox && (MODE == ZM_PLAIN || MODE == ZM_PRESS)) {
+    if (dox && (MODE == ZM_PLAIN || MODE == ZM_PRESS || MODE == ZM_AUXSUB)) {
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
             double x = xacc[j];
@@ -195,6 +197,7 @@ k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<doub
             x += __shfl_xor(x, 32);
             const long col = p0 + 16 * j + li;
             if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)a.xrow * P + cj[j]];
+            if (MODE == ZM_AUXSUB) x = bct.p[f][(long)a.xrow * P + cj[j]] - x;
             if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = x;
         }
     }
@@ -221,7 +224,9 @@ template <int MODE>
 static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldList<double>& out,
                     const FieldList<double>& bt, const FieldList<double>& bb, int nf, hipStream_t s) {
     ZArgs a = a0;
-    a.xrow = ((MODE == ZM_PLAIN || MODE == ZM_PRESS) && a.nout > 16 && a.nout % 16 == 1) ? a.nout - 1 : -1;
+    a.xrow = ((MODE == ZM_PLAIN || MODE == ZM_PRESS || MODE == ZM_AUXSUB) && a.nout > 16 && a.nout % 16 == 1)
+                 ? a.nout - 1
+                 : -1;
     const int nrb = (a.xrow >= 0) ? a.nout / 16 : (a.nout + 15) / 16;
     a.gblocks = std::min(nrb, kZRB * kZMaxWaves);
     const int gy = (nrb + a.gblocks - 1) / a.gblocks;
@@ -242,8 +247,9 @@ static int launch_z(const ZArgs& a0, const FieldList<double>& in, const FieldLis
 typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kZRB32 = 4;
 
-template <class TI, class TO>
-__global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, FieldList<TI> in, FieldList<TO> out) {
+template <class TI, class TO, bool AUX>
+__global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, FieldList<TI> in, FieldList<TO> out,
+                                                                    FieldList<TO> aux) {
     constexpr int RB = kZRB32, CB = kZCB, PD = kZPD;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -328,7 +334,9 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
 #pragma unroll
                     for (int j = 0; j < CB; ++j) {
                         const long col = p0 + 16 * j + li;
-                        if (col < P) dst[(long)row * P + col] = (TO)acc[i][j][v];
+                        float x = acc[i][j][v];
+                        if (AUX) x = (float)aux.p[f][(long)row * P + cj[j]] - x;
+                        if (col < P) dst[(long)row * P + col] = (TO)x;
                     }
                 }
             }
@@ -341,6 +349,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
             x += __shfl_xor(x, 16);
             x += __shfl_xor(x, 32);
             const long col = p0 + 16 * j + li;
+            if (AUX) x = (float)aux.p[f][(long)a.xrow * P + cj[j]] - x;
             if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = (TO)x;
         }
     }
@@ -348,7 +357,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
 
 template <class TI, class TO>
 int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,
-                  int nf, hipStream_t s) {
+                  int nf, hipStream_t s, long P, const TO* const* aux) {
     if (nf < 1 || nf > kMaxFields) {
         set_error("zcontract: 1..16 fields per launch");
         return LSB_E_ARG;
@@ -358,12 +367,13 @@ int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const
     a.ldm = ldm;
     a.nout = nout;
     a.nin = nin;
-    a.P = (long)c->plane_s;
+    a.P = P > 0 ? P : (long)c->plane_s;
     FieldList<TI> fi{};
-    FieldList<TO> fo{};
+    FieldList<TO> fo{}, fa{};
     for (int f = 0; f < nf; ++f) {
         fi.p[f] = const_cast<TI*>(in[f]);
         fo.p[f] = out[f];
+        fa.p[f] = aux ? const_cast<TO*>(aux[f]) : nullptr;
     }
     a.xrow = (nout > 16 && nout % 16 == 1) ? nout - 1 : -1;
     const int nrb = (a.xrow >= 0) ? nout / 16 : (nout + 15) / 16;
@@ -373,21 +383,30 @@ int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const
     dim3 grid((unsigned)((a.P + 16 * kZCB - 1) / (16 * kZCB)), (unsigned)gy, (unsigned)nf);
     int tok = -1;
     prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
-                    (double)a.P * nf * (nin * sizeof(TI) + nout * sizeof(TO)));
-    hipLaunchKernelGGL((k_zcontract_f32<TI, TO>), grid, dim3(64 * nw), 0, s, a, fi, fo);
+                    (double)a.P * nf * (nin * sizeof(TI) + nout * sizeof(TO) * (aux ? 2 : 1)));
+    if (aux)
+        hipLaunchKernelGGL((k_zcontract_f32<TI, TO, true>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
+    else
+        hipLaunchKernelGGL((k_zcontract_f32<TI, TO, false>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
 template int zcontract_f32<double, float>(lsb_ctx*, const double*, int, int, int, const double* const*,
-                                          float* const*, int, hipStream_t);
+                                          float* const*, int, hipStream_t, long, const float* const*);
 template int zcontract_f32<double, double>(lsb_ctx*, const double*, int, int, int, const double* const*,
-                                           double* const*, int, hipStream_t);
+                                           double* const*, int, hipStream_t, long, const double* const*);
 template int zcontract_f32<float, float>(lsb_ctx*, const double*, int, int, int, const float* const*,
-                                         float* const*, int, hipStream_t);
+                                         float* const*, int, hipStream_t, long, const float* const*);
 
 int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
               double* const* out, int nf, hipStream_t s) {
+    return zcontract_ex(c, M, ldm, nout, nin, in, out, nf, (long)c->plane_s, nullptr, s);
+}
+
+// P columns per plane (spectral nx*ny or physical (nx+2)*ny); with aux, out = aux - M in
+int zcontract_ex(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const double* const* in,
+                 double* const* out, int nf, long P, const double* const* aux, hipStream_t s) {
     if (nf < 1 || nf > kMaxFields) {
         set_error("zcontract: 1..16 fields per launch");
         return LSB_E_ARG;
@@ -397,20 +416,21 @@ int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const dou
     a.ldm = ldm;
     a.nout = nout;
     a.nin = nin;
-    a.P = (long)c->plane_s;
+    a.P = P;
     a.nx = c->nx;
     a.ny = c->ny;
     a.nzp = c->nzp;
-    FieldList<double> fi{}, fo{}, z{};
+    FieldList<double> fi{}, fo{}, fa{}, z{};
     for (int f = 0; f < nf; ++f) {
         fi.p[f] = const_cast<double*>(in[f]);
         fo.p[f] = out[f];
+        fa.p[f] = aux ? const_cast<double*>(aux[f]) : nullptr;
         z.p[f] = nullptr;
     }
     int tok = -1;
     prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
-                    (double)(nin + nout) * a.P * nf * sizeof(double));
-    int rc = launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
+                    (double)(nin + nout + (aux ? nout : 0)) * a.P * nf * sizeof(double));
+    int rc = aux ? launch_z<ZM_AUXSUB>(a, fi, fo, fa, z, nf, s) : launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
     prof_mark_end(c, tok, s);
     return rc;
 }
