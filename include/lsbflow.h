@@ -219,8 +219,10 @@ int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields
 /* Kernel timing for roofline accounting.  Between lsb_profile_begin and
  * lsb_profile_end every launch group is bracketed by HIP events on the launch
  * stream and tagged with its algorithmic work.  Per class c (0 FFT, 1 Chebyshev
- * contraction, 2 Helmholtz, 3 pointwise): total device ms, launch count,
- * algorithmic flops and algorithmic HBM bytes (arrays of 4). */
+ * contraction, 2 Helmholtz, 3 other pointwise, 4 right-hand-side assembly,
+ * 5 physical-space products and SGS model): total device ms, launch count,
+ * algorithmic flops and algorithmic HBM bytes (arrays of LSB_PROF_CLASSES). */
+#define LSB_PROF_CLASSES 6
 int lsb_profile_begin(lsb_ctx* ctx);
 int lsb_profile_end(lsb_ctx* ctx, double* ms, int* count, double* flops, double* bytes);
 

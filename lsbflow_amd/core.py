@@ -131,6 +131,7 @@ def diagonalize_matrix(D2):
 
 
 # ---------------------------------------------------------------- context
+PROF_CLASSES = 6  # LSB_PROF_CLASSES
 class Context:
     """One HIP context: module globals snapshot + device operators + state."""
 
@@ -398,13 +399,13 @@ class Context:
 
     def profile_end(self):
         """{class: dict(ms=total device ms, n=launch groups, flops=..., bytes=...)}."""
-        ms = np.zeros(4)
-        n = np.zeros(4, dtype=np.int32)
-        fl = np.zeros(4)
-        by = np.zeros(4)
+        ms = np.zeros(PROF_CLASSES)
+        n = np.zeros(PROF_CLASSES, dtype=np.int32)
+        fl = np.zeros(PROF_CLASSES)
+        by = np.zeros(PROF_CLASSES)
         check(self._L.lsb_profile_end(self.h, _p(ms), n.ctypes.data_as(C.POINTER(C.c_int)), _p(fl), _p(by)),
               "profile_end")
-        keys = ("fft", "zcontract", "helmholtz", "pointwise")
+        keys = ("fft", "zcontract", "helmholtz", "pointwise", "rhs", "nonlinear")
         return {k: dict(ms=float(ms[i]), n=int(n[i]), flops=float(fl[i]), bytes=float(by[i])) for i, k in enumerate(keys)}
 
     def synchronize(self):

@@ -1086,8 +1086,8 @@ extern "C" int lsb_synchronize(lsb_ctx* c) {
 extern "C" int lsb_profile_end(lsb_ctx* c, double* ms, int* count, double* flops, double* bytes) {
     LSB_TRY(check_ctx(c));
     LSB_HIP(hipStreamSynchronize(c->stream));
-    double t_ms[4] = {0, 0, 0, 0}, fl[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
-    int cnt[4] = {0, 0, 0, 0};
+    double t_ms[kProfClasses] = {}, fl[kProfClasses] = {}, by[kProfClasses] = {};
+    int cnt[kProfClasses] = {};
     for (auto& ev : c->prof) {
         float t = 0.f;
         if (hipEventElapsedTime(&t, ev.a, ev.b) == hipSuccess) {
@@ -1101,7 +1101,7 @@ extern "C" int lsb_profile_end(lsb_ctx* c, double* ms, int* count, double* flops
     }
     c->prof.clear();
     c->profiling = false;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < kProfClasses; ++i) {
         if (ms) ms[i] = t_ms[i];
         if (count) count[i] = cnt[i];
         if (flops) flops[i] = fl[i];

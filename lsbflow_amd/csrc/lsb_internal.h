@@ -195,7 +195,9 @@ int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p);
 int upload_context(lsb_ctx* c);
 
 // profiling helpers
-enum { PC_FFT = 0, PC_Z = 1, PC_HELM = 2, PC_PW = 3 };
+// profiling classes: 2-D FFTs, Chebyshev contractions, Helmholtz projections, the RHS
+// assembly, the physical-space products/SGS pass, other pointwise kernels
+enum { PC_FFT = 0, PC_Z = 1, PC_HELM = 2, PC_PW = 3, PC_RHS = 4, PC_NL = 5, kProfClasses = 6 };
 int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token, double flops = 0.0, double bytes = 0.0);
 int prof_mark_end(lsb_ctx* c, int token, hipStream_t s);
 

@@ -419,7 +419,7 @@ static int launch_nonlinear(lsb_ctx* c, int mode, const NlArgsT<T>& a, hipStream
     // 12 fields in, 3 + up to 9 out
     int nh = 0;
     for (int q = 0; q < 9; ++q) nh += a.h[q] ? 1 : 0;
-    prof_mark_begin(c, PC_PW, s, &tok, 0.0, (double)(15 + nh) * d.npts * sizeof(T));
+    prof_mark_begin(c, PC_NL, s, &tok, 0.0, (double)(15 + nh) * d.npts * sizeof(T));
     const dim3 g = ph_grid(d.npts);
     switch (mode) {
         case NL_NONE: hipLaunchKernelGGL((k_nonlinear<T, NL_NONE>), g, dim3(256), 0, s, d, a); break;
@@ -478,7 +478,8 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
                double* tmp, hipStream_t s) {
     PhDims d = ph_dims(c);
     int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
+    // x-y pass and z pass, each one read and one write per field
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, 4.0 * nf * (double)d.npts * sizeof(double));
     const double* w;
     if (a == c->cfg.a && b == c->cfg.b)
         w = c->d_fz;

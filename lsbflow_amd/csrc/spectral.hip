@@ -175,8 +175,11 @@ int sp_dfdy(lsb_ctx* c, const double* in, double* out, hipStream_t s) {
 }
 int sp_divuv(lsb_ctx* c, const double* u, const double* v, double* out, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, 3.0 * 16.0 * (double)d.npairs);
     hipLaunchKernelGGL(k_divuv, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)u,
                        (const c2*)v, (c2*)out);
+    prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
@@ -342,7 +345,13 @@ __global__ void k_project(SpDims d, const double* __restrict__ wavx, const doubl
 int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
     int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
+    {   // per component: the N pieces (nlt, h0, h1, hz if present), 5 FP64 history/velocity
+        // fields read, N and rhs written
+        const double tn = a.n32 ? 4.0 : 8.0;
+        const double pieces = a.hz[0] ? 4.0 : 3.0;
+        const double per_pair = 2.0 * (pieces * tn + 5.0 * 8.0 + 2.0 * 8.0);
+        prof_mark_begin(c, PC_RHS, s, &tok, 0.0, per_pair * (double)d.npairs * 3.0);
+    }
     if (a.n32)
         hipLaunchKernelGGL(k_rhs_vel<float>, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
     else
@@ -361,7 +370,8 @@ int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, do
                double bdfv, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
     int tok = -1;
-    prof_mark_begin(c, PC_PW, s, &tok);
+    // reads dwdz, divf, u, v; writes p, u, v
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, 7.0 * 16.0 * (double)d.npairs);
     hipLaunchKernelGGL(k_project, sp_grid(d.npairs), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, (const c2*)dwdz,
                        (const c2*)divf, (c2*)p, (c2*)u, (c2*)v, dt, bdfv);
     prof_mark_end(c, tok, s);
