@@ -537,7 +537,7 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
 // arithmetic type (double: FP64 form; float: mixed form).
 template <class RT>
 int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
-                    RT* const* dudy, int nplanes, hipStream_t s) {
+                    RT* const* dudy, int nplanes, const double* ftab, hipStream_t s) {
     if (!horfft_has_ops(hp)) {
         set_error("horfft: the fused gradient transform needs power-of-two nx/2 and ny in 16..256");
         return LSB_E_UNSUPPORTED;
@@ -564,15 +564,15 @@ int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT*
     const cpx<RT>* twy = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_y : (void*)hp.tw_y32);
     const cpx<RT>* twx = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_x : (void*)hp.tw_x32);
     const cpx<RT>* twr = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_r : (void*)hp.tw_r32);
-    LSB_TRY((fast_yinv3<RT, double, RT>(in, mid, nplanes, 3, lh, ny, lh, twy, c->d_wavx, c->d_wavy, s)));
+    LSB_TRY((fast_yinv3<RT, double, RT>(in, mid, nplanes, 3, lh, ny, lh, twy, c->d_wavx, c->d_wavy, ftab, s)));
     LSB_TRY((fast_xinv<RT, RT, RT>(mid, out, nplanes * ny, 9, lh, nx, nxpp, twx, twr, s)));
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }
 template int horfft_inv_grad<double>(lsb_ctx*, const HorPlan&, const double* const*, double* const*, double* const*,
-                                     double* const*, int, hipStream_t);
+                                     double* const*, int, const double*, hipStream_t);
 template int horfft_inv_grad<float>(lsb_ctx*, const HorPlan&, const double* const*, float* const*, float* const*,
-                                    float* const*, int, hipStream_t);
+                                    float* const*, int, const double*, hipStream_t);
 
 // Mixed form: FP32 physical fields, FP64 spectra, FP32 arithmetic.
 template <class TO>

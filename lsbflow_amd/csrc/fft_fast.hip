@@ -391,7 +391,8 @@ template <class R, class TI, class TO, int N1, int N2, int C>
 __global__ void __launch_bounds__(C * N2) k_yinv3_fast(FieldList<TI> in, FieldList<TO> out, int nplanes, int lh,
                                                        int ph, const cpx<R>* __restrict__ tw,
                                                        const double* __restrict__ wavx,
-                                                       const double* __restrict__ wavy) {
+                                                       const double* __restrict__ wavy,
+                                                       const double* __restrict__ ftab) {
     constexpr int TS = N1 * (N2 + 1) + 1;
     __shared__ cpx<R> T[C * TS];
     const int tid = threadIdx.x;
@@ -418,6 +419,14 @@ __global__ void __launch_bounds__(C * N2) k_yinv3_fast(FieldList<TI> in, FieldLi
             const TI* q = src + 2 * ((long)(N2 * n1 + n2) * lh + m);
             re[n1] = (double)q[0];
             im[n1] = (double)q[1];
+            if (ftab) {
+                // the spectral filter as a real factor per mode: the reference's
+                // magnitude/phase rebuild (specfilter_mode) equals it up to the rounding
+                // of re/|u|, i.e. ~eps |re|^2/|im| on modes with |im| << |re|
+                const double fac = ftab[(N2 * n1 + n2) * lh + m];
+                re[n1] *= fac;
+                im[n1] *= fac;
+            }
         }
     }
     cpx<R>* Tc = T + c * TS;
@@ -554,14 +563,14 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 
 template <class R, class TI, class TO>
 int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
-               const cpx<R>* tw, const double* wavx, const double* wavy, hipStream_t s) {
+               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s) {
     constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nsrc;
     switch (ny) {
 #define X(LL, A, B)                                                                                           \
     case LL:                                                                                                  \
         hipLaunchKernelGGL((k_yinv3_fast<R, TI, TO, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, in, out, \
-                           nplanes, lh, ph, tw, wavx, wavy);                                                  \
+                           nplanes, lh, ph, tw, wavx, wavy, ftab);                                            \
         break;
         LSB_FAST_CASES(X)
 #undef X
@@ -581,7 +590,7 @@ int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, i
     template int fast_yinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
                                       const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t); \
     template int fast_yinv3<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,       \
-                                       const cpx<R>*, const double*, const double*, hipStream_t);
+                                       const cpx<R>*, const double*, const double*, const double*, hipStream_t);
 
 LSB_INST_FAST(double, double, double)
 LSB_INST_FAST(float, float, float)    // mixed: x passes (physical <-> compact staging)

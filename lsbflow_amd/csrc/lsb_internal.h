@@ -216,7 +216,7 @@ int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, dou
 // u, du/dx, du/dy of 3 spectra (one load each); RT = physical storage and arithmetic type
 template <class RT>
 int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
-                    RT* const* dudy, int nplanes, hipStream_t s);
+                    RT* const* dudy, int nplanes, const double* ftab, hipStream_t s);
 // mixed form: FP32 physical fields, FP64 spectra
 template <class TO>
 int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO* const* spec, int nf, int nplanes,
@@ -224,6 +224,26 @@ int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO
 template <class TI>
 int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,
                      const int* ops, hipStream_t s);
+
+// filters.f90:281-335 specfilter_exp_xy on one mode: rebuilt from its magnitude and
+// phase, fmag*filt*(cos(phase), sin(phase)) with cos(phase) = re/fmag and the phase
+// signed by im; cos(acos(c)) = c and |sin(acos(c))| = sqrt((1-c)(1+c)) are the same
+// functions of the rounded c, so this keeps the reference's loss of Im parts that are
+// tiny next to Re without the transcendental round trip.
+__host__ __device__ inline void specfilter_mode(double& re, double& im, double filt) {
+    const double fmag = sqrt(re * re + im * im);
+    if (fmag == 0.0) {
+        re = 0.0;
+        im = 0.0;
+        return;
+    }
+    const double cosc = re / fmag;
+    double sn = sqrt(fmax((1.0 - cosc) * (1.0 + cosc), 0.0));
+    if (im < 0.0) sn = -sn;
+    const double fmagf = fmag * filt;
+    re = fmagf * cosc;
+    im = fmagf * sn;
+}
 
 // register-resident power-of-two passes (fft_fast.hip); R arithmetic, TI/TO
 // element types of input and output
@@ -245,9 +265,10 @@ template <class R, class TI, class TO>
 int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
               const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s);
 // nsrc spectra -> 3 nsrc outputs (u, du/dx, du/dy per source), one load per source
+// ftab (lh*ny factors, index m + lh*ky) applies the spectral filter on load; null: none
 template <class R, class TI, class TO>
 int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
-               const cpx<R>* tw, const double* wavx, const double* wavy, hipStream_t s);
+               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s);
 
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).

@@ -41,6 +41,7 @@ int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s);
 int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s);
 int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, double* u, double* v, double dt,
                double bdfv, hipStream_t s);
+const double* sp_filter_table(lsb_ctx* c, double kc, double p, double amp, int filtery, hipStream_t s);
 int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,
                   int filtery, hipStream_t s);
 

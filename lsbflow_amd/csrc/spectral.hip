@@ -95,12 +95,8 @@ __global__ void k_filt_table(SpDims d, const double* __restrict__ wavx, const do
     tab[t] = filt;
 }
 
-// filters.f90:281-335 specfilter_exp_xy: the mode is rebuilt from its magnitude and
-// phase, fmag*filt*(cos(phase), sin(phase)) with cos(phase) = fr/fmag and phase
-// signed by fi.  cos(acos(c)) = c and |sin(acos(c))| = sqrt((1-c)(1+c)) are the
-// same functions of the rounded c = fr/fmag, so the form keeps the reference's
-// loss of Im parts that are tiny next to Re (c rounds to +-1) without the
-// transcendental round trip.  One plane per blockIdx.y, no 64-bit index division.
+// filters.f90:281-335 specfilter_exp_xy (specfilter_mode per element).  One plane
+// per blockIdx.y, no 64-bit index division.
 __global__ void k_specfilter(SpDims d, const double* __restrict__ tab, FieldList<double> in, FieldList<double> out,
                              int nzp) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,16 +104,9 @@ __global__ void k_specfilter(SpDims d, const double* __restrict__ tab, FieldList
     const int f = blockIdx.y / nzp;
     const long q = (long)(blockIdx.y - f * nzp) * d.plane_pairs + t;
     const c2 u = reinterpret_cast<const c2*>(in.p[f])[q];
-    const double fmag = sqrt(u.x * u.x + u.y * u.y);
-    c2 r = make_double2(0.0, 0.0);
-    if (fmag != 0.0) {
-        const double cosc = u.x / fmag;
-        double sn = sqrt(fmax((1.0 - cosc) * (1.0 + cosc), 0.0));
-        if (u.y < 0.0) sn = -sn;
-        const double fmagf = fmag * tab[t];
-        r = make_double2(fmagf * cosc, fmagf * sn);
-    }
-    reinterpret_cast<c2*>(out.p[f])[q] = r;
+    double re = u.x, im = u.y;
+    specfilter_mode(re, im, tab[t]);
+    reinterpret_cast<c2*>(out.p[f])[q] = make_double2(re, im);
 }
 
 // horfft.f90:153-224 norm on the kx = 0 column and the ky-Nyquist row.
@@ -190,27 +179,38 @@ int sp_lapl_combine(lsb_ctx* c, const double* d2u, const double* u, double* out,
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
-int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,
-                  int filtery, hipStream_t s) {
+// Factors exp(-amp theta^(2p)) of one plane into c->d_ftab (returned).
+const double* sp_filter_table(lsb_ctx* c, double kc, double p, double amp, int filtery, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
     const double wavxmax = c->nxh * c->alpha;
     const double wavymax = c->nyh * c->beta;
     double akmax = wavxmax;
     if (filtery == 1) akmax = sqrt(wavxmax * wavxmax + wavymax * wavymax);
+    if (!c->d_ftab && dev_alloc((void**)&c->d_ftab, (size_t)d.plane_pairs * sizeof(double)) != LSB_OK) return nullptr;
+    const unsigned nb = (unsigned)((d.plane_pairs + 255) / 256);
+    hipLaunchKernelGGL(k_filt_table, dim3(nb), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, c->d_ftab, akmax, kc, p, amp,
+                       filtery);
+    if (hipGetLastError() != hipSuccess) return nullptr;
+    return c->d_ftab;
+}
+
+int sp_specfilter(lsb_ctx* c, const double* const* in, double* const* out, int nf, double kc, double p, double amp,
+                  int filtery, hipStream_t s) {
+    SpDims d = sp_dims(c, c->nzp);
     FieldList<double> fi{}, fo{};
     for (int f = 0; f < nf; ++f) {
         fi.p[f] = const_cast<double*>(in[f]);
         fo.p[f] = out[f];
     }
-    if (!c->d_ftab) LSB_TRY(dev_alloc((void**)&c->d_ftab, (size_t)d.plane_pairs * sizeof(double)));
-    const unsigned nb = (unsigned)((d.plane_pairs + 255) / 256);
     int tok = -1;
     prof_mark_begin(c, PC_PW, s, &tok, 0.0, 2.0 * (double)d.npairs * nf * 2 * sizeof(double));
-    hipLaunchKernelGGL(k_filt_table, dim3(nb), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, c->d_ftab, akmax, kc, p, amp,
-                       filtery);
-    LSB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_specfilter, dim3(nb, (unsigned)(c->nzp * nf)), dim3(256), 0, s, d, c->d_ftab, fi, fo,
-                       c->nzp);
+    const double* tab = sp_filter_table(c, kc, p, amp, filtery, s);
+    if (!tab) {
+        set_error("specfilter: table allocation/launch failed");
+        return LSB_E_HIP;
+    }
+    const unsigned nb = (unsigned)((d.plane_pairs + 255) / 256);
+    hipLaunchKernelGGL(k_specfilter, dim3(nb, (unsigned)(c->nzp * nf)), dim3(256), 0, s, d, tab, fi, fo, c->nzp);
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;

@@ -414,7 +414,7 @@ int isc_full(lsb_ctx* c, double* ui, double* tau, double* work, hipStream_t s) {
 // N_l written to nlt (3 spectral).  Work comes from the step layout.
 // FP64 physical chain: inverse transforms, products + SGS model, forward transforms.
 static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
-                       bool ops_on_load, hipStream_t s) {
+                       bool ops_on_load, const double* ftab, hipStream_t s) {
     StepLayout& L = S->L;
     // inverse FFTs: ui (3) + g (9).  d/dz acts along pencils and commutes with the
     // horizontal transform, so with the register-resident FFT the z derivatives are
@@ -428,7 +428,7 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
             gy[l] = L.phys[P_G0 + l + 3];
             gz[l] = L.phys[P_G0 + l + 6];
         }
-        LSB_TRY(horfft_inv_grad<double>(c, c->hp, uif, u, gx, gy, c->nzp, s));
+        LSB_TRY(horfft_inv_grad<double>(c, c->hp, uif, u, gx, gy, c->nzp, ftab, s));
         const double* uc[3] = {u[0], u[1], u[2]};
         LSB_TRY(zcontract_ex(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uc, gz, 3, pphys, nullptr, s));
     } else {
@@ -557,7 +557,7 @@ static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED;
 // rounding); that part lives only on the (kx, ky) = (0, 0) mode, where h_00 enters
 // N only through i kx h_00 = 0, so it is not restored.
 static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3],
-                             double* hf[9], hipStream_t s) {
+                             double* hf[9], const double* ftab, hipStream_t s) {
     StepLayout& L = S->L;
     const bool sgs = c->cfg.sgsflag == 1;
     const int model = c->cfg.sgsmodel;
@@ -575,7 +575,7 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
             gx[l] = ph(P_G0 + l);          // (l, x)
             gy[l] = ph(P_G0 + l + 3);      // (l, y)
         }
-        LSB_TRY(horfft_inv_grad<float>(c, c->hp, uif, u, gx, gy, c->nzp, s));
+        LSB_TRY(horfft_inv_grad<float>(c, c->hp, uif, u, gx, gy, c->nzp, ftab, s));
         // d/dz of the physical velocity (commutes with the horizontal transform)
         const float* uc[3] = {u[0], u[1], u[2]};
         float* gz[3] = {ph(P_G0 + 6), ph(P_G0 + 7), ph(P_G0 + 8)};
@@ -634,8 +634,10 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
 // ------------------------------------------------------- nonlinear term
 // navierstokes.f90:658-795 for T = 0: uif (3 spectral, already filtered) ->
 // N_l written to nlt (3 spectral).  Work comes from the step layout.
+// ftab (fast transform sizes only): the spectral filter's factors, applied to uif as the
+// inverse transform loads it; null when uif is used as given.
 static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
-                       hipStream_t s) {
+                       const double* ftab, hipStream_t s) {
     StepLayout& L = S->L;
     // With the register-resident FFT, d/dx and d/dy are applied as the inverse transform
     // loads uif and d/dz is taken in physical space; otherwise all three are spectral passes
@@ -652,8 +654,12 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         double* dz[3] = {L.spec[S_DZ0], L.spec[S_DZ0 + 1], L.spec[S_DZ0 + 2]};
         LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, uif, dz, 3, s));
     }
-    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, s);
-    return nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, s);
+    if (ftab && !ops_on_load) {
+        set_error("nlterm: filter-on-load needs the register-resident FFT sizes");
+        return LSB_E_ARG;
+    }
+    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, ftab, s);
+    return nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, ftab, s);
 }
 
 static void hf_view(StepState* S, bool sym, double* hf[9]) {
@@ -674,7 +680,7 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
     double* hf[9];
     hf_view(S, sym, hf);
     double* nltf[3] = {S->L.spec[S_NLTF0], S->L.spec[S_NLTF0 + 1], S->L.spec[S_NLTF0 + 2]};
-    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, s));
+    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, nullptr, s));
     // N_l = nltf - dfdx(h_l0) - dfdy(h_l1) (nltf holds N' = nlt - dh_lz/dz): reuse the rhs kernel with zero-weighted
     // history (finite FP64 fields) and its right-hand side written to the x-derivative
     // slots, which the inverse transform has consumed (the step's rotating slots stay intact)
@@ -756,8 +762,18 @@ static int update_once(lsb_ctx* c) {
     const int istep = c->istep;
     const double dt = c->dt;
     // velocities used for the nonlinear term (navierstokes.f90:1213-1229)
+    // (filterqspec, navierstokes.f90:1206-1209): with the register-resident FFT the filter
+    // is applied as the fused inverse transform loads u; otherwise as its own pass
     const double* uif[3];
-    if (g.dealiasflag >= 2) {
+    const double* ftab = nullptr;
+    if (g.dealiasflag >= 2 && horfft_has_ops(c->hp)) {
+        ftab = sp_filter_table(c, g.kc, g.filt_order, g.filt_amp, 1, s);
+        if (!ftab) {
+            set_error("filterqspec: filter table");
+            return LSB_E_HIP;
+        }
+        for (int l = 0; l < 3; ++l) uif[l] = L.spec[S->u[l]];
+    } else if (g.dealiasflag >= 2) {
         const double* in[3] = {L.spec[S->u[0]], L.spec[S->u[1]], L.spec[S->u[2]]};
         double* out[3] = {L.spec[S_UNL0], L.spec[S_UNL0 + 1], L.spec[S_UNL0 + 2]};
         LSB_TRY(sp_specfilter(c, in, out, 3, g.kc, g.filt_order, g.filt_amp, 1, s));
@@ -769,7 +785,7 @@ static int update_once(lsb_ctx* c) {
     double* hf[9];
     hf_view(S, sym, hf);
     double* nltf[3] = {L.spec[S_NLTF0], L.spec[S_NLTF0 + 1], L.spec[S_NLTF0 + 2]};
-    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, s));
+    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, ftab, s));
     // the unfiltered velocity is u^n for dealiasflag 0..2 (navierstokes.f90:1225-1229)
     const double* un_[3];
     for (int l = 0; l < 3; ++l) un_[l] = (g.dealiasflag > 2) ? uif[l] : L.spec[S->u[l]];
