@@ -116,14 +116,18 @@ __device__ __forceinline__ void f_dft(cpx<R>* x) {
 }
 
 // stage 1 twiddle: v[k1] *= W_L^(n2 k1) with W_L = exp(S 2 pi i / L), table tw[m] = exp(-2 pi i m/L)
+// (n2 = 0 multiplies by tw[0] = 1 exactly: no branch, so the N1 - 1 loads issue
+// together instead of each waiting inside a divergent region)
 template <class R, int S, int N1>
 __device__ __forceinline__ void f_twiddle(cpx<R>* v, int n2, const cpx<R>* __restrict__ tw) {
-    if (n2 == 0) return;
+    cpx<R> w[N1];
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) w[k1] = tw[n2 * k1];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) {
-        cpx<R> w = tw[n2 * k1];
-        if (S > 0) w.im = -w.im;
-        v[k1] = cmul(v[k1], w);
+        if (S > 0) w[k1].im = -w[k1].im;
+        v[k1] = cmul(v[k1], w[k1]);
     }
 }
 
@@ -169,18 +173,16 @@ __global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<TI> in, FieldList<T
     const int tid = threadIdx.x;
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
-    const bool valid = r < (long)nrows * nf;
-    const TI* src = nullptr;
-    TO* dst = nullptr;
-    if (valid) {
-        const int f = (int)(r / nrows);
-        const long rr = r - (long)f * nrows;
-        src = in.p[f] + rr * istride;
-        dst = out.p[f] + rr * ostride;
-    }
+    const long rlast = (long)nrows * nf - 1;
+    const bool valid = r <= rlast;
+    const long rc = valid ? r : rlast;  // lines past the end read the last line, store nothing
+    const int f = (int)(rc / nrows);
+    const long rr = rc - (long)f * nrows;
+    const TI* src = in.p[f] + rr * istride;
+    TO* dst = out.p[f] + rr * ostride;
     cpx<R> v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? ld_c<R>(src, N2 * n1 + n2) : cpx<R>{0, 0};
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ld_c<R>(src, N2 * n1 + n2);
     f_dft<R, -1, N1>(v);
     f_twiddle<R, -1, N1>(v, n2, tw);
     cpx<R>* Tl = T + l * TS;
@@ -229,20 +231,23 @@ __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<TI> in, FieldList<T
     const int tid = threadIdx.x;
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
-    const bool valid = r < (long)nrows * nf;
-    const TI* rin = nullptr;
-    TO* row = nullptr;
-    if (valid) {
-        const int f = (int)(r / nrows);
-        const long rr = r - (long)f * nrows;
-        rin = in.p[f] + rr * istride;
-        row = out.p[f] + rr * nxpp;
-    }
+    const long rlast = (long)nrows * nf - 1;
+    const bool valid = r <= rlast;
+    // lines past the end read the last line (unconditional loads issue back to
+    // back; a load under a per-element branch feeding the LDS store would wait
+    // for each in turn) and store nothing
+    const long rc = valid ? r : rlast;
+    const int f = (int)(rc / nrows);
+    const long rr = rc - (long)f * nrows;
+    const TI* rin = in.p[f] + rr * istride;
+    TO* row = out.p[f] + rr * nxpp;
     cpx<R>* Tl = T + l * TS;
+    {
+        cpx<R> t[N1];
 #pragma unroll
-    for (int j = 0; j < N1; ++j) {
-        const int k = n2 + N2 * j;
-        Tl[k] = valid ? ld_c<R>(rin, k) : cpx<R>{0, 0};
+        for (int j = 0; j < N1; ++j) t[j] = ld_c<R>(rin, n2 + N2 * j);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) Tl[n2 + N2 * j] = t[j];
     }
     __syncthreads();
     // Z'_n = (X_n + conj X_{L-n}) + i W^-n (X_n - conj X_{L-n}); X_L = 0, Im X_0 ignored
@@ -303,9 +308,10 @@ __global__ void __launch_bounds__(C * N2) k_yfwd_fast(FieldList<TI> in, FieldLis
     TO* dst = out.p[f] + pk * (long)(N1 * N2) * (2 * lh);
     const int m = tile * C + c;
     const bool valid = m < lh;
+    const int mc = valid ? m : lh - 1;  // columns past the end read the last one, store nothing
     cpx<R> v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = valid ? ld_c<R>(src, (long)(N2 * n1 + n2) * lh + m) : cpx<R>{0, 0};
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = ld_c<R>(src, (long)(N2 * n1 + n2) * lh + mc);
     f_dft<R, -1, N1>(v);
     f_twiddle<R, -1, N1>(v, n2, tw);
     cpx<R>* Tc = T + c * TS;
@@ -347,20 +353,25 @@ __global__ void __launch_bounds__(C * N2) k_yinv_fast(FieldList<TI> in, FieldLis
     TO* dst = out.p[f] + pk * (long)(N1 * N2) * (2 * ph);
     const int m = tile * C + c;
     const bool valid = m < lh;
+    const int mc = valid ? m : lh - 1;  // columns past the end read the last one, store nothing
     const int op = ops.op[f];
-    const double kx = (op == FOP_DX && valid) ? wavx[2 * m] : 0.0;
-    cpx<R> v[N1];
+    TI qre[N1], qim[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-        const int n = N2 * n1 + n2;
-        if (!valid) {
-            v[n1] = {0, 0};
-        } else if (op == FOP_NONE) {
-            v[n1] = ld_c<R>(src, (long)n * lh + m);
-        } else {
-            const TI* q = src + 2 * ((long)n * lh + m);
-            const double k = (op == FOP_DX) ? kx : wavy[n];
-            const double re = (double)q[0], im = (double)q[1];
+        const TI* q = src + 2 * ((long)(N2 * n1 + n2) * lh + mc);
+        qre[n1] = q[0];
+        qim[n1] = q[1];
+    }
+    cpx<R> v[N1];
+    if (op == FOP_NONE) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = {(R)qre[n1], (R)qim[n1]};
+    } else {
+        const double kx = (op == FOP_DX) ? wavx[2 * mc] : 0.0;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const double k = (op == FOP_DX) ? kx : wavy[N2 * n1 + n2];
+            const double re = (double)qre[n1], im = (double)qim[n1];
             v[n1] = {(R)(-k * im), (R)(k * re)};
         }
     }
@@ -409,24 +420,27 @@ __global__ void __launch_bounds__(C * N2) k_yinv3_fast(FieldList<TI> in, FieldLi
     TO* dy = out.p[3 * f + 2] + ooff;
     const int m = tile * C + c;
     const bool valid = m < lh;
-    const double kx = valid ? wavx[2 * m] : 0.0;
+    const int mc = valid ? m : lh - 1;  // columns past the end read the last one, store nothing
+    const double kx = wavx[2 * mc];
     double re[N1], im[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-        re[n1] = 0.0;
-        im[n1] = 0.0;
-        if (valid) {
-            const TI* q = src + 2 * ((long)(N2 * n1 + n2) * lh + m);
-            re[n1] = (double)q[0];
-            im[n1] = (double)q[1];
-            if (ftab) {
-                // the spectral filter as a real factor per mode: the reference's
-                // magnitude/phase rebuild (specfilter_mode) equals it up to the rounding
-                // of re/|u|, i.e. ~eps |re|^2/|im| on modes with |im| << |re|
-                const double fac = ftab[(N2 * n1 + n2) * lh + m];
-                re[n1] *= fac;
-                im[n1] *= fac;
-            }
+        const TI* q = src + 2 * ((long)(N2 * n1 + n2) * lh + mc);
+        re[n1] = (double)q[0];
+        im[n1] = (double)q[1];
+    }
+    if (ftab) {
+        // the spectral filter as a real factor per mode: the reference's
+        // magnitude/phase rebuild (specfilter_mode) equals it up to the rounding
+        // of re/|u|, i.e. ~eps |re|^2/|im| on modes with |im| << |re|
+        double fac[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) fac[n1] = ftab[(N2 * n1 + n2) * lh + mc];
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their uses
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            re[n1] *= fac[n1];
+            im[n1] *= fac[n1];
         }
     }
     cpx<R>* Tc = T + c * TS;
@@ -485,6 +499,7 @@ template <class R, class TI, class TO>
 int fast_xfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride,
               int ostride, const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
     const long rows = (long)nrows * nf;
+    if (rows <= 0) return LSB_OK;
     switch (lh) {
 #define X(LL, A, B)                                                                                                 \
     case LL:                                                                                                        \
@@ -503,6 +518,7 @@ template <class R, class TI, class TO>
 int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int nf, int lh, int istride, int nxpp,
               const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
     const long rows = (long)nrows * nf;
+    if (rows <= 0) return LSB_OK;
     switch (lh) {
 #define X(LL, A, B)                                                                                                 \
     case LL:                                                                                                        \
@@ -528,6 +544,7 @@ int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
               const cpx<R>* tw, R scale, hipStream_t s) {
     constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
+    if (nblk <= 0) return LSB_OK;
     switch (ny) {
 #define X(LL, A, B)                                                                                          \
     case LL:                                                                                                 \
@@ -547,6 +564,7 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
               const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s) {
     constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
+    if (nblk <= 0) return LSB_OK;
     switch (ny) {
 #define X(LL, A, B)                                                                                          \
     case LL:                                                                                                 \
@@ -566,6 +584,7 @@ int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, i
                const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s) {
     constexpr int C = ytile_cols<R>();
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nsrc;
+    if (nblk <= 0) return LSB_OK;
     switch (ny) {
 #define X(LL, A, B)                                                                                           \
     case LL:                                                                                                  \

@@ -154,10 +154,38 @@ k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<doub
             }
         }
     }
+    constexpr bool kAux = (MODE == ZM_PRESS || MODE == ZM_AUXSUB);
+    const double* __restrict__ auxf = kAux ? bct.p[f] : nullptr;
     double* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
         if (i < myn) {
+            // every operand of the row block is loaded before its first store: the
+            // compiler cannot prove the output disjoint from them, and a load placed
+            // after a store would wait for its own round trip
+            double ep[4][5], ax[4][CB];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rc = min((rb0 + i) * 16 + lk + 4 * v, a.nout - 1);
+                if (MODE == ZM_HELM1) {
+                    ep[v][0] = a.lam[rc];
+                    if (nhom) {
+                        const double* e = a.elift;
+                        const int n = a.nout;
+                        ep[v][1] = e[rc];
+                        ep[v][2] = e[n + rc];
+                        ep[v][3] = e[2 * n + rc];
+                        ep[v][4] = e[3 * n + rc];
+                    }
+                }
+                if (MODE == ZM_HELM2 && nhom) {
+                    ep[v][0] = a.lift[rc + 1];
+                    ep[v][1] = a.lift[a.nzp + rc + 1];
+                }
+                if (kAux)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) ax[v][j] = auxf[(long)rc * P + cj[j]];
+            }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = (rb0 + i) * 16 + lk + 4 * v;
@@ -170,34 +198,32 @@ k_zcontract(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<doub
                             if (nhom) {
                                 // Einv (q - L(h_bot) bc_bot - L(h_top) bc_top), L(h) = h'' + dfac h,
                                 // with Einv h and Einv h'' precomputed per interior row
-                                const double* e = a.elift;
-                                const int n = a.nout;
-                                x = x - vbb[j] * (e[2 * n + row] + dfac[j] * e[row]);
-                                x = x - vbt[j] * (e[3 * n + row] + dfac[j] * e[n + row]);
+                                x = x - vbb[j] * (ep[v][3] + dfac[j] * ep[v][1]);
+                                x = x - vbt[j] * (ep[v][4] + dfac[j] * ep[v][2]);
                             }
-                            x = x / (a.lam[row] + dfac[j]);
+                            x = x / (ep[v][0] + dfac[j]);
                         }
-                        if (MODE == ZM_HELM2 && nhom) {
-                            const int k = row + 1;
-                            x = x + a.lift[k] * vbb[j] + a.lift[a.nzp + k] * vbt[j];
-                        }
-                        if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)row * P + cj[j]];
-                        if (MODE == ZM_AUXSUB) x = bct.p[f][(long)row * P + cj[j]] - x;
+                        if (MODE == ZM_HELM2 && nhom) x = x + ep[v][0] * vbb[j] + ep[v][1] * vbt[j];
+                        if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[v][j];
+                        if (MODE == ZM_AUXSUB) x = ax[v][j] - x;
                         if (col < P) dst[(long)row * P + col] = x;
                     }
                 }
             }
         }
     }
-    if (dox && (MODE == ZM_PLAIN || MODE == ZM_PRESS || MODE == ZM_AUXSUB)) {
+    if (dox && (MODE == ZM_PLAIN || kAux)) {
+        double ax[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) ax[j] = kAux ? auxf[(long)a.xrow * P + cj[j]] : 0.0;
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
             double x = xacc[j];
             x += __shfl_xor(x, 16);
             x += __shfl_xor(x, 32);
             const long col = p0 + 16 * j + li;
-            if (MODE == ZM_PRESS) x = -x + dfac[j] * bct.p[f][(long)a.xrow * P + cj[j]];
-            if (MODE == ZM_AUXSUB) x = bct.p[f][(long)a.xrow * P + cj[j]] - x;
+            if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[j];
+            if (MODE == ZM_AUXSUB) x = ax[j] - x;
             if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = x;
         }
     }
@@ -323,10 +349,21 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
             }
         }
     }
+    const TO* __restrict__ auxf = AUX ? aux.p[f] : nullptr;
     TO* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
         if (i < myn) {
+            // the row block's aux operands are loaded before its first store (see the
+            // FP64 kernel)
+            float ax[4][CB];
+            if (AUX)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rc = min((rb0 + i) * 16 + Z32_ROW(lk, v), a.nout - 1);
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) ax[v][j] = (float)auxf[(long)rc * P + cj[j]];
+                }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = (rb0 + i) * 16 + Z32_ROW(lk, v);
@@ -335,7 +372,7 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
                     for (int j = 0; j < CB; ++j) {
                         const long col = p0 + 16 * j + li;
                         float x = acc[i][j][v];
-                        if (AUX) x = (float)aux.p[f][(long)row * P + cj[j]] - x;
+                        if (AUX) x = ax[v][j] - x;
                         if (col < P) dst[(long)row * P + col] = (TO)x;
                     }
                 }
@@ -343,13 +380,16 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
         }
     }
     if (dox) {
+        float ax[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) ax[j] = AUX ? (float)auxf[(long)a.xrow * P + cj[j]] : 0.f;
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
             float x = xacc[j];
             x += __shfl_xor(x, 16);
             x += __shfl_xor(x, 32);
             const long col = p0 + 16 * j + li;
-            if (AUX) x = (float)aux.p[f][(long)a.xrow * P + cj[j]] - x;
+            if (AUX) x = ax[j] - x;
             if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = (TO)x;
         }
     }
