@@ -135,6 +135,7 @@ int upload_context(lsb_ctx* c) {
     }
     lift.insert(lift.end(), elift.begin(), elift.end());
     LSB_TRY(put(&c->d_lift, lift));
+    LSB_TRY(zsplit_setup(c, elift));
     std::vector<double> par;
     par.insert(par.end(), c->ubar.begin(), c->ubar.end());
     par.insert(par.end(), c->ubarp.begin(), c->ubarp.end());
@@ -261,6 +262,10 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     for (double* p : ds)
         if (p) (void)hipFree(p);
     if (c->d_walls) (void)hipFree(c->d_walls);
+    for (double* p : {c->zs_dz.d_m, c->zs_h1.d_m, c->zs_h2.d_m, c->d_helm_s})
+        if (p) (void)hipFree(p);
+    for (float* p : {c->zs_dz.d_m32, c->zs_h1.d_m32, c->zs_h2.d_m32})
+        if (p) (void)hipFree(p);
     for (void* p : c->stage)
         if (p) (void)hipFree(p);
     if (c->st) (void)hipFree(c->st);

@@ -80,6 +80,17 @@ struct HorPlan {
 };
 
 // A list of field base pointers passed by value to batched kernels.
+// A z operator split by reflection parity for a symmetric grid (zops.hip):
+// P and Q, me x me each, column-major with leading dimension ldm, Q at d_m + qoff.
+struct ZSplitOp {
+    bool ok = false;
+    int n = 0, me = 0, ldm = 0;
+    double osgn = 1.0;
+    double* d_m = nullptr;
+    float* d_m32 = nullptr;   // the same block in FP32 (FP32 split kernel)
+    size_t qoff = 0;
+};
+
 constexpr int kMaxFields = 16;
 // leading dimension of the two-row boundary operator d_md2b (rows 0 and nzp-1 of d2fdz2)
 constexpr int kLdmBnd = 64;
@@ -140,6 +151,11 @@ struct lsb_ctx {
     double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
     int* d_walls = nullptr;     // wall planes of the static model's utau average
     int nwalls = 0;
+    // parity-split forms of dfdz (d_mdz) and of the two Helmholtz projections, with
+    // the eigenvalues and Einv-lifting data in parity-ordered mode rows (d_helm_s:
+    // nzm, then 4*nzm); not built (ok = false) on a grid without the symmetry
+    lsb::ZSplitOp zs_dz, zs_h1, zs_h2;
+    double* d_helm_s = nullptr;
     float* d_mdz32 = nullptr;
     float* d_smat32 = nullptr;
     float* d_simat32 = nullptr;
@@ -193,6 +209,7 @@ int dev_alloc(void** p, size_t bytes);
 int upload(void* dst, const void* src, size_t bytes, hipStream_t s);
 int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p);
 int upload_context(lsb_ctx* c);
+int zsplit_setup(lsb_ctx* c, const std::vector<double>& elift);
 
 // profiling helpers
 // profiling classes: 2-D FFTs, Chebyshev contractions, Helmholtz projections, the RHS

@@ -25,6 +25,11 @@
 
 #include "lsb_internal.h"
 
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
 namespace lsb {
 
 // ZM_PRESS: out = -acc - (kx^2 + ky^2) aux; ZM_AUXSUB: out = aux - acc (aux a field at the
@@ -49,6 +54,13 @@ struct ZArgs {
     const double* lift;   // hbot, htop, d2hbot, d2htop (4*nzp)
     const double* elift;  // Einv applied to interior hbot, htop, d2hbot, d2htop (4*nzm)
     int nzp;
+    // parity-split contractions: M holds P, M2 holds Q; nsp rows in the physical
+    // (or parity-ordered mode) layout; osgn the reflection sign of the output
+    const double* M2;
+    int nsp;
+    double osgn;
+    const float* M32;   // FP32 copy of [P | Q] for the FP32 split kernel, Q at M32 + q32
+    long q32;
 };
 
 constexpr int kZRB = 3;   // row blocks (of 16) per wave
@@ -395,6 +407,614 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
     }
 }
 
+// ------------------------------------------------------------ parity split
+// On a grid symmetric about the mid-plane (Gauss-Lobatto points under mapping 0
+// or 3) the z operators are centro-symmetric or centro-antisymmetric
+// (M[n-1-i][n-1-j] = sgn M[i][j]: d2 and E, Einv by modes; d with sgn = -1) and
+// the Helmholtz eigenvectors are each even or odd.  With s_j = x_j + x_{n-1-j},
+// d_j = x_j - x_{n-1-j} (j < me = (n+1)/2; the middle point of odd n gives
+// s = 2 x_m, d = 0):
+//   E_i = sum_j P[i][j] s_j,  O_i = sum_j Q[i][j] d_j,
+//   P[i][j] = (M[i][j] + M[i][n-1-j]) / 2,  Q[i][j] = (M[i][j] - M[i][n-1-j]) / 2,
+//   y_i = E_i + O_i,  y_{n-1-i} = sgn (E_i - O_i)          (i < me)
+// -- two half-size products, half the matrix-core work of the dense one.  The
+// Helmholtz mode space is kept in parity order (even mode a in row a, odd mode b
+// in row n-1-b): stage 1 (Einv) folds its input and writes E_i, O_i to those rows
+// unmixed; stage 2 (E) reads them unmixed and unfolds into physical rows.  A wave
+// owns the same rows of both halves (two accumulator sets, two column blocks).
+constexpr int kZSCB = 2;
+constexpr int kZSRB = 3;  // FP64 row blocks per wave (two 3 x 2 tile sets, 96 accumulator VGPRs)
+
+// element of a field or matrix at a 32-bit element offset from a wave-uniform base
+// (the address is base + zero-extended byte offset: the load takes the base in
+// scalar registers)
+template <class T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, unsigned eoff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(eoff * (unsigned)sizeof(T)));
+}
+
+template <int MODE, bool XR, int RB = kZSRB, int CB = kZSCB, int PD = kZPD, int WPE = 3>
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double> bct, FieldList<double> bcb) {
+    constexpr bool kInMix = (MODE != ZM_HELM2);   // fold a physical input
+    constexpr bool kOutMix = (MODE != ZM_HELM1);  // unfold into physical rows
+    constexpr bool kAux = (MODE == ZM_PRESS || MODE == ZM_AUXSUB);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int f = blockIdx.z;
+    const int me = a.nout, n = a.nsp, mo = n - me;
+    const int nrb_all = XR ? (me >> 4) : ((me + 15) >> 4);
+    const int g0 = blockIdx.y * a.gblocks;
+    const int nrb = min(a.gblocks, nrb_all - g0);
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    if (myn <= 0) return;
+    const int rb0 = g0 + wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * (16 * CB);
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    // The k loop has no branches, so that the loads of k-step s + PD stay in flight
+    // while step s multiplies: the k-steps are padded to a multiple of PD (the matrix
+    // has zero columns up to a multiple of 16), all RB row blocks are multiplied (rows
+    // past the wave's share are zero or unused), the prefetch position is clamped
+    // (matrix: last padded step; panel: last k-step whose 4 rows fit), and the middle
+    // row (XR) is accumulated by every wave from per-lane loads.  Offsets are 32-bit
+    // element offsets (a field holds < 2^29 elements).
+    const double* __restrict__ X = in.p[f] + (long)a.in_row0 * P;
+    const double* __restrict__ Mp = a.M;
+    const double* __restrict__ Mq = a.M2;
+    const unsigned xstep = 4u * (unsigned)P, mstep = 4u * (unsigned)a.ldm;
+    unsigned ot[CB], ob[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        ot[j] = (unsigned)lk * (unsigned)P + (unsigned)cj[j];
+        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + (unsigned)cj[j];
+    }
+    const unsigned om = (unsigned)(rb0 * 16 + li + lk * a.ldm);
+    const unsigned omx = (unsigned)((XR ? a.xrow : 0) + lk * a.ldm);
+    const int nks = (me + 3) >> 2;
+    const int nkp = (nks + PD - 1) / PD * PD;
+    const int kmax = (n - 4) >> 2;
+    v4d accE[RB][CB], accO[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            accO[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
+    double xacc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) xacc[j] = 0.0;
+    double fa[PD][RB], fq[PD][RB], fx[PD][CB], fy[PD][CB], fxp[PD], fxq[PD];
+    auto load = [&](int s, int k) {
+        const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+        const unsigned xk = (unsigned)min(k, kmax) * xstep;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            fa[s][i] = ldo(Mp, om + mk + 16u * i);
+            fq[s][i] = ldo(Mq, om + mk + 16u * i);
+        }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            fx[s][j] = ldo(X, ot[j] + xk);
+            fy[s][j] = ldo(X, ob[j] - xk);
+        }
+        if (XR) {
+            fxp[s] = ldo(Mp, omx + mk);
+            fxq[s] = ldo(Mq, omx + mk);
+        }
+        // keeps the prefetch where it is written: without the asm the compiler turns
+        // the phi of the two loads (prologue / previous step) into one load at the
+        // use; without the scheduling barrier the loads sink below the next step's
+        // products
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+    };
+#pragma unroll
+    for (int s = 0; s < PD; ++s) load(s, s);
+    for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+        for (int s = 0; s < PD; ++s) {
+            double sv[CB], dv[CB];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                sv[j] = kInMix ? fx[s][j] + fy[s][j] : fx[s][j];
+                dv[j] = kInMix ? fx[s][j] - fy[s][j] : fy[s][j];
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                    accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                }
+            if (XR)
+#pragma unroll
+                for (int j = 0; j < CB; ++j) xacc[j] = fma(fxq[s], dv[j], fma(fxp[s], sv[j], xacc[j]));
+            load(s, k0 + s + PD);
+        }
+    }
+    const bool dox = XR && (g0 == 0) && (wave == 0);
+    // per-pencil epilogue data
+    double dfac[CB], vbt[CB], vbb[CB];
+    const bool nhom = (MODE == ZM_HELM1 || MODE == ZM_HELM2) && (bct.p[f] != nullptr);
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        dfac[j] = 0.0;
+        vbt[j] = 0.0;
+        vbb[j] = 0.0;
+        if (MODE != ZM_PLAIN) {
+            const int c = (int)cj[j];
+            const double wx = a.wavx[c % a.nx], wy = a.wavy[c / a.nx];
+            dfac[j] = -(wx * wx) - wy * wy - a.sigma;
+            if (nhom) {
+                vbt[j] = bct.p[f][c];
+                vbb[j] = bcb.p[f][c];
+            }
+        }
+    }
+    const double* __restrict__ auxf = kAux ? bct.p[f] : nullptr;
+    double* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
+    const double osgn = a.osgn;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                // the stored row of (row block i, element v) in this half: r or n-1-r
+                // (operands are gathered before the first store, as in k_zcontract)
+                double ep[4][5], ax[4][CB];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rb0 + i) * 16 + lk + 4 * v, me - 1);
+                    const int rc = half ? n - 1 - r : r;
+                    if (MODE == ZM_HELM1) {
+                        ep[v][0] = a.lam[rc];
+                        if (nhom) {
+                            ep[v][1] = a.elift[rc];
+                            ep[v][2] = a.elift[n + rc];
+                            ep[v][3] = a.elift[2 * n + rc];
+                            ep[v][4] = a.elift[3 * n + rc];
+                        }
+                    }
+                    if (MODE == ZM_HELM2 && nhom) {
+                        ep[v][0] = a.lift[rc + 1];
+                        ep[v][1] = a.lift[a.nzp + rc + 1];
+                    }
+                    if (kAux)
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) ax[v][j] = auxf[(long)rc * P + cj[j]];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + lk + 4 * v;
+                    // the bottom half holds rows r < mo only (the middle row of odd n,
+                    // and the even-only last mode, have no mirror)
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const long col = p0 + 16 * j + li;
+                            const double e = accE[i][j][v], o = accO[i][j][v];
+                            double x;
+                            if (kOutMix)
+                                x = half ? osgn * (e - o) : e + o;
+                            else
+                                x = half ? o : e;
+                            if (MODE == ZM_HELM1) {
+                                if (nhom) {
+                                    x = x - vbb[j] * (ep[v][3] + dfac[j] * ep[v][1]);
+                                    x = x - vbt[j] * (ep[v][4] + dfac[j] * ep[v][2]);
+                                }
+                                x = x / (ep[v][0] + dfac[j]);
+                            }
+                            if (MODE == ZM_HELM2 && nhom) x = x + ep[v][0] * vbb[j] + ep[v][1] * vbt[j];
+                            if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[v][j];
+                            if (MODE == ZM_AUXSUB) x = ax[v][j] - x;
+                            if (col < P) dst[(long)row * P + col] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (dox && (MODE == ZM_PLAIN || kAux)) {
+        double ax[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) ax[j] = kAux ? auxf[(long)a.xrow * P + cj[j]] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            double x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            const long col = p0 + 16 * j + li;
+            if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[j];
+            if (MODE == ZM_AUXSUB) x = ax[j] - x;
+            if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = x;
+        }
+    }
+    if (MODE == ZM_HELM2 && g0 == 0 && wave == 0 && lk == 0) {
+        // boundary planes 0 and nz of the full field (out_row0 == 1)
+        double* full = out.p[f];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const long col = p0 + 16 * j + li;
+            double v0 = 0.0, vn = 0.0;
+            if (nhom) {
+                v0 = 0.0 + a.lift[0] * vbb[j] + a.lift[a.nzp + 0] * vbt[j];
+                vn = 0.0 + a.lift[a.nzp - 1] * vbb[j] + a.lift[a.nzp + a.nzp - 1] * vbt[j];
+            }
+            if (col < P) {
+                full[col] = v0;
+                full[(long)(a.nzp - 1) * P + col] = vn;
+            }
+        }
+    }
+}
+
+// FP32 form of k_zsplit for the mixed nonlinear chain (folded physical input,
+// unfolded output; aux subtraction as in k_zcontract_f32), same branch-free
+// pipelined k loop; the matrix halves are read from their FP32 copies.
+template <class TI, class TO, bool AUX, bool XR, int RB = kZRB32, int CB = kZSCB, int PD = kZPD, int WPE = 3>
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zsplit_f32(ZArgs a, FieldList<TI> in, FieldList<TO> out, FieldList<TO> aux) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int f = blockIdx.z;
+    const int me = a.nout, n = a.nsp, mo = n - me;
+    const int nrb_all = XR ? (me >> 4) : ((me + 15) >> 4);
+    const int g0 = blockIdx.y * a.gblocks;
+    const int nrb = min(a.gblocks, nrb_all - g0);
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    if (myn <= 0) return;
+    const int rb0 = g0 + wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * (16 * CB);
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    const TI* __restrict__ X = in.p[f] + (long)a.in_row0 * P;
+    const float* __restrict__ Mp = a.M32;
+    const float* __restrict__ Mq = a.M32 + a.q32;
+    const unsigned xstep = 4u * (unsigned)P, mstep = 4u * (unsigned)a.ldm;
+    unsigned ot[CB], ob[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        ot[j] = (unsigned)lk * (unsigned)P + (unsigned)cj[j];
+        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + (unsigned)cj[j];
+    }
+    const unsigned om = (unsigned)(rb0 * 16 + li + lk * a.ldm);
+    const unsigned omx = (unsigned)((XR ? a.xrow : 0) + lk * a.ldm);
+    const int nks = (me + 3) >> 2;
+    const int nkp = (nks + PD - 1) / PD * PD;
+    const int kmax = (n - 4) >> 2;
+    v4f accE[RB][CB], accO[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+            accO[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        }
+    float xacc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) xacc[j] = 0.f;
+    float fa[PD][RB], fq[PD][RB], fxp[PD], fxq[PD];
+    TI fx[PD][CB], fy[PD][CB];
+    auto load = [&](int s, int k) {
+        const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+        const unsigned xk = (unsigned)min(k, kmax) * xstep;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            fa[s][i] = ldo(Mp, om + mk + 16u * i);
+            fq[s][i] = ldo(Mq, om + mk + 16u * i);
+        }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            fx[s][j] = ldo(X, ot[j] + xk);
+            fy[s][j] = ldo(X, ob[j] - xk);
+        }
+        if (XR) {
+            fxp[s] = ldo(Mp, omx + mk);
+            fxq[s] = ldo(Mq, omx + mk);
+        }
+        asm volatile("" ::: "memory");  // keep the prefetch in place (see k_zsplit)
+        __builtin_amdgcn_sched_barrier(0);
+    };
+#pragma unroll
+    for (int s = 0; s < PD; ++s) load(s, s);
+    for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+        for (int s = 0; s < PD; ++s) {
+            float sv[CB], dv[CB];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const float x0 = (float)fx[s][j], x1 = (float)fy[s][j];
+                sv[j] = x0 + x1;
+                dv[j] = x0 - x1;
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    accE[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                    accO[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                }
+            if (XR)
+#pragma unroll
+                for (int j = 0; j < CB; ++j) xacc[j] = fmaf(fxq[s], dv[j], fmaf(fxp[s], sv[j], xacc[j]));
+            load(s, k0 + s + PD);
+        }
+    }
+    const bool dox = XR && (g0 == 0) && (wave == 0);
+    const TO* __restrict__ auxf = AUX ? aux.p[f] : nullptr;
+    TO* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
+    const float osgn = (float)a.osgn;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                float ax[4][CB];
+                if (AUX)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = min((rb0 + i) * 16 + Z32_ROW(lk, v), me - 1);
+                        const int rc = half ? n - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) ax[v][j] = (float)auxf[(long)rc * P + cj[j]];
+                    }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + Z32_ROW(lk, v);
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const long col = p0 + 16 * j + li;
+                            const float e = accE[i][j][v], o = accO[i][j][v];
+                            float x = half ? osgn * (e - o) : e + o;
+                            if (AUX) x = ax[v][j] - x;
+                            if (col < P) dst[(long)row * P + col] = (TO)x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (dox) {
+        float ax[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) ax[j] = AUX ? (float)auxf[(long)a.xrow * P + cj[j]] : 0.f;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            float x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            const long col = p0 + 16 * j + li;
+            if (AUX) x = ax[j] - x;
+            if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = (TO)x;
+        }
+    }
+}
+
+// Row-block grouping shared by the split launches; the middle row of odd n goes to
+// the VALU when me % 16 == 1 (modes without a per-row epilogue only).
+static void zsplit_shape(ZArgs& a, const ZSplitOp& op, bool allow_xrow, int rb, int* gy, int* nw) {
+    a.M = op.d_m;
+    a.M2 = op.d_m + op.qoff;
+    a.M32 = op.d_m32;
+    a.q32 = (long)op.qoff;
+    a.ldm = op.ldm;
+    a.nout = op.me;
+    a.nin = op.me;
+    a.nsp = op.n;
+    a.osgn = op.osgn;
+    a.xrow = (allow_xrow && (op.n & 1) && op.me > 16 && op.me % 16 == 1) ? op.me - 1 : -1;
+    const int nrb = (a.xrow >= 0) ? op.me / 16 : (op.me + 15) / 16;
+    a.gblocks = std::min(nrb, rb * kZMaxWaves);
+    *gy = (nrb + a.gblocks - 1) / a.gblocks;
+    *nw = (a.gblocks + rb - 1) / rb;
+}
+
+template <int MODE>
+static int launch_zs(const ZArgs& a0, const ZSplitOp& op, const FieldList<double>& in, const FieldList<double>& out,
+                     const FieldList<double>& bt, const FieldList<double>& bb, int nf, hipStream_t s) {
+    ZArgs a = a0;
+    int gy = 1, nw = 1;
+    zsplit_shape(a, op, MODE == ZM_PLAIN || MODE == ZM_PRESS || MODE == ZM_AUXSUB, kZSRB, &gy, &nw);
+    dim3 grid((unsigned)((a.P + 16 * kZSCB - 1) / (16 * kZSCB)), (unsigned)gy, (unsigned)nf);
+    if (a.xrow >= 0)
+        hipLaunchKernelGGL((k_zsplit<MODE, true>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
+    else
+        hipLaunchKernelGGL((k_zsplit<MODE, false>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+// the split kernels address a field with 32-bit element offsets
+static bool zs_fits(const ZSplitOp& op, long P) { return op.ok && (long)op.n * P < (1L << 29) && op.n >= 8; }
+
+// flops of one split contraction: two me x me products per column
+static double zsplit_flops(const ZSplitOp& op, double cols) { return 4.0 * op.me * (double)op.me * cols; }
+
+// ------------------------------------------------------- parity split: setup
+namespace {
+
+// column-major n x n element (i, j)
+inline double cm(const std::vector<double>& m, int n, int i, int j) { return m[(size_t)j * n + i]; }
+
+bool centro(const std::vector<double>& m, int n, double sgn, double tol) {
+    double mx = 0.0;
+    for (double v : m) mx = std::max(mx, std::fabs(v));
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i)
+            if (std::fabs(cm(m, n, n - 1 - i, n - 1 - j) - sgn * cm(m, n, i, j)) > tol * mx) return false;
+    return true;
+}
+
+// P, Q (me x me, row-major host arrays) -> device block [P | Q], column-major, ld = ldm
+int zs_upload(lsb_ctx* c, ZSplitOp* op, int n, double osgn, const std::vector<double>& P,
+              const std::vector<double>& Q) {
+    const int me = (n + 1) / 2;
+    const int ldm = ((me + 63) / 64) * 64 + 64;
+    const int kc = (me + 15) / 16 * 16;  // zero columns: k-steps padded to the prefetch depth
+    std::vector<double> h((size_t)2 * ldm * kc, 0.0);
+    const size_t qoff = (size_t)ldm * kc;
+    for (int j = 0; j < me; ++j)
+        for (int i = 0; i < me; ++i) {
+            h[(size_t)j * ldm + i] = P[(size_t)i * me + j];
+            h[qoff + (size_t)j * ldm + i] = Q[(size_t)i * me + j];
+        }
+    if (!op->d_m) LSB_TRY(dev_alloc((void**)&op->d_m, h.size() * sizeof(double)));
+    LSB_HIP(hipMemcpy(op->d_m, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<float> h32(h.begin(), h.end());
+    if (!op->d_m32) LSB_TRY(dev_alloc((void**)&op->d_m32, h32.size() * sizeof(float)));
+    LSB_HIP(hipMemcpy(op->d_m32, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
+    op->n = n;
+    op->me = me;
+    op->ldm = ldm;
+    op->osgn = osgn;
+    op->qoff = qoff;
+    op->ok = true;
+    return LSB_OK;
+}
+
+// Fold of a physical input for an operator M (nout = nin = n) whose rows i < me are
+// used: P[i][j] = (M[i][j] + M[i][n-1-j])/2, Q[i][j] = (M[i][j] - M[i][n-1-j])/2 for
+// j < mo; the middle column of odd n (s_m = 2 x_m, d_m = 0) carries M[i][m]/2 in P.
+void fold_cols(const std::vector<double>& m, int n, std::vector<double>& P, std::vector<double>& Q) {
+    const int me = (n + 1) / 2, mo = n / 2;
+    P.assign((size_t)me * me, 0.0);
+    Q.assign((size_t)me * me, 0.0);
+    for (int i = 0; i < me; ++i) {
+        for (int j = 0; j < mo; ++j) {
+            const double a = cm(m, n, i, j), b = cm(m, n, i, n - 1 - j);
+            P[(size_t)i * me + j] = 0.5 * (a + b);
+            Q[(size_t)i * me + j] = 0.5 * (a - b);
+        }
+        if (n & 1) P[(size_t)i * me + mo] = 0.5 * cm(m, n, i, mo);
+    }
+}
+
+}  // namespace
+
+int zsplit_setup(lsb_ctx* c, const std::vector<double>& elift) {
+    c->zs_dz.ok = c->zs_h1.ok = c->zs_h2.ok = false;
+    const char* off = std::getenv("LSB_NO_ZSPLIT");
+    if (off && off[0] && std::strcmp(off, "0") != 0) return LSB_OK;
+    std::vector<double> P, Q;
+    // dfdz: centro-antisymmetric on a symmetric grid
+    const int nzp = c->nzp;
+    if (nzp >= 8 && centro(c->mdz, nzp, -1.0, 1e-12)) {
+        fold_cols(c->mdz, nzp, P, Q);
+        LSB_TRY(zs_upload(c, &c->zs_dz, nzp, -1.0, P, Q));
+    }
+    // Helmholtz: modes of definite parity (columns of E, rows of Einv)
+    const int n = c->nzm;
+    if (n < 8) return LSB_OK;
+    const std::vector<double>& E = c->smat;   // E[i][k]
+    const std::vector<double>& Ei = c->simat; // Einv[k][j]
+    std::vector<int> ev, od;
+    for (int k = 0; k < n; ++k) {
+        double se = 0.0, sr = 0.0, fe = 0.0, fr = 0.0, me_ = 0.0, mi = 0.0;
+        for (int i = 0; i < n; ++i) {
+            se += cm(E, n, i, k) * cm(E, n, n - 1 - i, k);
+            sr += cm(E, n, i, k) * cm(E, n, i, k);
+            fe += cm(Ei, n, k, i) * cm(Ei, n, k, n - 1 - i);
+            fr += cm(Ei, n, k, i) * cm(Ei, n, k, i);
+            me_ = std::max(me_, std::fabs(cm(E, n, i, k)));
+            mi = std::max(mi, std::fabs(cm(Ei, n, k, i)));
+        }
+        if (sr <= 0.0 || fr <= 0.0) return LSB_OK;
+        const double pe = se / sr, pi = fe / fr;
+        const double sg = pe > 0.0 ? 1.0 : -1.0;
+        if (std::fabs(pe - sg) > 1e-6 || std::fabs(pi - sg) > 1e-6) return LSB_OK;
+        for (int i = 0; i < n; ++i)
+            if (std::fabs(cm(E, n, n - 1 - i, k) - sg * cm(E, n, i, k)) > 1e-6 * me_ ||
+                std::fabs(cm(Ei, n, k, n - 1 - i) - sg * cm(Ei, n, k, i)) > 1e-6 * mi)
+                return LSB_OK;
+        (sg > 0.0 ? ev : od).push_back(k);
+    }
+    const int me = (n + 1) / 2, mo = n / 2;
+    if ((int)ev.size() != me || (int)od.size() != mo) return LSB_OK;
+    // stage 1: Einv folded; even mode a -> row a (P), odd mode b -> row n-1-b (Q)
+    std::vector<double> P1((size_t)me * me, 0.0), Q1((size_t)me * me, 0.0);
+    for (int a = 0; a < me; ++a) {
+        for (int j = 0; j < mo; ++j) P1[(size_t)a * me + j] = 0.5 * (cm(Ei, n, ev[a], j) + cm(Ei, n, ev[a], n - 1 - j));
+        if (n & 1) P1[(size_t)a * me + mo] = 0.5 * cm(Ei, n, ev[a], mo);
+    }
+    for (int b = 0; b < mo; ++b)
+        for (int j = 0; j < mo; ++j) Q1[(size_t)b * me + j] = 0.5 * (cm(Ei, n, od[b], j) - cm(Ei, n, od[b], n - 1 - j));
+    // stage 2: E unfolded from parity-ordered modes
+    std::vector<double> P2((size_t)me * me, 0.0), Q2((size_t)me * me, 0.0);
+    for (int i = 0; i < me; ++i) {
+        const bool mid = (n & 1) && i == mo;
+        for (int a = 0; a < me; ++a)
+            P2[(size_t)i * me + a] = mid ? cm(E, n, i, ev[a]) : 0.5 * (cm(E, n, i, ev[a]) + cm(E, n, n - 1 - i, ev[a]));
+        if (!mid)
+            for (int b = 0; b < mo; ++b)
+                Q2[(size_t)i * me + b] = 0.5 * (cm(E, n, i, od[b]) - cm(E, n, n - 1 - i, od[b]));
+    }
+    // the projected pair must still invert each other
+    {
+        std::vector<double> Ep((size_t)n * n, 0.0), Eip((size_t)n * n, 0.0);  // row-major
+        for (int i = 0; i < me; ++i) {
+            for (int a = 0; a < me; ++a) {
+                Ep[(size_t)i * n + ev[a]] = P2[(size_t)i * me + a];
+                if (i < mo) Ep[(size_t)(n - 1 - i) * n + ev[a]] = P2[(size_t)i * me + a];
+            }
+            for (int b = 0; b < mo; ++b) {
+                Ep[(size_t)i * n + od[b]] = Q2[(size_t)i * me + b];
+                if (i < mo) Ep[(size_t)(n - 1 - i) * n + od[b]] = -Q2[(size_t)i * me + b];
+            }
+        }
+        for (int a = 0; a < me; ++a) {
+            for (int j = 0; j < mo; ++j) {
+                Eip[(size_t)ev[a] * n + j] = P1[(size_t)a * me + j];
+                Eip[(size_t)ev[a] * n + n - 1 - j] = P1[(size_t)a * me + j];
+            }
+            if (n & 1) Eip[(size_t)ev[a] * n + mo] = 2.0 * P1[(size_t)a * me + mo];
+        }
+        for (int b = 0; b < mo; ++b)
+            for (int j = 0; j < mo; ++j) {
+                Eip[(size_t)od[b] * n + j] = Q1[(size_t)b * me + j];
+                Eip[(size_t)od[b] * n + n - 1 - j] = -Q1[(size_t)b * me + j];
+            }
+        double err = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc += Ep[(size_t)i * n + k] * Eip[(size_t)k * n + j];
+                err = std::max(err, std::fabs(acc - (i == j ? 1.0 : 0.0)));
+            }
+        if (err > 1e-9) return LSB_OK;
+    }
+    // eigenvalues and Einv-lifting data in parity-ordered rows
+    std::vector<double> hs((size_t)5 * n, 0.0);
+    for (int a = 0; a < me; ++a) {
+        hs[a] = c->diag[ev[a]];
+        for (int q = 0; q < 4; ++q) hs[(size_t)n + (size_t)q * n + a] = elift[(size_t)q * n + ev[a]];
+    }
+    for (int b = 0; b < mo; ++b) {
+        hs[n - 1 - b] = c->diag[od[b]];
+        for (int q = 0; q < 4; ++q) hs[(size_t)n + (size_t)q * n + n - 1 - b] = elift[(size_t)q * n + od[b]];
+    }
+    if (!c->d_helm_s) LSB_TRY(dev_alloc((void**)&c->d_helm_s, hs.size() * sizeof(double)));
+    LSB_HIP(hipMemcpy(c->d_helm_s, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+    LSB_TRY(zs_upload(c, &c->zs_h1, n, 1.0, P1, Q1));
+    LSB_TRY(zs_upload(c, &c->zs_h2, n, 1.0, P2, Q2));
+    return LSB_OK;
+}
+
 template <class TI, class TO>
 int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const TI* const* in, TO* const* out,
                   int nf, hipStream_t s, long P, const TO* const* aux) {
@@ -415,15 +1035,33 @@ int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const
         fo.p[f] = out[f];
         fa.p[f] = aux ? const_cast<TO*>(aux[f]) : nullptr;
     }
+    const double bytes = (double)a.P * nf * (nin * sizeof(TI) + nout * sizeof(TO) * (aux ? 2 : 1));
+    int tok = -1;
+    if (M == c->d_mdz && zs_fits(c->zs_dz, a.P) && nout == c->nzp && nin == c->nzp) {
+        int gy = 1, nw = 1;
+        zsplit_shape(a, c->zs_dz, true, kZRB32, &gy, &nw);
+        dim3 grid((unsigned)((a.P + 16 * kZSCB - 1) / (16 * kZSCB)), (unsigned)gy, (unsigned)nf);
+        prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P) * nf, bytes);
+        const bool xr = a.xrow >= 0;
+        if (aux && xr)
+            hipLaunchKernelGGL((k_zsplit_f32<TI, TO, true, true>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
+        else if (aux)
+            hipLaunchKernelGGL((k_zsplit_f32<TI, TO, true, false>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
+        else if (xr)
+            hipLaunchKernelGGL((k_zsplit_f32<TI, TO, false, true>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
+        else
+            hipLaunchKernelGGL((k_zsplit_f32<TI, TO, false, false>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
+        prof_mark_end(c, tok, s);
+        LSB_HIP(hipGetLastError());
+        return LSB_OK;
+    }
     a.xrow = (nout > 16 && nout % 16 == 1) ? nout - 1 : -1;
     const int nrb = (a.xrow >= 0) ? nout / 16 : (nout + 15) / 16;
     a.gblocks = std::min(nrb, kZRB32 * kZMaxWaves);
     const int gy = (nrb + a.gblocks - 1) / a.gblocks;
     const int nw = (a.gblocks + kZRB32 - 1) / kZRB32;
     dim3 grid((unsigned)((a.P + 16 * kZCB - 1) / (16 * kZCB)), (unsigned)gy, (unsigned)nf);
-    int tok = -1;
-    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
-                    (double)a.P * nf * (nin * sizeof(TI) + nout * sizeof(TO) * (aux ? 2 : 1)));
+    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf, bytes);
     if (aux)
         hipLaunchKernelGGL((k_zcontract_f32<TI, TO, true>), grid, dim3(64 * nw), 0, s, a, fi, fo, fa);
     else
@@ -468,9 +1106,16 @@ int zcontract_ex(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const 
         z.p[f] = nullptr;
     }
     int tok = -1;
-    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf,
-                    (double)(nin + nout + (aux ? nout : 0)) * a.P * nf * sizeof(double));
-    int rc = aux ? launch_z<ZM_AUXSUB>(a, fi, fo, fa, z, nf, s) : launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
+    const double bytes = (double)(nin + nout + (aux ? nout : 0)) * a.P * nf * sizeof(double);
+    int rc;
+    if (M == c->d_mdz && zs_fits(c->zs_dz, a.P) && nout == c->nzp && nin == c->nzp) {
+        prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P) * nf, bytes);
+        rc = aux ? launch_zs<ZM_AUXSUB>(a, c->zs_dz, fi, fo, fa, z, nf, s)
+                 : launch_zs<ZM_PLAIN>(a, c->zs_dz, fi, fo, z, z, nf, s);
+    } else {
+        prof_mark_begin(c, PC_Z, s, &tok, 2.0 * nout * nin * (double)a.P * nf, bytes);
+        rc = aux ? launch_z<ZM_AUXSUB>(a, fi, fo, fa, z, nf, s) : launch_z<ZM_PLAIN>(a, fi, fo, z, z, nf, s);
+    }
     prof_mark_end(c, tok, s);
     return rc;
 }
@@ -498,8 +1143,15 @@ int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs
     aux.p[0] = const_cast<double*>(w);
     z.p[0] = nullptr;
     int tok = -1;
-    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * c->nzp * (double)c->nzp * a.P, 3.0 * c->nzp * (double)a.P * sizeof(double));
-    int rc = launch_z<ZM_PRESS>(a, fi, fo, aux, z, 1, s);
+    const double bytes = 3.0 * c->nzp * (double)a.P * sizeof(double);
+    int rc;
+    if (zs_fits(c->zs_dz, a.P)) {
+        prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P), bytes);
+        rc = launch_zs<ZM_PRESS>(a, c->zs_dz, fi, fo, aux, z, 1, s);
+    } else {
+        prof_mark_begin(c, PC_Z, s, &tok, 2.0 * c->nzp * (double)c->nzp * a.P, bytes);
+        rc = launch_z<ZM_PRESS>(a, fi, fo, aux, z, 1, s);
+    }
     prof_mark_end(c, tok, s);
     return rc;
 }
@@ -538,9 +1190,25 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
     a.lift = c->d_lift;
     a.elift = c->d_lift + 4 * c->nzp;
     int tok = -1;
+    const double bytes = 2.0 * c->nzp * (double)a.P * nf * sizeof(double);
+    if (zs_fits(c->zs_h1, a.P) && zs_fits(c->zs_h2, a.P)) {
+        // parity-split projections; the mode buffer is in parity order
+        prof_mark_begin(c, PC_HELM, s, &tok, 2.0 * zsplit_flops(c->zs_h1, (double)a.P) * nf, bytes);
+        a.lam = c->d_helm_s;
+        a.elift = c->d_helm_s + nzm;
+        a.in_row0 = 1;
+        a.out_row0 = 0;
+        int rc = launch_zs<ZM_HELM1>(a, c->zs_h1, fr, ft, bt, bb, nf, s);
+        if (rc == LSB_OK) {
+            a.in_row0 = 0;
+            a.out_row0 = 1;
+            rc = launch_zs<ZM_HELM2>(a, c->zs_h2, ft, fr, bt, bb, nf, s);
+        }
+        prof_mark_end(c, tok, s);
+        return rc;
+    }
     // two nzm x nzm projections per pencil; read rhs and write the solution once each
-    prof_mark_begin(c, PC_HELM, s, &tok, 4.0 * nzm * (double)nzm * a.P * nf,
-                    2.0 * c->nzp * (double)a.P * nf * sizeof(double));
+    prof_mark_begin(c, PC_HELM, s, &tok, 4.0 * nzm * (double)nzm * a.P * nf, bytes);
     a.M = c->d_simat;
     a.in_row0 = 1;
     a.out_row0 = 0;

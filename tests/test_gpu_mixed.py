@@ -52,7 +52,12 @@ def check(errs):
     assert errs["pf"] < P_BAR, errs
 
 
-def test_mixed_dns_small():
+@pytest.mark.parametrize("zpath", ["split", "dense"])
+def test_mixed_dns_small(zpath, monkeypatch):
+    """Both z-contraction paths of the FP32 chain (parity-split on this symmetric
+    grid; LSB_NO_ZSPLIT=1 the dense one)."""
+    if zpath == "dense":
+        monkeypatch.setenv("LSB_NO_ZSPLIT", "1")
     case = L.cases.CASES["c1_dns"]
     check(run_pair(case, case.nsteps))
 

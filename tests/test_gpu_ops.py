@@ -31,6 +31,18 @@ def make(nx=32, ny=16, nz=40, **kw):
     return o, c
 
 
+@pytest.fixture(params=["split", "dense"])
+def zpath(request, monkeypatch):
+    """On a grid symmetric about the mid-plane the z operators run parity-split (two
+    half-size products); LSB_NO_ZSPLIT=1 keeps the dense contraction that any other
+    grid uses.  Both must match the oracle."""
+    if request.param == "dense":
+        monkeypatch.setenv("LSB_NO_ZSPLIT", "1")
+    else:
+        monkeypatch.delenv("LSB_NO_ZSPLIT", raising=False)
+    return request.param
+
+
 def spec_field(o, rng, n=None):
     sh = (o.nzp, o.ny, o.nx) if n is None else (n, o.nzp, o.ny, o.nx)
     return rng.standard_normal(sh)
@@ -86,7 +98,7 @@ def test_norm(rng):
 
 
 @pytest.mark.parametrize("nz", [40, 16])
-def test_spectral_derivatives(nz, rng):
+def test_spectral_derivatives(nz, rng, zpath):
     o, c = make(32, 16, nz)
     u, v, w = (spec_field(o, rng) for _ in range(3))
     assert np.array_equal(c.dfdx(u), o.dfdx(u))
@@ -101,7 +113,7 @@ def test_spectral_derivatives(nz, rng):
     assert abs(c.divcheck(u, v, w) - o.divcheck(u, v, w)) < 1e-10 * o.divcheck(u, v, w)
 
 
-def test_dpdz(rng):
+def test_dpdz(rng, zpath):
     o, c = make(32, 16, 40)
     u = rng.standard_normal((o.nzp, 3, 5))
     for p in (1, 2):
@@ -109,7 +121,7 @@ def test_dpdz(rng):
 
 
 @pytest.mark.parametrize("sigma", [0.0, 37.5, 1.0e4])
-def test_solve_helmholtz(sigma, rng):
+def test_solve_helmholtz(sigma, rng, zpath):
     o, c = make(32, 16, 40)
     r = spec_field(o, rng)
     got = c.solve_helmholtz(r, sigma)
@@ -122,7 +134,7 @@ def test_solve_helmholtz(sigma, rng):
 
 
 @pytest.mark.parametrize("sigma", [0.0, 250.0])
-def test_solve_nhom_helmholtz(sigma, rng):
+def test_solve_nhom_helmholtz(sigma, rng, zpath):
     o, c = make(32, 16, 40)
     r = spec_field(o, rng)
     bc = spec_field(o, rng)
@@ -134,7 +146,7 @@ def test_solve_nhom_helmholtz(sigma, rng):
     assert np.max(np.abs(got[-1] - bc[-1])) < 1e-12 * np.max(np.abs(bc))
 
 
-def test_helmholtz_matches_with_driver_matrices(rng):
+def test_helmholtz_matches_with_driver_matrices(rng, zpath):
     """lsb_set_mats with the oracle's (LAPACK) eigenbasis gives the same solution."""
     o, c = make(32, 16, 40)
     c.set_mats(diag=o.diag, smat=o.smat, simat=o.simat)

@@ -95,6 +95,15 @@ def test_dealias_modes_and_homogeneous_bc(dealias):
     check_fields(st, g)
 
 
+def test_c1_dns_dense_z_operators(monkeypatch):
+    """The dense z contractions (the path of grids without mid-plane symmetry) in a
+    full step; the default path on this grid is the parity-split one."""
+    monkeypatch.setenv("LSB_NO_ZSPLIT", "1")
+    case = L.cases.CASES["c1_dns"]
+    st, g = run_both(case, case.nsteps)
+    check_fields(st, g)
+
+
 def test_matmul_derivative_path():
     """nz <= 31 uses matmul(d, .) instead of the DCT operator (derivatives.f90:122-129)."""
     case = L.cases.CASES["c1_dns"].with_(nx=16, ny=8, nz=24, pert_rms=0.02)
