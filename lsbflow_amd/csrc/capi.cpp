@@ -992,6 +992,15 @@ extern "C" int lsb_state_set(lsb_ctx* c, const double* uf, const double* vf, con
     LSB_TRY(step_alloc(c));
     const size_t n = c->nspec;
     auto put = [&](int which, const double* h) -> int {
+        if (step_hist32(c, which)) {  // FP32 N history: convert on the device
+            double* tmp = nullptr;
+            if (h) {
+                LSB_TRY(st.get(0, n, &tmp));
+                LSB_TRY(st.in(tmp, h, n));
+            }
+            LSB_TRY(step_hist_set(c, which, tmp, c->stream));
+            return st.sync();
+        }
         double* d = step_field(c, which);
         if (h)
             LSB_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1027,7 +1036,15 @@ extern "C" int lsb_state_get(lsb_ctx* c, double* uf, double* vf, double* wf, dou
     }
     const size_t n = c->nspec;
     auto get = [&](int which, double* h) -> int {
-        if (h) LSB_HIP(hipMemcpyAsync(h, step_field(c, which), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (!h) return LSB_OK;
+        if (step_hist32(c, which)) {  // FP32 N history: widen on the device
+            double* tmp = nullptr;
+            LSB_TRY(st.get(0, n, &tmp));
+            LSB_TRY(step_hist_get(c, which, tmp, c->stream));
+            LSB_TRY(st.out(h, tmp, n));
+            return st.sync();
+        }
+        LSB_HIP(hipMemcpyAsync(h, step_field(c, which), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         return LSB_OK;
     };
     LSB_TRY(get(0, uf));

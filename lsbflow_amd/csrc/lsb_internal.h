@@ -325,6 +325,9 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm);
 int step_prepare(lsb_ctx* c);
 void step_free(lsb_ctx* c);
 double* step_field(lsb_ctx* c, int which);
+bool step_hist32(const lsb_ctx* c, int which);
+int step_hist_set(lsb_ctx* c, int which, const double* src, hipStream_t s);
+int step_hist_get(lsb_ctx* c, int which, double* dst, hipStream_t s);
 int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStream_t s);
 // full 9-component tensors (tier A): work must hold 69 (dynamic) / 40 (interscale) physical fields
 int dyn_full(lsb_ctx* c, double* ui, double* uiuj, double* g, double* tau, double* work, hipStream_t s);

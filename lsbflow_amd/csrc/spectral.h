@@ -12,9 +12,12 @@ struct StepRhsArgs {
     const void* h1[3];     // hij(l, y) spectral
     const void* hz[3];     // dfdz(hij(l, z)); null when already folded into nlt
     int n32;
-    double* nout[3];       // N_l (becomes nlnif)
-    const double* nl[3];   // nlnif
-    const double* nlm[3];  // nlnmif
+    // N history (nout, nl, nlm): double, or float when h32 = 1 (mixed form: N is
+    // FP32-accurate there, so its history is kept in FP32)
+    int h32;
+    void* nout[3];         // N_l (becomes nlnif)
+    const void* nl[3];     // nlnif
+    const void* nlm[3];    // nlnmif
     const double* u[3];
     const double* un[3];
     const double* unm[3];

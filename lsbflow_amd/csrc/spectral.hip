@@ -243,6 +243,13 @@ __device__ __forceinline__ c2 ld_n(const void* p, long q) {
 }
 
 template <class TN>
+__device__ __forceinline__ void st_n(void* p, long q, double re, double im) {
+    TN* t = reinterpret_cast<TN*>(p) + 2 * q;
+    t[0] = (TN)re;
+    t[1] = (TN)im;
+}
+
+template <class TN, class TH>
 __global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const double* __restrict__ wavy, StepRhsArgs a) {
     const long n = d.npairs * 3;
     SP_LOOP(n) {
@@ -264,9 +271,15 @@ __global__ void k_rhs_vel(SpDims d, const double* __restrict__ wavx, const doubl
         ni = ni - (wy * h1.x);
         nr = nr - hz.x;
         ni = ni - hz.y;
-        reinterpret_cast<c2*>(a.nout[l])[q] = make_double2(nr, ni);
-        const c2 n1 = reinterpret_cast<const c2*>(a.nl[l])[q];
-        const c2 n2 = reinterpret_cast<const c2*>(a.nlm[l])[q];
+        if (sizeof(TH) < sizeof(double)) {
+            // the FP32 history holds N rounded once; the right-hand side uses the same
+            // rounded value this step as the next two do
+            nr = (double)(TH)nr;
+            ni = (double)(TH)ni;
+        }
+        st_n<TH>(a.nout[l], q, nr, ni);
+        const c2 n1 = ld_n<TH>(a.nl[l], q);
+        const c2 n2 = ld_n<TH>(a.nlm[l], q);
         const c2 u0 = reinterpret_cast<const c2*>(a.u[l])[q];
         const c2 u1 = reinterpret_cast<const c2*>(a.un[l])[q];
         const c2 u2 = reinterpret_cast<const c2*>(a.unm[l])[q];
@@ -345,17 +358,26 @@ __global__ void k_project(SpDims d, const double* __restrict__ wavx, const doubl
 int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);
     int tok = -1;
-    {   // per component: the N pieces (nlt, h0, h1, hz if present), 5 FP64 history/velocity
-        // fields read, N and rhs written
-        const double tn = a.n32 ? 4.0 : 8.0;
+    {   // per component: the N pieces (nlt, h0, h1, hz if present) and two N history
+        // fields, three FP64 velocity fields read, N and rhs written
+        const double tn = a.n32 ? 4.0 : 8.0, th = a.h32 ? 4.0 : 8.0;
         const double pieces = a.hz[0] ? 4.0 : 3.0;
-        const double per_pair = 2.0 * (pieces * tn + 5.0 * 8.0 + 2.0 * 8.0);
+        const double per_pair = 2.0 * (pieces * tn + 3.0 * th + 3.0 * 8.0 + 8.0);
         prof_mark_begin(c, PC_RHS, s, &tok, 0.0, per_pair * (double)d.npairs * 3.0);
     }
-    if (a.n32)
-        hipLaunchKernelGGL(k_rhs_vel<float>, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
-    else
-        hipLaunchKernelGGL(k_rhs_vel<double>, sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy, a);
+    if (a.n32 && a.h32)
+        hipLaunchKernelGGL((k_rhs_vel<float, float>), sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx, c->d_wavy,
+                           a);
+    else if (a.n32)
+        hipLaunchKernelGGL((k_rhs_vel<float, double>), sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx,
+                           c->d_wavy, a);
+    else if (!a.h32)
+        hipLaunchKernelGGL((k_rhs_vel<double, double>), sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx,
+                           c->d_wavy, a);
+    else {
+        set_error("rhs_vel: FP32 N history needs FP32 N pieces");
+        return LSB_E_ARG;
+    }
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;

@@ -169,6 +169,40 @@ double* step_field(lsb_ctx* c, int which) {
     }
 }
 
+// N history fields (state slots 9..14) are stored in FP32 in the mixed form, in the
+// first half of their FP64-sized slot; the state interface stays FP64.
+static bool mixed_form(const lsb_ctx* c);
+bool step_hist32(const lsb_ctx* c, int which) { return which >= 9 && which <= 14 && mixed_form(c); }
+
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        b[t] = (float)a[t];
+}
+__global__ void k_f2d(const float* __restrict__ a, double* __restrict__ b, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        b[t] = (double)a[t];
+}
+
+// slot <- FP64 device data (src == nullptr: zero)
+int step_hist_set(lsb_ctx* c, int which, const double* src, hipStream_t s) {
+    float* d = reinterpret_cast<float*>(step_field(c, which));
+    const long n = (long)c->nspec;
+    if (!src) {
+        LSB_HIP(hipMemsetAsync(d, 0, n * sizeof(float), s));
+        return LSB_OK;
+    }
+    hipLaunchKernelGGL(k_d2f, dim3(2048), dim3(256), 0, s, src, d, n);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+// FP64 device data <- slot
+int step_hist_get(lsb_ctx* c, int which, double* dst, hipStream_t s) {
+    const float* d = reinterpret_cast<const float*>(step_field(c, which));
+    hipLaunchKernelGGL(k_f2d, dim3(2048), dim3(256), 0, s, d, dst, (long)c->nspec);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
 // ------------------------------------------------------------ bound()
 // navierstokes.f90:214-355 physical boundary data on planes 0 (ceiling) and
 // nzp-1 (wall) at unit ramp; returned as two contiguous physical planes.
@@ -810,6 +844,7 @@ static int update_once(lsb_ctx* c) {
         r.rhs[l] = L.spec[S->rhs[l]];
     }
     r.n32 = mixed_form(c) ? 1 : 0;
+    r.h32 = mixed_form(c) ? 1 : 0;
     r.ubarpp = c->d_paral + 2 * c->nzp;
     r.ab0 = c->abfac[0];
     r.ab1 = c->abfac[1];

@@ -423,6 +423,9 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) k_zcontract_f32(ZArgs a, Fiel
 // unmixed; stage 2 (E) reads them unmixed and unfolds into physical rows.  A wave
 // owns the same rows of both halves (two accumulator sets, two column blocks).
 constexpr int kZSCB = 2;
+// FP32 split: 3 x 4 tiles per set at two waves per SIMD (measured best of 3-4 row
+// by 1-8 column blocks on the 193-row physical-space contraction)
+constexpr int kZSRB32 = 3, kZSCB32 = 4;
 constexpr int kZSRB = 3;  // FP64 row blocks per wave (two 3 x 2 tile sets, 96 accumulator VGPRs)
 
 // element of a field or matrix at a 32-bit element offset from a wave-uniform base
@@ -660,7 +663,7 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
 // FP32 form of k_zsplit for the mixed nonlinear chain (folded physical input,
 // unfolded output; aux subtraction as in k_zcontract_f32), same branch-free
 // pipelined k loop; the matrix halves are read from their FP32 copies.
-template <class TI, class TO, bool AUX, bool XR, int RB = kZRB32, int CB = kZSCB, int PD = kZPD, int WPE = 3>
+template <class TI, class TO, bool AUX, bool XR, int RB = kZSRB32, int CB = kZSCB32, int PD = kZPD, int WPE = 2>
 __global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zsplit_f32(ZArgs a, FieldList<TI> in, FieldList<TO> out, FieldList<TO> aux) {
     const int lane = threadIdx.x & 63;
@@ -1039,8 +1042,8 @@ int zcontract_f32(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const
     int tok = -1;
     if (M == c->d_mdz && zs_fits(c->zs_dz, a.P) && nout == c->nzp && nin == c->nzp) {
         int gy = 1, nw = 1;
-        zsplit_shape(a, c->zs_dz, true, kZRB32, &gy, &nw);
-        dim3 grid((unsigned)((a.P + 16 * kZSCB - 1) / (16 * kZSCB)), (unsigned)gy, (unsigned)nf);
+        zsplit_shape(a, c->zs_dz, true, kZSRB32, &gy, &nw);
+        dim3 grid((unsigned)((a.P + 16 * kZSCB32 - 1) / (16 * kZSCB32)), (unsigned)gy, (unsigned)nf);
         prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P) * nf, bytes);
         const bool xr = a.xrow >= 0;
         if (aux && xr)

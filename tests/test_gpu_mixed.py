@@ -85,3 +85,28 @@ def test_mixed_refuses_dynamic_model():
             c.timestep(1)
     finally:
         c.close()
+
+
+def test_mixed_state_roundtrip_history():
+    """The mixed form keeps the N history (nlnif, nlnmif) in FP32 on the device:
+    state_set/state_get still exchange FP64 arrays, the history coming back rounded
+    to FP32 and every other field unchanged."""
+    case = L.cases.CASES["c1_dns"]
+    rng = np.random.default_rng(7)
+    u, v, w = L.cases.initial_condition(case)
+    c = setup_case(case, precision=1)
+    try:
+        s = (3,) + u.shape
+        hist = {k: rng.standard_normal(s) for k in ("unif", "unmif", "nlnif", "nlnmif")}
+        p, pn = rng.standard_normal(u.shape), rng.standard_normal(u.shape)
+        c.state_set(u, v, w, pf=p, pnf=pn, istep=5, t=0.25, **hist)
+        g = c.state_get()
+        for k in ("uf", "vf", "wf"):
+            assert np.array_equal(g[k], {"uf": u, "vf": v, "wf": w}[k])
+        assert np.array_equal(g["pf"], p) and np.array_equal(g["pnf"], pn)
+        for k in ("unif", "unmif"):
+            assert np.array_equal(g[k], hist[k])
+        for k in ("nlnif", "nlnmif"):
+            assert np.array_equal(g[k], hist[k].astype(np.float32).astype(np.float64))
+    finally:
+        c.close()
