@@ -608,6 +608,61 @@ template int horfft_fwd_mixed<double>(lsb_ctx*, const HorPlan&, const float* con
 template int horfft_fwd_mixed<float>(lsb_ctx*, const HorPlan&, const float* const*, float* const*, int, int,
                                      hipStream_t);
 
+// x pass only of nf forward transforms: physical rows -> rows of lh complex (the
+// layout of a spectral field before its y pass), for a fused consumer of the y pass.
+template <class T>
+int horfft_xfwd(lsb_ctx* c, const HorPlan& hp, const T* const* phys, T* const* rows, int nf, int nplanes,
+                hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields || !horfft_has_ops(hp)) {
+        set_error("horfft (x pass): 1..16 fields, power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
+    }
+    FieldList<T> in{}, out{};
+    for (int f = 0; f < nf; ++f) {
+        in.p[f] = const_cast<T*>(phys[f]);
+        out.p[f] = rows[f];
+    }
+    int tok = -1;
+    const double planes = (double)nf * nplanes;
+    prof_mark_begin(c, PC_FFT, s, &tok, planes * 2.5 * hp.nx * hp.ny * std::log2((double)hp.nx),
+                    planes * ((double)(hp.nx + 2) * hp.ny + (double)hp.nx * hp.ny) * sizeof(T));
+    const cpx<T>* twx = (const cpx<T>*)(sizeof(T) == 8 ? (void*)hp.tw_x : (void*)hp.tw_x32);
+    const cpx<T>* twr = (const cpx<T>*)(sizeof(T) == 8 ? (void*)hp.tw_r : (void*)hp.tw_r32);
+    LSB_TRY((fast_xfwd<T, T, T>(in, out, nplanes * hp.ny, nf, hp.lh, hp.nx + 2, hp.nx, twx, twr, s)));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+template int horfft_xfwd<double>(lsb_ctx*, const HorPlan&, const double* const*, double* const*, int, int,
+                                 hipStream_t);
+template int horfft_xfwd<float>(lsb_ctx*, const HorPlan&, const float* const*, float* const*, int, int, hipStream_t);
+
+// y pass of the three pieces of each N_l fused with the velocity right-hand side
+// (fast_yfwd_rhs); pieces[3 l + q] hold x-transformed rows.
+template <class R, class TH>
+int horfft_yfwd_rhs(lsb_ctx* c, const HorPlan& hp, R* const* pieces, int nplanes, const YRhsArgs& a, hipStream_t s) {
+    if (!horfft_has_ops(hp)) {
+        set_error("horfft (fused y pass): power-of-two nx/2 and ny in 16..256");
+        return LSB_E_UNSUPPORTED;
+    }
+    FieldList<R> pc{};
+    for (int q = 0; q < 9; ++q) pc.p[q] = pieces[q];
+    int tok = -1;
+    {
+        const double pairs = (double)nplanes * hp.ny * hp.lh * 3.0;
+        const double per = 3.0 * 2 * sizeof(R) + 3.0 * 2 * sizeof(TH) + 4.0 * 16.0;
+        prof_mark_begin(c, PC_RHS, s, &tok, 9.0 * nplanes * 2.5 * hp.lh * hp.ny * std::log2((double)hp.ny),
+                        pairs * per);
+    }
+    const cpx<R>* twy = (const cpx<R>*)(sizeof(R) == 8 ? (void*)hp.tw_y : (void*)hp.tw_y32);
+    LSB_TRY((fast_yfwd_rhs<R, TH>(pc, nplanes, hp.lh, hp.ny, twy, (R)(1.0 / ((double)hp.nx * hp.ny)), a, s)));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+template int horfft_yfwd_rhs<double, double>(lsb_ctx*, const HorPlan&, double* const*, int, const YRhsArgs&,
+                                             hipStream_t);
+template int horfft_yfwd_rhs<float, float>(lsb_ctx*, const HorPlan&, float* const*, int, const YRhsArgs&,
+                                           hipStream_t);
+
 template <class TI>
 int horfft_inv_mixed(lsb_ctx* c, const HorPlan& hp, const TI* const* spec, float* const* phys, int nf, int nplanes,
                      const int* ops, hipStream_t s) {

@@ -131,6 +131,17 @@ __device__ __forceinline__ void f_twiddle(cpx<R>* v, int n2, const cpx<R>* __res
     }
 }
 
+// the same from a copy of the table in LDS (short latency: no batching needed)
+template <class R, int S, int N1>
+__device__ __forceinline__ void f_twiddle_lds(cpx<R>* v, int n2, const cpx<R>* tw) {
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        cpx<R> w = tw[n2 * k1];
+        if (S > 0) w.im = -w.im;
+        v[k1] = cmul(v[k1], w);
+    }
+}
+
 // --------------------------------------------------------------- X passes
 template <class R, class T>
 __device__ __forceinline__ cpx<R> ld_c(const T* p, long i) {
@@ -598,6 +609,153 @@ int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, i
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
+
+// Forward y pass fused with the velocity right-hand side (one block: C kx-columns
+// of one plane, one component l).  The three pieces are transformed in turn
+// through the same LDS tile; N accumulates in FP64 in the order of the separate
+// passes (nlt, then - i kx h_x, then - i ky h_y, each piece rounded to R after
+// scaling as the separate forward pass stores it), so results are bitwise those of
+// fast_yfwd followed by the right-hand-side kernel.
+template <class TN>
+__device__ __forceinline__ void ld_h(const void* p, long q, double& re, double& im) {
+    const TN* t = reinterpret_cast<const TN*>(p) + 2 * q;
+    re = (double)t[0];
+    im = (double)t[1];
+}
+template <class TN>
+__device__ __forceinline__ void st_h(void* p, long q, double re, double im) {
+    TN* t = reinterpret_cast<TN*>(p) + 2 * q;
+    t[0] = (TN)re;
+    t[1] = (TN)im;
+}
+
+template <class R, class TH, int N1, int N2, int C>
+__global__ void __launch_bounds__(C * N2) k_yfwd_rhs(FieldList<R> pc, int nplanes, int lh,
+                                                     const cpx<R>* __restrict__ tw, R scale, YRhsArgs a) {
+    constexpr int L = N1 * N2;
+    constexpr int TS = N1 * (N2 + 1) + 1;
+    constexpr int NQ = N1 / N2;
+    __shared__ cpx<R> T[C * TS];
+    __shared__ cpx<R> Tw[L];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < L; t += C * N2) Tw[t] = tw[t];
+    const int c = tid % C, n2 = tid / C;
+    const int ntile = (lh + C - 1) / C;
+    const int tile = blockIdx.x % ntile;
+    const long rest = blockIdx.x / ntile;
+    const int pk = (int)(rest % nplanes);
+    const int l = (int)(rest / nplanes);
+    const int m = tile * C + c;
+    const bool valid = m < lh;
+    const int mc = valid ? m : lh - 1;  // columns past the end read the last one, store nothing
+    const double wx = a.wavx[2 * mc];
+    double nr[NQ][N2], ni[NQ][N2];
+    cpx<R>* Tc = T + c * TS;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        // one piece at a time: the next piece's loads are not hoisted over this
+        // piece's second stage (three live tiles would halve the occupancy)
+        __builtin_amdgcn_sched_barrier(0);
+        const R* src = pc.p[3 * l + q] + (long)pk * L * (2 * lh);
+        cpx<R> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = ld_c<R>(src, (long)(N2 * n1 + n2) * lh + mc);
+        f_dft<R, -1, N1>(v);
+        __syncthreads();  // the twiddle table is in place; the previous piece has read its tile
+        f_twiddle_lds<R, -1, N1>(v, n2, Tw);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) Tc[k1 * (N2 + 1) + n2] = v[k1];
+        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < NQ; ++qq) {
+            const int k1 = n2 + N2 * qq;
+            cpx<R> u[N2];
+#pragma unroll
+            for (int j = 0; j < N2; ++j) u[j] = Tc[k1 * (N2 + 1) + j];
+            f_dft<R, -1, N2>(u);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const double vr = (double)(R)(u[k2].re * scale), vi = (double)(R)(u[k2].im * scale);
+                if (q == 0) {
+                    nr[qq][k2] = vr;
+                    ni[qq][k2] = vi;
+                } else if (q == 1) {
+                    nr[qq][k2] = nr[qq][k2] - (-wx * vi);
+                    ni[qq][k2] = ni[qq][k2] - (wx * vr);
+                } else {
+                    const double wy = a.wavy[k1 + N1 * k2];
+                    nr[qq][k2] = nr[qq][k2] - (-wy * vi);
+                    ni[qq][k2] = ni[qq][k2] - (wy * vr);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const double ub = (l == 0 && m == 0) ? a.ubarpp[pk] : 0.0;
+#pragma unroll
+    for (int qq = 0; qq < NQ; ++qq) {
+        const int k1 = n2 + N2 * qq;
+#pragma unroll
+        for (int b0 = 0; b0 < N2; b0 += 4) {
+            // the history of four outputs is loaded before their stores (and not
+            // earlier: the barrier bounds the live registers)
+            __builtin_amdgcn_sched_barrier(0);
+            double h1r[4], h1i[4], h2r[4], h2i[4], u0r[4], u0i[4], u1r[4], u1i[4], u2r[4], u2i[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int ky = k1 + N1 * (b0 + b);
+                const long qd = ((long)pk * L + ky) * lh + m;
+                ld_h<TH>(a.nl[l], qd, h1r[b], h1i[b]);
+                ld_h<TH>(a.nlm[l], qd, h2r[b], h2i[b]);
+                ld_h<double>(a.u[l], qd, u0r[b], u0i[b]);
+                ld_h<double>(a.un[l], qd, u1r[b], u1i[b]);
+                ld_h<double>(a.unm[l], qd, u2r[b], u2i[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int k2 = b0 + b;
+                const int ky = k1 + N1 * k2;
+                const long qd = ((long)pk * L + ky) * lh + m;
+                double xr = nr[qq][k2], xi = ni[qq][k2];
+                if (sizeof(TH) < sizeof(double)) {
+                    xr = (double)(TH)xr;
+                    xi = (double)(TH)xi;
+                }
+                st_h<TH>(a.nout[l], qd, xr, xi);
+                const double cr = a.ab0 * xr + a.ab1 * h1r[b] + a.ab2 * h2r[b];
+                const double ci = a.ab0 * xi + a.ab1 * h1i[b] + a.ab2 * h2i[b];
+                double rr = -a.dtnui * (a.dt * cr + a.bdf0 * u0r[b] + a.bdf1 * u1r[b] + a.bdf2 * u2r[b]);
+                const double ri = -a.dtnui * (a.dt * ci + a.bdf0 * u0i[b] + a.bdf1 * u1i[b] + a.bdf2 * u2i[b]);
+                if (ky == 0) rr = rr - ub;
+                st_h<double>(a.rhs[l], qd, rr, ri);
+            }
+        }
+    }
+}
+
+template <class R, class TH>
+int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
+                  const YRhsArgs& a, hipStream_t s) {
+    constexpr int C = ytile_cols<R>();
+    const long nblk = (long)((lh + C - 1) / C) * nplanes * 3;
+    if (nblk <= 0) return LSB_OK;
+    switch (ny) {
+#define X(LL, A, B)                                                                                            \
+    case LL:                                                                                                   \
+        hipLaunchKernelGGL((k_yfwd_rhs<R, TH, A, B, C>), dim3((unsigned)nblk), dim3(C * B), 0, s, pieces, nplanes, \
+                           lh, tw, scale, a);                                                                  \
+        break;
+        LSB_FAST_CASES(X)
+#undef X
+        default: return LSB_E_ARG;
+    }
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+template int fast_yfwd_rhs<double, double>(const FieldList<double>&, int, int, int, const cpx<double>*, double,
+                                           const YRhsArgs&, hipStream_t);
+template int fast_yfwd_rhs<float, float>(const FieldList<float>&, int, int, int, const cpx<float>*, float,
+                                         const YRhsArgs&, hipStream_t);
 
 #define LSB_INST_FAST(R, TI, TO)                                                                                  \
     template int fast_xfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \

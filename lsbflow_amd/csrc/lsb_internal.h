@@ -228,6 +228,12 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
                int nplanes, hipStream_t s);
 // inverse with per-field d/dx, d/dy applied on load (FOP_*); fast lengths only
 bool horfft_has_ops(const HorPlan& hp);
+template <class T>
+int horfft_xfwd(lsb_ctx* c, const HorPlan& hp, const T* const* phys, T* const* rows, int nf, int nplanes,
+                hipStream_t s);
+struct YRhsArgs;
+template <class R, class TH>
+int horfft_yfwd_rhs(lsb_ctx* c, const HorPlan& hp, R* const* pieces, int nplanes, const YRhsArgs& a, hipStream_t s);
 int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
                    int nplanes, const int* ops, hipStream_t s);
 // u, du/dx, du/dy of 3 spectra (one load each); RT = physical storage and arithmetic type
@@ -286,6 +292,26 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 template <class R, class TI, class TO>
 int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
                const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s);
+
+// Forward y pass of the three pieces of N_l (nlt_l, h_(l,x), h_(l,y), as x-transformed
+// rows) fused with the EX3/BDF3 velocity right-hand side: N_l and rhs_l are written,
+// the transformed pieces never leave registers.  pieces.p[3 l + q], q = nlt, h_x, h_y.
+struct YRhsArgs {
+    const double* wavx;    // nx (kx at 2 m)
+    const double* wavy;    // ny
+    const double* ubarpp;  // nzp
+    void* nout[3];         // N_l (TH)
+    const void* nl[3];     // N history (TH)
+    const void* nlm[3];
+    const double* u[3];
+    const double* un[3];
+    const double* unm[3];
+    double* rhs[3];
+    double ab0, ab1, ab2, bdf0, bdf1, bdf2, dt, dtnui;
+};
+template <class R, class TH>
+int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
+                  const YRhsArgs& a, hipStream_t s);
 
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).

@@ -591,7 +591,7 @@ static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED;
 // rounding); that part lives only on the (kx, ky) = (0, 0) mode, where h_00 enters
 // N only through i kx h_00 = 0, so it is not restored.
 static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3],
-                             double* hf[9], const double* ftab, hipStream_t s) {
+                             double* hf[9], const double* ftab, bool xonly, hipStream_t s) {
     StepLayout& L = S->L;
     const bool sgs = c->cfg.sgsflag == 1;
     const int model = c->cfg.sgsmodel;
@@ -660,7 +660,11 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
             in[n] = ph(P_H0 + map[uq[q]]);
             out[n++] = reinterpret_cast<float*>(hf[uq[q]]);
         }
-        LSB_TRY(horfft_fwd_mixed<float>(c, c->hp, in, out, n, c->nzp, s));
+        // xonly: the rows stay x-transformed for the fused y pass + right-hand side
+        if (xonly)
+            LSB_TRY(horfft_xfwd<float>(c, c->hp, in, out, n, c->nzp, s));
+        else
+            LSB_TRY(horfft_fwd_mixed<float>(c, c->hp, in, out, n, c->nzp, s));
     }
     return LSB_OK;
 }
@@ -671,7 +675,7 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
 // ftab (fast transform sizes only): the spectral filter's factors, applied to uif as the
 // inverse transform loads it; null when uif is used as given.
 static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3], double* hf[9],
-                       const double* ftab, hipStream_t s) {
+                       const double* ftab, hipStream_t s, bool xonly = false) {
     StepLayout& L = S->L;
     // With the register-resident FFT, d/dx and d/dy are applied as the inverse transform
     // loads uif and d/dz is taken in physical space; otherwise all three are spectral passes
@@ -692,7 +696,11 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         set_error("nlterm: filter-on-load needs the register-resident FFT sizes");
         return LSB_E_ARG;
     }
-    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, ftab, s);
+    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, ftab, xonly, s);
+    if (xonly) {
+        set_error("nlterm: the fused forward pass is built for the mixed form");
+        return LSB_E_ARG;
+    }
     return nlterm_phys(c, S, uif, nltf_out, hf, ops_on_load, ftab, s);
 }
 
@@ -819,7 +827,10 @@ static int update_once(lsb_ctx* c) {
     double* hf[9];
     hf_view(S, sym, hf);
     double* nltf[3] = {L.spec[S_NLTF0], L.spec[S_NLTF0 + 1], L.spec[S_NLTF0 + 2]};
-    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, ftab, s));
+    // mixed form on the register-resident FFT: the forward y pass of the N pieces is
+    // fused with the right-hand side (the transformed pieces stay in registers)
+    const bool fuse = mixed_form(c) && horfft_has_ops(c->hp) && sym;
+    LSB_TRY(nlterm_core(c, S, uif, nltf, hf, ftab, s, fuse));
     // the unfiltered velocity is u^n for dealiasflag 0..2 (navierstokes.f90:1225-1229)
     const double* un_[3];
     for (int l = 0; l < 3; ++l) un_[l] = (g.dealiasflag > 2) ? uif[l] : L.spec[S->u[l]];
@@ -854,7 +865,36 @@ static int update_once(lsb_ctx* c) {
     r.bdf2 = c->bdf[2];
     r.dt = dt;
     r.dtnui = dtnui;
-    LSB_TRY(sp_rhs_vel(c, r, s));
+    if (fuse) {
+        YRhsArgs y{};
+        y.wavx = c->d_wavx;
+        y.wavy = c->d_wavy;
+        y.ubarpp = r.ubarpp;
+        float* pieces[9];
+        for (int l = 0; l < 3; ++l) {
+            y.nout[l] = r.nout[l];
+            y.nl[l] = r.nl[l];
+            y.nlm[l] = r.nlm[l];
+            y.u[l] = r.u[l];
+            y.un[l] = r.un[l];
+            y.unm[l] = r.unm[l];
+            y.rhs[l] = r.rhs[l];
+            pieces[3 * l + 0] = reinterpret_cast<float*>(nltf[l]);
+            pieces[3 * l + 1] = reinterpret_cast<float*>(hf[l]);
+            pieces[3 * l + 2] = reinterpret_cast<float*>(hf[l + 3]);
+        }
+        y.ab0 = r.ab0;
+        y.ab1 = r.ab1;
+        y.ab2 = r.ab2;
+        y.bdf0 = r.bdf0;
+        y.bdf1 = r.bdf1;
+        y.bdf2 = r.bdf2;
+        y.dt = r.dt;
+        y.dtnui = r.dtnui;
+        LSB_TRY((horfft_yfwd_rhs<float, float>(c, c->hp, pieces, c->nzp, y, s)));
+    } else {
+        LSB_TRY(sp_rhs_vel(c, r, s));
+    }
     // boundary data (navierstokes.f90:1321-1373)
     const bool nhom = (g.bctype > 0 && c->t >= g.istartbound);
     {
