@@ -319,6 +319,11 @@ int zcontract(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const dou
               double* const* out, int nf, hipStream_t s);
 // pressure right-hand side -d(div_h)/dz - k^2 w in one contraction
 int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs, hipStream_t s);
+// fused pressure-step contractions (parity-split dfdz only; zsplit_dz tells)
+bool zsplit_dz(const lsb_ctx* c);
+int zcontract_pressdiv(lsb_ctx* c, const double* u, const double* v, const double* w, double* rhs, hipStream_t s);
+int zcontract_proj(lsb_ctx* c, const double* phi, double* p, double* u, double* v, double dt, double bdfv,
+                   hipStream_t s);
 // mixed form: FP32 matrix-core contraction, panel TI, output TO (FP32 products and sums)
 // P columns per plane (0: spectral nx*ny); with aux, out = aux - M in (out may alias aux)
 template <class TI, class TO>

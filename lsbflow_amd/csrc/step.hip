@@ -964,22 +964,30 @@ static int update_once(lsb_ctx* c) {
     double* vN = L.spec[S->rhs[1]];
     double* wS = L.spec[S->rhs[2]];
     double* r3 = L.spec[S->rhs[3]];
-    // divf = ikx u + iky v; rhs3 = -d(divf)/dz - k^2 w (navierstokes.f90:1411-1418)
-    LSB_TRY(sp_divuv(c, uN, vN, L.spec[S_DIVF], s));
-    LSB_TRY(zcontract_press(c, L.spec[S_DIVF], wS, r3, s));
+    // divf = ikx u + iky v; rhs3 = -d(divf)/dz - k^2 w (navierstokes.f90:1411-1418); on a
+    // symmetric grid divf is formed inside the contractions (here and in the projection)
+    const bool fused_p = zsplit_dz(c);
+    if (fused_p) {
+        LSB_TRY(zcontract_pressdiv(c, uN, vN, wS, r3, s));
+    } else {
+        LSB_TRY(sp_divuv(c, uN, vN, L.spec[S_DIVF], s));
+        LSB_TRY(zcontract_press(c, L.spec[S_DIVF], wS, r3, s));
+    }
     {
         double* rh[1] = {r3};
         const double* bt[1] = {sc_top};
         const double* bb[1] = {sc_bot};
         LSB_TRY(helmholtz(c, rh, nhom ? bt : nullptr, nhom ? bb : nullptr, 1, 0.0, s));
     }
-    {
+    double* pnew = L.spec[S->pn];
+    if (fused_p) {
+        LSB_TRY(zcontract_proj(c, r3, pnew, uN, vN, dt, c->bdfv, s));
+    } else {
         const double* in[1] = {r3};
         double* out[1] = {L.spec[S_DWDZ]};
         LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 1, s));
+        LSB_TRY(sp_project(c, L.spec[S_DWDZ], L.spec[S_DIVF], pnew, uN, vN, dt, c->bdfv, s));
     }
-    double* pnew = L.spec[S->pn];
-    LSB_TRY(sp_project(c, L.spec[S_DWDZ], L.spec[S_DIVF], pnew, uN, vN, dt, c->bdfv, s));
     // rotate (navierstokes.f90:1308-1312, 1332, 1399-1401, 1464)
     int ou[3], oun[3], ounm[3], onl[3], onlm[3], onn[3], orhs[4];
     for (int l = 0; l < 3; ++l) {

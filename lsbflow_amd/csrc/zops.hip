@@ -34,7 +34,13 @@ namespace lsb {
 
 // ZM_PRESS: out = -acc - (kx^2 + ky^2) aux; ZM_AUXSUB: out = aux - acc (aux a field at the
 // output positions, passed in the bct slot; out may alias aux)
-enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2, ZM_PRESS = 3, ZM_AUXSUB = 4 };
+// Parity-split kernel only:
+// ZM_PRESSDIV: out = -M div_h - (kx^2 + ky^2) aux with div_h = i kx u + i ky v formed as
+//   the panel is loaded (in.p[0] = u, in.p[1] = v, aux = w);
+// ZM_PROJ: the projection of navierstokes.f90:1466-1488 applied to y = M phi as it is
+//   stored: p = -(y + div_h)/(k^2 dt bdfv), u -= dt bdfv i kx p, v -= dt bdfv i ky p
+//   (out.p[0] = p, bct.p[0] = u, bct.p[1] = v, updated in place).
+enum { ZM_PLAIN = 0, ZM_HELM1 = 1, ZM_HELM2 = 2, ZM_PRESS = 3, ZM_AUXSUB = 4, ZM_PRESSDIV = 5, ZM_PROJ = 6 };
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -61,6 +67,7 @@ struct ZArgs {
     double osgn;
     const float* M32;   // FP32 copy of [P | Q] for the FP32 split kernel, Q at M32 + q32
     long q32;
+    double pdt, pbdfv;  // ZM_PROJ: dt and bdfv
 };
 
 constexpr int kZRB = 3;   // row blocks (of 16) per wave
@@ -441,7 +448,8 @@ __global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_p
 k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double> bct, FieldList<double> bcb) {
     constexpr bool kInMix = (MODE != ZM_HELM2);   // fold a physical input
     constexpr bool kOutMix = (MODE != ZM_HELM1);  // unfold into physical rows
-    constexpr bool kAux = (MODE == ZM_PRESS || MODE == ZM_AUXSUB);
+    constexpr bool kAux = (MODE == ZM_PRESS || MODE == ZM_AUXSUB || MODE == ZM_PRESSDIV);
+    constexpr bool kDiv = (MODE == ZM_PRESSDIV), kProj = (MODE == ZM_PROJ);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nw = blockDim.x >> 6;
@@ -468,14 +476,31 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
     // row (XR) is accumulated by every wave from per-lane loads.  Offsets are 32-bit
     // element offsets (a field holds < 2^29 elements).
     const double* __restrict__ X = in.p[f] + (long)a.in_row0 * P;
+    const double* __restrict__ Xv = kDiv ? in.p[1] + (long)a.in_row0 * P : nullptr;
     const double* __restrict__ Mp = a.M;
     const double* __restrict__ Mq = a.M2;
     const unsigned xstep = 4u * (unsigned)P, mstep = 4u * (unsigned)a.ldm;
+    // the pencil's wavenumbers; sgc = -1 on a real column, +1 on an imaginary one, so
+    // that (i k x)_c = sgc k x_(c^1) (derivatives.f90:306-326 divuv)
+    double wxc[CB], wyc[CB], sgc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        wxc[j] = 0.0;
+        wyc[j] = 0.0;
+        sgc[j] = 0.0;
+        if (kDiv || kProj) {
+            const int c = (int)cj[j];
+            wxc[j] = a.wavx[2 * ((c % a.nx) >> 1)];
+            wyc[j] = a.wavy[c / a.nx];
+            sgc[j] = (c & 1) ? 1.0 : -1.0;
+        }
+    }
     unsigned ot[CB], ob[CB];
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
-        ot[j] = (unsigned)lk * (unsigned)P + (unsigned)cj[j];
-        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + (unsigned)cj[j];
+        const unsigned cl = (unsigned)(kDiv ? (cj[j] ^ 1) : cj[j]);  // div_h reads the partner column
+        ot[j] = (unsigned)lk * (unsigned)P + cl;
+        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + cl;
     }
     const unsigned om = (unsigned)(rb0 * 16 + li + lk * a.ldm);
     const unsigned omx = (unsigned)((XR ? a.xrow : 0) + lk * a.ldm);
@@ -493,7 +518,7 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
     double xacc[CB];
 #pragma unroll
     for (int j = 0; j < CB; ++j) xacc[j] = 0.0;
-    double fa[PD][RB], fq[PD][RB], fx[PD][CB], fy[PD][CB], fxp[PD], fxq[PD];
+    double fa[PD][RB], fq[PD][RB], fx[PD][CB], fy[PD][CB], gx[PD][CB], gy[PD][CB], fxp[PD], fxq[PD];
     auto load = [&](int s, int k) {
         const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
         const unsigned xk = (unsigned)min(k, kmax) * xstep;
@@ -506,6 +531,10 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
         for (int j = 0; j < CB; ++j) {
             fx[s][j] = ldo(X, ot[j] + xk);
             fy[s][j] = ldo(X, ob[j] - xk);
+            if (kDiv) {
+                gx[s][j] = ldo(Xv, ot[j] + xk);
+                gy[s][j] = ldo(Xv, ob[j] - xk);
+            }
         }
         if (XR) {
             fxp[s] = ldo(Mp, omx + mk);
@@ -526,8 +555,13 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
             double sv[CB], dv[CB];
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
-                sv[j] = kInMix ? fx[s][j] + fy[s][j] : fx[s][j];
-                dv[j] = kInMix ? fx[s][j] - fy[s][j] : fy[s][j];
+                double xt = fx[s][j], xb = fy[s][j];
+                if (kDiv) {
+                    xt = sgc[j] * (wxc[j] * fx[s][j] + wyc[j] * gx[s][j]);
+                    xb = sgc[j] * (wxc[j] * fy[s][j] + wyc[j] * gy[s][j]);
+                }
+                sv[j] = kInMix ? xt + xb : xt;
+                dv[j] = kInMix ? xt - xb : xb;
             }
 #pragma unroll
             for (int i = 0; i < RB; ++i)
@@ -564,6 +598,25 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
     const double* __restrict__ auxf = kAux ? bct.p[f] : nullptr;
     double* __restrict__ dst = out.p[f] + (long)a.out_row0 * P;
     const double osgn = a.osgn;
+    double* __restrict__ pu = kProj ? bct.p[0] : nullptr;
+    double* __restrict__ pv = kProj ? bct.p[1] : nullptr;
+    // ZM_PROJ store of y at (row, col): every lane of a 16-lane quarter calls it (the
+    // partner column c^1 is the neighbouring lane), stores are masked by `st`
+    auto proj_store = [&](long row, int j, double y, double uc, double vc, bool st) {
+        const double up = __shfl_xor(uc, 1), vp = __shfl_xor(vc, 1);
+        const double divf = sgc[j] * (wxc[j] * up + wyc[j] * vp);
+        const double wav2 = wxc[j] * wxc[j] + wyc[j] * wyc[j];
+        double pc = 0.0;
+        if (wav2 != 0.0) pc = -(y + divf) / (wav2 * a.pdt * a.pbdfv);
+        const double pp = __shfl_xor(pc, 1);
+        const double dtb = a.pdt * a.pbdfv;
+        if (st) {
+            const long o = row * P + cj[j];
+            dst[o] = pc;
+            pu[o] = uc - dtb * (sgc[j] * wxc[j] * pp);
+            pv[o] = vc - dtb * (sgc[j] * wyc[j] * pp);
+        }
+    };
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
         if (i < myn) {
@@ -571,11 +624,17 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
             for (int half = 0; half < 2; ++half) {
                 // the stored row of (row block i, element v) in this half: r or n-1-r
                 // (operands are gathered before the first store, as in k_zcontract)
-                double ep[4][5], ax[4][CB];
+                double ep[4][5], ax[4][CB], ay[4][CB];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = min((rb0 + i) * 16 + lk + 4 * v, me - 1);
                     const int rc = half ? n - 1 - r : r;
+                    if (kProj)
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            ax[v][j] = pu[(long)rc * P + cj[j]];
+                            ay[v][j] = pv[(long)rc * P + cj[j]];
+                        }
                     if (MODE == ZM_HELM1) {
                         ep[v][0] = a.lam[rc];
                         if (nhom) {
@@ -617,28 +676,37 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
                                 x = x / (ep[v][0] + dfac[j]);
                             }
                             if (MODE == ZM_HELM2 && nhom) x = x + ep[v][0] * vbb[j] + ep[v][1] * vbt[j];
-                            if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[v][j];
+                            if (MODE == ZM_PRESS || kDiv) x = -x + dfac[j] * ax[v][j];
                             if (MODE == ZM_AUXSUB) x = ax[v][j] - x;
-                            if (col < P) dst[(long)row * P + col] = x;
+                            if (kProj)
+                                proj_store(row, j, x, ax[v][j], ay[v][j], col < P);
+                            else if (col < P)
+                                dst[(long)row * P + col] = x;
                         }
                     }
                 }
             }
         }
     }
-    if (dox && (MODE == ZM_PLAIN || kAux)) {
-        double ax[CB];
+    if (dox && (MODE == ZM_PLAIN || kAux || kProj)) {
+        double ax[CB], ay[CB];
 #pragma unroll
-        for (int j = 0; j < CB; ++j) ax[j] = kAux ? auxf[(long)a.xrow * P + cj[j]] : 0.0;
+        for (int j = 0; j < CB; ++j) {
+            ax[j] = kAux ? auxf[(long)a.xrow * P + cj[j]] : (kProj ? pu[(long)a.xrow * P + cj[j]] : 0.0);
+            ay[j] = kProj ? pv[(long)a.xrow * P + cj[j]] : 0.0;
+        }
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
             double x = xacc[j];
             x += __shfl_xor(x, 16);
             x += __shfl_xor(x, 32);
             const long col = p0 + 16 * j + li;
-            if (MODE == ZM_PRESS) x = -x + dfac[j] * ax[j];
+            if (MODE == ZM_PRESS || kDiv) x = -x + dfac[j] * ax[j];
             if (MODE == ZM_AUXSUB) x = ax[j] - x;
-            if (lk == 0 && col < P) dst[(long)a.xrow * P + col] = x;
+            if (kProj)
+                proj_store(a.xrow, j, x, ax[j], ay[j], lk == 0 && col < P);
+            else if (lk == 0 && col < P)
+                dst[(long)a.xrow * P + col] = x;
         }
     }
     if (MODE == ZM_HELM2 && g0 == 0 && wave == 0 && lk == 0) {
@@ -833,12 +901,14 @@ static int launch_zs(const ZArgs& a0, const ZSplitOp& op, const FieldList<double
                      const FieldList<double>& bt, const FieldList<double>& bb, int nf, hipStream_t s) {
     ZArgs a = a0;
     int gy = 1, nw = 1;
-    zsplit_shape(a, op, MODE == ZM_PLAIN || MODE == ZM_PRESS || MODE == ZM_AUXSUB, kZSRB, &gy, &nw);
+    // two input fields or a wider epilogue: two row blocks per wave keep three waves per SIMD
+    constexpr int RB = (MODE == ZM_PRESSDIV || MODE == ZM_PROJ) ? 2 : kZSRB;
+    zsplit_shape(a, op, MODE != ZM_HELM1 && MODE != ZM_HELM2, RB, &gy, &nw);
     dim3 grid((unsigned)((a.P + 16 * kZSCB - 1) / (16 * kZSCB)), (unsigned)gy, (unsigned)nf);
     if (a.xrow >= 0)
-        hipLaunchKernelGGL((k_zsplit<MODE, true>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
+        hipLaunchKernelGGL((k_zsplit<MODE, true, RB>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
     else
-        hipLaunchKernelGGL((k_zsplit<MODE, false>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
+        hipLaunchKernelGGL((k_zsplit<MODE, false, RB>), grid, dim3(64 * nw), 0, s, a, in, out, bt, bb);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
@@ -1155,6 +1225,67 @@ int zcontract_press(lsb_ctx* c, const double* divh, const double* w, double* rhs
         prof_mark_begin(c, PC_Z, s, &tok, 2.0 * c->nzp * (double)c->nzp * a.P, bytes);
         rc = launch_z<ZM_PRESS>(a, fi, fo, aux, z, 1, s);
     }
+    prof_mark_end(c, tok, s);
+    return rc;
+}
+
+bool zsplit_dz(const lsb_ctx* c) { return zs_fits(c->zs_dz, (long)c->plane_s); }
+
+// Pressure right-hand side with the horizontal divergence formed as the contraction
+// loads u and v (no div_h field): rhs = -d(i kx u + i ky v)/dz - (kx^2 + ky^2) w.
+int zcontract_pressdiv(lsb_ctx* c, const double* u, const double* v, const double* w, double* rhs, hipStream_t s) {
+    if (!zsplit_dz(c)) {
+        set_error("zcontract_pressdiv: needs the parity-split dfdz (grid symmetric about the mid-plane)");
+        return LSB_E_UNSUPPORTED;
+    }
+    ZArgs a{};
+    a.P = (long)c->plane_s;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nzp = c->nzp;
+    a.sigma = 0.0;
+    a.wavx = c->d_wavx;
+    a.wavy = c->d_wavy;
+    FieldList<double> fi{}, fo{}, aux{}, z{};
+    fi.p[0] = const_cast<double*>(u);
+    fi.p[1] = const_cast<double*>(v);
+    fo.p[0] = rhs;
+    aux.p[0] = const_cast<double*>(w);
+    int tok = -1;
+    prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P),
+                    4.0 * c->nzp * (double)a.P * sizeof(double));
+    const int rc = launch_zs<ZM_PRESSDIV>(a, c->zs_dz, fi, fo, aux, z, 1, s);
+    prof_mark_end(c, tok, s);
+    return rc;
+}
+
+// d(phi)/dz with the projection (navierstokes.f90:1466-1488) applied as it is stored:
+// p = -(dphi/dz + div_h)/(k^2 dt bdfv) (0 at k = 0), u -= dt bdfv i kx p,
+// v -= dt bdfv i ky p; div_h = i kx u + i ky v from the u, v it updates.
+int zcontract_proj(lsb_ctx* c, const double* phi, double* p, double* u, double* v, double dt, double bdfv,
+                   hipStream_t s) {
+    if (!zsplit_dz(c)) {
+        set_error("zcontract_proj: needs the parity-split dfdz (grid symmetric about the mid-plane)");
+        return LSB_E_UNSUPPORTED;
+    }
+    ZArgs a{};
+    a.P = (long)c->plane_s;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nzp = c->nzp;
+    a.wavx = c->d_wavx;
+    a.wavy = c->d_wavy;
+    a.pdt = dt;
+    a.pbdfv = bdfv;
+    FieldList<double> fi{}, fo{}, uv{}, z{};
+    fi.p[0] = const_cast<double*>(phi);
+    fo.p[0] = p;
+    uv.p[0] = u;
+    uv.p[1] = v;
+    int tok = -1;
+    prof_mark_begin(c, PC_Z, s, &tok, zsplit_flops(c->zs_dz, (double)a.P),
+                    6.0 * c->nzp * (double)a.P * sizeof(double));
+    const int rc = launch_zs<ZM_PROJ>(a, c->zs_dz, fi, fo, uv, z, 1, s);
     prof_mark_end(c, tok, s);
     return rc;
 }
