@@ -68,6 +68,8 @@ struct ZArgs {
     const float* M32;   // FP32 copy of [P | Q] for the FP32 split kernel, Q at M32 + q32
     long q32;
     double pdt, pbdfv;  // ZM_PROJ: dt and bdfv
+    const double* M3;   // k_zhelm: stage-2 P and Q (stage 1 in M, M2)
+    const double* M4;
 };
 
 constexpr int kZRB = 3;   // row blocks (of 16) per wave
@@ -916,6 +918,247 @@ static int launch_zs(const ZArgs& a0, const ZSplitOp& op, const FieldList<double
 // the split kernels address a field with 32-bit element offsets
 static bool zs_fits(const ZSplitOp& op, long P) { return op.ok && (long)op.n * P < (1L << 29) && op.n >= 8; }
 
+// Both Helmholtz projections of a block's pencils in one kernel: stage 1 (folded
+// rhs, Einv halves, lifting and 1/(lam - k^2 - sigma)) leaves the parity-ordered
+// mode values of its 16 CB pencils in LDS; after a barrier stage 2 (E halves, h bc,
+// boundary planes) reads them from there.  The mode-space buffer of the two-kernel
+// form (two HBM passes over nzm planes per field) disappears.  The block owns its
+// pencils for all rows, so the in-place update of rhs is race-free.
+template <int RB, int CB, int PD, int WPE>
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zhelm(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double> bcb) {
+    extern __shared__ double T[];  // n rows x W pencils
+    constexpr int W = 16 * CB;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int f = blockIdx.z;
+    const int me = a.nout, n = a.nsp, mo = n - me;
+    const int nrb = (me + 15) >> 4;
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    const int rb0 = wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * W;
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    double* __restrict__ Fq = fld.p[f];
+    const double* __restrict__ X = Fq + P;  // interior rows 1..nz
+    const unsigned xstep = 4u * (unsigned)P, mstep = 4u * (unsigned)a.ldm;
+    unsigned ot[CB], ob[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        ot[j] = (unsigned)lk * (unsigned)P + (unsigned)cj[j];
+        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + (unsigned)cj[j];
+    }
+    const unsigned om = (unsigned)(rb0 * 16 + li + lk * a.ldm);
+    const int nks = (me + 3) >> 2;
+    const int nkp = (nks + PD - 1) / PD * PD;
+    const int kmax = (n - 4) >> 2;
+    // per-pencil data of both stages
+    double dfac[CB], vbt[CB], vbb[CB];
+    const bool nhom = bct.p[f] != nullptr;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        const int c = (int)cj[j];
+        const double wx = a.wavx[c % a.nx], wy = a.wavy[c / a.nx];
+        dfac[j] = -(wx * wx) - wy * wy - a.sigma;
+        vbt[j] = nhom ? bct.p[f][c] : 0.0;
+        vbb[j] = nhom ? bcb.p[f][c] : 0.0;
+    }
+    v4d accE[RB][CB], accO[RB][CB];
+    double fa[PD][RB], fq[PD][RB], fx[PD][CB], fy[PD][CB];
+    // ---------------------------------------------------------------- stage 1
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            accO[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
+    {
+        const double* __restrict__ Mp = a.M;
+        const double* __restrict__ Mq = a.M2;
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+            const unsigned xk = (unsigned)min(k, kmax) * xstep;
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(Mp, om + mk + 16u * i);
+                fq[s][i] = ldo(Mq, om + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = ldo(X, ot[j] + xk);
+                fy[s][j] = ldo(X, ob[j] - xk);
+            }
+            asm volatile("" ::: "memory");  // keep the prefetch in place (see k_zsplit)
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                double sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    sv[j] = fx[s][j] + fy[s][j];
+                    dv[j] = fx[s][j] - fy[s][j];
+                }
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                        accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                    }
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // stage-1 epilogue into LDS: even mode r -> row r, odd mode r -> row n-1-r
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                double ep[4][5];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rb0 + i) * 16 + lk + 4 * v, me - 1);
+                    const int rc = half ? n - 1 - r : r;
+                    ep[v][0] = a.lam[rc];
+                    if (nhom) {
+                        ep[v][1] = a.elift[rc];
+                        ep[v][2] = a.elift[n + rc];
+                        ep[v][3] = a.elift[2 * n + rc];
+                        ep[v][4] = a.elift[3 * n + rc];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + lk + 4 * v;
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            double x = half ? accO[i][j][v] : accE[i][j][v];
+                            if (nhom) {
+                                x = x - vbb[j] * (ep[v][3] + dfac[j] * ep[v][1]);
+                                x = x - vbt[j] * (ep[v][4] + dfac[j] * ep[v][2]);
+                            }
+                            x = x / (ep[v][0] + dfac[j]);
+                            T[row * W + 16 * j + li] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---------------------------------------------------------------- stage 2
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            accO[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
+    {
+        const double* __restrict__ Mp = a.M3;
+        const double* __restrict__ Mq = a.M4;
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(Mp, om + mk + 16u * i);
+                fq[s][i] = ldo(Mq, om + mk + 16u * i);
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                // mode values from LDS: even mode kr in row kr, odd mode kr in row n-1-kr
+                // (rows past me - 1 meet zero matrix columns; clamped to stay in the tile)
+                const int kr = min(4 * (k0 + s) + lk, me - 1);
+                double sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    sv[j] = T[kr * W + 16 * j + li];
+                    dv[j] = T[(n - 1 - kr) * W + 16 * j + li];
+                }
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                        accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                    }
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // stage-2 epilogue: physical rows r and n-1-r (interior row index; field row + 1)
+    double* __restrict__ dst = Fq + P;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                double ep[4][2];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rb0 + i) * 16 + lk + 4 * v, me - 1);
+                    const int rc = half ? n - 1 - r : r;
+                    ep[v][0] = nhom ? a.lift[rc + 1] : 0.0;
+                    ep[v][1] = nhom ? a.lift[a.nzp + rc + 1] : 0.0;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + lk + 4 * v;
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const long col = p0 + 16 * j + li;
+                            const double e = accE[i][j][v], o = accO[i][j][v];
+                            double x = half ? (e - o) : e + o;
+                            if (nhom) x = x + ep[v][0] * vbb[j] + ep[v][1] * vbt[j];
+                            if (col < P) dst[(long)row * P + col] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (wave == 0 && lk == 0) {
+        // boundary planes 0 and nz
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const long col = p0 + 16 * j + li;
+            double v0 = 0.0, vn = 0.0;
+            if (nhom) {
+                v0 = 0.0 + a.lift[0] * vbb[j] + a.lift[a.nzp + 0] * vbt[j];
+                vn = 0.0 + a.lift[a.nzp - 1] * vbb[j] + a.lift[a.nzp + a.nzp - 1] * vbt[j];
+            }
+            if (col < P) {
+                Fq[col] = v0;
+                Fq[(long)(a.nzp - 1) * P + col] = vn;
+            }
+        }
+    }
+}
+// two row blocks x two column blocks per wave set (measured against 1-3 x 1-2: the
+// 49 KB LDS tile per block leaves room for three blocks per CU)
+constexpr int kZHRB = 2, kZHCB = 2;
+
 // flops of one split contraction: two me x me products per column
 static double zsplit_flops(const ZSplitOp& op, double cols) { return 4.0 * op.me * (double)op.me * cols; }
 
@@ -1299,16 +1542,20 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
         return LSB_E_ARG;
     }
     const int nzm = c->nzm;
-    const size_t tbytes = (size_t)nf * nzm * c->plane_s * sizeof(double);
-    void* tbuf = nullptr;
-    LSB_TRY(stage_get(c, 7, tbytes, &tbuf));
     FieldList<double> fr{}, ft{}, bt{}, bb{};
     for (int f = 0; f < nf; ++f) {
         fr.p[f] = rhs[f];
-        ft.p[f] = (double*)tbuf + (size_t)f * nzm * c->plane_s;
         bt.p[f] = bc_top ? const_cast<double*>(bc_top[f]) : nullptr;
         bb.p[f] = bc_bot ? const_cast<double*>(bc_bot[f]) : nullptr;
     }
+    // the two-kernel forms pass the mode values through a staging buffer
+    auto stage_modes = [&]() -> int {
+        const size_t tbytes = (size_t)nf * nzm * c->plane_s * sizeof(double);
+        void* tbuf = nullptr;
+        LSB_TRY(stage_get(c, 7, tbytes, &tbuf));
+        for (int f = 0; f < nf; ++f) ft.p[f] = (double*)tbuf + (size_t)f * nzm * c->plane_s;
+        return LSB_OK;
+    };
     ZArgs a{};
     a.ldm = ((nzm + 63) / 64) * 64 + 64;
     a.nout = nzm;
@@ -1325,9 +1572,27 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
     a.elift = c->d_lift + 4 * c->nzp;
     int tok = -1;
     const double bytes = 2.0 * c->nzp * (double)a.P * nf * sizeof(double);
-    if (zs_fits(c->zs_h1, a.P) && zs_fits(c->zs_h2, a.P)) {
-        // parity-split projections; the mode buffer is in parity order
+    if (zs_fits(c->zs_h1, a.P) && zs_fits(c->zs_h2, a.P) && c->zs_h1.ldm == c->zs_h2.ldm) {
+        // both parity-split projections in one kernel, the mode values in LDS
         prof_mark_begin(c, PC_HELM, s, &tok, 2.0 * zsplit_flops(c->zs_h1, (double)a.P) * nf, bytes);
+        a.lam = c->d_helm_s;
+        a.elift = c->d_helm_s + nzm;
+        int gy = 1, nw = 1;
+        zsplit_shape(a, c->zs_h1, false, kZHRB, &gy, &nw);
+        a.M3 = c->zs_h2.d_m;
+        a.M4 = c->zs_h2.d_m + c->zs_h2.qoff;
+        const size_t lds = (size_t)nzm * 16 * kZHCB * sizeof(double);
+        dim3 grid((unsigned)((a.P + 16 * kZHCB - 1) / (16 * kZHCB)), 1, (unsigned)nf);
+        const char* off = std::getenv("LSB_NO_ZHELM");  // the two-kernel form, for tests
+        const bool ok = gy == 1 && lds <= 64 * 1024 && !(off && off[0] && std::strcmp(off, "0") != 0);
+        if (ok) hipLaunchKernelGGL((k_zhelm<kZHRB, kZHCB, kZPD, 3>), grid, dim3(64 * nw), lds, s, a, fr, bt, bb);
+        if (ok) {
+            prof_mark_end(c, tok, s);
+            LSB_HIP(hipGetLastError());
+            return LSB_OK;
+        }
+        // larger nz: two kernels through the mode-space buffer
+        LSB_TRY(stage_modes());
         a.lam = c->d_helm_s;
         a.elift = c->d_helm_s + nzm;
         a.in_row0 = 1;
@@ -1342,6 +1607,7 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
         return rc;
     }
     // two nzm x nzm projections per pencil; read rhs and write the solution once each
+    LSB_TRY(stage_modes());
     prof_mark_begin(c, PC_HELM, s, &tok, 4.0 * nzm * (double)nzm * a.P * nf, bytes);
     a.M = c->d_simat;
     a.in_row0 = 1;

@@ -31,15 +31,19 @@ def make(nx=32, ny=16, nz=40, **kw):
     return o, c
 
 
-@pytest.fixture(params=["split", "dense"])
+@pytest.fixture(params=["split", "split2", "dense"])
 def zpath(request, monkeypatch):
     """On a grid symmetric about the mid-plane the z operators run parity-split (two
-    half-size products); LSB_NO_ZSPLIT=1 keeps the dense contraction that any other
-    grid uses.  Both must match the oracle."""
+    half-size products) and the two Helmholtz projections run as one kernel with the
+    mode values in LDS; LSB_NO_ZHELM=1 keeps them as two kernels through HBM (the
+    form for nz > 256), LSB_NO_ZSPLIT=1 the dense contraction that any other grid
+    uses.  All must match the oracle."""
+    monkeypatch.delenv("LSB_NO_ZSPLIT", raising=False)
+    monkeypatch.delenv("LSB_NO_ZHELM", raising=False)
     if request.param == "dense":
         monkeypatch.setenv("LSB_NO_ZSPLIT", "1")
-    else:
-        monkeypatch.delenv("LSB_NO_ZSPLIT", raising=False)
+    elif request.param == "split2":
+        monkeypatch.setenv("LSB_NO_ZHELM", "1")
     return request.param
 
 

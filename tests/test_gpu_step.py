@@ -104,6 +104,16 @@ def test_c1_dns_dense_z_operators(monkeypatch):
     check_fields(st, g)
 
 
+def test_c1_dns_two_kernel_helmholtz(monkeypatch):
+    """The parity-split Helmholtz projections as two kernels through the mode-space
+    buffer (the form for nz > 256); the default on this grid keeps the mode values
+    of a block's pencils in LDS between the two projections."""
+    monkeypatch.setenv("LSB_NO_ZHELM", "1")
+    case = L.cases.CASES["c1_dns"]
+    st, g = run_both(case, case.nsteps)
+    check_fields(st, g)
+
+
 def test_matmul_derivative_path():
     """nz <= 31 uses matmul(d, .) instead of the DCT operator (derivatives.f90:122-129)."""
     case = L.cases.CASES["c1_dns"].with_(nx=16, ny=8, nz=24, pert_rms=0.02)
