@@ -57,25 +57,32 @@ __global__ void k_zero_pads(FieldList<double> f, int nf, long nrows, int nx, int
 }
 
 // ---------------------------------------------------------- SGS per point
-// sgsterm.f90:283-423 sigma model: eddy viscosity from the singular values of g.
+// sgsterm.f90:283-423 sigma model: eddy viscosity from the singular values of g,
+// through the invariants of G = g^T g and the trigonometric roots of its
+// characteristic cubic.  The same quantities as the reference with fewer
+// operations: G is symmetric (6 dot products), tr(G^2) is the sum of squares of
+// G, the three roots use cos(pi/3 +- a) = cos(a)/2 -+ (sqrt 3/2) sin(a) from one
+// sincos (a in [0, pi/3]), and the constant divisions are multiplications.
 __device__ __forceinline__ double sigma_nu(const double g[3][3], double delta) {
     const double pi = 3.141592653589793;
-    double G[3][3], G2[3][3];
+    double G[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) G[a][b] = g[0][a] * g[0][b] + g[1][a] * g[1][b] + g[2][a] * g[2][b];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) G2[a][b] = G[a][0] * G[0][b] + G[a][1] * G[1][b] + G[a][2] * G[2][b];
+        for (int b = a; b < 3; ++b) G[a][b] = g[0][a] * g[0][b] + g[1][a] * g[1][b] + g[2][a] * g[2][b];
+    G[1][0] = G[0][1];
+    G[2][0] = G[0][2];
+    G[2][1] = G[1][2];
     const double i1 = G[0][0] + G[1][1] + G[2][2];
-    const double i2 = 0.5 * (i1 * i1 - (G2[0][0] + G2[1][1] + G2[2][2]));
+    const double trg2 = G[0][0] * G[0][0] + G[1][1] * G[1][1] + G[2][2] * G[2][2] +
+                        2.0 * (G[0][1] * G[0][1] + G[0][2] * G[0][2] + G[1][2] * G[1][2]);
+    const double i2 = 0.5 * (i1 * i1 - trg2);
     const double i3 = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
                       G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
                       G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
-    double a1 = i1 * i1 / 9. - i2 / 3.;
-    const double a2 = i1 * i1 * i1 / 27. - i1 * i2 / 6. + 0.5 * i3;
+    const double third = 1.0 / 3.0;
+    double a1 = i1 * i1 * (1.0 / 9.0) - i2 * third;
+    const double a2 = i1 * i1 * i1 * (1.0 / 27.0) - i1 * i2 * (1.0 / 6.0) + 0.5 * i3;
     if (a1 < 0.0) a1 = fabs(a1);
     const double sa = sqrt(a1);
     const double pre = a2 / (sa * sa * sa);
@@ -85,10 +92,17 @@ __device__ __forceinline__ double sigma_nu(const double g[3][3], double delta) {
     else if (pre <= -1.)
         a3 = pi / 3.;
     else
-        a3 = acos(pre) / 3.;
-    const double s1 = sqrt(fabs(i1 / 3. + 2. * sa * cos(a3)));
-    const double s2 = sqrt(fabs(i1 / 3. - 2. * sa * cos(pi / 3. + a3)));
-    const double s3 = sqrt(fabs(i1 / 3. - 2. * sa * cos(pi / 3. - a3)));
+        a3 = acos(pre) * third;
+    double sn, cs;
+    sincos(a3, &sn, &cs);
+    const double r3 = 0.86602540378443865;  // sqrt(3)/2
+    const double c1 = cs;                   // cos(a3)
+    const double c2 = 0.5 * cs - r3 * sn;   // cos(pi/3 + a3)
+    const double c3 = 0.5 * cs + r3 * sn;   // cos(pi/3 - a3)
+    const double t = i1 * third, s2a = 2. * sa;
+    const double s1 = sqrt(fabs(t + s2a * c1));
+    const double s2 = sqrt(fabs(t - s2a * c2));
+    const double s3 = sqrt(fabs(t - s2a * c3));
     double ds = s3 * (s1 - s2) * (s2 - s3);
     if (s1 != 0.) ds = ds / (s1 * s1);
     if (ds < 0.0 || isnan(ds)) ds = 0.0;
