@@ -472,7 +472,7 @@ k_zsplit(ZArgs a, FieldList<double> in, FieldList<double> out, FieldList<double>
     for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
     // The k loop has no branches, so that the loads of k-step s + PD stay in flight
     // while step s multiplies: the k-steps are padded to a multiple of PD (the matrix
-    // has zero columns up to a multiple of 16), all RB row blocks are multiplied (rows
+    // has zero columns up to a multiple of 48), all RB row blocks are multiplied (rows
     // past the wave's share are zero or unused), the prefetch position is clamped
     // (matrix: last padded step; panel: last k-step whose 4 rows fit), and the middle
     // row (XR) is accumulated by every wave from per-lane loads.  Offsets are 32-bit
@@ -1070,12 +1070,21 @@ k_zhelm(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double>
     {
         const double* __restrict__ Mp = a.M3;
         const double* __restrict__ Mq = a.M4;
+        // mode values from LDS: even mode kr in row kr, odd mode kr in row n-1-kr (rows
+        // past me - 1 meet zero matrix columns; clamped to stay in the tile); they are
+        // prefetched with the matrix fragments, as the panel is in stage 1
         auto load = [&](int s, int k) {
             const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+            const int kr = min(4 * k + lk, me - 1);
 #pragma unroll
             for (int i = 0; i < RB; ++i) {
                 fa[s][i] = ldo(Mp, om + mk + 16u * i);
                 fq[s][i] = ldo(Mq, om + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = T[kr * W + 16 * j + li];
+                fy[s][j] = T[(n - 1 - kr) * W + 16 * j + li];
             }
             asm volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
@@ -1085,21 +1094,12 @@ k_zhelm(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double>
         for (int k0 = 0; k0 < nkp; k0 += PD) {
 #pragma unroll
             for (int s = 0; s < PD; ++s) {
-                // mode values from LDS: even mode kr in row kr, odd mode kr in row n-1-kr
-                // (rows past me - 1 meet zero matrix columns; clamped to stay in the tile)
-                const int kr = min(4 * (k0 + s) + lk, me - 1);
-                double sv[CB], dv[CB];
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    sv[j] = T[kr * W + 16 * j + li];
-                    dv[j] = T[(n - 1 - kr) * W + 16 * j + li];
-                }
 #pragma unroll
                 for (int i = 0; i < RB; ++i)
 #pragma unroll
                     for (int j = 0; j < CB; ++j) {
-                        accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
-                        accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                        accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], fx[s][j], accE[i][j], 0, 0, 0);
+                        accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], fy[s][j], accO[i][j], 0, 0, 0);
                     }
                 load(s, k0 + s + PD);
             }
@@ -1182,7 +1182,9 @@ int zs_upload(lsb_ctx* c, ZSplitOp* op, int n, double osgn, const std::vector<do
               const std::vector<double>& Q) {
     const int me = (n + 1) / 2;
     const int ldm = ((me + 63) / 64) * 64 + 64;
-    const int kc = (me + 15) / 16 * 16;  // zero columns: k-steps padded to the prefetch depth
+    // zero columns: the k-steps (4 columns each) are padded to a multiple of the
+    // prefetch depth PD; 48 = 4 * lcm(1, 2, 3, 4) covers every PD up to 4
+    const int kc = (me + 47) / 48 * 48;
     std::vector<double> h((size_t)2 * ldm * kc, 0.0);
     const size_t qoff = (size_t)ldm * kc;
     for (int j = 0; j < me; ++j)
@@ -1585,7 +1587,7 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
         dim3 grid((unsigned)((a.P + 16 * kZHCB - 1) / (16 * kZHCB)), 1, (unsigned)nf);
         const char* off = std::getenv("LSB_NO_ZHELM");  // the two-kernel form, for tests
         const bool ok = gy == 1 && lds <= 64 * 1024 && !(off && off[0] && std::strcmp(off, "0") != 0);
-        if (ok) hipLaunchKernelGGL((k_zhelm<kZHRB, kZHCB, kZPD, 3>), grid, dim3(64 * nw), lds, s, a, fr, bt, bb);
+        if (ok) hipLaunchKernelGGL((k_zhelm<kZHRB, kZHCB, 3, 3>), grid, dim3(64 * nw), lds, s, a, fr, bt, bb);
         if (ok) {
             prof_mark_end(c, tok, s);
             LSB_HIP(hipGetLastError());
