@@ -324,6 +324,9 @@ bool zsplit_dz(const lsb_ctx* c);
 int zcontract_pressdiv(lsb_ctx* c, const double* u, const double* v, const double* w, double* rhs, hipStream_t s);
 int zcontract_proj(lsb_ctx* c, const double* phi, double* p, double* u, double* v, double dt, double bdfv,
                    hipStream_t s);
+// the whole pressure step in one kernel; LSB_E_UNSUPPORTED when it does not apply
+int zpress_step(lsb_ctx* c, double* u, double* v, const double* w, double* wout, double* p, const double* bc_top,
+                const double* bc_bot, double dt, double bdfv, hipStream_t s);
 // mixed form: FP32 matrix-core contraction, panel TI, output TO (FP32 products and sums)
 // P columns per plane (0: spectral nx*ny); with aux, out = aux - M in (out may alias aux)
 template <class TI, class TO>

@@ -966,6 +966,12 @@ static int update_once(lsb_ctx* c) {
     double* r3 = L.spec[S->rhs[3]];
     // divf = ikx u + iky v; rhs3 = -d(divf)/dz - k^2 w (navierstokes.f90:1411-1418); on a
     // symmetric grid divf is formed inside the contractions (here and in the projection)
+    double* pnew = L.spec[S->pn];
+    // the whole step in one kernel, the intermediate fields of a block's pencils in LDS
+    int rcp = zpress_step(c, uN, vN, wS, r3, pnew, nhom ? sc_top : nullptr, nhom ? sc_bot : nullptr, dt, c->bdfv, s);
+    if (rcp != LSB_E_UNSUPPORTED) {
+        LSB_TRY(rcp);
+    } else {
     const bool fused_p = zsplit_dz(c);
     if (fused_p) {
         LSB_TRY(zcontract_pressdiv(c, uN, vN, wS, r3, s));
@@ -979,7 +985,6 @@ static int update_once(lsb_ctx* c) {
         const double* bb[1] = {sc_bot};
         LSB_TRY(helmholtz(c, rh, nhom ? bt : nullptr, nhom ? bb : nullptr, 1, 0.0, s));
     }
-    double* pnew = L.spec[S->pn];
     if (fused_p) {
         LSB_TRY(zcontract_proj(c, r3, pnew, uN, vN, dt, c->bdfv, s));
     } else {
@@ -987,6 +992,7 @@ static int update_once(lsb_ctx* c) {
         double* out[1] = {L.spec[S_DWDZ]};
         LSB_TRY(zcontract(c, c->d_mdz, c->ldm_z, c->nzp, c->nzp, in, out, 1, s));
         LSB_TRY(sp_project(c, L.spec[S_DWDZ], L.spec[S_DIVF], pnew, uN, vN, dt, c->bdfv, s));
+    }
     }
     // rotate (navierstokes.f90:1308-1312, 1332, 1399-1401, 1464)
     int ou[3], oun[3], ounm[3], onl[3], onlm[3], onn[3], orhs[4];

@@ -1159,6 +1159,471 @@ k_zhelm(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double>
 // 49 KB LDS tile per block leaves room for three blocks per CU)
 constexpr int kZHRB = 2, kZHCB = 2;
 
+// The pressure step of navierstokes.f90:1411-1488 for a block's 16 CB pencils in
+// one kernel, the intermediate field of each stage in one LDS tile (nzp rows):
+//   A  r = -d(i kx u + i ky v)/dz - k^2 w          (dfdz split, div_h in the loads)
+//   B  modes = Einv-split fold of r's interior rows, lifting, 1/(lam - k^2)
+//   C  w_new = E-split unfold of the modes + h bc, boundary planes  (-> HBM and LDS)
+//   D  y = dw_new/dz, p = -(y + div_h)/(k^2 dt bdfv), u, v -= dt bdfv (i kx, i ky) p
+// Each stage repeats the operations of its stand-alone kernel (k_zsplit PRESSDIV,
+// k_zhelm, k_zsplit PROJ), so the results are those of the three-kernel sequence.
+struct ZPressArgs {
+    const double *Dp, *Dq;                      // dfdz halves (n = nzp)
+    const double *H1p, *H1q, *H2p, *H2q;        // Helmholtz halves (n = nzm)
+    int ldD, ldH, nD, meD, nH, meH, xrow;       // xrow: middle row of dfdz on the VALU (-1: none)
+    long P;
+    int nx;
+    const double* wavx;
+    const double* wavy;
+    const double* lam;    // parity-ordered eigenvalues, then Einv of the lifting vectors
+    const double* elift;
+    const double* lift;   // hbot, htop (nzp each)
+    const double* bct;    // Dirichlet data of the Poisson solve (plane 0 / plane nzp-1), nullable
+    const double* bcb;
+    double* u;            // updated in place
+    double* v;
+    const double* w;      // w at the start of the pressure step
+    double* wout;         // w_new (all nzp planes)
+    double* p;            // pressure
+    double dt, bdfv;
+};
+
+template <int RB, int CB, int PD, bool XR>
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(3)))
+k_zpress(ZPressArgs a) {
+    extern __shared__ double T[];  // nzp rows x W pencils
+    constexpr int W = 16 * CB;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * W;
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    // per-pencil constants (as the stand-alone kernels compute them)
+    double wxc[CB], wyc[CB], sgc[CB], dfac[CB], vbt[CB], vbb[CB];
+    const bool nhom = a.bct != nullptr;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        const int c = (int)cj[j];
+        wxc[j] = a.wavx[2 * ((c % a.nx) >> 1)];
+        wyc[j] = a.wavy[c / a.nx];
+        sgc[j] = (c & 1) ? 1.0 : -1.0;
+        const double wx = a.wavx[c % a.nx], wy = a.wavy[c / a.nx];
+        dfac[j] = -(wx * wx) - wy * wy - 0.0;
+        vbt[j] = nhom ? a.bct[c] : 0.0;
+        vbb[j] = nhom ? a.bcb[c] : 0.0;
+    }
+    // row blocks of the two operator sizes
+    const int nD = a.nD, meD = a.meD, moD = nD - meD;
+    const int nH = a.nH, meH = a.meH, moH = nH - meH;
+    const int nrbD = XR ? (meD >> 4) : ((meD + 15) >> 4);
+    const int nrbH = (meH + 15) >> 4;
+    const int bD = nrbD / nw, xD = nrbD % nw, mynD = bD + (wave < xD ? 1 : 0), rbD = wave * bD + min(wave, xD);
+    const int bH = nrbH / nw, xH = nrbH % nw, mynH = bH + (wave < xH ? 1 : 0), rbH = wave * bH + min(wave, xH);
+    const unsigned omD = (unsigned)(rbD * 16 + li + lk * a.ldD), omH = (unsigned)(rbH * 16 + li + lk * a.ldH);
+    const unsigned omx = (unsigned)((XR ? a.xrow : 0) + lk * a.ldD);
+    const unsigned mstD = 4u * (unsigned)a.ldD, mstH = 4u * (unsigned)a.ldH;
+    const int nkD = (meD + 3) >> 2, nkpD = (nkD + PD - 1) / PD * PD, kmaxD = (nD - 4) >> 2;
+    const int nkH = (meH + 3) >> 2, nkpH = (nkH + PD - 1) / PD * PD;
+    const bool dox = XR && wave == 0;
+    v4d accE[RB][CB], accO[RB][CB];
+    double fa[PD][RB], fq[PD][RB], fx[PD][CB], fy[PD][CB], gx[PD][CB], gy[PD][CB], fxp[PD], fxq[PD];
+    double xacc[CB];
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                accE[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+                accO[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) xacc[j] = 0.0;
+    };
+    auto mfma_step = [&](int s, const double* sv, const double* dv) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                accE[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                accO[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+            }
+    };
+    // ------------------------------------------------------------------ stage A
+    zero_acc();
+    {
+        const double* __restrict__ U = a.u;
+        const double* __restrict__ V = a.v;
+        unsigned ot[CB], ob[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const unsigned cl = (unsigned)(cj[j] ^ 1);
+            ot[j] = (unsigned)lk * (unsigned)P + cl;
+            ob[j] = (unsigned)(nD - 1 - lk) * (unsigned)P + cl;
+        }
+        const unsigned xstep = 4u * (unsigned)P;
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkpD - 1) * mstD;
+            const unsigned xk = (unsigned)min(k, kmaxD) * xstep;
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(a.Dp, omD + mk + 16u * i);
+                fq[s][i] = ldo(a.Dq, omD + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = ldo(U, ot[j] + xk);
+                fy[s][j] = ldo(U, ob[j] - xk);
+                gx[s][j] = ldo(V, ot[j] + xk);
+                gy[s][j] = ldo(V, ob[j] - xk);
+            }
+            if (XR) {
+                fxp[s] = ldo(a.Dp, omx + mk);
+                fxq[s] = ldo(a.Dq, omx + mk);
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkpD; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                double sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    const double xt = sgc[j] * (wxc[j] * fx[s][j] + wyc[j] * gx[s][j]);
+                    const double xb = sgc[j] * (wxc[j] * fy[s][j] + wyc[j] * gy[s][j]);
+                    sv[j] = xt + xb;
+                    dv[j] = xt - xb;
+                }
+                mfma_step(s, sv, dv);
+                if (XR)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) xacc[j] = fma(fxq[s], dv[j], fma(fxp[s], sv[j], xacc[j]));
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // r = -acc + dfac w  -> LDS rows 0..nzp-1
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < mynD) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                double ax[4][CB];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rbD + i) * 16 + lk + 4 * v, meD - 1);
+                    const int rc = half ? nD - 1 - r : r;
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) ax[v][j] = a.w[(long)rc * P + cj[j]];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rbD + i) * 16 + lk + 4 * v;
+                    if (r < meD && (half == 0 || r < moD)) {
+                        const int row = half ? nD - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const double e = accE[i][j][v], o = accO[i][j][v];
+                            double x = half ? -1.0 * (e - o) : e + o;
+                            x = -x + dfac[j] * ax[v][j];
+                            T[row * W + 16 * j + li] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (dox) {
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const double ax = a.w[(long)a.xrow * P + cj[j]];
+            double x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            x = -x + dfac[j] * ax;
+            if (lk == 0) T[a.xrow * W + 16 * j + li] = x;
+        }
+    }
+    __syncthreads();
+    // ------------------------------------------------------------------ stage B
+    // Einv-split of the interior rows 1..nz (fold around the interior mid-row)
+    zero_acc();
+    {
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkpH - 1) * mstH;
+            const int kr = min(4 * k + lk, meH - 1);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(a.H1p, omH + mk + 16u * i);
+                fq[s][i] = ldo(a.H1q, omH + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = T[(1 + kr) * W + 16 * j + li];
+                fy[s][j] = T[(nH - kr) * W + 16 * j + li];
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkpH; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                double sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    sv[j] = fx[s][j] + fy[s][j];
+                    dv[j] = fx[s][j] - fy[s][j];
+                }
+                mfma_step(s, sv, dv);
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    __syncthreads();  // every wave has read r
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < mynH) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                double ep[4][5];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rbH + i) * 16 + lk + 4 * v, meH - 1);
+                    const int rc = half ? nH - 1 - r : r;
+                    ep[v][0] = a.lam[rc];
+                    if (nhom) {
+                        ep[v][1] = a.elift[rc];
+                        ep[v][2] = a.elift[nH + rc];
+                        ep[v][3] = a.elift[2 * nH + rc];
+                        ep[v][4] = a.elift[3 * nH + rc];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rbH + i) * 16 + lk + 4 * v;
+                    if (r < meH && (half == 0 || r < moH)) {
+                        const int row = half ? nH - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            double x = half ? accO[i][j][v] : accE[i][j][v];
+                            if (nhom) {
+                                x = x - vbb[j] * (ep[v][3] + dfac[j] * ep[v][1]);
+                                x = x - vbt[j] * (ep[v][4] + dfac[j] * ep[v][2]);
+                            }
+                            x = x / (ep[v][0] + dfac[j]);
+                            T[row * W + 16 * j + li] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ------------------------------------------------------------------ stage C
+    zero_acc();
+    {
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkpH - 1) * mstH;
+            const int kr = min(4 * k + lk, meH - 1);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(a.H2p, omH + mk + 16u * i);
+                fq[s][i] = ldo(a.H2q, omH + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = T[kr * W + 16 * j + li];
+                fy[s][j] = T[(nH - 1 - kr) * W + 16 * j + li];
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkpH; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                mfma_step(s, fx[s], fy[s]);
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    __syncthreads();  // every wave has read the modes
+    {
+        double* __restrict__ wo = a.wout + P;  // interior rows
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            if (i < mynH) {
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    double ep[4][2];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = min((rbH + i) * 16 + lk + 4 * v, meH - 1);
+                        const int rc = half ? nH - 1 - r : r;
+                        ep[v][0] = nhom ? a.lift[rc + 1] : 0.0;
+                        ep[v][1] = nhom ? a.lift[nD + rc + 1] : 0.0;
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = (rbH + i) * 16 + lk + 4 * v;
+                        if (r < meH && (half == 0 || r < moH)) {
+                            const int row = half ? nH - 1 - r : r;
+#pragma unroll
+                            for (int j = 0; j < CB; ++j) {
+                                const long col = p0 + 16 * j + li;
+                                const double e = accE[i][j][v], o = accO[i][j][v];
+                                double x = half ? (e - o) : e + o;
+                                if (nhom) x = x + ep[v][0] * vbb[j] + ep[v][1] * vbt[j];
+                                T[(row + 1) * W + 16 * j + li] = x;
+                                if (col < P) wo[(long)row * P + col] = x;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (wave == 0 && lk == 0) {
+            // boundary planes 0 and nz
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const long col = p0 + 16 * j + li;
+                double v0 = 0.0, vn = 0.0;
+                if (nhom) {
+                    v0 = 0.0 + a.lift[0] * vbb[j] + a.lift[nD + 0] * vbt[j];
+                    vn = 0.0 + a.lift[nD - 1] * vbb[j] + a.lift[nD + nD - 1] * vbt[j];
+                }
+                T[16 * j + li] = v0;
+                T[(nD - 1) * W + 16 * j + li] = vn;
+                if (col < P) {
+                    a.wout[col] = v0;
+                    a.wout[(long)(nD - 1) * P + col] = vn;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ------------------------------------------------------------------ stage D
+    zero_acc();
+    {
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkpD - 1) * mstD;
+            const int kr = min(4 * k + lk, 4 * kmaxD + 3);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(a.Dp, omD + mk + 16u * i);
+                fq[s][i] = ldo(a.Dq, omD + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = T[kr * W + 16 * j + li];
+                fy[s][j] = T[(nD - 1 - kr) * W + 16 * j + li];
+            }
+            if (XR) {
+                fxp[s] = ldo(a.Dp, omx + mk);
+                fxq[s] = ldo(a.Dq, omx + mk);
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkpD; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                double sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    sv[j] = fx[s][j] + fy[s][j];
+                    dv[j] = fx[s][j] - fy[s][j];
+                }
+                mfma_step(s, sv, dv);
+                if (XR)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) xacc[j] = fma(fxq[s], dv[j], fma(fxp[s], sv[j], xacc[j]));
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // projection at the store (lane pairs hold the Re/Im columns of a mode)
+    double* __restrict__ pu = a.u;
+    double* __restrict__ pv = a.v;
+    double* __restrict__ pp_ = a.p;
+    auto proj_store = [&](long row, int j, double y, double uc, double vc, bool st) {
+        const double up = __shfl_xor(uc, 1), vp = __shfl_xor(vc, 1);
+        const double divf = sgc[j] * (wxc[j] * up + wyc[j] * vp);
+        const double wav2 = wxc[j] * wxc[j] + wyc[j] * wyc[j];
+        double pc = 0.0;
+        if (wav2 != 0.0) pc = -(y + divf) / (wav2 * a.dt * a.bdfv);
+        const double pp = __shfl_xor(pc, 1);
+        const double dtb = a.dt * a.bdfv;
+        if (st) {
+            const long o = row * P + cj[j];
+            pp_[o] = pc;
+            pu[o] = uc - dtb * (sgc[j] * wxc[j] * pp);
+            pv[o] = vc - dtb * (sgc[j] * wyc[j] * pp);
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < mynD) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                double ax[4][CB], ay[4][CB];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rbD + i) * 16 + lk + 4 * v, meD - 1);
+                    const int rc = half ? nD - 1 - r : r;
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        ax[v][j] = pu[(long)rc * P + cj[j]];
+                        ay[v][j] = pv[(long)rc * P + cj[j]];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rbD + i) * 16 + lk + 4 * v;
+                    if (r < meD && (half == 0 || r < moD)) {
+                        const int row = half ? nD - 1 - r : r;
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const long col = p0 + 16 * j + li;
+                            const double e = accE[i][j][v], o = accO[i][j][v];
+                            const double y = half ? -1.0 * (e - o) : e + o;
+                            proj_store(row, j, y, ax[v][j], ay[v][j], col < P);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (dox) {
+        double ax[CB], ay[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            ax[j] = pu[(long)a.xrow * P + cj[j]];
+            ay[j] = pv[(long)a.xrow * P + cj[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            double x = xacc[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            const long col = p0 + 16 * j + li;
+            proj_store(a.xrow, j, x, ax[j], ay[j], lk == 0 && col < P);
+        }
+    }
+}
+constexpr int kZPRB = 2, kZPCB = 2;
+
 // flops of one split contraction: two me x me products per column
 static double zsplit_flops(const ZSplitOp& op, double cols) { return 4.0 * op.me * (double)op.me * cols; }
 
@@ -1533,6 +1998,66 @@ int zcontract_proj(lsb_ctx* c, const double* phi, double* p, double* u, double* 
     const int rc = launch_zs<ZM_PROJ>(a, c->zs_dz, fi, fo, uv, z, 1, s);
     prof_mark_end(c, tok, s);
     return rc;
+}
+
+// The pressure step in one kernel (k_zpress) when the parity-split operators and
+// one nzp-row LDS tile per block are available; LSB_E_UNSUPPORTED otherwise (the
+// caller then runs zcontract_pressdiv, helmholtz and zcontract_proj).
+int zpress_step(lsb_ctx* c, double* u, double* v, const double* w, double* wout, double* p, const double* bc_top,
+                const double* bc_bot, double dt, double bdfv, hipStream_t s) {
+    const long P = (long)c->plane_s;
+    const ZSplitOp &D = c->zs_dz, &H1 = c->zs_h1, &H2 = c->zs_h2;
+    const char* off = std::getenv("LSB_NO_ZPRESS");  // the three-kernel form, for tests
+    const size_t lds = (size_t)c->nzp * 16 * kZPCB * sizeof(double);
+    if (!zs_fits(D, P) || !zs_fits(H1, P) || !zs_fits(H2, P) || H1.ldm != H2.ldm || D.n != c->nzp ||
+        H1.n != c->nzm || lds > 64 * 1024 || (off && off[0] && std::strcmp(off, "0") != 0))
+        return LSB_E_UNSUPPORTED;
+    ZPressArgs a{};
+    a.Dp = D.d_m;
+    a.Dq = D.d_m + D.qoff;
+    a.H1p = H1.d_m;
+    a.H1q = H1.d_m + H1.qoff;
+    a.H2p = H2.d_m;
+    a.H2q = H2.d_m + H2.qoff;
+    a.ldD = D.ldm;
+    a.ldH = H1.ldm;
+    a.nD = D.n;
+    a.meD = D.me;
+    a.nH = H1.n;
+    a.meH = H1.me;
+    a.xrow = ((D.n & 1) && D.me > 16 && D.me % 16 == 1) ? D.me - 1 : -1;
+    a.P = P;
+    a.nx = c->nx;
+    a.wavx = c->d_wavx;
+    a.wavy = c->d_wavy;
+    a.lam = c->d_helm_s;
+    a.elift = c->d_helm_s + c->nzm;
+    a.lift = c->d_lift;
+    a.bct = bc_top;
+    a.bcb = bc_bot;
+    a.u = u;
+    a.v = v;
+    a.w = w;
+    a.wout = wout;
+    a.p = p;
+    a.dt = dt;
+    a.bdfv = bdfv;
+    const int nrbD = (a.xrow >= 0) ? D.me / 16 : (D.me + 15) / 16;
+    const int nrbH = (H1.me + 15) / 16;
+    const int nw = (std::max(nrbD, nrbH) + kZPRB - 1) / kZPRB;
+    if (nw > kZMaxWaves) return LSB_E_UNSUPPORTED;
+    dim3 grid((unsigned)((P + 16 * kZPCB - 1) / (16 * kZPCB)));
+    int tok = -1;
+    // four split contractions per pencil; reads u, v, w, u, v; writes w_new, p, u, v
+    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * (zsplit_flops(D, (double)P) + zsplit_flops(H1, (double)P)),
+                    9.0 * c->nzp * (double)P * sizeof(double));
+    if (a.xrow >= 0)
+        hipLaunchKernelGGL((k_zpress<kZPRB, kZPCB, kZPD, true>), grid, dim3(64 * nw), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_zpress<kZPRB, kZPCB, kZPD, false>), grid, dim3(64 * nw), lds, s, a);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
 }
 
 // In-place Helmholtz/Poisson solves of nf fields (interior rows 1..nz-1), with

@@ -114,6 +114,15 @@ def test_c1_dns_two_kernel_helmholtz(monkeypatch):
     check_fields(st, g)
 
 
+def test_c1_dns_three_kernel_pressure_step(monkeypatch):
+    """The pressure step as three kernels (pressure RHS, Poisson solve, dphi/dz with
+    the projection) instead of one kernel with its intermediate fields in LDS."""
+    monkeypatch.setenv("LSB_NO_ZPRESS", "1")
+    case = L.cases.CASES["c1_dns"]
+    st, g = run_both(case, case.nsteps)
+    check_fields(st, g)
+
+
 def test_matmul_derivative_path():
     """nz <= 31 uses matmul(d, .) instead of the DCT operator (derivatives.f90:122-129)."""
     case = L.cases.CASES["c1_dns"].with_(nx=16, ny=8, nz=24, pert_rms=0.02)
