@@ -129,8 +129,9 @@ def roofline(prof, precision, case=None, steps=1):
     sec = p["ms"] * 1e-3
     traffic = measured_traffic(case, cls, precision, p, steps)
     if cls in ("zcontract", "helmholtz"):
-        # the Helmholtz projections are FP64 in both forms; the mixed form's derivative
-        # contractions are FP32 (157.3 TFLOP/s), priced here against the FP64 peak
+        # the Helmholtz / Poisson projections (fused kernels of the velocity solves and of the
+        # pressure step) are FP64 in both forms; the mixed form's derivative contractions are
+        # FP32 (157.3 TFLOP/s), priced here against the FP64 peak
         peak = PEAK_FP64_TFLOPS
         ach = p["flops"] / sec / 1e12 if sec > 0 else 0.0
         return dict(bound="mfma", kernel=cls, achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
@@ -168,6 +169,43 @@ def cpu_baseline(case, sample_ny, sample_steps):
                        f"extent {sample_ny}/{case.ny} of the workload), {t1 - t0:.1f} s, finite={ok}")
 
 
+def diagnostics(case, reps):
+    """Wall time per call (host-synchronised; the records include their PCIe copy to
+    host memory) of the per-step diagnostics and the I/O records on the FP64 resident
+    state of `case`, after two steps.  The kernels' own device times come from a
+    rocprofv3 --kernel-trace run of this mode."""
+    import numpy as np
+    import lsbflow_amd as L
+    ctx = L.setup_case(case, precision=0)
+    ctx.state_set(*L.cases.initial_condition(case))
+    ctx.timestep(2)
+    ctx.synchronize()
+    n = case.nx * case.ny * case.nzp
+    snap = np.zeros(7 + 5 * n, dtype=np.float32)
+    rst = np.zeros(9 + 20 * n, dtype=np.float32)
+    calls = {
+        "state_maxcalc": lambda: ctx.state_maxcalc(1.0, 0.0),
+        "state_nancheck": ctx.state_nancheck,
+        "state_wall_shear": lambda: ctx.wall_shear(),
+        "state_snapshot_record": lambda: ctx._L.lsb_state_snapshot(ctx.h, 0, None, ctx._fp(snap)),
+        "write_restart_record": lambda: ctx._L.lsb_write_restart(ctx.h, None, ctx._fp(rst)),
+        "read_restart_record": lambda: ctx._L.lsb_read_restart(ctx.h, None, ctx._fp(rst)),
+    }
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rc = fn()
+            if isinstance(rc, int) and rc != 0:
+                raise RuntimeError(f"{name}: status {rc}")
+        out[name] = round(1e3 * (time.perf_counter() - t0) / reps, 3)
+        print(json.dumps({"diagnostics_step": name, "ms": out[name]}), flush=True)
+    ctx.close()
+    print(json.dumps({"diagnostics": case.name, "points": n, "reps": reps, "wall_ms_per_call": out}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +218,9 @@ def main(argv=None):
     ap.add_argument("--pmc-probe", action="store_true",
                     help="the workload of the HBM-traffic counter runs: a 1 GiB device copy (calibration), "
                          "then one step of each form after one warm-up step")
+    ap.add_argument("--diagnostics", action="store_true",
+                    help="time the snapshot / restart records and the per-step diagnostics (maxcalc, NaN "
+                         "scan, wall shear) on the resident state of the workload instead of the step")
     ap.add_argument("--cpu-sample-ny", type=int, default=16)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     args = ap.parse_args(argv)
@@ -192,6 +233,8 @@ def main(argv=None):
     case = L.cases.CASES[args.case]
     if args.pmc_probe:
         return pmc_probe(case)
+    if args.diagnostics:
+        return diagnostics(case, args.steps)
     ic = L.cases.initial_condition(case)
     pts = case.points
     secs64, prof64, finite64, st64 = gpu_run(case, args.steps, args.warmup, 0, ic)

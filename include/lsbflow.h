@@ -188,6 +188,9 @@ int lsb_nlterm_skewsym(lsb_ctx* ctx, const double* uif, double* nltermif);
 
 /* Core state (common.f90:162-175: uf, vf, wf, pf, pnf, unif, unmif, nlnif,
  * nlnmif) and time module values; any history pointer may be NULL (zero).
+ * istep sets both step counters: the run's own (start-up coefficients, adaptive-dt
+ * schedule) and the total (isum of snapshot/restart headers); state_get returns the
+ * total.  The step coefficients are recomputed on the first step after a set.
  * tempf must be zero everywhere (buoyancy/temperature is not part of this
  * path; LSB_E_UNSUPPORTED otherwise). */
 int lsb_state_set(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
@@ -196,7 +199,7 @@ int lsb_state_set(lsb_ctx* ctx, const double* uf, const double* vf, const double
 int lsb_state_get(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* pf, double* pnf,
                   double* unif, double* unmif, double* nlnif, double* nlnmif, int* istep,
                   double* t);
-/* navierstokes.f90:10-90 timestep() x nsteps (ifilt = 0, dtadapt = 0):
+/* navierstokes.f90:10-90 timestep() x nsteps (ifilt = 0; dtadapt per lsb_set_adaptive_dt):
  * navierstokes_update + norm + ifrz + time update, all on the device. */
 int lsb_timestep(lsb_ctx* ctx, int nsteps);
 /* navierstokes.f90:1157-1496 navierstokes_update (one step, no norm). */
@@ -208,6 +211,62 @@ int lsb_state_device_ptrs(lsb_ctx* ctx, void** uf, void** vf, void** wf, void** 
 void* lsb_stream(lsb_ctx* ctx);
 /* Wait for all work queued on the context stream (lsb_timestep only enqueues). */
 int lsb_synchronize(lsb_ctx* ctx);
+
+/* ------------------------- snapshots, restart files, per-step diagnostics */
+
+/* mpi_io.f90:339-497 mpi_snapshot(uf, vf, wf, tempf, pf, name) as the record it writes:
+ * rec = 7 floats [isum, t, dt, nx, ny, nzp, nypl] then u, v, w, temp, p in physical
+ * space, float32, x fastest then z then y (5*nx*ny*nzp floats).  u += ubar and
+ * temp += tbar unless name == 2.  tempf and tbar may be NULL (zero).  With one rank
+ * nypl = ny; the body is the same for any rank count. */
+int lsb_snapshot_record(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+                        const double* tempf, const double* pf, const double* tbar, int name,
+                        int isum, double t, double dt, float* rec);
+/* The same record from the resident state (temp = 0; isum, t, dt of the context).
+ * path != NULL writes the file (an existing one is renamed prev_<name>, mpi_io.f90:400-404;
+ * the caller names it q%05d.dat / qavg%05d.dat / q_ref.dat as :396-402); rec may be NULL. */
+int lsb_state_snapshot(lsb_ctx* ctx, int name, const char* path, float* rec);
+/* mpi_io.f90:500-650 mpi_read_snapshot: q(nx, ny, nzp, 5) physical (Fortran order) from a
+ * snapshot file; dimension mismatch -> LSB_E_ARG.  Host-only. */
+int lsb_read_snapshot(lsb_ctx* ctx, const char* path, double* q, int* isum, double* t, double* dt);
+/* mpi_io.f90:9-160 mpi_write_restart_file: 9 floats [isum, t, dt, dt2, dt3, nx, ny, nzp,
+ * nxpl] then 20 float32 spectral fields (uf, vf, wf, tempf, unif(3), unmif(3), nlnif(3),
+ * nlnmif(3), tnf, tnmf, nltnf, nltnmf), z fastest then y then x.  Slots 14-16 hold
+ * nlnmif (the reference writes nlnif twice there, quirk Q12).  Temperature slots are 0.
+ * From the resident state; path and/or rec (9 + 20*nx*ny*nzp floats) may be NULL. */
+int lsb_write_restart(lsb_ctx* ctx, const char* path, float* rec);
+/* mpi_io.f90:163-336 mpi_read_restart_file into the resident state (rec, else the file):
+ * the total step count = nint(isum), t, dt, dt2, dt3 from the header; the run's own step
+ * counter starts again at 0 (main.f90:61), so the next step follows the nstart start-up
+ * rule.  pf/pnf keep their values (zero on a fresh context).  Nonzero temperature
+ * slots -> LSB_E_UNSUPPORTED. */
+int lsb_read_restart(lsb_ctx* ctx, const char* path, const float* rec);
+
+/* navierstokes.f90:358-511 maxcalc(uf, vf, wf, tempf, dtmax, dt, dt2, dt3) with the hidden
+ * inputs t (time), fsize (fringe size) and tbar (may be NULL).  stats[11] = umax, umin,
+ * vmax, vmin, wmax, wmin, tmax, tmin (u + ubar, temp + tbar), x-, y-, z-CFL.  On a CFL
+ * outside the limits dt2 = dt and dt = dt/1.25 (or dt*1.25); *change_dt = 1 then.
+ * Extrema run over the nx physical points of each row (the reference also reads the
+ * pad junk, quirk Q10). */
+int lsb_maxcalc(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+                const double* tempf, const double* tbar, double dtmax, double t, double fsize,
+                double* dt, double* dt2, double* dt3, double* stats, int* change_dt);
+/* maxcalc on the resident state; updates the context's dt and dt2. */
+int lsb_state_maxcalc(lsb_ctx* ctx, double dtmax, double fsize, double* stats, int* change_dt);
+/* navierstokes.f90:77-81 / main.f90:79: any(isnan(.)) over the spectral fields (NULL
+ * fields skipped). */
+int lsb_nancheck(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
+                 const double* tempf, int* has_nan);
+int lsb_state_nancheck(lsb_ctx* ctx, int* has_nan);
+/* Skin friction cf(nx) of run/compare_results.m:63,76-79 (spanwise mean of u + ubar,
+ * xnu (u(nz) - u(nzp))/(zpts(nz) - zpts(nzp)) / (0.5 u0^2)) and the first zero crossings
+ * of cf from >= 0 to < 0 (x_sep) and back (x_reatt), linear in xpts; NaN if none. */
+int lsb_wall_shear(lsb_ctx* ctx, const double* uf, double* cf, double* x_sep, double* x_reatt);
+int lsb_state_wall_shear(lsb_ctx* ctx, double* cf, double* x_sep, double* x_reatt);
+/* runparam dtadapt, iomod, fsize for lsb_timestep: with dtadapt = 1, after every step
+ * whose count is a multiple of 5 or of iomod, a NaN check (LSB_E_NUMERIC) and
+ * maxcalc(dtmax = 1.0) on the state (navierstokes.f90:74-87). */
+int lsb_set_adaptive_dt(lsb_ctx* ctx, int dtadapt, int iomod, double fsize);
 
 /* ------------------------------------- device-pointer kernel entry points */
 /* Same semantics as the tier-A routines, on device memory, asynchronous on

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblsbflow_amd.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 
@@ -86,6 +87,20 @@ _SIGS = {
     "lsb_profile_begin": (C.c_int, [_vp]),
     "lsb_profile_end": (C.c_int, [_vp, _dp, _ip, _dp, _dp]),
     "lsb_synchronize": (C.c_int, [_vp]),
+    "lsb_snapshot_record": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      _fp]),
+    "lsb_state_snapshot": (C.c_int, [_vp, C.c_int, C.c_char_p, _fp]),
+    "lsb_read_snapshot": (C.c_int, [_vp, C.c_char_p, _dp, _ip, _dp, _dp]),
+    "lsb_write_restart": (C.c_int, [_vp, C.c_char_p, _fp]),
+    "lsb_read_restart": (C.c_int, [_vp, C.c_char_p, _fp]),
+    "lsb_maxcalc": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                              _ip]),
+    "lsb_state_maxcalc": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _ip]),
+    "lsb_nancheck": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _ip]),
+    "lsb_state_nancheck": (C.c_int, [_vp, _ip]),
+    "lsb_wall_shear": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "lsb_state_wall_shear": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "lsb_set_adaptive_dt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
 }
 
 

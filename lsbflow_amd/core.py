@@ -17,6 +17,7 @@ Every call runs the HIP kernels; there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -410,6 +411,87 @@ class Context:
 
     def synchronize(self):
         check(self._L.lsb_synchronize(self.h), "synchronize")
+
+    # ------------------------------------------ snapshots, restart, diagnostics
+    def _fp(self, a):
+        return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+    def snapshot_record(self, uf, vf, wf, tempf, pf, name=0, isum=0, t=0.0, dt=0.0, tbar=None):
+        """mpi_snapshot record (7 + 5*nx*ny*nzp float32) of host spectral fields."""
+        n = self.nx * self.ny * self.nzp
+        rec = np.zeros(7 + 5 * n, dtype=np.float32)
+        a = [None if f is None else _f64(f) for f in (uf, vf, wf, tempf, pf, tbar)]
+        check(self._L.lsb_snapshot_record(self.h, *[_p(x) for x in a], name, isum, t, dt, self._fp(rec)),
+              "snapshot_record")
+        return rec
+
+    def state_snapshot(self, name=0, path=None, record=True):
+        n = self.nx * self.ny * self.nzp
+        rec = np.zeros(7 + 5 * n, dtype=np.float32) if record else None
+        check(self._L.lsb_state_snapshot(self.h, name, None if path is None else os.fsencode(path), self._fp(rec)),
+              "state_snapshot")
+        return rec
+
+    def read_snapshot(self, path):
+        """(q (5, nzp, ny, nx), isum, t, dt) of a snapshot file."""
+        q = np.zeros((5,) + self.sshape)
+        isum, t, dt = C.c_int(), C.c_double(), C.c_double()
+        check(self._L.lsb_read_snapshot(self.h, os.fsencode(path), _p(q), C.byref(isum), C.byref(t), C.byref(dt)),
+              "read_snapshot")
+        return q, isum.value, t.value, dt.value
+
+    def write_restart(self, path=None, record=True):
+        n = self.nx * self.ny * self.nzp
+        rec = np.zeros(9 + 20 * n, dtype=np.float32) if record else None
+        check(self._L.lsb_write_restart(self.h, None if path is None else os.fsencode(path), self._fp(rec)),
+              "write_restart")
+        return rec
+
+    def read_restart(self, path=None, record=None):
+        rec = None if record is None else np.ascontiguousarray(record, dtype=np.float32)
+        check(self._L.lsb_read_restart(self.h, None if path is None else os.fsencode(path), self._fp(rec)),
+              "read_restart")
+
+    def maxcalc(self, uf, vf, wf, tempf=None, dtmax=1.0, dt=None, dt2=None, dt3=None, t=0.0, fsize=0.0, tbar=None):
+        """(stats[11], dt, dt2, dt3, change_dt) as navierstokes.f90 maxcalc."""
+        a = [None if f is None else _f64(f) for f in (uf, vf, wf, tempf, tbar)]
+        d1, d2, d3 = (C.c_double(self.cfg.dt if x is None else x) for x in (dt, dt2, dt3))
+        stats = np.zeros(11)
+        ch = C.c_int()
+        check(self._L.lsb_maxcalc(self.h, *[_p(x) for x in a], dtmax, t, fsize, C.byref(d1), C.byref(d2),
+                                  C.byref(d3), _p(stats), C.byref(ch)), "maxcalc")
+        return stats, d1.value, d2.value, d3.value, bool(ch.value)
+
+    def state_maxcalc(self, dtmax=1.0, fsize=0.0):
+        stats = np.zeros(11)
+        ch = C.c_int()
+        check(self._L.lsb_state_maxcalc(self.h, dtmax, fsize, _p(stats), C.byref(ch)), "state_maxcalc")
+        return stats, bool(ch.value)
+
+    def nancheck(self, uf, vf=None, wf=None, tempf=None):
+        a = [None if f is None else _f64(f) for f in (uf, vf, wf, tempf)]
+        r = C.c_int()
+        check(self._L.lsb_nancheck(self.h, *[_p(x) for x in a], C.byref(r)), "nancheck")
+        return bool(r.value)
+
+    def state_nancheck(self):
+        r = C.c_int()
+        check(self._L.lsb_state_nancheck(self.h, C.byref(r)), "state_nancheck")
+        return bool(r.value)
+
+    def wall_shear(self, uf=None):
+        """(cf (nx,), x_sep, x_reatt) of host spectral uf, or of the resident state."""
+        cf = np.zeros(self.nx)
+        xs, xr = C.c_double(), C.c_double()
+        if uf is None:
+            check(self._L.lsb_state_wall_shear(self.h, _p(cf), C.byref(xs), C.byref(xr)), "state_wall_shear")
+        else:
+            u = _f64(uf)
+            check(self._L.lsb_wall_shear(self.h, _p(u), _p(cf), C.byref(xs), C.byref(xr)), "wall_shear")
+        return cf, xs.value, xr.value
+
+    def set_adaptive_dt(self, dtadapt=1, iomod=5, fsize=0.0):
+        check(self._L.lsb_set_adaptive_dt(self.h, dtadapt, iomod, fsize), "set_adaptive_dt")
 
 
 def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:

@@ -258,7 +258,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     step_free(c);
     double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
-                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab};
+                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red};
     for (double* p : ds)
         if (p) (void)hipFree(p);
     if (c->d_walls) (void)hipFree(c->d_walls);
@@ -1020,6 +1020,8 @@ extern "C" int lsb_state_set(lsb_ctx* c, const double* uf, const double* vf, con
     LSB_TRY(put(15, pf));
     LSB_TRY(put(16, pnf));
     c->istep = istep;
+    c->isum = istep;
+    c->coef_valid = false;
     c->t = t;
     c->dt = c->dt2 = c->dt3 = c->cfg.dt;
     LSB_TRY(step_prepare(c));
@@ -1058,7 +1060,7 @@ extern "C" int lsb_state_get(lsb_ctx* c, double* uf, double* vf, double* wf, dou
     }
     LSB_TRY(get(15, pf));
     LSB_TRY(get(16, pnf));
-    if (istep) *istep = c->istep;
+    if (istep) *istep = c->isum;
     if (t) *t = c->t;
     return st.sync();
 }
@@ -1131,3 +1133,477 @@ extern "C" int lsb_profile_end(lsb_ctx* c, double* ms, int* count, double* flops
     }
     return LSB_OK;
 }
+
+// ======================================= snapshots, restart files, diagnostics
+namespace {
+
+constexpr int kSnapHead = 7;      // mpi_io.f90:408
+constexpr int kRestartHead = 9;   // mpi_io.f90:44
+constexpr int kRestartFields = 20;
+
+// mpi_io.f90:400-404: an existing file of that name becomes prev_<name> (same folder).
+int write_record(const char* path, const float* head, size_t nh, const float* body, size_t nb) {
+    std::string p(path);
+    FILE* probe = std::fopen(path, "rb");
+    if (probe) {
+        std::fclose(probe);
+        const size_t cut = p.find_last_of('/');
+        const std::string prev = (cut == std::string::npos) ? "prev_" + p : p.substr(0, cut + 1) + "prev_" +
+                                                                                  p.substr(cut + 1);
+        if (std::rename(path, prev.c_str()) != 0) {
+            set_error("snapshot: cannot rename " + p + " to " + prev);
+            return LSB_E_ARG;
+        }
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open " + p + " for writing");
+        return LSB_E_ARG;
+    }
+    const bool ok = std::fwrite(head, sizeof(float), nh, f) == nh && std::fwrite(body, sizeof(float), nb, f) == nb;
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) {
+        set_error("short write on " + p);
+        return LSB_E_ARG;
+    }
+    return LSB_OK;
+}
+
+int read_record(const char* path, std::vector<float>* rec, size_t expect) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("cannot open ") + path);
+        return LSB_E_ARG;
+    }
+    rec->resize(expect);
+    const size_t got = std::fread(rec->data(), sizeof(float), expect, f);
+    std::fclose(f);
+    if (got != expect) {
+        set_error(std::string("short record in ") + path);
+        return LSB_E_ARG;
+    }
+    return LSB_OK;
+}
+
+void snap_header(const lsb_ctx* c, int isum, double t, double dt, float* h) {
+    const double v[kSnapHead] = {(double)isum, t, dt, (double)c->nx, (double)c->ny, (double)c->nzp, (double)c->ny};
+    for (int q = 0; q < kSnapHead; ++q) h[q] = (float)v[q];
+}
+
+// Inverse transforms of the present spectral fields (device) into physical stage slots
+// 1.. and the (x, z, y) FP32 record body in stage slot 0.
+int snapshot_body(lsb_ctx* c, Stage& st, const double* const* spec, int name, const double* tbar_dev,
+                  float** body) {
+    double* dp;
+    LSB_TRY(st.get(1, c->nphys * 5, &dp));
+    const double* si[5];
+    double* so[5];
+    const double* phys[5];
+    int n = 0;
+    for (int l = 0; l < 5; ++l) {
+        phys[l] = nullptr;
+        if (!spec[l]) continue;
+        si[n] = spec[l];
+        so[n] = dp + (size_t)l * c->nphys;
+        phys[l] = so[n];
+        ++n;
+    }
+    LSB_TRY(horfft_inv(c, c->hp, si, so, n, c->nzp, c->stream));
+    double* b;
+    LSB_TRY(st.get(0, (c->nspec * 5 + 1) / 2, &b));
+    *body = (float*)b;
+    return snap_pack(c, phys, name != 2 ? c->d_paral : nullptr, name != 2 ? tbar_dev : nullptr, *body, c->stream);
+}
+
+// navierstokes.f90:452-467: the CFL limits and the 4/5 (or 5/4) change of dt.
+bool maxcalc_decide(const double* s, double dtmax, double* dt, double* dt2) {
+    const double cflx = s[8], cfly = s[9], cflz = s[10];
+    const bool exceed = cflz > 0.45 || cflx > 0.25 || cfly > 0.25;
+    const bool below = cflz < 0.25 && cflx < 0.10 && cfly < 0.12 && *dt < dtmax;
+    if (!(exceed || below)) return false;
+    const double cdt = exceed ? 1.25 : 1.0 / 1.25;
+    *dt2 = *dt;
+    *dt = *dt / cdt;
+    return true;
+}
+
+// maxcalc on three (four) spectral fields on the device: inverse transforms into stage
+// slots 5.., then the reduction.  stats[8..10] become the CFL numbers.
+int maxcalc_fields(lsb_ctx* c, Stage& st, const double* u, const double* v, const double* w, const double* temp,
+                   const double* tbar, double dt, double t, double fsize, double* stats) {
+    double* dp;
+    const int nf = temp ? 4 : 3;
+    LSB_TRY(st.get(5, c->nphys * nf, &dp));
+    const double* si[4] = {u, v, w, temp};
+    double* so[4] = {dp, dp + c->nphys, dp + 2 * c->nphys, dp + 3 * c->nphys};
+    LSB_TRY(horfft_inv(c, c->hp, si, so, nf, c->nzp, c->stream));
+    int findex = c->nx;
+    if (t > c->cfg.istartbound) {  // navierstokes.f90:442-447
+        findex = (int)((1.0 - 0.8 * fsize) * c->nx);
+        findex = std::max(0, std::min(c->nx, findex));
+    }
+    LSB_TRY(maxcalc_reduce(c, so[0], so[1], so[2], temp ? so[3] : nullptr, tbar, dt, findex, stats, c->stream));
+    stats[8] = (stats[8] * dt) / c->dx;  // maxval(abs(u))*dt/dx (:439)
+    stats[9] = (stats[9] * dt) / c->dy;
+    return LSB_OK;
+}
+
+// cf(x) from the spanwise mean of u + ubar at z rows nz and nzp (1-based), and the
+// separation / reattachment points (run/compare_results.m:63,76-79).
+int wall_shear_field(lsb_ctx* c, Stage& st, const double* uf_dev, double* cf, double* x_sep, double* x_reatt) {
+    double* ph;
+    LSB_TRY(st.get(9, c->plane_p * 2, &ph));
+    const double* si[1] = {uf_dev + (size_t)(c->nz - 1) * c->plane_s};
+    double* so[1] = {ph};
+    LSB_TRY(horfft_inv(c, c->hp, si, so, 1, 2, c->stream));
+    std::vector<double> mean(2 * (size_t)c->nx);
+    LSB_TRY(wall_mean(c, ph, mean.data(), c->stream));
+    const int nz = c->nz, nx = c->nx;
+    const double dz = c->zpts[nz - 1] - c->zpts[nz];
+    const double den = 0.5 * c->cfg.u0 * c->cfg.u0;
+    std::vector<double> cfv(nx);
+    for (int i = 0; i < nx; ++i) {
+        const double a = mean[i] + c->ubar[nz - 1], b = mean[nx + i] + c->ubar[nz];
+        cfv[i] = ((c->cfg.xnu * (a - b)) / dz) / den;
+    }
+    if (cf) std::memcpy(cf, cfv.data(), nx * sizeof(double));
+    double sep = NAN, reatt = NAN;
+    int i0 = -1;
+    for (int i = 1; i < nx; ++i)
+        if (cfv[i - 1] >= 0.0 && cfv[i] < 0.0) {
+            sep = c->xpts[i - 1] + (c->xpts[i] - c->xpts[i - 1]) * cfv[i - 1] / (cfv[i - 1] - cfv[i]);
+            i0 = i;
+            break;
+        }
+    if (i0 > 0)
+        for (int i = i0 + 1; i < nx; ++i)
+            if (cfv[i - 1] < 0.0 && cfv[i] >= 0.0) {
+                reatt = c->xpts[i - 1] + (c->xpts[i] - c->xpts[i - 1]) * cfv[i - 1] / (cfv[i - 1] - cfv[i]);
+                break;
+            }
+    if (x_sep) *x_sep = sep;
+    if (x_reatt) *x_reatt = reatt;
+    return LSB_OK;
+}
+
+// FP64 device view of state field `which` (a converted copy in stage slot 8 for the
+// FP32 history of the mixed form).
+int state_field_f64(lsb_ctx* c, Stage& st, int which, const double** p) {
+    if (!step_hist32(c, which)) {
+        *p = step_field(c, which);
+        return LSB_OK;
+    }
+    double* tmp;
+    LSB_TRY(st.get(8, c->nspec, &tmp));
+    LSB_TRY(step_hist_get(c, which, tmp, c->stream));
+    *p = tmp;
+    return LSB_OK;
+}
+
+int need_state(lsb_ctx* c, const char* who) {
+    if (!c->state_ready) {
+        set_error(std::string(who) + ": no state (lsb_state_set or lsb_read_restart first)");
+        return LSB_E_STATE;
+    }
+    return LSB_OK;
+}
+
+}  // namespace
+
+extern "C" int lsb_snapshot_record(lsb_ctx* c, const double* uf, const double* vf, const double* wf,
+                                   const double* tempf, const double* pf, const double* tbar, int name, int isum,
+                                   double t, double dt, float* rec) {
+    LSB_ENTER(c);
+    if (!uf || !vf || !wf || !pf || !rec) {
+        set_error("snapshot_record: null array");
+        return LSB_E_ARG;
+    }
+    const double* hs[5] = {uf, vf, wf, tempf, pf};
+    double *ds, *dtb = nullptr;
+    LSB_TRY(st.get(2, c->nspec * 5, &ds));
+    const double* spec[5];
+    for (int l = 0; l < 5; ++l) {
+        spec[l] = nullptr;
+        if (!hs[l]) continue;
+        double* d = ds + (size_t)l * c->nspec;
+        LSB_TRY(st.in(d, hs[l], c->nspec));
+        spec[l] = d;
+    }
+    if (tbar) {
+        LSB_TRY(st.get(3, c->nzp, &dtb));
+        LSB_TRY(st.in(dtb, tbar, c->nzp));
+    }
+    float* body;
+    LSB_TRY(snapshot_body(c, st, spec, name, dtb, &body));
+    snap_header(c, isum, t, dt, rec);
+    LSB_HIP(hipMemcpyAsync(rec + kSnapHead, body, c->nspec * 5 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return st.sync();
+}
+
+extern "C" int lsb_state_snapshot(lsb_ctx* c, int name, const char* path, float* rec) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_snapshot"));
+    const double* spec[5] = {step_field(c, 0), step_field(c, 1), step_field(c, 2), nullptr, step_field(c, 15)};
+    float* body;
+    LSB_TRY(snapshot_body(c, st, spec, name, nullptr, &body));
+    const size_t nb = c->nspec * 5;
+    std::vector<float> own;
+    float* hb = rec ? rec + kSnapHead : nullptr;
+    if (!hb) {
+        own.resize(nb);
+        hb = own.data();
+    }
+    float head[kSnapHead];
+    snap_header(c, c->isum, c->t, c->dt, head);
+    LSB_HIP(hipMemcpyAsync(hb, body, nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LSB_TRY(st.sync());
+    if (rec) std::memcpy(rec, head, sizeof(head));
+    if (path) LSB_TRY(write_record(path, head, kSnapHead, hb, nb));
+    return LSB_OK;
+}
+
+extern "C" int lsb_read_snapshot(lsb_ctx* c, const char* path, double* q, int* isum, double* t, double* dt) {
+    LSB_TRY(check_ctx(c));
+    if (!path) {
+        set_error("read_snapshot: null path");
+        return LSB_E_ARG;
+    }
+    const size_t res = c->nspec;  // nx*ny*nzp
+    std::vector<float> rec;
+    LSB_TRY(read_record(path, &rec, kSnapHead));
+    const int nx_in = (int)std::lround(rec[3]), ny_in = (int)std::lround(rec[4]), nzp_in = (int)std::lround(rec[5]);
+    if (nx_in != c->nx || ny_in != c->ny || nzp_in != c->nzp) {  // mpi_io.f90:575-583
+        set_error("read_snapshot: dimension mismatch (file " + std::to_string(nx_in) + " " + std::to_string(ny_in) +
+                  " " + std::to_string(nzp_in) + ")");
+        return LSB_E_ARG;
+    }
+    LSB_TRY(read_record(path, &rec, kSnapHead + 5 * res));
+    if (isum) *isum = (int)std::lround(rec[0]);
+    if (t) *t = rec[1];
+    if (dt) *dt = rec[2];
+    if (q) {  // q(i, j, k, l) = buf(ii, l), ii over x, z, y (mpi_io.f90:623-636)
+        const int nx = c->nx, ny = c->ny, nzp = c->nzp;
+        for (int l = 0; l < 5; ++l) {
+            const float* b = rec.data() + kSnapHead + (size_t)l * res;
+            double* o = q + (size_t)l * res;
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nzp; ++k)
+                    for (int i = 0; i < nx; ++i)
+                        o[((size_t)k * ny + j) * nx + i] = b[((size_t)j * nzp + k) * nx + i];
+        }
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_write_restart(lsb_ctx* c, const char* path, float* rec) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "write_restart"));
+    double* dbody;
+    LSB_TRY(st.get(0, c->nspec * kRestartFields / 2, &dbody));
+    float* body = (float*)dbody;
+    // file slot -> state field (-1: temperature slots, zero here); slots 14-16 hold nlnmif
+    const int which[kRestartFields] = {0, 1, 2, -1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, -1, -1, -1, -1};
+    for (int f = 0; f < kRestartFields; ++f) {
+        const double* src[1] = {nullptr};
+        if (which[f] >= 0) LSB_TRY(state_field_f64(c, st, which[f], &src[0]));
+        LSB_TRY(restart_pack(c, src, 1, body + (size_t)f * c->nspec, c->stream));
+    }
+    const size_t nb = c->nspec * kRestartFields;
+    std::vector<float> own;
+    float* hb = rec ? rec + kRestartHead : nullptr;
+    if (!hb) {
+        own.resize(nb);
+        hb = own.data();
+    }
+    const double hv[kRestartHead] = {(double)c->isum, c->t, c->dt, c->dt2, c->dt3,
+                                     (double)c->nx, (double)c->ny, (double)c->nzp, (double)c->nx};
+    float head[kRestartHead];
+    for (int q = 0; q < kRestartHead; ++q) head[q] = (float)hv[q];
+    LSB_HIP(hipMemcpyAsync(hb, body, nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LSB_TRY(st.sync());
+    if (rec) std::memcpy(rec, head, sizeof(head));
+    if (path) LSB_TRY(write_record(path, head, kRestartHead, hb, nb));
+    return LSB_OK;
+}
+
+extern "C" int lsb_read_restart(lsb_ctx* c, const char* path, const float* rec) {
+    LSB_ENTER(c);
+    if (!path && !rec) {
+        set_error("read_restart: no file and no record");
+        return LSB_E_ARG;
+    }
+    if (c->cfg.dealiasflag == 1) {
+        set_error("read_restart: dealiasflag 1 runs through tier A (dealiasing_2d_fftw) only");
+        return LSB_E_UNSUPPORTED;
+    }
+    const size_t nb = c->nspec * kRestartFields;
+    std::vector<float> own;
+    const float* r = rec;
+    if (!r) {
+        LSB_TRY(read_record(path, &own, kRestartHead));
+        r = own.data();
+    }
+    const int nx_in = (int)std::lround(r[5]), ny_in = (int)std::lround(r[6]), nzp_in = (int)std::lround(r[7]);
+    if (nx_in != c->nx || ny_in != c->ny || nzp_in != c->nzp) {  // mpi_io.f90:243-252
+        set_error("read_restart: dimension mismatch (file " + std::to_string(nx_in) + " " + std::to_string(ny_in) +
+                  " " + std::to_string(nzp_in) + ")");
+        return LSB_E_ARG;
+    }
+    if (!rec) {
+        LSB_TRY(read_record(path, &own, kRestartHead + nb));
+        r = own.data();
+    }
+    // the temperature slots must be zero: this path carries no temperature
+    const int tslots[5] = {3, 16, 17, 18, 19};
+    for (int f : tslots) {
+        const float* b = r + kRestartHead + (size_t)f * c->nspec;
+        for (size_t q = 0; q < c->nspec; ++q)
+            if (b[q] != 0.0f) {
+                set_error("read_restart: temperature fields are not zero (temperature is not part of this path)");
+                return LSB_E_UNSUPPORTED;
+            }
+    }
+    const double hv[5] = {r[0], r[1], r[2], r[3], r[4]};
+    LSB_TRY(step_alloc(c));
+    const bool fresh = !c->state_ready;
+    double* dbody;
+    LSB_TRY(st.get(0, nb / 2, &dbody));
+    float* body = (float*)dbody;
+    LSB_HIP(hipMemcpyAsync(body, r + kRestartHead, nb * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const int which[kRestartFields] = {0, 1, 2, -1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, -1, -1, -1, -1};
+    double* tmp;
+    LSB_TRY(st.get(8, c->nspec, &tmp));
+    for (int f = 0; f < kRestartFields; ++f) {
+        const int w = which[f];
+        if (w < 0) continue;
+        const bool h32 = step_hist32(c, w);
+        double* dst[1] = {h32 ? tmp : step_field(c, w)};
+        LSB_TRY(restart_unpack(c, body + (size_t)f * c->nspec, dst, 1, c->stream));
+        if (h32) LSB_TRY(step_hist_set(c, w, tmp, c->stream));
+    }
+    if (fresh) {  // pf, pnf are not in the file; the reference keeps them as they were
+        LSB_HIP(hipMemsetAsync(step_field(c, 15), 0, c->nspec * sizeof(double), c->stream));
+        LSB_HIP(hipMemsetAsync(step_field(c, 16), 0, c->nspec * sizeof(double), c->stream));
+    }
+    // isum = nint(prebuf(1)) (mpi_io.f90:231); the restarted run's own step counter
+    // starts again at 1 (main.f90:61), so the start-up rule of nstart applies
+    c->isum = (int)std::lround(hv[0]);
+    c->istep = 0;
+    c->coef_valid = false;
+    c->t = hv[1];
+    c->dt = hv[2];
+    c->dt2 = hv[3];
+    c->dt3 = hv[4];
+    LSB_TRY(step_prepare(c));
+    c->state_ready = true;
+    return st.sync();
+}
+
+extern "C" int lsb_maxcalc(lsb_ctx* c, const double* uf, const double* vf, const double* wf, const double* tempf,
+                           const double* tbar, double dtmax, double t, double fsize, double* dt, double* dt2,
+                           double* dt3, double* stats, int* change_dt) {
+    LSB_ENTER(c);
+    if (!uf || !vf || !wf || !dt || !dt2 || !dt3) {
+        set_error("maxcalc: null argument");
+        return LSB_E_ARG;
+    }
+    double* ds;
+    const int nf = tempf ? 4 : 3;
+    LSB_TRY(st.get(4, c->nspec * nf, &ds));
+    const double* hs[4] = {uf, vf, wf, tempf};
+    for (int l = 0; l < nf; ++l) LSB_TRY(st.in(ds + (size_t)l * c->nspec, hs[l], c->nspec));
+    double s[11];
+    LSB_TRY(maxcalc_fields(c, st, ds, ds + c->nspec, ds + 2 * c->nspec, tempf ? ds + 3 * c->nspec : nullptr, tbar,
+                           *dt, t, fsize, s));
+    const bool ch = maxcalc_decide(s, dtmax, dt, dt2);
+    (void)dt3;  // dt3 is passed through unchanged (navierstokes.f90:462-464)
+    if (stats) std::memcpy(stats, s, sizeof(s));
+    if (change_dt) *change_dt = ch ? 1 : 0;
+    return LSB_OK;
+}
+
+extern "C" int lsb_state_maxcalc(lsb_ctx* c, double dtmax, double fsize, double* stats, int* change_dt) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_maxcalc"));
+    double s[11];
+    LSB_TRY(maxcalc_fields(c, st, step_field(c, 0), step_field(c, 1), step_field(c, 2), nullptr, nullptr, c->dt, c->t,
+                           fsize, s));
+    const bool ch = maxcalc_decide(s, dtmax, &c->dt, &c->dt2);
+    if (stats) std::memcpy(stats, s, sizeof(s));
+    if (change_dt) *change_dt = ch ? 1 : 0;
+    return LSB_OK;
+}
+
+extern "C" int lsb_nancheck(lsb_ctx* c, const double* uf, const double* vf, const double* wf, const double* tempf,
+                            int* has_nan) {
+    LSB_ENTER(c);
+    if (!has_nan) return LSB_E_ARG;
+    const double* hs[4] = {uf, vf, wf, tempf};
+    double* ds;
+    LSB_TRY(st.get(4, c->nspec * 4, &ds));
+    const double* f[4];
+    int n = 0;
+    for (int l = 0; l < 4; ++l) {
+        if (!hs[l]) continue;
+        double* d = ds + (size_t)n * c->nspec;
+        LSB_TRY(st.in(d, hs[l], c->nspec));
+        f[n++] = d;
+    }
+    bool found = false;
+    if (n) LSB_TRY(nan_scan(c, f, n, c->nspec, &found, c->stream));
+    *has_nan = found ? 1 : 0;
+    return LSB_OK;
+}
+
+extern "C" int lsb_state_nancheck(lsb_ctx* c, int* has_nan) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_nancheck"));
+    if (!has_nan) return LSB_E_ARG;
+    const double* f[3] = {step_field(c, 0), step_field(c, 1), step_field(c, 2)};
+    bool found = false;
+    LSB_TRY(nan_scan(c, f, 3, c->nspec, &found, c->stream));
+    *has_nan = found ? 1 : 0;
+    return LSB_OK;
+}
+
+extern "C" int lsb_wall_shear(lsb_ctx* c, const double* uf, double* cf, double* x_sep, double* x_reatt) {
+    LSB_ENTER(c);
+    if (!uf) return LSB_E_ARG;
+    double* d;
+    LSB_TRY(st.get(4, c->nspec, &d));
+    LSB_TRY(st.in(d, uf, c->nspec));
+    return wall_shear_field(c, st, d, cf, x_sep, x_reatt);
+}
+
+extern "C" int lsb_state_wall_shear(lsb_ctx* c, double* cf, double* x_sep, double* x_reatt) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_wall_shear"));
+    return wall_shear_field(c, st, step_field(c, 0), cf, x_sep, x_reatt);
+}
+
+extern "C" int lsb_set_adaptive_dt(lsb_ctx* c, int dtadapt, int iomod, double fsize) {
+    LSB_TRY(check_ctx(c));
+    if ((dtadapt != 0 && dtadapt != 1) || iomod < 1 || !(fsize >= 0.0 && fsize <= 1.25)) {
+        set_error("set_adaptive_dt: dtadapt 0/1, iomod >= 1, 0 <= fsize <= 1.25");
+        return LSB_E_ARG;
+    }
+    c->dtadapt = dtadapt;
+    c->iomod = iomod;
+    c->fsize = fsize;
+    return LSB_OK;
+}
+
+namespace lsb {
+// navierstokes.f90:74-87, after the time update of step istep: every 5th (or iomod-th)
+// step a NaN check (abort) and maxcalc(..., dtmax = 1.0, dt, dt2, dt3).
+int state_adapt_dt(lsb_ctx* c) {
+    if (c->dtadapt != 1 || (c->istep % c->iomod != 0 && c->istep % 5 != 0)) return LSB_OK;
+    int bad = 0;
+    LSB_TRY(lsb_state_nancheck(c, &bad));
+    if (bad) {
+        set_error("NaN detected at t=" + std::to_string(c->t) + " iteration #: " + std::to_string(c->isum));
+        return LSB_E_NUMERIC;
+    }
+    return lsb_state_maxcalc(c, 1.0, c->fsize, nullptr, nullptr);
+}
+}  // namespace lsb

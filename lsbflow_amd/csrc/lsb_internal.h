@@ -96,6 +96,8 @@ constexpr int kMaxFields = 16;
 constexpr int kLdmBnd = 64;
 // doubles in the reduction / small-table scratch d_red
 constexpr int kRedDoubles = 8192;
+// doubles in the scratch of the diagnostics reductions (d_diag_red)
+constexpr int kDiagDoubles = 16384;
 template <class R>
 struct FieldList {
     R* p[kMaxFields];
@@ -149,6 +151,7 @@ struct lsb_ctx {
     double* d_smag = nullptr;   // nzp (static Smagorinsky -(cs delta fvd)^2) + scratch
     double* d_red = nullptr;    // reduction scratch (kRedDoubles doubles)
     double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
+    double* d_diag_red = nullptr;  // diagnostics partials + small tables (kDiagDoubles)
     int* d_walls = nullptr;     // wall planes of the static model's utau average
     int nwalls = 0;
     // parity-split forms of dfdz (d_mdz) and of the two Helmholtz projections, with
@@ -172,8 +175,15 @@ struct lsb_ctx {
     bool state_ready = false;
     double* st = nullptr;       // contiguous block of all state + work fields
     size_t st_bytes = 0;
-    int istep = 0;
+    // istep: the run's step counter (main.f90:61 loop variable: start-up coefficients,
+    // adaptive-dt schedule); isum: steps taken in total (time module its/isum, the
+    // snapshot and restart headers).  coef_valid is cleared when state is loaded.
+    int istep = 0, isum = 0;
+    bool coef_valid = false;
     double t = 0.0, dt = 0.0, dt2 = 0.0, dt3 = 0.0;
+    // adaptive time step (navierstokes.f90:74-87; runparam dtadapt, iomod, fsize)
+    int dtadapt = 0, iomod = 1;
+    double fsize = 0.0;
     double abfac[3] = {0, 0, 0};
     double bdf[3] = {0, 0, 0};
     double bdfv = 0.0;
@@ -210,6 +220,24 @@ int upload(void* dst, const void* src, size_t bytes, hipStream_t s);
 int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p);
 int upload_context(lsb_ctx* c);
 int zsplit_setup(lsb_ctx* c, const std::vector<double>& elift);
+
+// snapshot / restart records and per-step diagnostics (iodiag.hip)
+// phys: 5 physical FP64 fields u, v, w, temp, p (null = zero); add_u / add_t: per-plane
+// profiles added to u / temp (null = none); out: 5*nx*ny*nzp floats in (x, z, y) order.
+int snap_pack(lsb_ctx* c, const double* const* phys, const double* add_u, const double* add_t, float* out,
+              hipStream_t s);
+// nf spectral FP64 fields (null = zero) <-> nf FP32 blocks in (z, y, x) order, z fastest.
+int restart_pack(lsb_ctx* c, const double* const* spec, int nf, float* out, hipStream_t s);
+int restart_unpack(lsb_ctx* c, const float* in, double* const* spec, int nf, hipStream_t s);
+// extrema / CFL partials of physical u, v, w, temp (temp, tbar may be null); stats[11]:
+// umax, umin, vmax, vmin, wmax, wmin, tmax, tmin, max|u+ubar|, max|v|, z-CFL.  Synchronises.
+int maxcalc_reduce(lsb_ctx* c, const double* u, const double* v, const double* w, const double* temp,
+                   const double* tbar_host, double dt, int findex, double* stats, hipStream_t s);
+int nan_scan(lsb_ctx* c, const double* const* f, int nf, size_t n, bool* found, hipStream_t s);
+// spanwise means of two physical planes -> host (2*nx doubles).  Synchronises.
+int wall_mean(lsb_ctx* c, const double* phys2, double* mean_host, hipStream_t s);
+// NaN check + maxcalc on the resident state after a step when dtadapt is on (capi.cpp)
+int state_adapt_dt(lsb_ctx* c);
 
 // profiling helpers
 // profiling classes: 2-D FFTs, Chebyshev contractions, Helmholtz projections, the RHS

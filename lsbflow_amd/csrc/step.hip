@@ -835,9 +835,10 @@ static int update_once(lsb_ctx* c) {
     const double* un_[3];
     for (int l = 0; l < 3; ++l) un_[l] = (g.dealiasflag > 2) ? uif[l] : L.spec[S->u[l]];
     // coefficients (navierstokes.f90:1236-1243)
-    if (istep <= 3 || c->dt != c->dt2 || c->dt2 != c->dt3) {
+    if (!c->coef_valid || istep <= 3 || c->dt != c->dt2 || c->dt2 != c->dt3) {
         abcoef(g.nstart, istep, c->abfac);
         bdfcoef(g.nstart, istep, c->dt, c->dt2, c->dt3, c->bdf, &c->bdfv);
+        c->coef_valid = true;
     }
     const double dtnui = 1. / (g.xnu * dt);
     StepRhsArgs r{};
@@ -1048,8 +1049,10 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
                                              (size_t)c->nzp, c->stream));
             }
             c->t = c->t + c->dt;
+            c->isum += 1;
             c->dt3 = c->dt2;
             c->dt2 = c->dt;
+            LSB_TRY(state_adapt_dt(c));
         }
     }
     return LSB_OK;

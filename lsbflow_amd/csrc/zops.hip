@@ -2049,7 +2049,7 @@ int zpress_step(lsb_ctx* c, double* u, double* v, const double* w, double* wout,
     dim3 grid((unsigned)((P + 16 * kZPCB - 1) / (16 * kZPCB)));
     int tok = -1;
     // four split contractions per pencil; reads u, v, w, u, v; writes w_new, p, u, v
-    prof_mark_begin(c, PC_Z, s, &tok, 2.0 * (zsplit_flops(D, (double)P) + zsplit_flops(H1, (double)P)),
+    prof_mark_begin(c, PC_HELM, s, &tok, 2.0 * (zsplit_flops(D, (double)P) + zsplit_flops(H1, (double)P)),
                     9.0 * c->nzp * (double)P * sizeof(double));
     if (a.xrow >= 0)
         hipLaunchKernelGGL((k_zpress<kZPRB, kZPCB, kZPD, true>), grid, dim3(64 * nw), lds, s, a);

@@ -354,9 +354,89 @@ module lsb_capi
     integer(c_int) function lsb_profile_end(ctx, ms, count, flops, bytes) bind(C, name="lsb_profile_end")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: ms(4)
-      integer(c_int), intent(out) :: count(4)
-      real(c_double), intent(out) :: flops(4), bytes(4)
+      real(c_double), intent(out) :: ms(6)
+      integer(c_int), intent(out) :: count(6)
+      real(c_double), intent(out) :: flops(6), bytes(6)
+    end function
+
+    ! ------------------------------ snapshots, restart files, diagnostics
+    ! path / rec arguments are type(c_ptr): c_loc of a NUL-terminated character
+    ! array / real(c_float) buffer, or c_null_ptr
+    integer(c_int) function lsb_snapshot_record(ctx, uf, vf, wf, tempf, pf, tbar, name, isum, t, dt, rec) &
+        bind(C, name="lsb_snapshot_record")
+      import :: c_int, c_ptr, c_double, c_float
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*), vf(*), wf(*), tempf(*), pf(*), tbar(*)
+      integer(c_int), value :: name, isum
+      real(c_double), value :: t, dt
+      real(c_float), intent(out) :: rec(*)
+    end function
+    integer(c_int) function lsb_state_snapshot(ctx, name, path, rec) bind(C, name="lsb_state_snapshot")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: name
+      type(c_ptr), value :: path, rec
+    end function
+    integer(c_int) function lsb_read_snapshot(ctx, path, q, isum, t, dt) bind(C, name="lsb_read_snapshot")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(out) :: q(*)
+      integer(c_int), intent(out) :: isum
+      real(c_double), intent(out) :: t, dt
+    end function
+    integer(c_int) function lsb_write_restart(ctx, path, rec) bind(C, name="lsb_write_restart")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path, rec
+    end function
+    integer(c_int) function lsb_read_restart(ctx, path, rec) bind(C, name="lsb_read_restart")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path, rec
+    end function
+    integer(c_int) function lsb_maxcalc(ctx, uf, vf, wf, tempf, tbar, dtmax, t, fsize, dt, dt2, dt3, stats, &
+        change_dt) bind(C, name="lsb_maxcalc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*), vf(*), wf(*), tempf(*), tbar(*)
+      real(c_double), value :: dtmax, t, fsize
+      real(c_double), intent(inout) :: dt, dt2, dt3
+      real(c_double), intent(out) :: stats(11)
+      integer(c_int), intent(out) :: change_dt
+    end function
+    integer(c_int) function lsb_state_maxcalc(ctx, dtmax, fsize, stats, change_dt) bind(C, name="lsb_state_maxcalc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtmax, fsize
+      real(c_double), intent(out) :: stats(11)
+      integer(c_int), intent(out) :: change_dt
+    end function
+    integer(c_int) function lsb_nancheck(ctx, uf, vf, wf, tempf, has_nan) bind(C, name="lsb_nancheck")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*), vf(*), wf(*), tempf(*)
+      integer(c_int), intent(out) :: has_nan
+    end function
+    integer(c_int) function lsb_state_nancheck(ctx, has_nan) bind(C, name="lsb_state_nancheck")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: has_nan
+    end function
+    integer(c_int) function lsb_wall_shear(ctx, uf, cf, x_sep, x_reatt) bind(C, name="lsb_wall_shear")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*)
+      real(c_double), intent(out) :: cf(*), x_sep, x_reatt
+    end function
+    integer(c_int) function lsb_state_wall_shear(ctx, cf, x_sep, x_reatt) bind(C, name="lsb_state_wall_shear")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: cf(*), x_sep, x_reatt
+    end function
+    integer(c_int) function lsb_set_adaptive_dt(ctx, dtadapt, iomod, fsize) bind(C, name="lsb_set_adaptive_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dtadapt, iomod
+      real(c_double), value :: fsize
     end function
 
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")

@@ -25,6 +25,15 @@ Pinning (tests/test_oracle_*.py):
   navierstokes_update  -- divergence-free projection, Stokes (0,0)-mode decay,
                           linear-operator invariants; parity unpinned against
                           reference-produced fields (the reference cannot run here)
+  snapshot_record / read_snapshot -- run/q00019.dat round trip (read, transform,
+                          write back to 2 float32 spacings), run/prev_q00001.dat
+                          header and base profiles
+  restart_record / read_restart -- element order of the writer's loop nest, reader
+                          inverse, dimension check (no reference restart file ships)
+  maxcalc              -- run/output.txt Max|u+ubar|, x-CFL, Max|rho| lines; closed
+                          forms for the CFL planes, fringe range and the dt rule
+  wall_shear / nancheck -- single-mode closed forms (cf and its zero crossings);
+                          Inf vs NaN
 """
 
 from __future__ import annotations
@@ -1121,6 +1130,162 @@ class Oracle:
         divf = self.dfdx(uf) + self.dfdy(vf) + self.dfdz(wf)
         divp = self.horfft_inv(divf)
         return float(np.sum(np.abs(divp)) / (self.nx * self.ny * self.nzp))
+
+    # ------------------------------------------------- per-step diagnostics
+    def nancheck(self, *fields):
+        """navierstokes.f90:77-81 / main.f90:79: any(isnan(.)) over spectral fields (Inf passes)."""
+        return any(bool(np.isnan(f).any()) for f in fields if f is not None)
+
+    def maxcalc(self, uf, vf, wf, tempf, dtmax, dt, dt2, dt3, t, fsize=0.0):
+        """navierstokes.f90:358-511 maxcalc (one rank; the allreduces are identities).
+
+        Returns (stats, dt, dt2, dt3, change_dt); stats = [umax, umin, vmax, vmin, wmax, wmin,
+        tmax, tmin, cflx, cfly, cflz] with u = u + ubar(k), temp = temp + tbar(k) (:416-419).
+        The extrema run over the nx physical points of each x-row; the reference's maxval also
+        reads the two pad reals of each row, which hold FFT junk there (quirk Q10).
+        z-CFL (:428-437): (w dt)/(zpts(k) - zpts(k+1)) on planes 1..nz, then plane 1 a second
+        time with the same factor, plane nzp with (zpts(nz) - zpts(nzp)); over x < findex =
+        int((1 - 0.8 fsize) nx) once t > istartbound (:442-450)."""
+        c = self.cfg
+        nx, nz, nzp = self.nx, self.nz, self.nzp
+        zp = self.zpts
+        u = self.horfft_inv(uf)[:, :, :nx] + self.ubar[:, None, None]
+        v = self.horfft_inv(vf)[:, :, :nx]
+        w = self.horfft_inv(wf)[:, :, :nx]
+        tp = np.zeros_like(u) if tempf is None else self.horfft_inv(tempf)[:, :, :nx]
+        tp = tp + self.tbar[:, None, None]
+        vmax, vmin, wmax, wmin = v.max(), v.min(), w.max(), w.min()
+        umax, umin, tmax, tmin = u.max(), u.min(), tp.max(), tp.min()
+        ws = np.empty_like(w)
+        for k in range(nz):
+            ws[k] = (w[k] * dt) / (zp[k] - zp[k + 1])
+        ws[0] = (ws[0] * dt) / (zp[0] - zp[1])
+        ws[nz] = (w[nz] * dt) / (zp[nz - 1] - zp[nz])
+        cflx = (np.abs(u).max() * dt) / self.dx
+        cfly = (np.abs(v).max() * dt) / self.dy
+        findex = int((1.0 - 0.8 * fsize) * nx) if t > c.istartbound else nx
+        # maxval of an empty section is -huge (findex = 0)
+        cflz = np.abs(ws[:, :, :findex]).max() if findex > 0 else -np.finfo(np.float64).max
+        exceed = cflz > 0.45 or cflx > 0.25 or cfly > 0.25
+        below = cflz < 0.25 and cflx < 0.10 and cfly < 0.12 and dt < dtmax
+        change = exceed or below
+        if change:
+            cdt = 1.25 if exceed else 1.0 / 1.25
+            dt2 = dt
+            dt = dt / cdt
+        stats = np.array([umax, umin, vmax, vmin, wmax, wmin, tmax, tmin, cflx, cfly, cflz])
+        return stats, dt, dt2, dt3, change
+
+    def wall_shear(self, uf):
+        """Skin friction along x as run/compare_results.m:63,76-79 computes it from a snapshot:
+        u = spanwise mean of u + ubar (spanwiseavg.m:3), dz = zpts(nz) - zpts(nzp),
+        utau = xnu (u(:, nz) - u(:, nzp))/dz, cf = utau/(0.5 u0^2).
+
+        x_sep: first index i with cf(i-1) >= 0 > cf(i), x_reatt: the first later i with
+        cf(i-1) < 0 <= cf(i); both placed at the linear zero of cf between xpts(i-1) and
+        xpts(i), NaN when there is none.  Returns (cf, x_sep, x_reatt)."""
+        c = self.cfg
+        nx, ny, nz = self.nx, self.ny, self.nz
+        ph = self.horfft_inv(uf[nz - 1 : nz + 1])[:, :, :nx]
+        um = ph.sum(axis=1) / float(ny) + self.ubar[nz - 1 : nz + 1, None]
+        dz = self.zpts[nz - 1] - self.zpts[nz]
+        cf = ((c.xnu * (um[0] - um[1])) / dz) / (0.5 * c.u0**2)
+        return (cf,) + self._cf_crossings(cf)
+
+    def _cf_crossings(self, cf):
+        xs = self.xpts
+        sep = reatt = float("nan")
+        i0 = None
+        for i in range(1, len(cf)):
+            if cf[i - 1] >= 0.0 > cf[i]:
+                sep = xs[i - 1] + (xs[i] - xs[i - 1]) * cf[i - 1] / (cf[i - 1] - cf[i])
+                i0 = i
+                break
+        if i0 is not None:
+            for i in range(i0 + 1, len(cf)):
+                if cf[i - 1] < 0.0 <= cf[i]:
+                    reatt = xs[i - 1] + (xs[i] - xs[i - 1]) * cf[i - 1] / (cf[i - 1] - cf[i])
+                    break
+        return float(sep), float(reatt)
+
+    # --------------------------------------------- snapshot / restart records
+    SNAP_HEADER = 7
+    RESTART_HEADER = 9
+    RESTART_FIELDS = 20
+
+    def snapshot_record(self, uf, vf, wf, tempf, pf, name, isum, t, dt):
+        """mpi_io.f90:339-497 mpi_snapshot as the float32 values of the file it writes.
+
+        Header (:408) [isum, t, dt, nx, ny, nzp, nypl]; then the physical u, v, w, temp, p
+        (:380-384), each in x, z, y order with x fastest (:411-424); u += ubar(k) and
+        temp += tbar(k) unless name == 2 (:387-393).  With P ranks each rank writes its y-slab
+        at offset (ivar-1) res + myid bufsize (:449), so the file is the same for any P except
+        the nypl entry of the header (ny here, one rank)."""
+        nx, ny, nzp = self.nx, self.ny, self.nzp
+        head = np.array([isum, t, dt, nx, ny, nzp, ny], dtype=np.float64).astype(np.float32)
+        phys = [None if f is None else self.horfft_inv(f) for f in (uf, vf, wf, tempf, pf)]
+        return np.concatenate([head, self.snapshot_pack(phys, name)])
+
+    def snapshot_pack(self, phys, name):
+        """Body of the snapshot record from the five physical fields ((nzp, ny, nxpp) each,
+        None = zero): mpi_io.f90:387-393 base profiles, :411-424 (x, z, y) order, real(4)."""
+        nx, ny, nzp = self.nx, self.ny, self.nzp
+        body = np.empty((5, ny, nzp, nx), dtype=np.float32)
+        for l, ph in enumerate(phys):
+            p = np.zeros((nzp, ny, nx)) if ph is None else ph[:, :, :nx]
+            if name != 2 and l == 0:
+                p = p + self.ubar[:, None, None]
+            if name != 2 and l == 3:
+                p = p + self.tbar[:, None, None]
+            body[l] = p.transpose(1, 0, 2)
+        return body.ravel()
+
+    @staticmethod
+    def read_snapshot(rec, nx, ny, nzp):
+        """mpi_io.f90:500-650 mpi_read_snapshot (and run/readsnapshot.m): header values and
+        q(l, k, j, i) = buf((j-1) nzp nx + (k-1) nx + i, l), i.e. numpy (5, nzp, ny, nx)."""
+        head = np.asarray(rec[:7], dtype=np.float64)
+        if (int(round(head[3])), int(round(head[4])), int(round(head[5]))) != (nx, ny, nzp):
+            raise ValueError("snapshot: dimension mismatch")
+        body = np.asarray(rec[7 : 7 + 5 * nx * ny * nzp]).reshape(5, ny, nzp, nx)
+        return head, body.transpose(0, 2, 1, 3).astype(np.float64)
+
+    def restart_record(self, st, q12_compat=False):
+        """mpi_io.f90:9-160 mpi_write_restart_file as the float32 values of the file.
+
+        Header (:44) [isum, t, dt, dt2, dt3, nx, ny, nzp, nxpl]; 20 spectral fields (:53-70)
+        uf, vf, wf, tempf, unif(1:3), unmif(1:3), nlnif(1:3), slots 14-16, tnf, tnmf, nltnf,
+        nltnmf, each in z, y, x order with z fastest (:49-51).  The reference fills slots
+        14-16 with nlnif again (:61-62, quirk Q12) while mpi_read_restart_file reads them
+        into nlnmif (:264-267); this writer stores nlnmif there (q12_compat keeps the
+        duplicate)."""
+        nx, ny, nzp = self.nx, self.ny, self.nzp
+        head = np.array([st.get("isum", st["istep"]), st["t"], st["dt"], st["dt2"], st["dt3"], nx, ny, nzp, nx],
+                        dtype=np.float64).astype(np.float32)
+        slot14 = st["nlnif"] if q12_compat else st["nlnmif"]
+        fields = [st["uf"], st["vf"], st["wf"], st["tempf"]]
+        fields += [st["unif"][l] for l in range(3)] + [st["unmif"][l] for l in range(3)]
+        fields += [st["nlnif"][l] for l in range(3)] + [slot14[l] for l in range(3)]
+        fields += [st.get(k, np.zeros((nzp, ny, nx))) for k in ("tnf", "tnmf", "nltnf", "nltnmf")]
+        body = np.stack([np.asarray(f).transpose(2, 1, 0) for f in fields]).astype(np.float32)
+        return np.concatenate([head, body.ravel()])
+
+    def read_restart(self, rec):
+        """mpi_io.f90:163-336 mpi_read_restart_file: header (:229-240, isum = nint), the
+        dimension check (:243-252) and the 20 fields back into the core arrays (:255-285).
+        The restarted run's own step counter istep starts again at 0 (main.f90:61 loops
+        istep = 1..nsteps), so its first steps follow the nstart start-up rule."""
+        nx, ny, nzp = self.nx, self.ny, self.nzp
+        h = np.asarray(rec[:9], dtype=np.float64)
+        if (int(round(h[5])), int(round(h[6])), int(round(h[7]))) != (nx, ny, nzp):
+            raise ValueError("restart: dimension mismatch")
+        body = np.asarray(rec[9 : 9 + 20 * nx * ny * nzp]).reshape(20, nx, ny, nzp)
+        f = [b.transpose(2, 1, 0).astype(np.float64) for b in body]
+        st = dict(isum=int(round(h[0])), istep=0, t=float(h[1]), dt=float(h[2]), dt2=float(h[3]), dt3=float(h[4]),
+                  uf=f[0], vf=f[1], wf=f[2], tempf=f[3], unif=np.stack(f[4:7]), unmif=np.stack(f[7:10]),
+                  nlnif=np.stack(f[10:13]), nlnmif=np.stack(f[13:16]), tnf=f[16], tnmf=f[17],
+                  nltnf=f[18], nltnmf=f[19])
+        return st
 
 
 __all__ = ["Oracle", "OracleConfig", "PI", "field"]

@@ -7,13 +7,19 @@ Sources (data files of the reference repository, not code):
                   (write_paral, io.f90:251-275)
   run/output.txt  log of the same run: the 31 eigenvalues of d2(2:nz,2:nz) printed by
                   diagonalize_matrix (helmholtz.f90:112-120), the Blasius iteration count,
-                  and the initial dt
+                  the initial dt and the maxcalc lines (navierstokes.f90:494-506)
+  run/q00019.dat  mpi_snapshot file (mpi_io.f90:339-497) of a 224 x 24 x 33 run, stored whole
+                  as float32 in q00019.npz
+  run/prev_q00001.dat  the snapshot of step 10 of the run/input.dat case (Blasius base flow,
+                  perturbations ~1e-17): its header and the y = 0 planes of u and temp
 Run from the repository root:  python tests/golden/make_reference_fixtures.py
 """
 
 import json
 import os
 import shutil
+
+import numpy as np
 
 REF = "/root/reference/run"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_run")
@@ -30,6 +36,8 @@ def main():
     dt = float(txt.split("based input cfl, grid, and initial dt =")[1].split()[0])
     div = [float(x.split(",")[0]) for x in txt.split("divergence avg error =")[1:]]
     maxu = [float(x.split()[0]) for x in txt.split("Max|u+ubar|=")[1:]]
+    cflx = [float(x.split()[0]) for x in txt.split("Max. x-CFL#:")[1:]]
+    rho = [float(x.split()[0]) for x in txt.split("Max|rho|=")[1:]]
     meta = {
         "case": "run/input.dat: nx=224 ny=24 nz=32 mapping=2 xlen=333.3333 ylen=20 zlen=33.3333 x0=8.3333 "
                 "u0=1 xnu=0.001 blfac=8 cfl=0.1 bctype=0 sgsflag=0 dealiasflag=2",
@@ -39,9 +47,17 @@ def main():
         "initial_dt": dt,
         "divergence_avg_error": div,
         "max_u_plus_ubar": maxu,
+        "max_x_cfl": cflx,
+        "max_rho": rho,
     }
     with open(os.path.join(OUT, "output_txt.json"), "w") as fh:
         json.dump(meta, fh, indent=1)
+    q19 = np.fromfile(os.path.join(REF, "q00019.dat"), dtype="<f4")
+    np.savez_compressed(os.path.join(OUT, "q00019.npz"), record=q19)
+    q1 = np.fromfile(os.path.join(REF, "prev_q00001.dat"), dtype="<f4")
+    nx, ny, nzp = (int(v) for v in q1[3:6])
+    body = q1[7:].reshape(5, ny, nzp, nx)
+    np.savez_compressed(os.path.join(OUT, "prev_q00001_y0.npz"), header=q1[:7], u_y0=body[0, 0], temp_y0=body[3, 0])
     print("wrote", OUT)
 
 

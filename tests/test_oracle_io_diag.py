@@ -1,0 +1,250 @@
+"""Pin the oracle's snapshot / restart records and per-step diagnostics.
+
+* mpi_snapshot (mpi_io.f90:339-497): the reference's own snapshot files in run/
+  (q00019.dat, prev_q00001.dat; fixtures in tests/golden/reference_run/) read with
+  the layout of run/readsnapshot.m, transformed to spectral space and written back
+  by the oracle: the float32 values come back to within one float32 spacing.
+* mpi_write_restart_file / mpi_read_restart_file (mpi_io.f90:9-336): the element
+  order of the writer's loop nest at sampled indices and the reader's inverse.
+* maxcalc (navierstokes.f90:358-511): the Max|u+ubar|, x-CFL and Max|rho| lines of
+  run/output.txt (the base-flow state of run/input.dat), the adaptive-dt rule on
+  hand-sized CFL numbers and the fringe restriction of the z-CFL.
+* wall_shear (run/compare_results.m:76-79): a single streamwise mode whose skin
+  friction and zero crossings have closed forms.
+"""
+
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import Oracle, OracleConfig
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_run")
+
+
+def fortran_records(path):
+    data = open(path, "rb").read()
+    off, out = 0, []
+    while off < len(data):
+        n = struct.unpack("<i", data[off:off + 4])[0]
+        out.append(data[off + 4: off + 4 + n])
+        off += 8 + n
+    return out
+
+
+def run_input_case(**kw):
+    """run/input.dat (224 x 24 x 32, mapping 2) with the reference's ubar/tbar of paral.dat."""
+    cfg = OracleConfig(nx=224, ny=24, nz=32, xlen=333.3333, ylen=20.0, zlen=33.3333, x0=8.3333, u0=1.0,
+                       xnu=0.001, mapping=2, blfac=8.0, bctype=0, sgsflag=0, dealiasflag=2, **kw)
+    o = Oracle(cfg, build_mats=False)
+    r = fortran_records(os.path.join(GOLD, "paral.dat"))
+    o.ubar = np.frombuffer(r[1], "<f8").copy()
+    o.tbar = np.frombuffer(r[7], "<f8").copy()
+    return o
+
+
+def within_f32_spacing(a, b, k=1):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    sp = np.spacing(np.maximum(np.abs(a), np.abs(b)).astype(np.float32))
+    return np.all(np.abs(a.astype(np.float64) - b.astype(np.float64)) <= k * sp.astype(np.float64))
+
+
+# ------------------------------------------------------------------ snapshots
+def test_snapshot_reference_file_roundtrip():
+    """q00019.dat: header layout, then read (readsnapshot.m) -> spectral -> oracle writer."""
+    rec = np.load(os.path.join(GOLD, "q00019.npz"))["record"]
+    head = rec[:7]
+    nx, ny, nzp = int(head[3]), int(head[4]), int(head[5])
+    assert (int(head[0]), nx, ny, nzp) == (1900, 224, 24, 33)
+    assert rec.size == 7 + 5 * nx * ny * nzp
+    o = run_input_case()
+    h, q = Oracle.read_snapshot(rec, nx, ny, nzp)
+    # readsnapshot.m semantics: u(i, j, k) = qout(i, k, j, 1); spot-check one element
+    i, j, k = 17, 5, 9
+    assert q[0, k, j, i] == rec[7 + (j * nzp + k) * nx + i]
+    # no-slip wall row (k = nzp) and free stream at the top: the (x, z, y) reading is right
+    assert np.max(np.abs(q[0, -1])) < 1e-9 and abs(np.mean(q[0, 0]) - 1.0) < 0.01
+    spec = []
+    for l in range(5):
+        ph = np.zeros((nzp, ny, nx + 2))
+        ph[:, :, :nx] = q[l]
+        spec.append(o.horfft_fwd(ph))
+    mine = o.snapshot_record(*spec, name=2, isum=int(head[0]), t=float(head[1]), dt=float(head[2]))
+    assert np.array_equal(mine[:6], head[:6])
+    assert mine[6] == ny            # one rank: nypl = ny (the file has 4 ranks, nypl = 6)
+    # the file holds float32 roundings of band-limited fields; the transform pair only
+    # removes that rounding noise from the kx-Nyquist mode: a few spacings of each field's
+    # largest value
+    res = nx * ny * nzp
+    for l in range(5):
+        a = mine[7 + l * res: 7 + (l + 1) * res].astype(np.float64)
+        b = rec[7 + l * res: 7 + (l + 1) * res].astype(np.float64)
+        scale = float(np.spacing(np.float32(np.max(np.abs(b))))) if np.any(b) else 0.0
+        assert np.max(np.abs(a - b)) <= 2 * scale, l
+
+
+def test_snapshot_base_flow_prev_q00001():
+    """prev_q00001.dat (step 10 of run/input.dat, name = 0): header, temp = tbar(k) of
+    paral.dat bit for bit after the float32 rounding, and u = ubar(k) on the top and
+    wall rows, where the run's mean flow has not moved (its interior rows hold the
+    evolved mean profile, up to 2% off ubar in the boundary layer)."""
+    fx = np.load(os.path.join(GOLD, "prev_q00001_y0.npz"))
+    head = fx["header"]
+    o = run_input_case()
+    z = np.zeros((o.nzp, o.ny, o.nx))
+    mine = o.snapshot_record(z, z, z, z, z, name=0, isum=int(head[0]), t=float(head[1]), dt=float(head[2]))
+    assert np.array_equal(mine[:6], head[:6])
+    body = mine[7:].reshape(5, o.ny, o.nzp, o.nx)
+    assert np.array_equal(body[3, 0], fx["temp_y0"])
+    assert np.array_equal(body[0, 0][0], fx["u_y0"][0])
+    assert np.max(np.abs(body[0, 0][-1].astype(np.float64) - fx["u_y0"][-1])) < 1e-16
+    # name == 2 skips the base profiles
+    mine2 = o.snapshot_record(z, z, z, z, z, name=2, isum=10, t=0.0, dt=0.0)
+    assert not np.any(mine2[7:])
+
+
+# -------------------------------------------------------------------- restart
+def test_restart_record_layout_and_roundtrip(rng):
+    o = Oracle(OracleConfig(nx=16, ny=8, nz=8), build_mats=False)
+    s = (o.nzp, o.ny, o.nx)
+    st = o.new_state(rng.standard_normal(s), rng.standard_normal(s), rng.standard_normal(s))
+    for k in ("unif", "unmif", "nlnif", "nlnmif"):
+        st[k] = rng.standard_normal((3,) + s)
+    st.update(istep=1234, t=5.25, dt=1e-3, dt2=1.1e-3, dt3=1.2e-3)
+    rec = o.restart_record(st)
+    res = o.nx * o.ny * o.nzp
+    assert rec.size == 9 + 20 * res
+    assert list(rec[:9]) == [np.float32(v) for v in (1234, 5.25, 1e-3, 1.1e-3, 1.2e-3, 16, 8, 9, 16)]
+    # mpi_io.f90:47-72: loop i (x) outer, j, k (z) inner -> offset ((i ny + j) nzp + k)
+    slots = {0: st["uf"], 1: st["vf"], 2: st["wf"], 5: st["unif"][1], 9: st["unmif"][2], 11: st["nlnif"][1],
+             13: st["nlnmif"][0], 15: st["nlnmif"][2]}
+    for f, arr in slots.items():
+        for (i, j, k) in [(0, 0, 0), (15, 7, 8), (3, 5, 2), (9, 0, 6)]:
+            assert rec[9 + f * res + (i * o.ny + j) * o.nzp + k] == np.float32(arr[k, j, i])
+    assert not np.any(rec[9 + 3 * res: 9 + 4 * res]) and not np.any(rec[9 + 16 * res:])
+    back = o.read_restart(rec)
+    for k in ("uf", "vf", "wf", "unif", "unmif", "nlnif", "nlnmif"):
+        assert np.array_equal(back[k], np.asarray(st[k], np.float32).astype(np.float64)), k
+    assert back["isum"] == 1234 and back["istep"] == 0 and back["dt2"] == float(np.float32(1.1e-3))
+    # quirk Q12: the reference's writer duplicates nlnif into slots 14-16
+    dup = o.restart_record(st, q12_compat=True)
+    assert np.array_equal(dup[9 + 13 * res: 9 + 16 * res], dup[9 + 10 * res: 9 + 13 * res])
+    with pytest.raises(ValueError):
+        Oracle(OracleConfig(nx=16, ny=8, nz=10), build_mats=False).read_restart(rec)
+
+
+# -------------------------------------------------------------------- maxcalc
+def test_maxcalc_output_txt():
+    """run/output.txt:136-143: Max|u+ubar|, x-CFL and Max|rho| of the base-flow state."""
+    meta = json.load(open(os.path.join(GOLD, "output_txt.json")))
+    o = run_input_case()
+    z = np.zeros((o.nzp, o.ny, o.nx))
+    dt = meta["initial_dt"]
+    stats, dt1, dt2, dt3, ch = o.maxcalc(z, z, z, z, 1.0, dt, dt, dt, t=10 * dt)
+    assert abs(max(abs(stats[0]), abs(stats[1])) - meta["max_u_plus_ubar"][0]) < 1e-15
+    assert abs(stats[8] - meta["max_x_cfl"][0]) < 1e-15
+    assert max(abs(stats[6]), abs(stats[7])) == meta["max_rho"][0]
+    assert stats[9] == 0.0 and stats[10] == 0.0
+    assert not ch and dt1 == dt          # cflx = 0.1 is inside the limits
+
+
+def _one_mode(o, amp, m=1, plane=None):
+    """Spectral field of amp*cos(2 pi m i / nx) on every plane (or one plane)."""
+    f = np.zeros((o.nzp, o.ny, o.nx))
+    ks = range(o.nzp) if plane is None else [plane]
+    for k in ks:
+        f[k, 0, 2 * m] = 0.5 * amp
+    return f
+
+
+def test_maxcalc_decision_rule():
+    o = Oracle(OracleConfig(nx=32, ny=8, nz=16, mapping=3), build_mats=False)
+    z = np.zeros((o.nzp, o.ny, o.nx))
+    u = _one_mode(o, 2.0)                       # |u| max = 2 (ubar = 0)
+    # x-CFL = 2 dt/dx = 0.3 > 0.25: dt shrinks by 1/1.25, dt2 takes the old dt, dt3 stays
+    dt = 0.15 * o.dx
+    stats, d1, d2, d3, ch = o.maxcalc(u, z, z, None, 1.0, dt, 7.0, 8.0, t=0.0)
+    assert abs(stats[0] - 2.0) < 1e-14 and abs(stats[1] + 2.0) < 1e-14
+    assert abs(stats[8] - 0.3) < 1e-14
+    assert ch and d1 == dt / 1.25 and d2 == dt and d3 == 8.0
+    # all CFLs below the lower limits and dt < dtmax: dt grows by 1.25
+    dt = 0.01 * o.dx
+    _, d1, d2, _, ch = o.maxcalc(u, z, z, None, 1.0, dt, 7.0, 8.0, t=0.0)
+    assert ch and d1 == dt / (1.0 / 1.25) and d2 == dt
+    # ... unless dt already reached dtmax
+    _, d1, _, _, ch = o.maxcalc(u, z, z, None, dt, dt, 7.0, 8.0, t=0.0)
+    assert not ch and d1 == dt
+
+
+def test_maxcalc_z_cfl_planes_and_fringe():
+    """z-CFL: plane 1 (top) takes dt/dz twice, plane nzp uses the cell above; past
+    istartbound only x < int((1 - 0.8 fsize) nx) counts."""
+    o = Oracle(OracleConfig(nx=32, ny=8, nz=16, mapping=3, istartbound=1.0), build_mats=False)
+    z = np.zeros((o.nzp, o.ny, o.nx))
+    zp = o.zpts
+    dt = 1e-3
+    w = np.zeros_like(z)
+    w[5, 0, 0] = 1.0                             # w = 1 on plane 6 (1-based)
+    s, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=0.0)
+    assert abs(s[10] - dt / (zp[5] - zp[6])) < 1e-15 * s[10]
+    w = np.zeros_like(z)
+    w[0, 0, 0] = 1.0
+    s, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=0.0)
+    assert abs(s[10] - dt * dt / (zp[0] - zp[1]) ** 2) < 1e-14 * s[10]
+    w = np.zeros_like(z)
+    w[o.nz, 0, 0] = 1.0
+    s, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=0.0)
+    assert abs(s[10] - dt / (zp[o.nz - 1] - zp[o.nz])) < 1e-15 * s[10]
+    # fringe: w = cos(2 pi i/nx) peaks at i = 0 and is -1 at i = nx/2; with fsize = 0.5
+    # findex = int(0.6 nx) = 19 keeps i = 16 (|w| = 1) and drops nothing larger
+    w = _one_mode(o, 1.0, plane=5)
+    s_all, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=0.0, fsize=0.5)
+    s_fr, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=2.0, fsize=1.25)   # findex = 0
+    assert s_all[10] > 0 and s_fr[10] == -np.finfo(np.float64).max
+    # fsize = 1.0: findex = int(0.2*32) = 6 -> max over i < 6 of |cos| = 1 (i = 0)
+    s_f1, *_ = o.maxcalc(z, z, w, None, 1.0, dt, dt, dt, t=2.0, fsize=1.0)
+    assert abs(s_f1[10] - s_all[10]) < 1e-15 * s_all[10]
+
+
+def test_nancheck():
+    o = Oracle(OracleConfig(nx=16, ny=8, nz=8), build_mats=False)
+    a = np.zeros((o.nzp, o.ny, o.nx))
+    b = a.copy()
+    b[3, 2, 5] = np.inf
+    assert not o.nancheck(a, b, None)            # Inf is not NaN (isnan)
+    b[1, 1, 1] = np.nan
+    assert o.nancheck(a, b)
+
+
+# ----------------------------------------------------------------- wall shear
+def test_wall_shear_closed_form():
+    """u = c + a cos(2 pi m i/nx) on row nz, 0 on the wall row: cf(i) = xnu (c + a cos)/dz/0.5;
+    the first + -> - crossing sits at i* = nx acos(-c/a)/(2 pi m), the next - -> + one at
+    nx (2 pi - acos(-c/a))/(2 pi m); xpts = x0 + i dx (dx = xlen/(nx-1))."""
+    o = Oracle(OracleConfig(nx=256, ny=8, nz=16, mapping=3, xnu=1e-3, x0=2.0), build_mats=False)
+    a, c, m = 1.0, 0.3, 1
+    f = np.zeros((o.nzp, o.ny, o.nx))
+    f[o.nz - 1, 0, 0] = c
+    f[o.nz - 1, 0, 2 * m] = 0.5 * a
+    cf, xs, xr = o.wall_shear(f)
+    i = np.arange(o.nx)
+    dz = o.zpts[o.nz - 1] - o.zpts[o.nz]
+    want = o.cfg.xnu * (c + a * np.cos(2 * math.pi * m * i / o.nx)) / dz / 0.5
+    assert np.max(np.abs(cf - want)) < 1e-13 * np.max(np.abs(want))
+    th = math.acos(-c / a)
+    i_sep = o.nx * th / (2 * math.pi * m)
+    i_reatt = o.nx * (2 * math.pi - th) / (2 * math.pi * m)
+    # linear interpolation between samples one index apart puts the root of f off by at
+    # most |f''|/(8 |f'|) = w |cot(theta)|/8 index units (w = 2 pi m/nx)
+    tol = (2 * math.pi * m / o.nx) * abs(math.cos(th) / math.sin(th)) / 8 * o.dx * 1.1
+    assert abs(xs - (o.cfg.x0 + i_sep * o.dx)) < tol
+    assert abs(xr - (o.cfg.x0 + i_reatt * o.dx)) < tol
+    # base flow enters through ubar; a positive cf everywhere has no crossing
+    o.ubar[o.nz - 1] = 2.0
+    _, xs2, xr2 = o.wall_shear(f)
+    assert math.isnan(xs2) and math.isnan(xr2)
