@@ -55,7 +55,7 @@ typedef struct lsb_config {
     int nx, ny, nz;                 /* &dimen (common.f90:20)                  */
     int mapping;                    /* &run  mapping 0..3 (init.f90:188-254)    */
     int bctype;                     /* 0 homogeneous, 1 Spalart, 3 ceiling V(x) */
-    int forcing;                    /* must be 0 (sponge is outside this path)   */
+    int forcing;                    /* 0 periodic, 1 sponge layer (lsb_set_sponge) */
     int sgsflag, sgsmodel;          /* sgsterm.f90:21-35                        */
     int dealiasflag;                /* 0 none, 1 3/2-rule, 2 spectral filter     */
     int dealias_ref_compat;         /* 1: reference ky placement (dealiasing.f90:329,435) */
@@ -263,6 +263,15 @@ int lsb_state_nancheck(lsb_ctx* ctx, int* has_nan);
  * of cf from >= 0 to < 0 (x_sep) and back (x_reatt), linear in xpts; NaN if none. */
 int lsb_wall_shear(lsb_ctx* ctx, const double* uf, double* cf, double* x_sep, double* x_reatt);
 int lsb_state_wall_shear(lsb_ctx* ctx, double* cf, double* x_sep, double* x_reatt);
+/* navierstokes.f90:119-187 force(): the sponge coefficient lambda(x) (nx values) of the
+ * fringe from xpts, xlen, u0, fsize, lfsize, sfsize.  Host only. */
+int lsb_sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize,
+                      double sfsize, double* lam);
+/* runparam fsize, lfsize, sfsize of the sponge layer (config forcing = 1; lsb_create uses
+ * run/input.dat's 0.15, 0.25, 0.70 until this is called).  With forcing = 1 the step adds
+ * lambda(x) (ubar - u, -v, -w) to the nonlinear term (navierstokes.f90:750-756) and the
+ * velocity solves take their boundary values (navierstokes.f90:1389). */
+int lsb_set_sponge(lsb_ctx* ctx, double fsize, double lfsize, double sfsize);
 /* runparam dtadapt, iomod, fsize for lsb_timestep: with dtadapt = 1, after every step
  * whose count is a multiple of 5 or of iomod, a NaN check (LSB_E_NUMERIC) and
  * maxcalc(dtmax = 1.0) on the state (navierstokes.f90:74-87). */

@@ -101,6 +101,8 @@ _SIGS = {
     "lsb_wall_shear": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "lsb_state_wall_shear": (C.c_int, [_vp, _dp, _dp, _dp]),
     "lsb_set_adaptive_dt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
+    "lsb_sponge_lambda": (C.c_int, [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+    "lsb_set_sponge": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
 }
 
 

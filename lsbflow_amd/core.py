@@ -110,6 +110,14 @@ def _pf(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def sponge_lambda(xpts, xlen: float, u0: float, fsize: float, lfsize: float, sfsize: float):
+    """navierstokes.f90:119-187: the fringe coefficient lambda(x) on xpts (host)."""
+    xp = _f64(xpts)
+    lam = np.zeros(xp.size)
+    check(_lib.lib().lsb_sponge_lambda(xp.size, _p(xp), xlen, u0, fsize, lfsize, sfsize, _p(lam)), "sponge_lambda")
+    return lam
+
+
 def dct_diff(y, xl: float, gf, p: int):
     """chebyshev.f90:204-270 along axis 0 of y (n, ...)."""
     y = _f64(y)
@@ -492,6 +500,9 @@ class Context:
 
     def set_adaptive_dt(self, dtadapt=1, iomod=5, fsize=0.0):
         check(self._L.lsb_set_adaptive_dt(self.h, dtadapt, iomod, fsize), "set_adaptive_dt")
+
+    def set_sponge(self, fsize=0.15, lfsize=0.25, sfsize=0.70):
+        check(self._L.lsb_set_sponge(self.h, fsize, lfsize, sfsize), "set_sponge")
 
 
 def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:

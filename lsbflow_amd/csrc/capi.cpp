@@ -141,6 +141,13 @@ int upload_context(lsb_ctx* c) {
     par.insert(par.end(), c->ubarp.begin(), c->ubarp.end());
     par.insert(par.end(), c->ubarpp.begin(), c->ubarpp.end());
     LSB_TRY(put(&c->d_paral, par));
+    std::vector<double> lam;  // kept alive until the stream is synchronised
+    if (c->cfg.forcing == 1) {  // navierstokes.f90:119-187, from the current xpts
+        lam.assign(c->nxpp, 0.0);
+        sponge_lambda(c->nx, c->xpts.data(), c->cfg.xlen, c->cfg.u0, c->sponge[0], c->sponge[1], c->sponge[2],
+                      lam.data());
+        LSB_TRY(put(&c->d_lam, lam));
+    }
     std::vector<double> del;
     del.insert(del.end(), c->delta_static.begin(), c->delta_static.end());
     del.insert(del.end(), c->delta_dyn.begin(), c->delta_dyn.end());
@@ -196,9 +203,9 @@ extern "C" int lsb_create(const lsb_config* cfg, lsb_ctx** out) {
         set_error("lsb_create: need even nx >= 4, even ny >= 2, nz >= 4");
         return LSB_E_ARG;
     }
-    if (g.forcing != 0) {
-        set_error("lsb_create: forcing (sponge layer) is outside this path");
-        return LSB_E_UNSUPPORTED;
+    if (g.forcing != 0 && g.forcing != 1) {
+        set_error("lsb_create: forcing must be 0 (periodic) or 1 (sponge layer)");
+        return LSB_E_ARG;
     }
     if (g.dealiasflag < 0 || g.dealiasflag > 2) {
         set_error("lsb_create: dealiasflag must be 0, 1 or 2");
@@ -258,7 +265,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     step_free(c);
     double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
-                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red};
+                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red, c->d_lam};
     for (double* p : ds)
         if (p) (void)hipFree(p);
     if (c->d_walls) (void)hipFree(c->d_walls);
@@ -1607,3 +1614,29 @@ int state_adapt_dt(lsb_ctx* c) {
     return lsb_state_maxcalc(c, 1.0, c->fsize, nullptr, nullptr);
 }
 }  // namespace lsb
+
+extern "C" int lsb_sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize,
+                                 double sfsize, double* lam) {
+    if (nx < 4 || !xpts || !lam || !(fsize > 0.0) || !(lfsize > 0.0) || !(sfsize > 0.0) ||
+        !(sfsize + lfsize < 1.0)) {
+        set_error("sponge_lambda: need nx >= 4, fsize, lfsize, sfsize > 0 and lfsize + sfsize < 1");
+        return LSB_E_ARG;
+    }
+    sponge_lambda(nx, xpts, xlen, u0, fsize, lfsize, sfsize, lam);
+    return LSB_OK;
+}
+
+extern "C" int lsb_set_sponge(lsb_ctx* c, double fsize, double lfsize, double sfsize) {
+    LSB_ENTER(c);
+    std::vector<double> lam(c->nxpp, 0.0);
+    LSB_TRY(lsb_sponge_lambda(c->nx, c->xpts.data(), c->cfg.xlen, c->cfg.u0, fsize, lfsize, sfsize, lam.data()));
+    c->sponge[0] = fsize;
+    c->sponge[1] = lfsize;
+    c->sponge[2] = sfsize;
+    c->fsize = fsize;
+    if (c->cfg.forcing == 1) {
+        if (!c->d_lam) LSB_TRY(dev_alloc((void**)&c->d_lam, (size_t)c->nxpp * sizeof(double)));
+        LSB_HIP(hipMemcpyAsync(c->d_lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    return st.sync();
+}

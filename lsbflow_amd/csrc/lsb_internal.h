@@ -152,6 +152,8 @@ struct lsb_ctx {
     double* d_red = nullptr;    // reduction scratch (kRedDoubles doubles)
     double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
     double* d_diag_red = nullptr;  // diagnostics partials + small tables (kDiagDoubles)
+    double* d_lam = nullptr;    // sponge lambda(x), nxpp (forcing = 1; pads 0)
+    double sponge[3] = {0.15, 0.25, 0.70};  // fsize, lfsize, sfsize (run/input.dat values)
     int* d_walls = nullptr;     // wall planes of the static model's utau average
     int nwalls = 0;
     // parity-split forms of dfdz (d_mdz) and of the two Helmholtz projections, with
@@ -213,6 +215,8 @@ void cheb_matrix(int nzp, double xl, const double* gf, double* dor, double* d, d
 void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m);
 int eigen_decompose(const double* a, int n, double* lam, double* e, double* ei);
 void lifting_functions(lsb_ctx* c);
+void sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize, double sfsize,
+                   double* lam);
 
 // device memory helpers (capi.cpp)
 int dev_alloc(void** p, size_t bytes);

@@ -21,6 +21,7 @@ struct NlArgsT {
     T* h[9];               // h_lm = 1/2 u_l u_m + tau_lm (NL_PREP: u_l u_m); null entries skipped
     T* uo[3];              // NL_PREP: velocity with ubar added (may alias ui)
     T* go[9];              // NL_PREP: g with ubarp added at (0, 2) (may alias g)
+    const double* lam;     // sponge coefficient lambda(x), nx values, or null (forcing = 0)
     int write_base;
     int sub_base;          // store h_00 - ubar^2/2 (the plane-constant base part, restored in spectral space)
 };

@@ -136,11 +136,15 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
             a.uo[2][off] = (T)u[2];
             a.go[0 + 3 * 2][off] = (T)g[0][2];
         }
+        // sponge (navierstokes.f90:190-200, 750-756): F = lambda(x) (ubar - u, -v, -w)
+        const double lam = a.lam ? a.lam[i] : 0.0;
+        const double tgt[3] = {a.ubar[k], 0.0, 0.0};
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             double s = 0.0;
 #pragma unroll
             for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * g[l][m];
+            if (a.lam) s = s + lam * (tgt[l] - u[l]);
             a.nlt[l][off] = (T)s;
         }
         double tau[3][3];

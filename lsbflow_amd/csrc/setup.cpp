@@ -678,4 +678,39 @@ int build_derived(lsb_ctx* c) {
     return LSB_OK;
 }
 
+// navierstokes.f90:119-187 force(), ftype = 1: the fringe's Rayleigh-damping coefficient.
+// kappa = 2 u0/dxf (-ln 1e-5); r(i) = kappa (s0 + s): s0 = exp(-eta0^2) ramps up over dxs
+// (1 on the plateau, 0 once eta >= 0), s = exp(-eta^2) decays over the left part dxl;
+// the profile is then rotated right by xshift, the last i <= nx/4 (1-based) with
+// xpts(i) < xpts(1) + dxl/1.5.
+void sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize, double sfsize,
+                   double* lam) {
+    const double epsx = 1.0e-5;
+    const double sfs = sfsize * fsize, lfs = lfsize * fsize;
+    const double rfsize = fsize - sfs - lfs;
+    const double dxs = sfs * xlen, dxl = lfs * xlen, dxr = rfsize * xlen;
+    const double dxf = dxs + dxl + dxr;
+    const double kappa = 2.0 * u0 / dxf * (-std::log(epsx));
+    std::vector<double> r(nx);
+    for (int i = 0; i < nx; ++i) {
+        const double eta0 = (xpts[i] - (xpts[nx - 1] - dxf + dxs)) / (dxs / 2.0);
+        const double eta = (xpts[i] - (xpts[nx - 1] - dxl)) / (dxl / 2.5);
+        double s0 = std::exp(-(eta0 * eta0));
+        double s = std::exp(-(eta * eta));
+        if (eta0 > 0.0 && eta < 0.0) s0 = 1.0;
+        if (eta < 0.0) s = 0.0;
+        if (eta >= 0.0) s0 = 0.0;
+        r[i] = kappa * (s0 + s);
+    }
+    int xshift = 0;
+    for (int i = 1; i <= nx / 4; ++i)
+        if (xpts[i - 1] < xpts[0] + dxl / 1.5) xshift = i;
+    if (xshift > 1) {
+        for (int i = 0; i < nx - xshift; ++i) lam[xshift + i] = r[i];
+        for (int i = 0; i < xshift; ++i) lam[i] = r[nx - xshift + i];
+    } else {
+        for (int i = 0; i < nx; ++i) lam[i] = r[i];
+    }
+}
+
 }  // namespace lsb

@@ -495,6 +495,7 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
     }
     a.ubar = c->d_paral;
     a.ubarp = c->d_paral + c->nzp;
+    a.lam = (c->cfg.forcing == 1) ? c->d_lam : nullptr;
     a.delta = c->d_delta + 2 * c->nzp;
     a.smag = c->d_smag;
     double* hph[9];
@@ -627,6 +628,7 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
     }
     a.ubar = c->d_paral;
     a.ubarp = c->d_paral + c->nzp;
+    a.lam = (c->cfg.forcing == 1) ? c->d_lam : nullptr;
     a.delta = c->d_delta + 2 * c->nzp;
     a.smag = c->d_smag;
     a.sub_base = 1;
@@ -896,8 +898,11 @@ static int update_once(lsb_ctx* c) {
     } else {
         LSB_TRY(sp_rhs_vel(c, r, s));
     }
-    // boundary data (navierstokes.f90:1321-1373)
+    // boundary data (navierstokes.f90:1321-1373): the imposed suction (bound) when
+    // bctype > 0 past istartbound; the velocity solves take the boundary values also
+    // with the sponge on (navierstokes.f90:1389)
     const bool nhom = (g.bctype > 0 && c->t >= g.istartbound);
+    const bool vel_nhom = nhom || g.forcing == 1;
     {
         const double* in[1] = {L.spec[S->u[2]]};
         double* out[1] = {S->d2w};
@@ -958,7 +963,7 @@ static int update_once(lsb_ctx* c) {
         double* rh[3] = {L.spec[S->rhs[0]], L.spec[S->rhs[1]], L.spec[S->rhs[2]]};
         const double* bt[3] = {S->bct[0], S->bct[1], S->bct[2]};
         const double* bb[3] = {S->bcb[0], S->bcb[1], S->bcb[2]};
-        LSB_TRY(helmholtz(c, rh, nhom ? bt : nullptr, nhom ? bb : nullptr, 3, dtnui / c->bdfv, s));
+        LSB_TRY(helmholtz(c, rh, vel_nhom ? bt : nullptr, vel_nhom ? bb : nullptr, 3, dtnui / c->bdfv, s));
     }
     // pressure step (navierstokes.f90:1411-1464)
     double* uN = L.spec[S->rhs[0]];

@@ -432,6 +432,19 @@ module lsb_capi
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: cf(*), x_sep, x_reatt
     end function
+    integer(c_int) function lsb_sponge_lambda(nx, xpts, xlen, u0, fsize, lfsize, sfsize, lam) &
+        bind(C, name="lsb_sponge_lambda")
+      import :: c_int, c_double
+      integer(c_int), value :: nx
+      real(c_double), intent(in) :: xpts(*)
+      real(c_double), value :: xlen, u0, fsize, lfsize, sfsize
+      real(c_double), intent(out) :: lam(*)
+    end function
+    integer(c_int) function lsb_set_sponge(ctx, fsize, lfsize, sfsize) bind(C, name="lsb_set_sponge")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: fsize, lfsize, sfsize
+    end function
     integer(c_int) function lsb_set_adaptive_dt(ctx, dtadapt, iomod, fsize) bind(C, name="lsb_set_adaptive_dt")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

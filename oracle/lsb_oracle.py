@@ -84,6 +84,9 @@ class OracleConfig:
     ifrz: int = 0
     nstart: int = 0
     dt: float = 1.0e-3
+    fsize: float = 0.15
+    lfsize: float = 0.25
+    sfsize: float = 0.70
 
     def with_(self, **kw) -> "OracleConfig":
         return replace(self, **kw)
@@ -962,8 +965,13 @@ class Oracle:
                 for m in range(3):
                     nlt[l] = nlt[l] - 0.5 * ui[m] * duidxj[l, m]
                     uiuj[m, l] = ui[m] * ui[l]
-        if c.forcing == 1:
-            raise NotImplementedError("sponge forcing is outside the restated path")
+        if c.forcing == 1:  # navierstokes.f90:750-756: nltermi += F_i
+            lam = np.zeros(self.nxpp)          # pads carry no forcing
+            lam[: self.nx] = self.sponge_lambda()
+            lam = lam[None, None, :]
+            nlt[0] = nlt[0] + lam * (ubk - ui[0])
+            nlt[1] = nlt[1] + lam * (0.0 - ui[1])
+            nlt[2] = nlt[2] + lam * (0.0 - ui[2])
         tau = np.zeros_like(uiuj)
         if sgsflag == 1:
             tau = self.tau_sgs(ui, uiuj, duidxj)
@@ -1130,6 +1138,49 @@ class Oracle:
         divf = self.dfdx(uf) + self.dfdy(vf) + self.dfdz(wf)
         divp = self.horfft_inv(divf)
         return float(np.sum(np.abs(divp)) / (self.nx * self.ny * self.nzp))
+
+    # ------------------------------------------------------------ sponge layer
+    def sponge_lambda(self):
+        """navierstokes.f90:119-187 force(): the Rayleigh-damping coefficient lambda(x) of the
+        fringe (ftype = 1, Herbst & Henningson), nx values.  kappa = 2 u0/dxf (-ln 1e-5);
+        r(i) = kappa (s0 + s) with s0 = exp(-eta0^2) rising over dxs (1 on the plateau,
+        0 once eta >= 0) and s = exp(-eta^2) decaying over the left part dxl; then shifted
+        right by xshift = the last i <= nx/4 with xpts(i) < xpts(1) + dxl/1.5."""
+        c = self.cfg
+        nx, xp = self.nx, self.xpts
+        epsx = 1.0e-5
+        sfs = c.sfsize * c.fsize
+        lfs = c.lfsize * c.fsize
+        rfsize = c.fsize - sfs - lfs
+        dxs = sfs * c.xlen
+        dxl = lfs * c.xlen
+        dxr = rfsize * c.xlen
+        dxf = dxs + dxl + dxr
+        kappa = 2.0 * c.u0 / dxf * (-math.log(epsx))
+        r = np.zeros(nx)
+        for i in range(nx):
+            eta0 = (xp[i] - (xp[nx - 1] - dxf + dxs)) / (dxs / 2.0)
+            eta = (xp[i] - (xp[nx - 1] - dxl)) / (dxl / 2.5)
+            s0 = math.exp(-(eta0 * eta0))    # Fortran x**2 with a constant exponent is x*x
+            s = math.exp(-(eta * eta))
+            if eta0 > 0.0 and eta < 0.0:
+                s0 = 1.0
+            if eta < 0.0:
+                s = 0.0
+            if eta >= 0.0:
+                s0 = 0.0
+            r[i] = kappa * (s0 + s)
+        xshift = 0
+        for i in range(1, nx // 4 + 1):          # Fortran i = 1..nx/4
+            if xp[i - 1] < xp[0] + dxl / 1.5:
+                xshift = i
+        lam = np.zeros(nx)
+        if xshift > 1:
+            lam[xshift:] = r[: nx - xshift]
+            lam[:xshift] = r[nx - xshift:]
+        else:
+            lam[:] = r
+        return lam
 
     # ------------------------------------------------- per-step diagnostics
     def nancheck(self, *fields):

@@ -148,3 +148,38 @@ def test_step_divergence_free():
     scale = np.sqrt(np.mean(c.inv(g["uf"])[..., : c.nx] ** 2))
     c.close()
     assert d < 1e-9 * max(scale, 1e-3) * c.nx
+
+
+@pytest.mark.parametrize("bctype", [3, 0])
+def test_c1_dns_sponge_forcing(bctype):
+    """forcing = 1 (navierstokes.f90:92-211, 750-756, 1389): lambda(x) (ubar - u, -v, -w) in the
+    nonlinear term and velocity solves that take their boundary values; fringe fsize = 0.3 so
+    that it covers a fifth of the 64 x-points.  bctype 0 isolates the switch of the solves."""
+    case = L.cases.CASES["c1_dns"]
+    kw = dict(forcing=1, bctype=bctype)
+    o = oracle_for(case, fsize=0.3, lfsize=0.25, sfsize=0.70, **kw)
+    u, v, w = L.cases.initial_condition(case)
+    st = o.new_state(u, v, w)
+    for _ in range(case.nsteps):
+        o.timestep(st)
+    rng = np.random.default_rng(99)
+    pert = [f * (1.0 + 1e-15 * rng.standard_normal(f.shape)) for f in (u, v, w)]
+    st2 = o.new_state(*pert)
+    for _ in range(case.nsteps):
+        o.timestep(st2)
+    c = setup_case(case, **kw)
+    try:
+        c.set_sponge(0.3, 0.25, 0.70)
+        c.state_set(u, v, w)
+        c.timestep(case.nsteps)
+        g = c.state_get()
+    finally:
+        c.close()
+    g["_sens"] = {k: rel_l2(st2[k], st[k]) for k in FIELDS}
+    check_fields(st, g)
+    # the sponge changes the solution well beyond the bar
+    o0 = oracle_for(case, bctype=bctype)
+    st0 = o0.new_state(u, v, w)
+    for _ in range(case.nsteps):
+        o0.timestep(st0)
+    assert rel_l2(st0["uf"], st["uf"]) > 1e-6

@@ -248,3 +248,37 @@ def test_wall_shear_closed_form():
     o.ubar[o.nz - 1] = 2.0
     _, xs2, xr2 = o.wall_shear(f)
     assert math.isnan(xs2) and math.isnan(xr2)
+
+
+# -------------------------------------------------------------------- sponge
+@pytest.mark.parametrize("nx,fsize", [(64, 0.3), (224, 0.15), (512, 0.15)])
+def test_sponge_lambda(nx, fsize):
+    """navierstokes.f90:119-187 force(): the library's host lambda(x) equals the oracle's bit for
+    bit; the oracle's profile has the closed-form properties of the Herbst-Henningson fringe:
+    plateau value kappa = 2 u0/dxf (-ln 1e-5), zero outside the fringe, the rise over dxs
+    ending at the plateau with exp(-eta0^2), and the whole profile rotated by xshift."""
+    import lsbflow_amd as L
+    cfg = OracleConfig(nx=nx, ny=8, nz=8, xlen=333.3333, x0=8.3333, u0=1.0, forcing=1, fsize=fsize,
+                       lfsize=0.25, sfsize=0.70)
+    o = Oracle(cfg, build_mats=False)
+    lam = o.sponge_lambda()
+    lib = L.sponge_lambda(o.xpts, cfg.xlen, cfg.u0, cfg.fsize, cfg.lfsize, cfg.sfsize)
+    assert np.array_equal(lib, lam)
+    dxf = fsize * cfg.xlen
+    kappa = 2.0 * cfg.u0 / dxf * (-math.log(1e-5))
+    assert abs(lam.max() - kappa) < 1e-12 * kappa
+    dxs, dxl = 0.70 * fsize * cfg.xlen, 0.25 * fsize * cfg.xlen
+    xshift = max(i for i in range(1, nx // 4 + 1) if o.xpts[i - 1] < o.xpts[0] + dxl / 1.5)
+    r = np.roll(lam, -xshift) if xshift > 1 else lam
+    x = o.xpts
+    start = x[-1] - dxf + dxs           # the plateau starts here (eta0 = 0)
+    left = x[-1] - dxl                  # ... and ends here (eta = 0)
+    for i in range(nx):
+        if x[i] < start - 3.5 * dxs / 2:            # far upstream: exp(-eta0^2) < 5e-6
+            assert r[i] < 5e-6 * kappa
+        elif x[i] <= start:                          # the rise
+            assert abs(r[i] - kappa * math.exp(-((x[i] - start) / (dxs / 2)) ** 2)) < 1e-12 * kappa
+        elif x[i] < left:                            # the plateau
+            assert r[i] == kappa
+        else:                                        # the decay over the left part
+            assert abs(r[i] - kappa * math.exp(-((x[i] - left) / (dxl / 2.5)) ** 2)) < 1e-12 * kappa
