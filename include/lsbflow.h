@@ -56,7 +56,7 @@ typedef struct lsb_config {
     int mapping;                    /* &run  mapping 0..3 (init.f90:188-254)    */
     int bctype;                     /* 0 homogeneous, 1 Spalart, 3 ceiling V(x) */
     int forcing;                    /* 0 periodic, 1 sponge layer (lsb_set_sponge) */
-    int sgsflag, sgsmodel;          /* sgsterm.f90:21-35                        */
+    int sgsflag, sgsmodel;          /* sgsterm.f90:21-35 (4: TNS, no SGS stress) */
     int dealiasflag;                /* 0 none, 1 3/2-rule, 2 spectral filter     */
     int dealias_ref_compat;         /* 1: reference ky placement (dealiasing.f90:329,435) */
     int ifrz;                       /* navierstokes.f90:53-59                    */
@@ -272,6 +272,24 @@ int lsb_sponge_lambda(int nx, const double* xpts, double xlen, double u0, double
  * lambda(x) (ubar - u, -v, -w) to the nonlinear term (navierstokes.f90:750-756) and the
  * velocity solves take their boundary values (navierstokes.f90:1389). */
 int lsb_set_sponge(lsb_ctx* ctx, double fsize, double lfsize, double sfsize);
+/* filters.f90:9-49 filterall3pt(uf, vf, wf, tempf, pf), in place: modecutFC (six 3-pt
+ * filter passes with the SGS pair (a, b), coefficients 6, -15, 20, -15, 6, -1) of the
+ * physical u, v, w, p, and temp when some |T| > 1e-10.  tempf may be NULL. */
+int lsb_filterall3pt(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* tempf, double* pf);
+/* filters.f90:52-161 tns_automatic_filtering(uf, vf, wf, tempf, pf), in place: modecutFC with
+ * (1, 6) and (1, 2); when the ratio of the kinetic energies they remove (trapezoidal z
+ * integrals, I(h)/I(delta)) exceeds 0.007 the (1, 6) result replaces u, v, w, p.  ih,
+ * idelta, applied (may be NULL) return the integrals and the decision.  tempf is not
+ * touched (as the reference). */
+int lsb_tns_automatic_filtering(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* tempf,
+                                double* pf, double* ih, double* idelta, int* applied);
+/* The same two on the resident state (u, v, w, pf). */
+int lsb_state_filterall3pt(lsb_ctx* ctx);
+int lsb_state_tns_filter(lsb_ctx* ctx, double* ih, double* idelta, int* applied);
+/* runparam ifilt, modfilt for lsb_timestep (navierstokes.f90:31-39): with ifilt >= 1, before
+ * every step whose run counter is a multiple of modfilt, tns_automatic_filtering when
+ * sgsflag = 1 and sgsmodel = 4, else filterall3pt. */
+int lsb_set_filtering(lsb_ctx* ctx, int ifilt, int modfilt);
 /* runparam dtadapt, iomod, fsize for lsb_timestep: with dtadapt = 1, after every step
  * whose count is a multiple of 5 or of iomod, a NaN check (LSB_E_NUMERIC) and
  * maxcalc(dtmax = 1.0) on the state (navierstokes.f90:74-87). */

@@ -103,6 +103,11 @@ _SIGS = {
     "lsb_set_adaptive_dt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double]),
     "lsb_sponge_lambda": (C.c_int, [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "lsb_set_sponge": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
+    "lsb_filterall3pt": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "lsb_tns_automatic_filtering": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "lsb_state_filterall3pt": (C.c_int, [_vp]),
+    "lsb_state_tns_filter": (C.c_int, [_vp, _dp, _dp, _ip]),
+    "lsb_set_filtering": (C.c_int, [_vp, C.c_int, C.c_int]),
 }
 
 

@@ -504,6 +504,33 @@ class Context:
     def set_sponge(self, fsize=0.15, lfsize=0.25, sfsize=0.70):
         check(self._L.lsb_set_sponge(self.h, fsize, lfsize, sfsize), "set_sponge")
 
+    def filterall3pt(self, uf, vf, wf, tempf, pf):
+        """filters.f90:9-49 on copies; returns (uf, vf, wf, tempf, pf)."""
+        a = [None if f is None else np.array(f, dtype=np.float64, copy=True, order="C")
+             for f in (uf, vf, wf, tempf, pf)]
+        check(self._L.lsb_filterall3pt(self.h, *[_p(x) for x in a]), "filterall3pt")
+        return tuple(a)
+
+    def tns_automatic_filtering(self, uf, vf, wf, tempf, pf):
+        """filters.f90:52-161 on copies; returns (uf, vf, wf, tempf, pf, Ih, Idelta, applied)."""
+        a = [None if f is None else np.array(f, dtype=np.float64, copy=True, order="C")
+             for f in (uf, vf, wf, tempf, pf)]
+        ih, idl, ap = C.c_double(), C.c_double(), C.c_int()
+        check(self._L.lsb_tns_automatic_filtering(self.h, *[_p(x) for x in a], C.byref(ih), C.byref(idl),
+                                                  C.byref(ap)), "tns_automatic_filtering")
+        return tuple(a) + (ih.value, idl.value, bool(ap.value))
+
+    def state_filterall3pt(self):
+        check(self._L.lsb_state_filterall3pt(self.h), "state_filterall3pt")
+
+    def state_tns_filter(self):
+        ih, idl, ap = C.c_double(), C.c_double(), C.c_int()
+        check(self._L.lsb_state_tns_filter(self.h, C.byref(ih), C.byref(idl), C.byref(ap)), "state_tns_filter")
+        return ih.value, idl.value, bool(ap.value)
+
+    def set_filtering(self, ifilt=1, modfilt=1):
+        check(self._L.lsb_set_filtering(self.h, ifilt, modfilt), "set_filtering")
+
 
 def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:
     """Context for a configuration of lsbflow_amd.cases with its base flow installed."""

@@ -156,6 +156,8 @@ int upload_context(lsb_ctx* c) {
     std::vector<double> fz;
     fz.insert(fz.end(), c->fz_ab.begin(), c->fz_ab.end());
     fz.insert(fz.end(), c->fz_11.begin(), c->fz_11.end());
+    fz.insert(fz.end(), c->fz_16.begin(), c->fz_16.end());
+    fz.insert(fz.end(), c->fz_12.begin(), c->fz_12.end());
     LSB_TRY(put(&c->d_fz, fz));
     if (!c->d_smag) LSB_TRY(dev_alloc((void**)&c->d_smag, (size_t)nzp * sizeof(double)));
     if (3 * nzp + 128 > kRedDoubles) {
@@ -882,7 +884,11 @@ extern "C" int lsb_tau_sgs(lsb_ctx* c, const double* ui, const double* uiuj, con
         case 1: return lsb_dynamic_smagorinsky(c, ui, uiuj, duidxj, tauij);
         case 2: return lsb_sigma_model(c, duidxj, tauij);
         case 3: return lsb_interscale_model(c, ui, tauij);
-        default: set_error("tau_sgs: sgsmodel must be 0..3"); return LSB_E_ARG;
+        case 4:  // TNS: no case in sgsterm.f90:21-35, tauij keeps the caller's zero
+            if (!tauij) return LSB_E_ARG;
+            std::memset(tauij, 0, c->nphys * 9 * sizeof(double));
+            return LSB_OK;
+        default: set_error("tau_sgs: sgsmodel must be 0..4"); return LSB_E_ARG;
     }
 }
 
@@ -1640,3 +1646,169 @@ extern "C" int lsb_set_sponge(lsb_ctx* c, double fsize, double lfsize, double sf
     }
     return st.sync();
 }
+
+// ================================================ TNS / mode-cut filtering (filters.f90)
+namespace {
+
+constexpr double kModecut[6] = {6., -15., 20., -15., 6., -1.};  // filters.f90:637
+
+// filters.f90:617-654 modecutFC: qb = sum_l coef_l F^l q on nf physical device fields, F the
+// 3-pt filter (a, b) with interpolating z weights.  A, B: nf-field ping-pong buffers, tmp: the
+// filter's own scratch.
+int modecut(lsb_ctx* c, const double* const* q, double* const* qb, int nf, double a, double b, double* A, double* B,
+            double* tmp) {
+    std::vector<const double*> src(q, q + nf);
+    std::vector<double*> dst(nf);
+    for (int l = 0; l < 6; ++l) {
+        double* base = (l % 2 == 0) ? A : B;
+        for (int f = 0; f < nf; ++f) dst[f] = base + (size_t)f * c->nphys;
+        LSB_TRY(ph_filter3(c, src.data(), dst.data(), nf, a, b, 1, tmp, c->stream));
+        LSB_TRY(ph_axpy(c, dst.data(), qb, nf, kModecut[l], l == 0, c->stream));
+        for (int f = 0; f < nf; ++f) src[f] = dst[f];
+    }
+    return LSB_OK;
+}
+
+struct TnsBufs {
+    double *q, *A, *B, *tmp, *qb1, *qb2;
+};
+
+int tns_bufs(lsb_ctx* c, Stage& st, int nf, TnsBufs* b) {
+    const size_t n = c->nphys * (size_t)nf;
+    LSB_TRY(st.get(10, n, &b->q));
+    LSB_TRY(st.get(11, n, &b->A));
+    LSB_TRY(st.get(12, n, &b->B));
+    LSB_TRY(st.get(13, n, &b->tmp));
+    LSB_TRY(st.get(14, n, &b->qb1));
+    LSB_TRY(st.get(15, n, &b->qb2));
+    return LSB_OK;
+}
+
+std::vector<double*> field_ptrs(double* base, int nf, size_t stride) {
+    std::vector<double*> p(nf);
+    for (int f = 0; f < nf; ++f) p[f] = base + (size_t)f * stride;
+    return p;
+}
+
+// filters.f90:9-49 filterall3pt on nf spectral device fields, in place.
+int filterall_dev(lsb_ctx* c, Stage& st, double* const* spec, int nf) {
+    TnsBufs b;
+    LSB_TRY(tns_bufs(c, st, nf, &b));
+    auto q = field_ptrs(b.q, nf, c->nphys), qb = field_ptrs(b.qb1, nf, c->nphys);
+    std::vector<const double*> sp(spec, spec + nf), qc(q.begin(), q.end()), qbc(qb.begin(), qb.end());
+    LSB_TRY(horfft_inv(c, c->hp, sp.data(), q.data(), nf, c->nzp, c->stream));
+    LSB_TRY(modecut(c, qc.data(), qb.data(), nf, c->cfg.a, c->cfg.b, b.A, b.B, b.tmp));
+    return horfft_fwd(c, c->hp, qbc.data(), spec, nf, c->nzp, c->stream);
+}
+
+// filters.f90:52-161 tns_automatic_filtering on u, v, w, p (spectral device fields, in place).
+int tns_dev(lsb_ctx* c, Stage& st, double* const* spec, double* ih, double* idelta, int* applied) {
+    const int nf = 4;
+    TnsBufs b;
+    LSB_TRY(tns_bufs(c, st, nf, &b));
+    auto q = field_ptrs(b.q, nf, c->nphys), q2 = field_ptrs(b.qb1, nf, c->nphys), qf = field_ptrs(b.qb2, nf, c->nphys);
+    std::vector<const double*> sp(spec, spec + nf), qc(q.begin(), q.end()), q2c(q2.begin(), q2.end()),
+        qfc(qf.begin(), qf.end());
+    LSB_TRY(horfft_inv(c, c->hp, sp.data(), q.data(), nf, c->nzp, c->stream));
+    LSB_TRY(modecut(c, qc.data(), q2.data(), nf, 1.0, 6.0, b.A, b.B, b.tmp));  // 1/8 3/4 1/8
+    LSB_TRY(modecut(c, qc.data(), qf.data(), nf, 1.0, 2.0, b.A, b.B, b.tmp));  // 1/4 1/2 1/4
+    std::vector<double> e_delta(c->nzp), e_h(c->nzp);
+    LSB_TRY(ph_plane_energy(c, qc.data(), qfc.data(), 3, e_delta.data(), c->stream));
+    LSB_TRY(ph_plane_energy(c, qc.data(), q2c.data(), 3, e_h.data(), c->stream));
+    const double area = (double)c->ny * (double)c->nx;
+    double Ih = 0.0, Id = 0.0;
+    for (int k = c->nz - 1; k >= 0; --k) {  // filters.f90:113-117, k = nz .. 1
+        const double dz = c->zpts[k] - c->zpts[k + 1];
+        Ih = Ih + 0.5 * (e_h[k] / area + e_h[k + 1] / area) * dz;
+        Id = Id + 0.5 * (e_delta[k] / area + e_delta[k + 1] / area) * dz;
+    }
+    const bool apply = Ih / Id > 0.007;  // ekin_ratio_limit (Tantikul & Domaradzki 2011)
+    if (apply) LSB_TRY(horfft_fwd(c, c->hp, q2c.data(), spec, nf, c->nzp, c->stream));
+    if (ih) *ih = Ih;
+    if (idelta) *idelta = Id;
+    if (applied) *applied = apply ? 1 : 0;
+    return LSB_OK;
+}
+
+}  // namespace
+
+extern "C" int lsb_filterall3pt(lsb_ctx* c, double* uf, double* vf, double* wf, double* tempf, double* pf) {
+    LSB_ENTER(c);
+    if (!uf || !vf || !wf || !pf) {
+        set_error("filterall3pt: null array");
+        return LSB_E_ARG;
+    }
+    bool hot = false;  // filters.f90:34: temperature only when some |T| > 1e-10
+    if (tempf)
+        for (size_t q = 0; q < c->nspec && !hot; ++q) hot = std::fabs(tempf[q]) > 1.0e-10;
+    double* hs[5] = {uf, vf, wf, pf, tempf};
+    const int nf = hot ? 5 : 4;
+    double* d;
+    LSB_TRY(st.get(0, c->nspec * nf, &d));
+    std::vector<double*> sp(nf);
+    for (int f = 0; f < nf; ++f) {
+        sp[f] = d + (size_t)f * c->nspec;
+        LSB_TRY(st.in(sp[f], hs[f], c->nspec));
+    }
+    LSB_TRY(filterall_dev(c, st, sp.data(), nf));
+    for (int f = 0; f < nf; ++f) LSB_TRY(st.out(hs[f], sp[f], c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_tns_automatic_filtering(lsb_ctx* c, double* uf, double* vf, double* wf, double* tempf, double* pf,
+                                           double* ih, double* idelta, int* applied) {
+    LSB_ENTER(c);
+    (void)tempf;  // the temperature is not filtered here (filters.f90:120-124)
+    if (!uf || !vf || !wf || !pf) {
+        set_error("tns_automatic_filtering: null array");
+        return LSB_E_ARG;
+    }
+    double* hs[4] = {uf, vf, wf, pf};
+    double* d;
+    LSB_TRY(st.get(0, c->nspec * 4, &d));
+    std::vector<double*> sp(4);
+    for (int f = 0; f < 4; ++f) {
+        sp[f] = d + (size_t)f * c->nspec;
+        LSB_TRY(st.in(sp[f], hs[f], c->nspec));
+    }
+    int ap = 0;
+    LSB_TRY(tns_dev(c, st, sp.data(), ih, idelta, &ap));
+    if (ap)
+        for (int f = 0; f < 4; ++f) LSB_TRY(st.out(hs[f], sp[f], c->nspec));
+    if (applied) *applied = ap;
+    return st.sync();
+}
+
+extern "C" int lsb_state_filterall3pt(lsb_ctx* c) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_filterall3pt"));
+    double* sp[4] = {step_field(c, 0), step_field(c, 1), step_field(c, 2), step_field(c, 15)};
+    return filterall_dev(c, st, sp, 4);
+}
+
+extern "C" int lsb_state_tns_filter(lsb_ctx* c, double* ih, double* idelta, int* applied) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_tns_filter"));
+    double* sp[4] = {step_field(c, 0), step_field(c, 1), step_field(c, 2), step_field(c, 15)};
+    return tns_dev(c, st, sp, ih, idelta, applied);
+}
+
+extern "C" int lsb_set_filtering(lsb_ctx* c, int ifilt, int modfilt) {
+    LSB_TRY(check_ctx(c));
+    if (ifilt < 0 || (ifilt >= 1 && modfilt < 1)) {
+        set_error("set_filtering: ifilt >= 0 and modfilt >= 1 when filtering");
+        return LSB_E_ARG;
+    }
+    c->ifilt = ifilt;
+    c->modfilt = modfilt;
+    return LSB_OK;
+}
+
+namespace lsb {
+// navierstokes.f90:31-39: before the update of step istep, every modfilt steps.
+int state_filter_hook(lsb_ctx* c) {
+    if (c->ifilt < 1 || c->istep % c->modfilt != 0) return LSB_OK;
+    if (c->cfg.sgsflag == 1 && c->cfg.sgsmodel == 4) return lsb_state_tns_filter(c, nullptr, nullptr, nullptr);
+    return lsb_state_filterall3pt(c);
+}
+}  // namespace lsb

@@ -130,6 +130,7 @@ struct lsb_ctx {
     std::vector<double> delta_static, delta_dyn, delta_sigma;
     // filter_q_3pt z weights for (a,b) and (1,1) (filters.f90:396-448)
     std::vector<double> fz_ab, fz_11;    // 3*nzp each: az, bz, cz
+    std::vector<double> fz_16, fz_12;    // (1, 6) and (1, 2): the TNS filters
 
     // device copies
     int device = 0;
@@ -147,7 +148,7 @@ struct lsb_ctx {
     double* d_lift = nullptr;   // 4*nzp: hbot, htop, d2hbot, d2htop; then 4*nzm: Einv of each
     double* d_paral = nullptr;  // 3*nzp: ubar, ubarp, ubarpp
     double* d_delta = nullptr;  // 3*nzp: static, dynamic, sigma
-    double* d_fz = nullptr;     // 6*nzp: (a,b) weights then (1,1) weights
+    double* d_fz = nullptr;     // 12*nzp: (a,b), (1,1), (1,6), (1,2) weights
     double* d_smag = nullptr;   // nzp (static Smagorinsky -(cs delta fvd)^2) + scratch
     double* d_red = nullptr;    // reduction scratch (kRedDoubles doubles)
     double* d_ftab = nullptr;   // spectral filter factors of one plane (lh*ny)
@@ -186,6 +187,8 @@ struct lsb_ctx {
     // adaptive time step (navierstokes.f90:74-87; runparam dtadapt, iomod, fsize)
     int dtadapt = 0, iomod = 1;
     double fsize = 0.0;
+    // filtering (navierstokes.f90:31-39; runparam ifilt, modfilt)
+    int ifilt = 0, modfilt = 1;
     double abfac[3] = {0, 0, 0};
     double bdf[3] = {0, 0, 0};
     double bdfv = 0.0;
@@ -242,6 +245,8 @@ int nan_scan(lsb_ctx* c, const double* const* f, int nf, size_t n, bool* found, 
 int wall_mean(lsb_ctx* c, const double* phys2, double* mean_host, hipStream_t s);
 // NaN check + maxcalc on the resident state after a step when dtadapt is on (capi.cpp)
 int state_adapt_dt(lsb_ctx* c);
+// filterall3pt / TNS filtering of the resident state before a step when ifilt >= 1 (capi.cpp)
+int state_filter_hook(lsb_ctx* c);
 
 // profiling helpers
 // profiling classes: 2-D FFTs, Chebyshev contractions, Helmholtz projections, the RHS

@@ -45,6 +45,11 @@ struct DynArgs {
 };
 
 int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s);
+// qb = (first ? 0 : qb) + coef*q on the physical points of nf fields
+int ph_axpy(lsb_ctx* c, const double* const* q, double* const* qb, int nf, double coef, bool first, hipStream_t s);
+// e_host[k] = sum over the nf fields and the physical points of plane k of (q - f)^2 (synchronises)
+int ph_plane_energy(lsb_ctx* c, const double* const* q, const double* const* f, int nf, double* e_host,
+                    hipStream_t s);
 int ph_nonlinear(lsb_ctx* c, int mode, const NlArgs& a, hipStream_t s);
 int ph_nonlinear_f32(lsb_ctx* c, int mode, const NlArgsT<float>& a, hipStream_t s);
 int ph_hcombine(lsb_ctx* c, const double* const* uu, const double* const* tau, double* const* h, int n,

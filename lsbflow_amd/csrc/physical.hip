@@ -420,7 +420,88 @@ __global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, 
     }
 }
 
+// ------------------------------------------------------ mode-cut accumulation
+// filters.f90:640-646: qb = qb + coef*q_filt over the physical points (first pass: coef*q).
+__global__ void k_axpy_fields(PhDims d, FieldList<const double> q, FieldList<double> qb, int nf, double coef,
+                              int first) {
+    const long n = d.npts * nf;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int f = (int)(t / d.npts);
+        const long e = t - (long)f * d.npts;
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        const double v = coef * q.p[f][off];
+        qb.p[f][off] = first ? v : qb.p[f][off] + v;
+    }
+}
+
+// filters.f90:103-108: per plane k, sum over the nf fields and the nx*ny physical points of
+// (q - f)^2.  Block (k, part): part of the rows of plane k; one partial per block,
+// part[k*kEnergyParts + p], added on the host in a fixed order.
+constexpr int kEnergyParts = 8;
+__global__ void __launch_bounds__(256) k_plane_energy(PhDims d, FieldList<const double> q, FieldList<const double> f,
+                                                      int nf, double* __restrict__ part) {
+    const int k = blockIdx.x, p = blockIdx.y;
+    const long per_plane = (long)d.ny * d.nx;
+    double s = 0.0;
+    for (int l = 0; l < nf; ++l)
+        for (long e = (long)p * blockDim.x + threadIdx.x; e < per_plane; e += (long)kEnergyParts * blockDim.x) {
+            const int j = (int)(e / d.nx), i = (int)(e - (long)j * d.nx);
+            const long off = ((long)k * d.ny + j) * d.nxpp + i;
+            const double r = q.p[l][off] - f.p[l][off];
+            s += r * r;
+        }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(long)k * kEnergyParts + p] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 // ------------------------------------------------------------- launchers
+int ph_axpy(lsb_ctx* c, const double* const* q, double* const* qb, int nf, double coef, bool first, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+        const int nb = (nf - f0 < kMaxFields) ? nf - f0 : kMaxFields;
+        FieldList<const double> fi{};
+        FieldList<double> fo{};
+        for (int l = 0; l < nb; ++l) {
+            fi.p[l] = q[f0 + l];
+            fo.p[l] = qb[f0 + l];
+        }
+        hipLaunchKernelGGL(k_axpy_fields, ph_grid(d.npts * nb), dim3(256), 0, s, d, fi, fo, nb, coef, first ? 1 : 0);
+        LSB_HIP(hipGetLastError());
+    }
+    return LSB_OK;
+}
+
+int ph_plane_energy(lsb_ctx* c, const double* const* q, const double* const* f, int nf, double* e_host,
+                    hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields || (long)c->nzp * kEnergyParts > kRedDoubles) {
+        set_error("plane_energy: bad field count or nzp too large");
+        return LSB_E_ARG;
+    }
+    PhDims d = ph_dims(c);
+    FieldList<const double> fq{}, ff{};
+    for (int l = 0; l < nf; ++l) {
+        fq.p[l] = q[l];
+        ff.p[l] = f[l];
+    }
+    hipLaunchKernelGGL(k_plane_energy, dim3((unsigned)c->nzp, (unsigned)kEnergyParts), dim3(256), 0, s, d, fq, ff, nf,
+                       c->d_red);
+    LSB_HIP(hipGetLastError());
+    std::vector<double> part((size_t)c->nzp * kEnergyParts);
+    LSB_HIP(hipMemcpyAsync(part.data(), c->d_red, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    LSB_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < c->nzp; ++k) {
+        double v = 0.0;
+        for (int p = 0; p < kEnergyParts; ++p) v += part[(size_t)k * kEnergyParts + p];
+        e_host[k] = v;
+    }
+    return LSB_OK;
+}
+
 int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s) {
     FieldList<double> fl{};
     for (int i = 0; i < nf; ++i) fl.p[i] = f[i];
@@ -503,6 +584,10 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
         w = c->d_fz;
     else if (a == 1.0 && b == 1.0)
         w = c->d_fz + 3 * c->nzp;
+    else if (a == 1.0 && b == 6.0)
+        w = c->d_fz + 6 * c->nzp;
+    else if (a == 1.0 && b == 2.0)
+        w = c->d_fz + 9 * c->nzp;
     else
         w = nullptr;
     std::vector<double> hw;
@@ -517,7 +602,7 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
                 hw[2 * c->nzp + k] = a;
             }
         } else {
-            set_error("filter_q_3pt: (a, b) must be the configured pair or (1, 1)");
+            set_error("filter_q_3pt: (a, b) must be the configured pair, (1, 1), (1, 6) or (1, 2)");
             return LSB_E_ARG;
         }
         wtmp = c->d_red + 64;

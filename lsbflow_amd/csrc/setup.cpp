@@ -636,8 +636,13 @@ int build_derived(lsb_ctx* c) {
     lifting_functions(c);
     c->fz_ab.assign(3 * nzp, 0.0);
     c->fz_11.assign(3 * nzp, 0.0);
+    c->fz_16.assign(3 * nzp, 0.0);
+    c->fz_12.assign(3 * nzp, 0.0);
     filter_z_weights(c, c->cfg.a, c->cfg.b, c->fz_ab.data());
     filter_z_weights(c, 1.0, 1.0, c->fz_11.data());
+    // the two filters of the automatic TNS filtering (filters.f90:98-100)
+    filter_z_weights(c, 1.0, 6.0, c->fz_16.data());
+    filter_z_weights(c, 1.0, 2.0, c->fz_12.data());
     // SGS filter widths
     const std::vector<double>& z = c->zpts;
     const double dx = c->dx, dy = c->dy;

@@ -480,7 +480,8 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
             }
         LSB_TRY(horfft_inv(c, c->hp, in, out, 12, c->nzp, s));
     }
-    const bool sgs = c->cfg.sgsflag == 1;
+    // sgsmodel 4 (TNS) has no SGS stress: tauij stays 0 (navierstokes.f90:696, sgsterm.f90:21-35)
+    const bool sgs = c->cfg.sgsflag == 1 && c->cfg.sgsmodel != 4;
     const int model = c->cfg.sgsmodel;
     const bool sym = !(sgs && model == 3);
     NlArgs a{};
@@ -594,7 +595,8 @@ static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED;
 static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif, double* nltf_out[3],
                              double* hf[9], const double* ftab, bool xonly, hipStream_t s) {
     StepLayout& L = S->L;
-    const bool sgs = c->cfg.sgsflag == 1;
+    // sgsmodel 4 (TNS) has no SGS stress: tauij stays 0 (navierstokes.f90:696, sgsterm.f90:21-35)
+    const bool sgs = c->cfg.sgsflag == 1 && c->cfg.sgsmodel != 4;
     const int model = c->cfg.sgsmodel;
     if (sgs && model != 0 && model != 2) {
         set_error("mixed precision: the dynamic and interscale models run in the FP64 form only");
@@ -1044,6 +1046,7 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
     }
     for (int n = 0; n < nsteps; ++n) {
         c->istep += 1;
+        if (with_norm) LSB_TRY(state_filter_hook(c));
         LSB_TRY(update_once(c));
         if (with_norm) {
             double* f[3] = {S->L.spec[S->u[0]], S->L.spec[S->u[1]], S->L.spec[S->u[2]]};

@@ -440,6 +440,34 @@ module lsb_capi
       real(c_double), value :: xlen, u0, fsize, lfsize, sfsize
       real(c_double), intent(out) :: lam(*)
     end function
+    integer(c_int) function lsb_filterall3pt(ctx, uf, vf, wf, tempf, pf) bind(C, name="lsb_filterall3pt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: uf(*), vf(*), wf(*), tempf(*), pf(*)
+    end function
+    integer(c_int) function lsb_tns_automatic_filtering(ctx, uf, vf, wf, tempf, pf, ih, idelta, applied) &
+        bind(C, name="lsb_tns_automatic_filtering")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: uf(*), vf(*), wf(*), tempf(*), pf(*)
+      real(c_double), intent(out) :: ih, idelta
+      integer(c_int), intent(out) :: applied
+    end function
+    integer(c_int) function lsb_state_filterall3pt(ctx) bind(C, name="lsb_state_filterall3pt")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lsb_state_tns_filter(ctx, ih, idelta, applied) bind(C, name="lsb_state_tns_filter")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ih, idelta
+      integer(c_int), intent(out) :: applied
+    end function
+    integer(c_int) function lsb_set_filtering(ctx, ifilt, modfilt) bind(C, name="lsb_set_filtering")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ifilt, modfilt
+    end function
     integer(c_int) function lsb_set_sponge(ctx, fsize, lfsize, sfsize) bind(C, name="lsb_set_sponge")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

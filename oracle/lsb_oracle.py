@@ -85,6 +85,8 @@ class OracleConfig:
     nstart: int = 0
     dt: float = 1.0e-3
     fsize: float = 0.15
+    ifilt: int = 0
+    modfilt: int = 0
     lfsize: float = 0.25
     sfsize: float = 0.70
 
@@ -787,6 +789,8 @@ class Oracle:
             return self.sigma_model(duidxj)
         if c.sgsmodel == 3:
             return self.interscale_model(ui)
+        if c.sgsmodel == 4:   # TNS: no case; tauij stays 0 (navierstokes.f90:696)
+            return np.zeros_like(uiuj)
         raise ValueError("sgsmodel")
 
     # --------------------------------------------------------------- dealiasing
@@ -1120,9 +1124,16 @@ class Oracle:
         st["uf"], st["vf"], st["wf"], st["pf"] = uf, vf, wf, pf
 
     def timestep(self, st):
-        """navierstokes.f90:10-90 with ifilt = 0, dtadapt = 0 (the per-step part)."""
+        """navierstokes.f90:10-90 with dtadapt = 0 (the per-step part; the filtering of
+        ifilt >= 1 every modfilt steps before the update)."""
         c = self.cfg
         st["istep"] += 1
+        if c.ifilt >= 1 and st["istep"] % c.modfilt == 0:   # navierstokes.f90:31-39
+            if c.sgsflag == 1 and c.sgsmodel == 4:
+                r = self.tns_automatic_filtering(st["uf"], st["vf"], st["wf"], st["tempf"], st["pf"])
+            else:
+                r = self.filterall3pt(st["uf"], st["vf"], st["wf"], st["tempf"], st["pf"])
+            st["uf"], st["vf"], st["wf"], st["tempf"], st["pf"] = r[:5]
         self.navierstokes_update(st)
         self.norm(st["uf"], st["vf"], st["wf"], st["tempf"])
         if c.ifrz == 1 and self.wavx[0] == 0:
@@ -1138,6 +1149,58 @@ class Oracle:
         divf = self.dfdx(uf) + self.dfdy(vf) + self.dfdz(wf)
         divp = self.horfft_inv(divf)
         return float(np.sum(np.abs(divp)) / (self.nx * self.ny * self.nzp))
+
+    # ----------------------------------------------------- TNS / mode-cut filters
+    MODECUT_COEF = (6.0, -15.0, 20.0, -15.0, 6.0, -1.0)
+
+    def modecut_fc(self, q, a, b):
+        """filters.f90:617-654 modecutFC: qb = sum_l coef_l F^l q with F the 3-pt filter
+        (filter_q_3pt with interpflag 1) applied l times, coef = (6, -15, 20, -15, 6, -1)."""
+        qb = np.zeros(q.shape)
+        f = q
+        for cl in self.MODECUT_COEF:
+            f = self.filter_q_3pt(f, a, b, 1)
+            qb = qb + cl * f
+        return qb
+
+    def filterall3pt(self, uf, vf, wf, tempf, pf):
+        """filters.f90:9-49: physical u, v, w, p (temp when any |T| > 1e-10), modecutFC with the
+        SGS test-filter pair (a, b), back to spectral space.  Returns (uf, vf, wf, tempf, pf)."""
+        c = self.cfg
+        hot = tempf is not None and bool(np.any(np.abs(tempf) > 1.0e-10))
+        fields = [uf, vf, wf, pf] + ([tempf] if hot else [])
+        q = np.stack([self.horfft_inv(f) for f in fields])
+        qb = self.modecut_fc(q, c.a, c.b)
+        out = [self.horfft_fwd(qb[l]) for l in range(len(fields))]
+        return out[0], out[1], out[2], (out[4] if hot else tempf), out[3]
+
+    def tns_automatic_filtering(self, uf, vf, wf, tempf, pf):
+        """filters.f90:52-161: modecutFC of physical u, v, w, p with (1, 6) and with (1, 2);
+        per plane the kinetic energy removed by each, summed over x and y over nx*ny (the
+        nx physical points of each row; the reference's sum also covers the two pad reals,
+        quirk Q10); trapezoidal z integrals I(h) (the (1, 6) filter) and I(delta) (the
+        (1, 2) filter); when I(h)/I(delta) > 0.007 the fields take the (1, 6) result.
+        Returns (uf, vf, wf, tempf, pf, Ih, Idelta, applied)."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        q = np.stack([self.horfft_inv(f) for f in (uf, vf, wf, pf)])
+        q2kc = self.modecut_fc(q, 1.0, 6.0)
+        qf = self.modecut_fc(q, 1.0, 2.0)
+        e_delta = np.zeros(self.nzp)
+        e_h = np.zeros(self.nzp)
+        for k in range(self.nzp):
+            for l in range(3):
+                e_delta[k] += np.sum((q[l, k, :, :nx] - qf[l, k, :, :nx]) ** 2) / (ny * nx)
+                e_h[k] += np.sum((q[l, k, :, :nx] - q2kc[l, k, :, :nx]) ** 2) / (ny * nx)
+        Ih = 0.0
+        Idelta = 0.0
+        for k in range(nz - 1, -1, -1):                 # Fortran k = nz .. 1
+            dz = self.zpts[k] - self.zpts[k + 1]
+            Ih = Ih + 0.5 * (e_h[k] + e_h[k + 1]) * dz
+            Idelta = Idelta + 0.5 * (e_delta[k] + e_delta[k + 1]) * dz
+        applied = Ih / Idelta > 0.007
+        if applied:
+            uf, vf, wf, pf = (self.horfft_fwd(q2kc[l]) for l in range(4))
+        return uf, vf, wf, tempf, pf, Ih, Idelta, applied
 
     # ------------------------------------------------------------ sponge layer
     def sponge_lambda(self):

@@ -282,3 +282,66 @@ def test_sponge_lambda(nx, fsize):
             assert r[i] == kappa
         else:                                        # the decay over the left part
             assert abs(r[i] - kappa * math.exp(-((x[i] - left) / (dxl / 2.5)) ** 2)) < 1e-12 * kappa
+
+
+# --------------------------------------------------------- TNS / mode-cut filters
+def _mode_field(o, mx, my, alpha=0.3, beta=0.7, amp=1.0):
+    """amp cos(2 pi mx i/nx) cos(2 pi my j/ny) (alpha + beta z) on the physical grid (pads 0)."""
+    x = 2 * np.pi * np.arange(o.nx) / o.nx
+    y = 2 * np.pi * np.arange(o.ny) / o.ny
+    f = np.zeros((o.nzp, o.ny, o.nx + 2))
+    f[:, :, : o.nx] = amp * ((alpha + beta * o.zpts)[:, None, None] * np.cos(my * y)[None, :, None]
+                             * np.cos(mx * x)[None, None, :])
+    return f
+
+
+def _transfer(o, a, b, mx, my):
+    tx = (b + 2 * a * math.cos(2 * math.pi * mx / o.nx)) / (2 * a + b)
+    ty = (b + 2 * a * math.cos(2 * math.pi * my / o.ny)) / (2 * a + b)
+    return tx * ty
+
+
+@pytest.mark.parametrize("a,b", [(1.0, 2.0), (1.0, 6.0), (1.0, 1.0)])
+def test_modecut_closed_form(a, b):
+    """filters.f90:617-654: on one Fourier mode times a profile linear in z (which the
+    interpolating z weights reproduce) each 3-pt pass multiplies by T = Tx Ty, and
+    sum_l coef_l T^l with coef = (6, -15, 20, -15, 6, -1) is 1 - (1 - T)^6."""
+    o = Oracle(OracleConfig(nx=32, ny=16, nz=16, mapping=3, a=1.0, b=2.0), build_mats=False)
+    f = _mode_field(o, 3, 2)
+    qb = o.modecut_fc(f, a, b)
+    T = _transfer(o, a, b, 3, 2)
+    assert np.max(np.abs(qb[..., : o.nx] - (1 - (1 - T) ** 6) * f[..., : o.nx])) < 1e-13
+    assert not np.any(qb[..., o.nx:])
+
+
+@pytest.mark.parametrize("mx,my", [(3, 2), (12, 6)])
+def test_tns_ratio_closed_form(mx, my):
+    """filters.f90:98-126: with u one mode and v = w = p = 0, q - modecut(q) = (1 - T)^6 q, so
+    I(h)/I(delta) = ((1 - T16)/(1 - T12))^12 whatever the z profile; low modes stay below the
+    0.007 limit (fields untouched), high modes exceed it (the (1, 6) result is applied)."""
+    o = Oracle(OracleConfig(nx=32, ny=16, nz=16, mapping=3), build_mats=False)
+    uf = o.horfft_fwd(_mode_field(o, mx, my))
+    z = np.zeros_like(uf)
+    r = o.tns_automatic_filtering(uf, z, z, None, z)
+    ih, idl, applied = r[5], r[6], r[7]
+    want = ((1 - _transfer(o, 1, 6, mx, my)) / (1 - _transfer(o, 1, 2, mx, my))) ** 12
+    assert abs(ih / idl - want) < 1e-10 * want
+    assert applied == (want > 0.007)
+    if applied:
+        T16 = _transfer(o, 1, 6, mx, my)
+        assert np.max(np.abs(r[0] - (1 - (1 - T16) ** 6) * uf)) < 1e-13 * np.max(np.abs(uf))
+    else:
+        assert r[0] is uf
+
+
+def test_filterall3pt_temperature_threshold():
+    """filters.f90:34,43: temp is filtered only when some |T| > 1e-10."""
+    o = Oracle(OracleConfig(nx=16, ny=8, nz=8, mapping=3), build_mats=False)
+    uf = o.horfft_fwd(_mode_field(o, 2, 1))
+    tiny = np.full_like(uf, 5e-11)
+    r = o.filterall3pt(uf, uf, uf, tiny, uf)
+    assert r[3] is tiny
+    big = o.horfft_fwd(_mode_field(o, 2, 1, amp=1e-3))
+    r = o.filterall3pt(uf, uf, uf, big, uf)
+    T = _transfer(o, 1.0, 2.0, 2, 1)
+    assert np.max(np.abs(r[3] - (1 - (1 - T) ** 6) * big)) < 1e-16
