@@ -171,8 +171,8 @@ def cpu_baseline(case, sample_ny, sample_steps):
 
 def diagnostics(case, reps):
     """Wall time per call (host-synchronised; the records include their PCIe copy to
-    host memory) of the per-step diagnostics and the I/O records on the FP64 resident
-    state of `case`, after two steps.  The kernels' own device times come from a
+    host memory) of the per-step diagnostics, the I/O records and the two filters (mode-cut
+    and TNS) on the FP64 resident state of `case`, after two steps.  The kernels' own device times come from a
     rocprofv3 --kernel-trace run of this mode."""
     import numpy as np
     import lsbflow_amd as L
@@ -190,6 +190,8 @@ def diagnostics(case, reps):
         "state_snapshot_record": lambda: ctx._L.lsb_state_snapshot(ctx.h, 0, None, ctx._fp(snap)),
         "write_restart_record": lambda: ctx._L.lsb_write_restart(ctx.h, None, ctx._fp(rst)),
         "read_restart_record": lambda: ctx._L.lsb_read_restart(ctx.h, None, ctx._fp(rst)),
+        "state_filterall3pt": ctx.state_filterall3pt,
+        "state_tns_filter": ctx.state_tns_filter,
     }
     out = {}
     for name, fn in calls.items():

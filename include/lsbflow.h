@@ -290,6 +290,18 @@ int lsb_state_tns_filter(lsb_ctx* ctx, double* ih, double* idelta, int* applied)
  * every step whose run counter is a multiple of modfilt, tns_automatic_filtering when
  * sgsflag = 1 and sgsmodel = 4, else filterall3pt. */
 int lsb_set_filtering(lsb_ctx* ctx, int ifilt, int modfilt);
+/* orrsomm.f90:625-666 ener(uf, vf, wf, itot, en): the kinetic energy 0.5 (u^2 + v^2 + w^2)
+ * summed over each plane's physical points (u + ubar when itot = 1) and integrated in z by
+ * the trapezoidal rule from row nz+1 up to row 2 (the top interval left out, k0 = 2). */
+int lsb_ener(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf, int itot, double* en);
+/* orrsomm.f90:566-622 lintest(uf, vf, wf, enr1, enr2, grow1, grow2) for step istep (1-based,
+ * the main loop's counter) with the time module's dt: enr1 = ener(total), enr2 = ener with
+ * the (0, 0) mode removed; grow = (E(istep) - E(istep-1))/(2 E(istep) dt), 0 at istep 1.
+ * The arrays hold at least istep values; entry istep-1 is written. */
+int lsb_lintest(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf, int istep, double dt,
+                double* enr1, double* enr2, double* grow1, double* grow2);
+/* lintest on the resident state after lsb_timestep (the run's step counter, the context's dt). */
+int lsb_state_lintest(lsb_ctx* ctx, double* enr1, double* enr2, double* grow1, double* grow2);
 /* runparam dtadapt, iomod, fsize for lsb_timestep: with dtadapt = 1, after every step
  * whose count is a multiple of 5 or of iomod, a NaN check (LSB_E_NUMERIC) and
  * maxcalc(dtmax = 1.0) on the state (navierstokes.f90:74-87). */

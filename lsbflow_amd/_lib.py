@@ -108,6 +108,9 @@ _SIGS = {
     "lsb_state_filterall3pt": (C.c_int, [_vp]),
     "lsb_state_tns_filter": (C.c_int, [_vp, _dp, _dp, _ip]),
     "lsb_set_filtering": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "lsb_ener": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, _dp]),
+    "lsb_lintest": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
+    "lsb_state_lintest": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
 }
 
 

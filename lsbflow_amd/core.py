@@ -528,6 +528,25 @@ class Context:
         check(self._L.lsb_state_tns_filter(self.h, C.byref(ih), C.byref(idl), C.byref(ap)), "state_tns_filter")
         return ih.value, idl.value, bool(ap.value)
 
+    def ener(self, uf, vf, wf, itot=1):
+        a = [_f64(f) for f in (uf, vf, wf)]
+        en = C.c_double()
+        check(self._L.lsb_ener(self.h, *[_p(x) for x in a], itot, C.byref(en)), "ener")
+        return en.value
+
+    def lintest(self, uf, vf, wf, istep, dt, enr1, enr2, grow1, grow2):
+        """orrsomm.f90 lintest: fills entry istep-1 of the four float64 arrays in place."""
+        a = [_f64(f) for f in (uf, vf, wf)]
+        for arr in (enr1, enr2, grow1, grow2):
+            assert arr.dtype == np.float64 and arr.flags.c_contiguous and arr.size >= istep
+        check(self._L.lsb_lintest(self.h, *[_p(x) for x in a], istep, dt, _p(enr1), _p(enr2), _p(grow1),
+                                  _p(grow2)), "lintest")
+
+    def state_lintest(self, enr1, enr2, grow1, grow2):
+        for arr in (enr1, enr2, grow1, grow2):
+            assert arr.dtype == np.float64 and arr.flags.c_contiguous
+        check(self._L.lsb_state_lintest(self.h, _p(enr1), _p(enr2), _p(grow1), _p(grow2)), "state_lintest")
+
     def set_filtering(self, ifilt=1, modfilt=1):
         check(self._L.lsb_set_filtering(self.h, ifilt, modfilt), "set_filtering")
 

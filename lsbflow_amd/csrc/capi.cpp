@@ -1812,3 +1812,85 @@ int state_filter_hook(lsb_ctx* c) {
     return lsb_state_filterall3pt(c);
 }
 }  // namespace lsb
+
+// =========================================== linear-stability energy (orrsomm.f90)
+namespace {
+
+// orrsomm.f90:625-666 ener on three spectral device fields (physical in stage slot 16)
+int ener_dev(lsb_ctx* c, Stage& st, const double* uf, const double* vf, const double* wf, int itot, double* en) {
+    double* ph;
+    LSB_TRY(st.get(16, c->nphys * 3, &ph));
+    const double* si[3] = {uf, vf, wf};
+    double* so[3] = {ph, ph + c->nphys, ph + 2 * c->nphys};
+    LSB_TRY(horfft_inv(c, c->hp, si, so, 3, c->nzp, c->stream));
+    std::vector<double> ee(c->nzp);
+    LSB_TRY(plane_ke(c, so[0], so[1], so[2], itot == 1, ee.data(), c->stream));
+    double e = 0.0;
+    for (int k = c->nz - 1; k >= 1; --k)  // Fortran k = nz .. k0 = 2
+        e = e + 0.5 * (ee[k + 1] + ee[k]) * (c->zpts[k] - c->zpts[k + 1]);
+    *en = e;
+    return LSB_OK;
+}
+
+// orrsomm.f90:566-622 on device fields; istep is 1-based, the arrays hold at least istep values
+int lintest_dev(lsb_ctx* c, Stage& st, const double* uf, const double* vf, const double* wf, int istep, double dt,
+                double* enr1, double* enr2, double* grow1, double* grow2) {
+    const int i = istep - 1;
+    LSB_TRY(ener_dev(c, st, uf, vf, wf, 1, &enr1[i]));
+    double* cp;
+    LSB_TRY(st.get(17, c->nspec * 3, &cp));
+    const double* src[3] = {uf, vf, wf};
+    double* dst[3] = {cp, cp + c->nspec, cp + 2 * c->nspec};
+    for (int l = 0; l < 3; ++l)
+        LSB_HIP(hipMemcpyAsync(dst[l], src[l], c->nspec * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    LSB_TRY(zero_mean_modes(c, dst, 3, c->stream));
+    LSB_TRY(ener_dev(c, st, dst[0], dst[1], dst[2], 0, &enr2[i]));
+    if (istep > 1) {
+        grow1[i] = 1. / (2.0 * enr1[i]) * (enr1[i] - enr1[i - 1]) / dt;
+        grow2[i] = 1. / (2.0 * enr2[i]) * (enr2[i] - enr2[i - 1]) / dt;
+    } else {
+        grow1[i] = 0.0;
+        grow2[i] = 0.0;
+    }
+    return LSB_OK;
+}
+
+}  // namespace
+
+extern "C" int lsb_ener(lsb_ctx* c, const double* uf, const double* vf, const double* wf, int itot, double* en) {
+    LSB_ENTER(c);
+    if (!uf || !vf || !wf || !en) return LSB_E_ARG;
+    double* d;
+    LSB_TRY(st.get(0, c->nspec * 3, &d));
+    LSB_TRY(st.in(d, uf, c->nspec));
+    LSB_TRY(st.in(d + c->nspec, vf, c->nspec));
+    LSB_TRY(st.in(d + 2 * c->nspec, wf, c->nspec));
+    return ener_dev(c, st, d, d + c->nspec, d + 2 * c->nspec, itot, en);
+}
+
+extern "C" int lsb_lintest(lsb_ctx* c, const double* uf, const double* vf, const double* wf, int istep, double dt,
+                           double* enr1, double* enr2, double* grow1, double* grow2) {
+    LSB_ENTER(c);
+    if (!uf || !vf || !wf || !enr1 || !enr2 || !grow1 || !grow2 || istep < 1) {
+        set_error("lintest: null array or istep < 1");
+        return LSB_E_ARG;
+    }
+    double* d;
+    LSB_TRY(st.get(0, c->nspec * 3, &d));
+    LSB_TRY(st.in(d, uf, c->nspec));
+    LSB_TRY(st.in(d + c->nspec, vf, c->nspec));
+    LSB_TRY(st.in(d + 2 * c->nspec, wf, c->nspec));
+    return lintest_dev(c, st, d, d + c->nspec, d + 2 * c->nspec, istep, dt, enr1, enr2, grow1, grow2);
+}
+
+extern "C" int lsb_state_lintest(lsb_ctx* c, double* enr1, double* enr2, double* grow1, double* grow2) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_lintest"));
+    if (!enr1 || !enr2 || !grow1 || !grow2 || c->istep < 1) {
+        set_error("state_lintest: null array or no step taken in this run");
+        return LSB_E_ARG;
+    }
+    // main.f90:65 calls lintest after timestep(): the time module's dt at that point
+    return lintest_dev(c, st, step_field(c, 0), step_field(c, 1), step_field(c, 2), c->istep, c->dt, enr1, enr2,
+                       grow1, grow2);
+}

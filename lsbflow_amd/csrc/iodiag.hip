@@ -209,6 +209,37 @@ __global__ void __launch_bounds__(256) k_wall_mean(const double* __restrict__ ph
     if (g == 0 && i < nx) out[p * nx + i] = (((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane]) / (double)ny;
 }
 
+// orrsomm.f90:650-656: per plane k, sum over the nx*ny physical points of
+// 0.5 (u^2 + v^2 + w^2), u + ubar(k) when ubar is given.  Block (k, part).
+constexpr int kKeParts = 8;
+__global__ void __launch_bounds__(256) k_plane_ke(const double* __restrict__ u, const double* __restrict__ v,
+                                                  const double* __restrict__ w, const double* __restrict__ ubar,
+                                                  int nx, int nxpp, int ny, double* __restrict__ part) {
+    const int k = blockIdx.x, p = blockIdx.y;
+    const long per_plane = (long)ny * nx;
+    const double ub = ubar ? ubar[k] : 0.0;
+    double s = 0.0;
+    for (long e = (long)p * blockDim.x + threadIdx.x; e < per_plane; e += (long)kKeParts * blockDim.x) {
+        const int j = (int)(e / nx), i = (int)(e - (long)j * nx);
+        const long off = ((long)k * ny + j) * nxpp + i;
+        const double a = u[off] + ub, b = v[off], cc = w[off];
+        s += 0.5 * (a * a + b * b + cc * cc);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(long)k * kKeParts + p] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// the (kx, ky) = (0, 0) mode (two reals at the start of every plane) set to zero
+__global__ void k_zero_mean(FieldList<double> f, int nf, int nzp, long plane) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf * nzp * 2) return;
+    const int l = t / (nzp * 2), r = t - l * nzp * 2;
+    f.p[l][(long)(r >> 1) * plane + (r & 1)] = 0.0;
+}
+
 int diag_scratch(lsb_ctx* c, double** p) {
     if (!c->d_diag_red) LSB_TRY(dev_alloc((void**)&c->d_diag_red, (size_t)kDiagDoubles * sizeof(double)));
     *p = c->d_diag_red;
@@ -296,6 +327,38 @@ int nan_scan(lsb_ctx* c, const double* const* f, int nf, size_t n, bool* found, 
     LSB_HIP(hipStreamSynchronize(s));
     *found = false;
     for (double v : part) *found = *found || (v != 0.0);
+    return LSB_OK;
+}
+
+int plane_ke(lsb_ctx* c, const double* u, const double* v, const double* w, bool add_ubar, double* ee_host,
+             hipStream_t s) {
+    if ((long)c->nzp * kKeParts > kDiagDoubles) {
+        set_error("ener: nzp too large for the reduction scratch");
+        return LSB_E_ARG;
+    }
+    double* scr;
+    LSB_TRY(diag_scratch(c, &scr));
+    hipLaunchKernelGGL(k_plane_ke, dim3((unsigned)c->nzp, (unsigned)kKeParts), dim3(256), 0, s, u, v, w,
+                       add_ubar ? c->d_paral : nullptr, c->nx, c->nxpp, c->ny, scr);
+    LSB_HIP(hipGetLastError());
+    std::vector<double> part((size_t)c->nzp * kKeParts);
+    LSB_HIP(hipMemcpyAsync(part.data(), scr, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    LSB_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < c->nzp; ++k) {
+        double e = 0.0;
+        for (int p = 0; p < kKeParts; ++p) e += part[(size_t)k * kKeParts + p];
+        ee_host[k] = e;
+    }
+    return LSB_OK;
+}
+
+int zero_mean_modes(lsb_ctx* c, double* const* spec, int nf, hipStream_t s) {
+    FieldList<double> f{};
+    for (int l = 0; l < nf; ++l) f.p[l] = spec[l];
+    const int n = nf * c->nzp * 2;
+    hipLaunchKernelGGL(k_zero_mean, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, nf, c->nzp,
+                       (long)c->plane_s);
+    LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
 

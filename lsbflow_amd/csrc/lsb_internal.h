@@ -243,6 +243,11 @@ int maxcalc_reduce(lsb_ctx* c, const double* u, const double* v, const double* w
 int nan_scan(lsb_ctx* c, const double* const* f, int nf, size_t n, bool* found, hipStream_t s);
 // spanwise means of two physical planes -> host (2*nx doubles).  Synchronises.
 int wall_mean(lsb_ctx* c, const double* phys2, double* mean_host, hipStream_t s);
+// per-plane sums of 0.5 (u^2 + v^2 + w^2) of physical fields (u + ubar when add_ubar) -> host
+int plane_ke(lsb_ctx* c, const double* u, const double* v, const double* w, bool add_ubar, double* ee_host,
+             hipStream_t s);
+// zero the (kx, ky) = (0, 0) mode of nf spectral fields
+int zero_mean_modes(lsb_ctx* c, double* const* spec, int nf, hipStream_t s);
 // NaN check + maxcalc on the resident state after a step when dtadapt is on (capi.cpp)
 int state_adapt_dt(lsb_ctx* c);
 // filterall3pt / TNS filtering of the resident state before a step when ifilt >= 1 (capi.cpp)

@@ -463,6 +463,27 @@ module lsb_capi
       real(c_double), intent(out) :: ih, idelta
       integer(c_int), intent(out) :: applied
     end function
+    integer(c_int) function lsb_ener(ctx, uf, vf, wf, itot, en) bind(C, name="lsb_ener")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*), vf(*), wf(*)
+      integer(c_int), value :: itot
+      real(c_double), intent(out) :: en
+    end function
+    integer(c_int) function lsb_lintest(ctx, uf, vf, wf, istep, dt, enr1, enr2, grow1, grow2) &
+        bind(C, name="lsb_lintest")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uf(*), vf(*), wf(*)
+      integer(c_int), value :: istep
+      real(c_double), value :: dt
+      real(c_double), intent(inout) :: enr1(*), enr2(*), grow1(*), grow2(*)
+    end function
+    integer(c_int) function lsb_state_lintest(ctx, enr1, enr2, grow1, grow2) bind(C, name="lsb_state_lintest")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: enr1(*), enr2(*), grow1(*), grow2(*)
+    end function
     integer(c_int) function lsb_set_filtering(ctx, ifilt, modfilt) bind(C, name="lsb_set_filtering")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -1202,6 +1202,40 @@ class Oracle:
             uf, vf, wf, pf = (self.horfft_fwd(q2kc[l]) for l in range(4))
         return uf, vf, wf, tempf, pf, Ih, Idelta, applied
 
+    # ------------------------------------------------- linear-stability energy
+    def ener(self, uf, vf, wf, itot):
+        """orrsomm.f90:625-666: ee(k) = sum over the nx*ny physical points of
+        0.5 (u^2 + v^2 + w^2) (u + ubar when itot = 1); en = sum_{k=nz..k0} 0.5 (ee(k+1) +
+        ee(k)) (zpts(k) - zpts(k+1)) with k0 = 2, i.e. the top interval is left out."""
+        nx = self.nx
+        u = self.horfft_inv(uf)[:, :, :nx]
+        v = self.horfft_inv(vf)[:, :, :nx]
+        w = self.horfft_inv(wf)[:, :, :nx]
+        if itot == 1:
+            u = u + self.ubar[:, None, None]
+        ee = np.sum(0.5 * (u * u + v * v + w * w), axis=(1, 2))
+        en = 0.0
+        for k in range(self.nz - 1, 0, -1):              # Fortran k = nz .. 2
+            en = en + 0.5 * (ee[k + 1] + ee[k]) * (self.zpts[k] - self.zpts[k + 1])
+        return float(en)
+
+    def lintest(self, uf, vf, wf, istep, dt, enr1, enr2, grow1, grow2):
+        """orrsomm.f90:566-622 for step istep (1-based): enr1 = ener(total), enr2 = ener with
+        the (kx, ky) = (0, 0) mode removed (no ubar); grow = (E(istep) - E(istep-1)) /
+        (2 E(istep) dt), 0 at istep 1.  Fills the arrays in place."""
+        i = istep - 1
+        enr1[i] = self.ener(uf, vf, wf, 1)
+        up, vp, wp = (np.array(f, copy=True) for f in (uf, vf, wf))
+        for f in (up, vp, wp):
+            f[:, 0, 0:2] = 0.0
+        enr2[i] = self.ener(up, vp, wp, 0)
+        if istep > 1:
+            grow1[i] = 1.0 / (2.0 * enr1[i]) * (enr1[i] - enr1[i - 1]) / dt
+            grow2[i] = 1.0 / (2.0 * enr2[i]) * (enr2[i] - enr2[i - 1]) / dt
+        else:
+            grow1[i] = 0.0
+            grow2[i] = 0.0
+
     # ------------------------------------------------------------ sponge layer
     def sponge_lambda(self):
         """navierstokes.f90:119-187 force(): the Rayleigh-damping coefficient lambda(x) of the

@@ -284,3 +284,38 @@ def test_adaptive_dt_in_timestep(case):
         assert abs(g["t"] - st["t"]) < 1e-15
     finally:
         c.close()
+
+
+# ------------------------------------------------------- linear-stability energy
+def test_ener_and_lintest_match_oracle(case, rng):
+    o = oracle_for(case)
+    c = setup_case(case)
+    try:
+        f = [random_spec(o, rng, 0.1) for _ in range(3)]
+        for itot in (0, 1):
+            e = c.ener(*f, itot=itot)
+            eo = o.ener(*f, itot)
+            assert abs(e - eo) < 1e-13 * abs(eo)
+        # three steps of the configuration with lintest after each (main.f90:63-65)
+        u, v, w = L.cases.initial_condition(case)
+        c.state_set(u, v, w)
+        n = 3
+        arrs = [np.zeros(n) for _ in range(4)]
+        tier_a = [np.zeros(n) for _ in range(4)]
+        oa = [np.zeros(n) for _ in range(4)]
+        for istep in range(1, n + 1):
+            c.timestep(1)
+            c.state_lintest(*arrs)
+            g = c.state_get()
+            c.lintest(g["uf"], g["vf"], g["wf"], istep, case.dt, *tier_a)
+            o.lintest(g["uf"], g["vf"], g["wf"], istep, case.dt, *oa)
+        for a, b in zip(arrs, tier_a):
+            assert np.array_equal(a, b)
+        for k in (0, 1):  # energies
+            assert np.max(np.abs(arrs[k] - oa[k]) / np.abs(oa[k])) < 1e-13
+        # growth rates: differences of energies that change by ~1e-6 per step
+        for k in (2, 3):
+            assert arrs[k][0] == 0.0
+            assert np.max(np.abs(arrs[k][1:] - oa[k][1:])) < 1e-6 * np.max(np.abs(oa[k][1:])) + 1e-12
+    finally:
+        c.close()

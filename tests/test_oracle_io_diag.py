@@ -345,3 +345,39 @@ def test_filterall3pt_temperature_threshold():
     r = o.filterall3pt(uf, uf, uf, big, uf)
     T = _transfer(o, 1.0, 2.0, 2, 1)
     assert np.max(np.abs(r[3] - (1 - (1 - T) ** 6) * big)) < 1e-16
+
+
+# ------------------------------------------------------- linear-stability energy
+def test_ener_and_lintest_closed_form():
+    """orrsomm.f90:566-666 on u = ubar(z) + A cos(2 pi m i/nx) g(z) + c(z), v = w = 0: per plane
+    sum over x of (ubar + c + A cos)^2 = nx (ubar + c)^2 + nx A^2 g^2/2 (the cross term
+    vanishes), so ener(itot=1) integrates nx ny/2 ((ubar + c)^2 + A^2 g^2/2), and lintest's
+    mean-removed energy drops c as well: nx ny A^2 g^2/4.  Scaling the field by s between two
+    steps gives grow2 = (s^2 - 1)/(2 s^2 dt)."""
+    o = Oracle(OracleConfig(nx=32, ny=8, nz=16, mapping=3), build_mats=False)
+    z = o.zpts
+    o.ubar = np.tanh(z)
+    g = 0.3 + z ** 2
+    cz = 0.05 * z
+    A, m = 0.2, 3
+    x = np.arange(o.nx)
+    ph = np.zeros((o.nzp, o.ny, o.nx + 2))
+    ph[:, :, : o.nx] = A * g[:, None, None] * np.cos(2 * np.pi * m * x / o.nx)[None, None, :] + cz[:, None, None]
+    uf = o.horfft_fwd(ph)
+    zero = np.zeros_like(uf)
+
+    def trap(e):
+        return sum(0.5 * (e[k + 1] + e[k]) * (z[k] - z[k + 1]) for k in range(o.nz - 1, 0, -1))
+
+    nn = o.nx * o.ny
+    e_tot = trap(nn / 2 * ((o.ubar + cz) ** 2 + A * A * g * g / 2))
+    e_pert = trap(nn * A * A * g * g / 4)
+    assert abs(o.ener(uf, zero, zero, 1) - e_tot) < 1e-12 * e_tot
+    enr1, enr2, gr1, gr2 = (np.zeros(2) for _ in range(4))
+    dt = 1e-3
+    o.lintest(uf, zero, zero, 1, dt, enr1, enr2, gr1, gr2)
+    assert abs(enr1[0] - e_tot) < 1e-12 * e_tot and abs(enr2[0] - e_pert) < 1e-12 * e_pert
+    assert gr1[0] == 0.0 and gr2[0] == 0.0
+    s = 1.01
+    o.lintest(s * uf, zero, zero, 2, dt, enr1, enr2, gr1, gr2)
+    assert abs(gr2[1] - (s * s - 1) / (2 * s * s * dt)) < 1e-9 * abs(gr2[1])
