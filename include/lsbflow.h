@@ -191,8 +191,7 @@ int lsb_nlterm_skewsym(lsb_ctx* ctx, const double* uif, double* nltermif);
  * istep sets both step counters: the run's own (start-up coefficients, adaptive-dt
  * schedule) and the total (isum of snapshot/restart headers); state_get returns the
  * total.  The step coefficients are recomputed on the first step after a set.
- * tempf must be zero everywhere (buoyancy/temperature is not part of this
- * path; LSB_E_UNSUPPORTED otherwise). */
+ * The temperature state is set apart (lsb_state_set_temperature). */
 int lsb_state_set(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
                   const double* pf, const double* pnf, const double* unif, const double* unmif,
                   const double* nlnif, const double* nlnmif, int istep, double t);
@@ -204,6 +203,17 @@ int lsb_state_get(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* pf, 
 int lsb_timestep(lsb_ctx* ctx, int nsteps);
 /* navierstokes.f90:1157-1496 navierstokes_update (one step, no norm). */
 int lsb_navierstokes_update(lsb_ctx* ctx);
+/* Temperature / buoyancy path (FP64 form; navierstokes.f90:800-845, 1251-1303): the paral
+ * profiles tbar, tbarp, tbarpp (nzp each, NULL = 0) and the buoyancy coex, (gx, gy, gz).
+ * Switches the path on. */
+int lsb_set_temperature(lsb_ctx* ctx, const double* tbar, const double* tbarp, const double* tbarpp,
+                        double coex, double gx, double gy, double gz);
+/* core temperature state tempf, tnf, tnmf, nltnf, nltnmf (spectral, NULL = 0); switches the
+ * path on.  The get returns zeros while the path is off. */
+int lsb_state_set_temperature(lsb_ctx* ctx, const double* tempf, const double* tnf, const double* tnmf,
+                              const double* nltnf, const double* nltnmf);
+int lsb_state_get_temperature(lsb_ctx* ctx, double* tempf, double* tnf, double* tnmf, double* nltnf,
+                              double* nltnmf);
 /* Device pointers of the resident state (for in-HBM benchmarks): fields
  * uf, vf, wf, pf (each nx*ny*nzp doubles). */
 int lsb_state_device_ptrs(lsb_ctx* ctx, void** uf, void** vf, void** wf, void** pf);

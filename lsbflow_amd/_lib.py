@@ -111,6 +111,9 @@ _SIGS = {
     "lsb_ener": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, _dp]),
     "lsb_lintest": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
     "lsb_state_lintest": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "lsb_set_temperature": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "lsb_state_set_temperature": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "lsb_state_get_temperature": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
 }
 
 

@@ -547,6 +547,20 @@ class Context:
             assert arr.dtype == np.float64 and arr.flags.c_contiguous
         check(self._L.lsb_state_lintest(self.h, _p(enr1), _p(enr2), _p(grow1), _p(grow2)), "state_lintest")
 
+    def set_temperature(self, tbar=None, tbarp=None, tbarpp=None, coex=0.0, gx=0.0, gy=0.0, gz=0.0):
+        a = [None if x is None else _f64(x) for x in (tbar, tbarp, tbarpp)]
+        check(self._L.lsb_set_temperature(self.h, *[_p(x) for x in a], coex, gx, gy, gz), "set_temperature")
+
+    def state_set_temperature(self, tempf=None, tnf=None, tnmf=None, nltnf=None, nltnmf=None):
+        a = [None if x is None else _f64(x) for x in (tempf, tnf, tnmf, nltnf, nltnmf)]
+        check(self._L.lsb_state_set_temperature(self.h, *[_p(x) for x in a]), "state_set_temperature")
+
+    def state_get_temperature(self):
+        keys = ("tempf", "tnf", "tnmf", "nltnf", "nltnmf")
+        d = {k: np.zeros(self.sshape) for k in keys}
+        check(self._L.lsb_state_get_temperature(self.h, *[_p(d[k]) for k in keys]), "state_get_temperature")
+        return d
+
     def set_filtering(self, ifilt=1, modfilt=1):
         check(self._L.lsb_set_filtering(self.h, ifilt, modfilt), "set_filtering")
 

@@ -266,6 +266,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     step_free(c);
+    temp_free(c);
     double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
                     c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red, c->d_lam};
     for (double* p : ds)
@@ -1356,9 +1357,10 @@ extern "C" int lsb_snapshot_record(lsb_ctx* c, const double* uf, const double* v
 extern "C" int lsb_state_snapshot(lsb_ctx* c, int name, const char* path, float* rec) {
     LSB_ENTER(c);
     LSB_TRY(need_state(c, "state_snapshot"));
-    const double* spec[5] = {step_field(c, 0), step_field(c, 1), step_field(c, 2), nullptr, step_field(c, 15)};
+    const double* spec[5] = {step_field(c, 0), step_field(c, 1), step_field(c, 2),
+                             temp_on(c) ? temp_field(c, 0) : nullptr, step_field(c, 15)};
     float* body;
-    LSB_TRY(snapshot_body(c, st, spec, name, nullptr, &body));
+    LSB_TRY(snapshot_body(c, st, spec, name, temp_tbar_dev(c), &body));
     const size_t nb = c->nspec * 5;
     std::vector<float> own;
     float* hb = rec ? rec + kSnapHead : nullptr;
@@ -1414,11 +1416,13 @@ extern "C" int lsb_write_restart(lsb_ctx* c, const char* path, float* rec) {
     double* dbody;
     LSB_TRY(st.get(0, c->nspec * kRestartFields / 2, &dbody));
     float* body = (float*)dbody;
-    // file slot -> state field (-1: temperature slots, zero here); slots 14-16 hold nlnmif
-    const int which[kRestartFields] = {0, 1, 2, -1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, -1, -1, -1, -1};
+    // file slot -> state field; -2..-6: tempf, tnf, tnmf, nltnf, nltnmf (zero without the
+    // temperature path); slots 14-16 hold nlnmif
+    const int which[kRestartFields] = {0, 1, 2, -2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, -3, -4, -5, -6};
     for (int f = 0; f < kRestartFields; ++f) {
         const double* src[1] = {nullptr};
         if (which[f] >= 0) LSB_TRY(state_field_f64(c, st, which[f], &src[0]));
+        else if (temp_on(c)) src[0] = temp_field(c, -2 - which[f]);
         LSB_TRY(restart_pack(c, src, 1, body + (size_t)f * c->nspec, c->stream));
     }
     const size_t nb = c->nspec * kRestartFields;
@@ -1466,16 +1470,19 @@ extern "C" int lsb_read_restart(lsb_ctx* c, const char* path, const float* rec) 
         LSB_TRY(read_record(path, &own, kRestartHead + nb));
         r = own.data();
     }
-    // the temperature slots must be zero: this path carries no temperature
+    // temperature slots: loaded into the temperature path (FP64 form), which a nonzero
+    // temperature switches on
     const int tslots[5] = {3, 16, 17, 18, 19};
+    bool hot = false;
     for (int f : tslots) {
         const float* b = r + kRestartHead + (size_t)f * c->nspec;
-        for (size_t q = 0; q < c->nspec; ++q)
-            if (b[q] != 0.0f) {
-                set_error("read_restart: temperature fields are not zero (temperature is not part of this path)");
-                return LSB_E_UNSUPPORTED;
-            }
+        for (size_t q = 0; q < c->nspec && !hot; ++q) hot = b[q] != 0.0f;
     }
+    if (hot && c->cfg.precision != LSB_FP64) {
+        set_error("read_restart: nonzero temperature fields need the FP64 form (the temperature path)");
+        return LSB_E_UNSUPPORTED;
+    }
+    if (hot) LSB_TRY(temp_enable(c));
     const double hv[5] = {r[0], r[1], r[2], r[3], r[4]};
     LSB_TRY(step_alloc(c));
     const bool fresh = !c->state_ready;
@@ -1494,6 +1501,11 @@ extern "C" int lsb_read_restart(lsb_ctx* c, const char* path, const float* rec) 
         LSB_TRY(restart_unpack(c, body + (size_t)f * c->nspec, dst, 1, c->stream));
         if (h32) LSB_TRY(step_hist_set(c, w, tmp, c->stream));
     }
+    if (temp_on(c))
+        for (int q = 0; q < 5; ++q) {
+            double* dst[1] = {temp_field(c, q)};
+            LSB_TRY(restart_unpack(c, body + (size_t)tslots[q] * c->nspec, dst, 1, c->stream));
+        }
     if (fresh) {  // pf, pnf are not in the file; the reference keeps them as they were
         LSB_HIP(hipMemsetAsync(step_field(c, 15), 0, c->nspec * sizeof(double), c->stream));
         LSB_HIP(hipMemsetAsync(step_field(c, 16), 0, c->nspec * sizeof(double), c->stream));
@@ -1539,8 +1551,8 @@ extern "C" int lsb_state_maxcalc(lsb_ctx* c, double dtmax, double fsize, double*
     LSB_ENTER(c);
     LSB_TRY(need_state(c, "state_maxcalc"));
     double s[11];
-    LSB_TRY(maxcalc_fields(c, st, step_field(c, 0), step_field(c, 1), step_field(c, 2), nullptr, nullptr, c->dt, c->t,
-                           fsize, s));
+    LSB_TRY(maxcalc_fields(c, st, step_field(c, 0), step_field(c, 1), step_field(c, 2),
+                           temp_on(c) ? temp_field(c, 0) : nullptr, temp_tbar_host(c), c->dt, c->t, fsize, s));
     const bool ch = maxcalc_decide(s, dtmax, &c->dt, &c->dt2);
     if (stats) std::memcpy(stats, s, sizeof(s));
     if (change_dt) *change_dt = ch ? 1 : 0;
@@ -1893,4 +1905,48 @@ extern "C" int lsb_state_lintest(lsb_ctx* c, double* enr1, double* enr2, double*
     // main.f90:65 calls lintest after timestep(): the time module's dt at that point
     return lintest_dev(c, st, step_field(c, 0), step_field(c, 1), step_field(c, 2), c->istep, c->dt, enr1, enr2,
                        grow1, grow2);
+}
+
+// ================================================ temperature state (common.f90:162-175)
+extern "C" int lsb_set_temperature(lsb_ctx* c, const double* tbar, const double* tbarp, const double* tbarpp,
+                                   double coex, double gx, double gy, double gz) {
+    LSB_ENTER(c);
+    if (c->cfg.precision != LSB_FP64) {
+        set_error("set_temperature: the temperature path runs in the FP64 form only");
+        return LSB_E_UNSUPPORTED;
+    }
+    return temp_set_params(c, tbar, tbarp, tbarpp, coex, gx, gy, gz);
+}
+
+extern "C" int lsb_state_set_temperature(lsb_ctx* c, const double* tempf, const double* tnf, const double* tnmf,
+                                         const double* nltnf, const double* nltnmf) {
+    LSB_ENTER(c);
+    if (c->cfg.precision != LSB_FP64) {
+        set_error("state_set_temperature: the temperature path runs in the FP64 form only");
+        return LSB_E_UNSUPPORTED;
+    }
+    LSB_TRY(temp_enable(c));
+    const double* hs[5] = {tempf, tnf, tnmf, nltnf, nltnmf};
+    for (int q = 0; q < 5; ++q) {
+        double* d = temp_field(c, q);
+        if (hs[q])
+            LSB_TRY(st.in(d, hs[q], c->nspec));
+        else
+            LSB_HIP(hipMemsetAsync(d, 0, c->nspec * sizeof(double), c->stream));
+    }
+    return st.sync();
+}
+
+extern "C" int lsb_state_get_temperature(lsb_ctx* c, double* tempf, double* tnf, double* tnmf, double* nltnf,
+                                         double* nltnmf) {
+    LSB_ENTER(c);
+    double* hs[5] = {tempf, tnf, tnmf, nltnf, nltnmf};
+    for (int q = 0; q < 5; ++q) {
+        if (!hs[q]) continue;
+        if (temp_on(c))
+            LSB_TRY(st.out(hs[q], temp_field(c, q), c->nspec));
+        else
+            std::memset(hs[q], 0, c->nspec * sizeof(double));
+    }
+    return st.sync();
 }

@@ -253,6 +253,19 @@ int state_adapt_dt(lsb_ctx* c);
 // filterall3pt / TNS filtering of the resident state before a step when ifilt >= 1 (capi.cpp)
 int state_filter_hook(lsb_ctx* c);
 
+// temperature / buoyancy path of the FP64 step (temperature.hip)
+bool temp_on(const lsb_ctx* c);
+int temp_enable(lsb_ctx* c);
+void temp_free(lsb_ctx* c);
+int temp_set_params(lsb_ctx* c, const double* tbar, const double* tbarp, const double* tbarpp, double coex,
+                    double gx, double gy, double gz);
+const double* temp_tbar_dev(const lsb_ctx* c);   // tbar, tbarp, tbarpp (3 nzp) or null
+const double* temp_tbar_host(const lsb_ctx* c);  // tbar (nzp) or null
+double* temp_field(lsb_ctx* c, int which);       // 0 tempf, 1 tnf, 2 tnmf, 3 nltnf, 4 nltnmf
+int temp_nlterm(lsb_ctx* c, const double* const* uif, bool filter_u, hipStream_t s);
+int temp_update(lsb_ctx* c, hipStream_t s);
+int temp_buoyancy(lsb_ctx* c, double* const* rhs, hipStream_t s);
+
 // profiling helpers
 // profiling classes: 2-D FFTs, Chebyshev contractions, Helmholtz projections, the RHS
 // assembly, the physical-space products/SGS pass, other pointwise kernels

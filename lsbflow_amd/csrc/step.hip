@@ -807,6 +807,10 @@ static int update_once(lsb_ctx* c) {
     const lsb_config& g = c->cfg;
     const int istep = c->istep;
     const double dt = c->dt;
+    if (temp_on(c) && mixed_form(c)) {
+        set_error("timestep: the temperature path runs in the FP64 form only");
+        return LSB_E_UNSUPPORTED;
+    }
     // velocities used for the nonlinear term (navierstokes.f90:1213-1229)
     // (filterqspec, navierstokes.f90:1206-1209): with the register-resident FFT the filter
     // is applied as the fused inverse transform loads u; otherwise as its own pass
@@ -835,6 +839,8 @@ static int update_once(lsb_ctx* c) {
     // fused with the right-hand side (the transformed pieces stay in registers)
     const bool fuse = mixed_form(c) && horfft_has_ops(c->hp) && sym;
     LSB_TRY(nlterm_core(c, S, uif, nltf, hf, ftab, s, fuse));
+    // temperature part of the nonlinear term (navierstokes.f90:800-845)
+    if (temp_on(c)) LSB_TRY(temp_nlterm(c, uif, ftab != nullptr, s));
     // the unfiltered velocity is u^n for dealiasflag 0..2 (navierstokes.f90:1225-1229)
     const double* un_[3];
     for (int l = 0; l < 3; ++l) un_[l] = (g.dealiasflag > 2) ? uif[l] : L.spec[S->u[l]];
@@ -844,6 +850,8 @@ static int update_once(lsb_ctx* c) {
         bdfcoef(g.nstart, istep, c->dt, c->dt2, c->dt3, c->bdf, &c->bdfv);
         c->coef_valid = true;
     }
+    // temperature update before the velocity right-hand side (navierstokes.f90:1251-1275)
+    LSB_TRY(temp_update(c, s));
     const double dtnui = 1. / (g.xnu * dt);
     StepRhsArgs r{};
     for (int l = 0; l < 3; ++l) {
@@ -900,6 +908,8 @@ static int update_once(lsb_ctx* c) {
     } else {
         LSB_TRY(sp_rhs_vel(c, r, s));
     }
+    // buoyancy with the new temperature (navierstokes.f90:1298-1303)
+    LSB_TRY(temp_buoyancy(c, r.rhs, s));
     // boundary data (navierstokes.f90:1321-1373): the imposed suction (bound) when
     // bctype > 0 past istartbound; the velocity solves take the boundary values also
     // with the sponge on (navierstokes.f90:1389)

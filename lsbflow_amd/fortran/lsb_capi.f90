@@ -484,6 +484,25 @@ module lsb_capi
       type(c_ptr), value :: ctx
       real(c_double), intent(inout) :: enr1(*), enr2(*), grow1(*), grow2(*)
     end function
+    integer(c_int) function lsb_set_temperature(ctx, tbar, tbarp, tbarpp, coex, gx, gy, gz) &
+        bind(C, name="lsb_set_temperature")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tbar(*), tbarp(*), tbarpp(*)
+      real(c_double), value :: coex, gx, gy, gz
+    end function
+    integer(c_int) function lsb_state_set_temperature(ctx, tempf, tnf, tnmf, nltnf, nltnmf) &
+        bind(C, name="lsb_state_set_temperature")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tempf(*), tnf(*), tnmf(*), nltnf(*), nltnmf(*)
+    end function
+    integer(c_int) function lsb_state_get_temperature(ctx, tempf, tnf, tnmf, nltnf, nltnmf) &
+        bind(C, name="lsb_state_get_temperature")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tempf(*), tnf(*), tnmf(*), nltnf(*), nltnmf(*)
+    end function
     integer(c_int) function lsb_set_filtering(ctx, ifilt, modfilt) bind(C, name="lsb_set_filtering")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -183,3 +183,41 @@ def test_c1_dns_sponge_forcing(bctype):
     for _ in range(case.nsteps):
         o0.timestep(st0)
     assert rel_l2(st0["uf"], st["uf"]) > 1e-6
+
+
+def test_c1_dns_temperature_buoyancy():
+    """Temperature / buoyancy path (navierstokes.f90:800-845, 1251-1303): base profile tbar =
+    1 + 0.1 tanh(z/0.2) with its derivatives, temperature perturbation 0.5 u' of the initial
+    condition, buoyancy coex (0, 0, -1); 10 steps, the temperature state included."""
+    case = L.cases.CASES["c1_dns"]
+    kw = dict(coex=1.0, gz=-1.0)
+    o = oracle_for(case, **kw)
+    z = o.zpts
+    th = np.tanh(z / 0.2)
+    tb, tbp, tbpp = 1.0 + 0.1 * th, 0.5 * (1 - th ** 2), -2 * 0.5 / 0.2 * th * (1 - th ** 2)
+    o.tbar, o.tbarp, o.tbarpp = tb, tbp, tbpp
+    u, v, w = L.cases.initial_condition(case)
+    T0 = 0.5 * u
+    st = o.new_state(u, v, w, T0)
+    for _ in range(case.nsteps):
+        o.timestep(st)
+    rng = np.random.default_rng(99)
+    pert = [f * (1.0 + 1e-15 * rng.standard_normal(f.shape)) for f in (u, v, w, T0)]
+    st2 = o.new_state(*pert)
+    for _ in range(case.nsteps):
+        o.timestep(st2)
+    c = setup_case(case)
+    try:
+        c.set_temperature(tb, tbp, tbpp, 1.0, 0.0, 0.0, -1.0)
+        c.state_set(u, v, w)
+        c.state_set_temperature(T0)
+        c.timestep(case.nsteps)
+        g = c.state_get()
+        gt = c.state_get_temperature()
+    finally:
+        c.close()
+    g["_sens"] = {k: rel_l2(st2[k], st[k]) for k in FIELDS}
+    check_fields(st, g)
+    for k in ("tempf", "tnf", "nltnf"):
+        bound = max(TOL, 1000.0 * rel_l2(st2[k], st[k]))
+        assert rel_l2(gt[k], st[k]) < bound, (k, rel_l2(gt[k], st[k]), bound)
