@@ -172,7 +172,8 @@ def cpu_baseline(case, sample_ny, sample_steps):
 def diagnostics(case, reps):
     """Wall time per call (host-synchronised; the records include their PCIe copy to
     host memory) of the per-step diagnostics, the I/O records and the two filters (mode-cut
-    and TNS) on the FP64 resident state of `case`, after two steps.  The kernels' own device times come from a
+    and TNS) on the FP64 resident state of `case`, after two steps; then one FP64 step
+    without and with the temperature / buoyancy path.  The kernels' own device times come from a
     rocprofv3 --kernel-trace run of this mode."""
     import numpy as np
     import lsbflow_amd as L
@@ -190,11 +191,29 @@ def diagnostics(case, reps):
         "state_snapshot_record": lambda: ctx._L.lsb_state_snapshot(ctx.h, 0, None, ctx._fp(snap)),
         "write_restart_record": lambda: ctx._L.lsb_write_restart(ctx.h, None, ctx._fp(rst)),
         "read_restart_record": lambda: ctx._L.lsb_read_restart(ctx.h, None, ctx._fp(rst)),
-        "state_filterall3pt": ctx.state_filterall3pt,
+        "state_filterall3pt": lambda: (ctx.state_filterall3pt(), ctx.synchronize()),
         "state_tns_filter": ctx.state_tns_filter,
     }
+
+    def step():
+        ctx.timestep(1)
+        ctx.synchronize()
+
+    def temperature_on():
+        z = ctx.grid()["zpts"]
+        th = np.tanh(z / 0.2)
+        ctx.set_temperature(1.0 + 0.1 * th, 0.5 * (1 - th ** 2), -2 * 0.5 / 0.2 * th * (1 - th ** 2), 1.0, 0.0, 0.0,
+                            -1.0)
+        ctx.state_set_temperature(0.5 * ctx.state_get()["uf"])
+
+    calls["timestep_fp64"] = step
+    calls["temperature_on"] = temperature_on
+    calls["timestep_fp64_with_temperature"] = step
     out = {}
     for name, fn in calls.items():
+        if name == "temperature_on":
+            fn()
+            continue
         fn()
         t0 = time.perf_counter()
         for _ in range(reps):

@@ -7,6 +7,7 @@
 // fields).
 
 #include <cmath>
+#include <cstdlib>
 
 #include "lsb_internal.h"
 #include "physical.h"
@@ -420,6 +421,95 @@ __global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, 
     }
 }
 
+// The same filter in one pass: block (R rows j, KC planes k, field) slides over k with the
+// xy-filtered values of planes k-1, k, k+1 in registers (threads over x, NXT points each), so
+// every input value is read from HBM about once (halo rows and planes aside) and the
+// xy-filtered field never goes through HBM.  Same per-element expressions as the two passes.
+template <int R, int NXT>
+__global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double> in, FieldList<double> out, double a,
+                                                     double b, const double* __restrict__ w, int KC) {
+    const int f = blockIdx.z;
+    const int J0 = blockIdx.x * R;
+    const int k0 = blockIdx.y * KC;
+    const int k1 = min(k0 + KC, d.nzp);
+    const double* q = in.p[f];
+    double* o = out.p[f];
+    const double s = 2. * a + b;
+    const double s2 = 2.00 * a + b;
+    const long plane = (long)d.ny * d.nxpp;
+    int jr[R + 2];
+#pragma unroll
+    for (int rr = 0; rr < R + 2; ++rr) {
+        int j = J0 - 1 + rr;
+        j = (j < 0) ? j + d.ny : j;
+        jr[rr] = j % d.ny;
+    }
+    auto xy = [&](int k, double (&y)[R][NXT]) {
+        double X[R + 2][NXT];
+#pragma unroll
+        for (int rr = 0; rr < R + 2; ++rr) {
+            const double* row = q + (long)k * plane + (long)jr[rr] * d.nxpp;
+#pragma unroll
+            for (int t = 0; t < NXT; ++t) {
+                const int i = threadIdx.x + 256 * t;
+                if (i < d.nx) {
+                    const int im = (i == 0) ? d.nx - 1 : i - 1;
+                    const int ip = (i == d.nx - 1) ? 0 : i + 1;
+                    X[rr][t] = (a * row[im] + b * row[i] + a * row[ip]) / s;
+                } else {
+                    X[rr][t] = 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int t = 0; t < NXT; ++t) y[r][t] = (a * X[r][t] + b * X[r + 1][t] + a * X[r + 2][t]) / s2;
+    };
+    double Yp[R][NXT], Yc[R][NXT], Yn[R][NXT];
+    xy(k0, Yc);
+    if (k0 > 0) xy(k0 - 1, Yp);
+    for (int k = k0; k < k1; ++k) {
+        if (k + 1 < d.nzp) xy(k + 1, Yn);
+        const bool interior = (k > 0 && k < d.nzp - 1);
+        const double w0 = w[k], w1 = w[d.nzp + k], w2 = w[2 * d.nzp + k];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = J0 + r;
+            if (j >= d.ny) continue;
+            double* orow = o + (long)k * plane + (long)j * d.nxpp;
+#pragma unroll
+            for (int t = 0; t < NXT; ++t) {
+                const int i = threadIdx.x + 256 * t;
+                if (i >= d.nx) continue;
+                double v;
+                if (interior) {
+                    v = w0 * Yp[r][t] + w1 * Yc[r][t] + w2 * Yn[r][t];
+                    v = v / s;
+                } else {
+                    v = Yc[r][t];
+                }
+                orow[i] = v;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int t = 0; t < NXT; ++t) {
+                Yp[r][t] = Yc[r][t];
+                Yc[r][t] = Yn[r][t];
+            }
+    }
+}
+
+template <int R, int NXT>
+static void launch_filt3_fused(const PhDims& d, const FieldList<double>& fi, const FieldList<double>& fo, int nb,
+                               double a, double b, const double* w, hipStream_t s) {
+    constexpr int KC = 16;
+    const dim3 grid((unsigned)((d.ny + R - 1) / R), (unsigned)((d.nzp + KC - 1) / KC), (unsigned)nb);
+    hipLaunchKernelGGL((k_filt3_fused<R, NXT>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
+}
+
 // ------------------------------------------------------ mode-cut accumulation
 // filters.f90:640-646: qb = qb + coef*q_filt over the physical points (first pass: coef*q).
 __global__ void k_axpy_fields(PhDims d, FieldList<const double> q, FieldList<double> qb, int nf, double coef,
@@ -573,6 +663,15 @@ int ph_smag_factor_f32(lsb_ctx* c, const float* g02, const double* ubarp, double
     return smag_factor<float>(c, g02, ubarp, cs, s);
 }
 
+// LSB_FILT3_TWO_PASS=1 selects the two-pass kernels (x-y, then z through tmp)
+static bool filt3_two_pass() {
+    static const bool v = [] {
+        const char* e = std::getenv("LSB_FILT3_TWO_PASS");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, double a, double b, int interpflag,
                double* tmp, hipStream_t s) {
     PhDims d = ph_dims(c);
@@ -617,9 +716,17 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
             ft.p[q] = tmp + (size_t)q * c->nphys;
             fo.p[q] = out[f0 + q];
         }
-        hipLaunchKernelGGL(k_filt_xy, ph_grid(d.npts * nb), dim3(256), 0, s, d, fi, ft, nb, a, b);
-        LSB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_filt_z, ph_grid(d.npts * nb), dim3(256), 0, s, d, ft, fo, nb, w, a, b);
+        if (d.nx <= 256 && !filt3_two_pass()) {
+            launch_filt3_fused<8, 1>(d, fi, fo, nb, a, b, w, s);
+        } else if (d.nx <= 512 && !filt3_two_pass()) {
+            launch_filt3_fused<8, 2>(d, fi, fo, nb, a, b, w, s);
+        } else if (d.nx <= 1024 && !filt3_two_pass()) {
+            launch_filt3_fused<4, 4>(d, fi, fo, nb, a, b, w, s);
+        } else {
+            hipLaunchKernelGGL(k_filt_xy, ph_grid(d.npts * nb), dim3(256), 0, s, d, fi, ft, nb, a, b);
+            LSB_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_filt_z, ph_grid(d.npts * nb), dim3(256), 0, s, d, ft, fo, nb, w, a, b);
+        }
         LSB_HIP(hipGetLastError());
     }
     prof_mark_end(c, tok, s);

@@ -43,11 +43,18 @@ dim3 tgrid(long n) {
 
 #define T_LOOP(n) for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (n); e += (long)gridDim.x * blockDim.x)
 
-// flag[0] = 1 when some |f| > thr (flag cleared by the caller's fill kernel)
-__global__ void k_flag_any_gt(const double* __restrict__ f, long n, double thr, int* __restrict__ flag) {
+// flag[0] = 1 when some |f| > thr (flag cleared by the caller's fill kernel).  A fixed grid
+// of kFlagBlocks blocks of 256 threads, one atomic per block that saw a hit.
+constexpr int kFlagBlocks = 1024;
+__global__ void __launch_bounds__(256) k_flag_any_gt(const double* __restrict__ f, long n, double thr,
+                                                     int* __restrict__ flag) {
     int hit = 0;
     T_LOOP(n) hit |= (fabs(f[e]) > thr);
-    if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    __shared__ int sh[4];
+    const int any = __any(hit);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = any;
+    __syncthreads();
+    if (threadIdx.x == 0 && (sh[0] | sh[1] | sh[2] | sh[3])) atomicOr(flag, 1);
 }
 
 __global__ void k_fill_int(int* p, int v) {
@@ -284,7 +291,7 @@ int temp_nlterm(lsb_ctx* c, const double* const* uif, bool filter_u, hipStream_t
     double* Tf = t->spec[TS_TF];
     if (g.dealiasflag >= 2) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, t->flags, 0);
-        hipLaunchKernelGGL(k_flag_any_gt, tgrid(nsp), dim3(256), 0, s, Tsrc, nsp, 1.0e-10, t->flags);
+        hipLaunchKernelGGL(k_flag_any_gt, dim3(kFlagBlocks), dim3(256), 0, s, Tsrc, nsp, 1.0e-10, t->flags);
         const double* in[1] = {Tsrc};
         double* out[1] = {t->spec[TS_TFILT]};
         LSB_TRY(sp_specfilter(c, in, out, 1, g.kc, g.filt_order, g.filt_amp, 1, s));
@@ -372,7 +379,7 @@ int temp_update(lsb_ctx* c, hipStream_t s) {
     t->W = oTnm;
     const long nsp = (long)c->nspec;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, t->flags + 1, 0);
-    hipLaunchKernelGGL(k_flag_any_gt, tgrid(nsp), dim3(256), 0, s, t->spec[t->T], nsp, 1.0e-10, t->flags + 1);
+    hipLaunchKernelGGL(k_flag_any_gt, dim3(kFlagBlocks), dim3(256), 0, s, t->spec[t->T], nsp, 1.0e-10, t->flags + 1);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
