@@ -312,6 +312,16 @@ int lsb_lintest(lsb_ctx* ctx, const double* uf, const double* vf, const double* 
                 double* enr1, double* enr2, double* grow1, double* grow2);
 /* lintest on the resident state after lsb_timestep (the run's step counter, the context's dt). */
 int lsb_state_lintest(lsb_ctx* ctx, double* enr1, double* enr2, double* grow1, double* grow2);
+/* orrsomm.f90:110-562 solve_orrsomm(uvec, vvec, wvec, eps, alphax, betax, omega, iglb) from the
+ * module arrays it reads: d, d2 (nzp x nzp column-major, mats), ubar, ubarp, ubarpp (paral),
+ * zpts, xnu.  omega[2] (Re, Im) is the guess in and the eigenvalue out; uvec/vvec/wvec get
+ * nzp complex values each (Re, Im interleaved), the mode scaled to energy eps.  Host only.
+ * iglb = 0 (the iteration from the guess alpha*cguess*u0) only: iglb < 0 (a global zggevx
+ * spectrum first) -> LSB_E_UNSUPPORTED.  A failed iteration, which the reference answers
+ * with a random restart, -> LSB_E_NUMERIC. */
+int lsb_solve_orrsomm(int nzp, const double* d, const double* d2, const double* ubar, const double* ubarp,
+                      const double* ubarpp, const double* zpts, double xnu, double eps, double alphax,
+                      double betax, double* omega, int iglb, double* uvec, double* vvec, double* wvec);
 /* runparam dtadapt, iomod, fsize for lsb_timestep: with dtadapt = 1, after every step
  * whose count is a multiple of 5 or of iomod, a NaN check (LSB_E_NUMERIC) and
  * maxcalc(dtmax = 1.0) on the state (navierstokes.f90:74-87). */

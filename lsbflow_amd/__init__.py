@@ -6,8 +6,8 @@ reference's module interfaces (``core``), the configurations of the
 benchmarks (``cases``) and the build recipe (``build``).
 """
 
-from .core import Config, Context, build_cheb_zder_matrix, dct_diff, diagonalize_matrix, setup_case, sponge_lambda  # noqa: F401,E501
+from .core import Config, Context, build_cheb_zder_matrix, dct_diff, diagonalize_matrix, setup_case, solve_orrsomm, sponge_lambda  # noqa: F401,E501
 from . import cases  # noqa: F401
 
 __all__ = ["Config", "Context", "build_cheb_zder_matrix", "dct_diff", "diagonalize_matrix", "setup_case",
-           "sponge_lambda", "cases"]
+           "solve_orrsomm", "sponge_lambda", "cases"]

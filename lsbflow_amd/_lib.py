@@ -112,6 +112,8 @@ _SIGS = {
     "lsb_lintest": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
     "lsb_state_lintest": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "lsb_set_temperature": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "lsb_solve_orrsomm": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _dp, C.c_int, _dp, _dp, _dp]),
     "lsb_state_set_temperature": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "lsb_state_get_temperature": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
 }

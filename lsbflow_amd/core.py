@@ -118,6 +118,20 @@ def sponge_lambda(xpts, xlen: float, u0: float, fsize: float, lfsize: float, sfs
     return lam
 
 
+def solve_orrsomm(d, d2, ubar, ubarp, ubarpp, zpts, xnu: float, eps: float, alphax: float, betax: float,
+                  omega: complex, iglb: int = 0):
+    """orrsomm.f90:110-562 on the host: (uvec, vvec, wvec, omega); d, d2 as (nzp, nzp) [row, col]."""
+    nzp = len(zpts)
+    a = [np.asfortranarray(d, dtype=np.float64), np.asfortranarray(d2, dtype=np.float64)]
+    v = [_f64(x) for x in (ubar, ubarp, ubarpp, zpts)]
+    om = np.array([complex(omega).real, complex(omega).imag])
+    out = [np.zeros(2 * nzp) for _ in range(3)]
+    check(_lib.lib().lsb_solve_orrsomm(nzp, _pf(a[0]), _pf(a[1]), *[_p(x) for x in v], xnu, eps, alphax, betax,
+                                       _p(om), iglb, *[_p(x) for x in out]), "solve_orrsomm")
+    u, vv, w = (x[0::2] + 1j * x[1::2] for x in out)
+    return u, vv, w, complex(om[0], om[1])
+
+
 def dct_diff(y, xl: float, gf, p: int):
     """chebyshev.f90:204-270 along axis 0 of y (n, ...)."""
     y = _f64(y)

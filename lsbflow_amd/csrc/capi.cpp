@@ -1950,3 +1950,19 @@ extern "C" int lsb_state_get_temperature(lsb_ctx* c, double* tempf, double* tnf,
     }
     return st.sync();
 }
+
+extern "C" int lsb_solve_orrsomm(int nzp, const double* d, const double* d2, const double* ubar, const double* ubarp,
+                                 const double* ubarpp, const double* zpts, double xnu, double eps, double alphax,
+                                 double betax, double* omega, int iglb, double* uvec, double* vvec, double* wvec) {
+    if (nzp < 4 || !d || !d2 || !ubar || !ubarp || !ubarpp || !zpts || !omega || !uvec || !vvec || !wvec ||
+        !(alphax != 0.0) || !(xnu > 0.0)) {
+        set_error("solve_orrsomm: bad arguments");
+        return LSB_E_ARG;
+    }
+    if (iglb < 0) {
+        set_error("solve_orrsomm: iglb < 0 (the global zggevx step) is not restated; pass the omega guess "
+                  "alpha*cguess*u0 with iglb = 0");
+        return LSB_E_UNSUPPORTED;
+    }
+    return orrsomm_solve(nzp, d, d2, ubar, ubarp, ubarpp, zpts, xnu, eps, alphax, betax, omega, uvec, vvec, wvec);
+}

@@ -220,6 +220,9 @@ int eigen_decompose(const double* a, int n, double* lam, double* e, double* ei);
 void lifting_functions(lsb_ctx* c);
 void sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize, double sfsize,
                    double* lam);
+int orrsomm_solve(int nzp, const double* d, const double* d2, const double* ubar, const double* ubarp,
+                  const double* ubarpp, const double* zpts, double xnu, double eps, double alphax, double betax,
+                  double* omega, double* uvec, double* vvec, double* wvec);
 
 // device memory helpers (capi.cpp)
 int dev_alloc(void** p, size_t bytes);

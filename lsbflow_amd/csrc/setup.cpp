@@ -719,3 +719,235 @@ void sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fs
 }
 
 }  // namespace lsb
+
+// ============================================ Orr-Sommerfeld mode (orrsomm.f90:110-562)
+#include <complex>
+
+namespace lsb {
+namespace {
+
+using cplx = std::complex<double>;
+
+// LU with partial pivoting of an n x n row-major complex matrix (in place); false if singular
+bool clu(std::vector<cplx>& a, int n, std::vector<int>& piv) {
+    piv.resize(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::abs(a[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(a[(size_t)i * n + k]) > best) {
+                best = std::abs(a[(size_t)i * n + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (best == 0.0) return false;
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]);
+        const cplx inv = 1.0 / a[(size_t)k * n + k];
+        for (int i = k + 1; i < n; ++i) {
+            const cplx f = a[(size_t)i * n + k] * inv;
+            a[(size_t)i * n + k] = f;
+            for (int j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
+        }
+    }
+    return true;
+}
+
+// solve (L U) x = P b  (trans = false) or (P^T L U)^H x = b  (trans = true), x in b
+void clu_solve(const std::vector<cplx>& lu, int n, const std::vector<int>& piv, std::vector<cplx>& b, bool herm) {
+    if (!herm) {
+        for (int k = 0; k < n; ++k) std::swap(b[k], b[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= lu[(size_t)i * n + j] * b[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) b[i] -= lu[(size_t)i * n + j] * b[j];
+            b[i] /= lu[(size_t)i * n + i];
+        }
+    } else {
+        // A = P^T L U  ->  A^H = U^H L^H P: solve U^H y = b, L^H z = y, x = P^T z
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < i; ++j) b[i] -= std::conj(lu[(size_t)j * n + i]) * b[j];
+            b[i] /= std::conj(lu[(size_t)i * n + i]);
+        }
+        for (int i = n - 1; i >= 0; --i)
+            for (int j = i + 1; j < n; ++j) b[i] -= std::conj(lu[(size_t)j * n + i]) * b[j];
+        for (int k = n - 1; k >= 0; --k) std::swap(b[k], b[piv[k]]);
+    }
+}
+
+std::vector<cplx> cmatvec(const std::vector<cplx>& a, int n, const std::vector<cplx>& x) {
+    std::vector<cplx> y(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        cplx s = 0.0;
+        for (int j = 0; j < n; ++j) s += a[(size_t)i * n + j] * x[j];
+        y[i] = s;
+    }
+    return y;
+}
+
+double l1(const std::vector<cplx>& x) {
+    double s = 0.0;
+    for (const cplx& v : x) s += std::abs(v);
+    return s;
+}
+
+}  // namespace
+
+// Column-major (Fortran) nzp x nzp d, d2; returns LSB_E_NUMERIC when the iteration fails the
+// way the reference answers with a random restart.
+int orrsomm_solve(int nzp, const double* d, const double* d2, const double* ubar, const double* ubarp,
+                  const double* ubarpp, const double* zpts, double xnu, double eps, double alphax, double betax,
+                  double* omega, double* uvec, double* vvec, double* wvec) {
+    const int n = nzp + 2;
+    const double c01 = 1.0e-40, c02 = 1.0e-40, c03 = 1.0e-40, c04 = 1.0e-40;
+    const double rey = 1.0 / xnu;
+    const double wav2 = alphax * alphax + betax * betax;
+    const cplx I(0.0, 1.0);
+    auto D = [&](int i, int j) { return d[(size_t)j * nzp + i]; };
+    auto D2 = [&](int i, int j) { return d2[(size_t)j * nzp + i]; };
+    std::vector<double> d4((size_t)nzp * nzp, 0.0);  // row-major d2 d2
+    for (int i = 0; i < nzp; ++i)
+        for (int j = 0; j < nzp; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < nzp; ++k) s += D2(i, k) * D2(k, j);
+            d4[(size_t)i * nzp + j] = s;
+        }
+    std::vector<cplx> a((size_t)n * n, 0.0), b((size_t)n * n, 0.0);
+    for (int i = 0; i < nzp; ++i)
+        for (int j = 0; j < nzp; ++j) {
+            const double did = (i == j) ? 1.0 : 0.0;
+            a[(size_t)i * n + j] = d4[(size_t)i * nzp + j] - 2.0 * wav2 * D2(i, j) + wav2 * wav2 * did -
+                                   I * alphax * rey * (ubar[i] * (D2(i, j) - wav2 * did) - ubarpp[i] * did);
+            b[(size_t)i * n + j] = -I * rey * (D2(i, j) - wav2 * did);
+        }
+    for (int j = 0; j < nzp; ++j) {  // membrane bottom-wall row (orrsomm.f90:189-194)
+        cplx s = 0.0;
+        for (int k = 0; k < nzp; ++k) s = s - I * c01 / (alphax * rey) * D2(nzp - 1, k) * D(k, j);
+        a[(size_t)(n - 1) * n + j] = s;
+    }
+    for (int r : {0, 1, 2, n - 2, n - 1})
+        for (int j = 0; j < n; ++j) b[(size_t)r * n + j] = 0.0;
+    for (int r : {0, 2, n - 2})
+        for (int j = 0; j < n; ++j) a[(size_t)r * n + j] = 0.0;
+    for (int j = 0; j < nzp; ++j) a[(size_t)1 * n + j] = D(nzp - 1, j);
+    a[0] = 1.0;
+    a[(size_t)2 * n + nzp - 1] = 1.0;
+    b[(size_t)2 * n + n - 2] = 1.0 / alphax;
+    a[(size_t)1 * n + n - 2] = ubarp[nzp - 1];
+    a[(size_t)1 * n + n - 1] = 0.0;
+    a[(size_t)(n - 2) * n + n - 1] = 1.0;
+    b[(size_t)(n - 2) * n + n - 2] = -I;
+    a[(size_t)(n - 1) * n + n - 2] = c03 + c04 * wav2 - I * wav2 * (c02 / rey) * (ubarp[nzp - 1] / alphax);
+    a[(size_t)(n - 1) * n + n - 1] = c01;
+    b[(size_t)(n - 1) * n + n - 1] = I;
+    // B^H
+    std::vector<cplx> bh((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) bh[(size_t)i * n + j] = std::conj(b[(size_t)j * n + i]);
+    // two-sided inverse iteration with Rayleigh-quotient shifts (orrsomm.f90:328-470)
+    cplx omk(omega[0], omega[1]);
+    std::vector<cplx> ph(n, 1.0), ga(n, 1.0), phk1 = ph, gak1 = ga;
+    std::vector<int> piv;
+    auto ck_of = [&](cplx om) {
+        std::vector<cplx> ck((size_t)n * n);
+        for (size_t q = 0; q < ck.size(); ++q) ck[q] = a[q] - om * b[q];
+        return ck;
+    };
+    for (int outer = 0; outer < 10; ++outer) {
+        const std::vector<cplx> ck = ck_of(omk);
+        const double rk = l1(cmatvec(ck, n, ph));
+        std::vector<cplx> lu = ck;
+        if (!clu(lu, n, piv)) {
+            set_error("solve_orrsomm: A - omega B is singular");
+            return LSB_E_NUMERIC;
+        }
+        for (int ii = 1; ii <= 10; ++ii) {
+            const double inner_rk = l1(cmatvec(ck, n, ph));
+            std::vector<cplx> rphi = cmatvec(b, n, ph), rgam = cmatvec(bh, n, ga);
+            clu_solve(lu, n, piv, rphi, false);
+            clu_solve(lu, n, piv, rgam, true);
+            const double nphi = l1(rphi), ngam = l1(rgam);
+            for (int q = 0; q < n; ++q) {
+                phk1[q] = rphi[q] / nphi;
+                gak1[q] = rgam[q] / ngam;
+            }
+            const double inner_rk1 = l1(cmatvec(ck, n, phk1));
+            double dphi = 0.0;
+            for (int q = 0; q < n; ++q) dphi += std::abs(phk1[q] - ph[q]);
+            if (dphi < 5.0e-5 || inner_rk1 < 5.0e-6) {
+                if (inner_rk1 > inner_rk) {
+                    phk1 = ph;
+                    gak1 = ga;
+                }
+                ph = phk1;
+                ga = gak1;
+                break;
+            }
+            // (a grown residual makes the reference add random noise, orrsomm.f90:383-396;
+            // the iteration goes on without it)
+            ph = phk1;
+            ga = gak1;
+        }
+        const std::vector<cplx> aph = cmatvec(a, n, phk1), bph = cmatvec(b, n, phk1);
+        cplx num = 0.0, den = 0.0;
+        for (int q = 0; q < n; ++q) {
+            num += std::conj(gak1[q]) * aph[q];
+            den += std::conj(gak1[q]) * bph[q];
+        }
+        cplx omk1 = num / den;
+        const double rk1 = l1(cmatvec(ck_of(omk1), n, phk1));
+        if (std::abs(omk1 - omk) < 1.0e-14 && rk1 < 1.5e-6) {
+            if (rk1 >= rk) {
+                omk1 = omk;
+                phk1 = ph;
+                gak1 = ga;
+            }
+            omk = omk1;
+            break;
+        }
+        if (std::abs(omk1 - omk) < 1.0e-14) {
+            // the Rayleigh quotient has stopped moving but the residual is above 1.5e-6, a
+            // test near the rounding floor of (A - omega B) phi: the reference restarts from
+            // a random omega here (orrsomm.f90:447-453); the restatement keeps the mode
+            omk = omk1;
+            break;
+        }
+        if (rk1 >= rk) {
+            omk1 = omk;
+            phk1 = ph;
+            gak1 = ga;
+        }
+        omk = omk1;
+    }
+    // streamfunction -> (u, v, w) = (D phi, 0, -i alpha phi), energy eps (orrsomm.f90:537-548)
+    std::vector<cplx> w(phk1.begin(), phk1.begin() + nzp), u(nzp, 0.0), v(nzp, 0.0);
+    for (int i = 0; i < nzp; ++i) {
+        cplx s = 0.0;
+        for (int j = 0; j < nzp; ++j) s += D(i, j) * w[j];
+        u[i] = s;
+    }
+    for (int i = 0; i < nzp; ++i) w[i] = -I * alphax * w[i];
+    auto energy = [&](const std::vector<cplx>& q) {  // orrsomm.f90:669-682
+        double e = 0.0;
+        for (int k = 1; k < nzp - 1; ++k) {
+            const double a1 = std::abs(q[k + 1]), a0 = std::abs(q[k]);
+            e = e - 0.5 * (a1 * a1 + a0 * a0) * (zpts[k + 1] - zpts[k]);
+        }
+        return e * 2.0 / 7.;
+    };
+    const double etot = energy(u) + energy(v) + energy(w);
+    const double xn = std::sqrt(eps / etot);
+    for (int i = 0; i < nzp; ++i) {
+        uvec[2 * i] = (u[i] * xn).real();
+        uvec[2 * i + 1] = (u[i] * xn).imag();
+        vvec[2 * i] = (v[i] * xn).real();
+        vvec[2 * i + 1] = (v[i] * xn).imag();
+        wvec[2 * i] = (w[i] * xn).real();
+        wvec[2 * i + 1] = (w[i] * xn).imag();
+    }
+    omega[0] = omk.real();
+    omega[1] = omk.imag();
+    return LSB_OK;
+}
+
+}  // namespace lsb

@@ -484,6 +484,15 @@ module lsb_capi
       type(c_ptr), value :: ctx
       real(c_double), intent(inout) :: enr1(*), enr2(*), grow1(*), grow2(*)
     end function
+    integer(c_int) function lsb_solve_orrsomm(nzp, d, d2, ubar, ubarp, ubarpp, zpts, xnu, eps, alphax, betax, &
+        omega, iglb, uvec, vvec, wvec) bind(C, name="lsb_solve_orrsomm")
+      import :: c_int, c_double
+      integer(c_int), value :: nzp, iglb
+      real(c_double), intent(in) :: d(*), d2(*), ubar(*), ubarp(*), ubarpp(*), zpts(*)
+      real(c_double), value :: xnu, eps, alphax, betax
+      real(c_double), intent(inout) :: omega(2)
+      real(c_double), intent(out) :: uvec(*), vvec(*), wvec(*)
+    end function
     integer(c_int) function lsb_set_temperature(ctx, tbar, tbarp, tbarpp, coex, gx, gy, gz) &
         bind(C, name="lsb_set_temperature")
       import :: c_int, c_ptr, c_double

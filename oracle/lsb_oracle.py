@@ -1202,6 +1202,127 @@ class Oracle:
             uf, vf, wf, pf = (self.horfft_fwd(q2kc[l]) for l in range(4))
         return uf, vf, wf, tempf, pf, Ih, Idelta, applied
 
+    # ------------------------------------------------- Orr-Sommerfeld mode
+    def orrsomm_matrices(self, alphax, betax):
+        """orrsomm.f90:159-213 (iglb <= 0): the n = nzp + 2 complex pair (A, B) of the
+        Orr-Sommerfeld problem on the Chebyshev grid, with the membrane-bottom-wall rows
+        (c01..c04 = 1e-40) and the boundary rows 1, 2, 3, n-1, n."""
+        nzp = self.nzp
+        n = nzp + 2
+        c01 = c02 = c03 = c04 = 1.0e-40
+        rey = 1.0 / self.cfg.xnu
+        wav2 = alphax * alphax + betax * betax
+        d, d2 = self.d, self.d2
+        d4 = d2 @ d2
+        I = np.eye(nzp)
+        a = np.zeros((n, n), dtype=np.complex128)
+        b = np.zeros((n, n), dtype=np.complex128)
+        ub = self.ubar[:, None]
+        a[:nzp, :nzp] = (d4 - 2.0 * wav2 * d2 + wav2 * wav2 * I
+                         - 1j * alphax * rey * (ub * (d2 - wav2 * I) - self.ubarpp[:, None] * I))
+        b[:nzp, :nzp] = -1j * rey * (d2 - wav2 * I)
+        row = np.zeros(nzp, dtype=np.complex128)
+        for j in range(nzp):
+            acc = 0.0 + 0.0j
+            for k in range(nzp):
+                acc = acc - 1j * c01 / (alphax * rey) * d2[nzp - 1, k] * d[k, j]
+            row[j] = acc
+        a[n - 1, :] = 0.0
+        a[n - 1, :nzp] = row
+        for r in (0, 1, 2, n - 2, n - 1):
+            b[r, :] = 0.0
+        for r in (0, 2, n - 2):
+            a[r, :] = 0.0
+        a[1, :nzp] = d[nzp - 1, :]
+        a[0, 0] = 1.0
+        a[2, nzp - 1] = 1.0
+        b[2, n - 2] = 1.0 / alphax
+        a[1, n - 2] = self.ubarp[nzp - 1]
+        a[1, n - 1] = 0.0
+        a[n - 2, n - 1] = 1.0
+        b[n - 2, n - 2] = -1j
+        a[n - 1, n - 2] = c03 + c04 * wav2 - 1j * wav2 * (c02 / rey) * (self.ubarp[nzp - 1] / alphax)
+        a[n - 1, n - 1] = c01
+        b[n - 1, n - 1] = 1j
+        return a, b
+
+    def solve_orrsomm(self, eps, alphax, betax, omega, iglb=-1):
+        """orrsomm.f90:110-562: (uvec, vvec, wvec, omega).  iglb < 0 first takes the most
+        unstable eigenvalue with 0 < c_r < 0.42 of the whole generalized spectrum (zggevx there,
+        scipy's QZ here; eigenvalues phi/|gam|, :238-244, 255); then the two-sided inverse
+        iteration with Rayleigh-quotient updates (:328-470; outer and inner loops of 10);
+        streamfunction -> (u, v, w) = (D phi, 0, -i alpha phi), scaled to energy eps.  The
+        reference's random nudges of a stalled iteration (:383-396) are left out, and where it
+        restarts from a random omega because a converged Rayleigh quotient still leaves a
+        residual above 1.5e-6 (:447-453) the restatement keeps the mode."""
+        import scipy.linalg as sla
+        a, b = self.orrsomm_matrices(alphax, betax)
+        n = a.shape[0]
+        if iglb < 0:
+            phi, gam = sla.eig(a, b, homogeneous_eigvals=True)[0]
+            mag = np.abs(gam)
+            reig = np.where(mag > 0, phi.real / np.where(mag > 0, mag, 1.0), phi.real)
+            aeig = np.where(mag > 0, phi.imag / np.where(mag > 0, mag, 1.0), phi.imag)
+            ok = (reig / alphax > 0) & (reig / alphax < 0.42)
+            j = int(np.argmax(np.where(ok, aeig / alphax, -np.inf)))
+            omega = complex(reig[j], aeig[j])
+        omk = complex(omega)
+        ph = np.ones(n, dtype=np.complex128)
+        ga = np.ones(n, dtype=np.complex128)
+        bh = b.conj().T
+        l1 = lambda x: float(np.sum(np.abs(x)))  # noqa: E731
+
+        phk1, gak1 = ph, ga
+        for _ in range(10):
+            ck = a - omk * b
+            rk = l1(ck @ ph)
+            lu = sla.lu_factor(ck)
+            for ii in range(1, 11):
+                inner_rk = l1(ck @ ph)
+                rphi = sla.lu_solve(lu, b @ ph, trans=0)
+                rgam = sla.lu_solve(lu, bh @ ga, trans=2)
+                phk1 = rphi / l1(rphi)
+                gak1 = rgam / l1(rgam)
+                inner_rk1 = l1(ck @ phk1)
+                if l1(phk1 - ph) < 5.0e-5 or inner_rk1 < 5.0e-6:
+                    if inner_rk1 > inner_rk:
+                        phk1, gak1 = ph, ga
+                    ph, ga = phk1, gak1
+                    break
+                # (a residual that grew here makes the reference add 1e-3 random noise to the
+                # iterates, :383-396; the restatement keeps iterating without it)
+                ph, ga = phk1, gak1
+            omk1 = np.vdot(gak1, a @ phk1) / np.vdot(gak1, b @ phk1)
+            rk1 = l1((a - omk1 * b) @ phk1)
+            if abs(omk1 - omk) < 1.0e-14 and rk1 < 1.5e-6:
+                if rk1 >= rk:
+                    omk1, phk1, gak1 = omk, ph, ga
+                omk = omk1
+                break
+            if abs(omk1 - omk) < 1.0e-14:
+                # stationary Rayleigh quotient with a residual above 1.5e-6 (a test near the
+                # rounding floor of (A - omega B) phi): the reference restarts from a random
+                # omega (:447-453); the restatement keeps the mode
+                omk = omk1
+                break
+            if rk1 >= rk:
+                omk1, phk1, gak1 = omk, ph, ga
+            omk = omk1
+        nzp = self.nzp
+        w = phk1[:nzp].copy()
+        u = self.d @ w
+        w = -1j * alphax * w
+        v = np.zeros(nzp, dtype=np.complex128)
+
+        def energy(q):
+            e = 0.0
+            for k in range(1, nzp - 1):          # Fortran k = 2 .. nzp-1
+                e = e - 0.5 * (abs(q[k + 1]) * abs(q[k + 1]) + abs(q[k]) * abs(q[k])) * (self.zpts[k + 1] - self.zpts[k])
+            return e * 2.0 / 7.0
+        etot = energy(u) + energy(v) + energy(w)
+        xn = math.sqrt(eps / etot)
+        return u * xn, v * xn, w * xn, omk
+
     # ------------------------------------------------- linear-stability energy
     def ener(self, uf, vf, wf, itot):
         """orrsomm.f90:625-666: ee(k) = sum over the nx*ny physical points of

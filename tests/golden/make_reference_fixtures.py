@@ -12,6 +12,9 @@ Sources (data files of the reference repository, not code):
                   as float32 in q00019.npz
   run/prev_q00001.dat  the snapshot of step 10 of the run/input.dat case (Blasius base flow,
                   perturbations ~1e-17): its header and the y = 0 planes of u and temp
+  run/energy.dat, run/output_lintest.txt  the linear-stability sweep (write_linstab_results,
+                  orrsomm.f90:63-107): per (Re_d, alpha*) omega_i, <dE/dt>, % error, and the last
+                  omega of the Orr-Sommerfeld iteration printed to 10 digits (orrsomm.f90:428)
 Run from the repository root:  python tests/golden/make_reference_fixtures.py
 """
 
@@ -58,6 +61,24 @@ def main():
     nx, ny, nzp = (int(v) for v in q1[3:6])
     body = q1[7:].reshape(5, ny, nzp, nx)
     np.savez_compressed(os.path.join(OUT, "prev_q00001_y0.npz"), header=q1[:7], u_y0=body[0, 0], temp_y0=body[3, 0])
+    rows = []
+    for line in open(os.path.join(REF, "energy.dat")).read().splitlines()[1:]:
+        v = [float(x) for x in line.replace(",", " ").split()]
+        if len(v) >= 5:
+            rows.append(dict(re_d=v[0], alphastar=v[1], omega_i=v[2], dedt=v[3], err_pct=v[4]))
+    cases = []
+    cur = None
+    for line in open(os.path.join(REF, "output_lintest.txt")):
+        if line.startswith(" Re_d ="):
+            parts = line.replace("(", " ").replace(")", " ").replace(",", " ").split()
+            cur = dict(re_d=float(parts[2]), x0=float(parts[5]), alphastar=float(parts[8]),
+                       cguess=float(parts[11]))
+            cases.append(cur)
+        elif cur is not None and line.strip().startswith("100,"):
+            v = [float(x) for x in line.replace(",", " ").split()]
+            cur["omega"] = [v[1], v[2]]
+    with open(os.path.join(OUT, "linstab.json"), "w") as fh:
+        json.dump({"energy_dat": rows, "orrsomm_omega": [c for c in cases if "omega" in c]}, fh, indent=1)
     print("wrote", OUT)
 
 
