@@ -317,7 +317,7 @@ int lsb_state_lintest(lsb_ctx* ctx, double* enr1, double* enr2, double* grow1, d
  * zpts, xnu.  omega[2] (Re, Im) is the guess in and the eigenvalue out; uvec/vvec/wvec get
  * nzp complex values each (Re, Im interleaved), the mode scaled to energy eps.  Host only.
  * iglb = 0 (the iteration from the guess alpha*cguess*u0) only: iglb < 0 (a global zggevx
- * spectrum first) -> LSB_E_UNSUPPORTED.  A failed iteration, which the reference answers
+ * spectrum first; init.f90:606 passes -1) and iglb > 0 (no solve) -> LSB_E_UNSUPPORTED.  A failed iteration, which the reference answers
  * with a random restart, -> LSB_E_NUMERIC. */
 int lsb_solve_orrsomm(int nzp, const double* d, const double* d2, const double* ubar, const double* ubarp,
                       const double* ubarpp, const double* zpts, double xnu, double eps, double alphax,

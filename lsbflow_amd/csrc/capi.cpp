@@ -1959,9 +1959,9 @@ extern "C" int lsb_solve_orrsomm(int nzp, const double* d, const double* d2, con
         set_error("solve_orrsomm: bad arguments");
         return LSB_E_ARG;
     }
-    if (iglb < 0) {
-        set_error("solve_orrsomm: iglb < 0 (the global zggevx step) is not restated; pass the omega guess "
-                  "alpha*cguess*u0 with iglb = 0");
+    if (iglb != 0) {
+        set_error("solve_orrsomm: only iglb = 0 is restated (iglb < 0 adds the global zggevx step, iglb > 0 "
+                  "skips the solve); pass the omega guess alpha*cguess*u0 with iglb = 0");
         return LSB_E_UNSUPPORTED;
     }
     return orrsomm_solve(nzp, d, d2, ubar, ubarp, ubarpp, zpts, xnu, eps, alphax, betax, omega, uvec, vvec, wvec);

@@ -844,7 +844,7 @@ int orrsomm_solve(int nzp, const double* d, const double* d2, const double* ubar
     std::vector<cplx> bh((size_t)n * n);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) bh[(size_t)i * n + j] = std::conj(b[(size_t)j * n + i]);
-    // two-sided inverse iteration with Rayleigh-quotient shifts (orrsomm.f90:328-470)
+    // two-sided inverse iteration with Rayleigh-quotient shifts (orrsomm.f90:336-498)
     cplx omk(omega[0], omega[1]);
     std::vector<cplx> ph(n, 1.0), ga(n, 1.0), phk1 = ph, gak1 = ga;
     std::vector<int> piv;
@@ -883,7 +883,7 @@ int orrsomm_solve(int nzp, const double* d, const double* d2, const double* ubar
                 ga = gak1;
                 break;
             }
-            // (a grown residual makes the reference add random noise, orrsomm.f90:383-396;
+            // (a grown residual makes the reference add random noise, orrsomm.f90:411-425;
             // the iteration goes on without it)
             ph = phk1;
             ga = gak1;
@@ -908,7 +908,7 @@ int orrsomm_solve(int nzp, const double* d, const double* d2, const double* ubar
         if (std::abs(omk1 - omk) < 1.0e-14) {
             // the Rayleigh quotient has stopped moving but the residual is above 1.5e-6, a
             // test near the rounding floor of (A - omega B) phi: the reference restarts from
-            // a random omega here (orrsomm.f90:447-453); the restatement keeps the mode
+            // a random omega here (orrsomm.f90:476-483); the restatement keeps the mode
             omk = omk1;
             break;
         }

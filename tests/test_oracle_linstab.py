@@ -47,9 +47,9 @@ def cguess(re_d):
 
 @pytest.mark.parametrize("case", FIX["orrsomm_omega"], ids=lambda c: f"{c['re_d']:.0f}-{c['alphastar']}")
 def test_orrsomm_omega_output_lintest(case):
-    """The 10-digit omega of run/output_lintest.txt (orrsomm.f90:428) from the oracle and from
+    """The 10-digit omega of run/output_lintest.txt (orrsomm.f90:443) from the oracle and from
     the library's host solver.  The iteration stops once the eigenvector changes by less than
-    5e-5 in L1 (orrsomm.f90:375), which fixes omega, a two-sided Rayleigh quotient, to a few
+    5e-5 in L1 (orrsomm.f90:394), which fixes omega, a two-sided Rayleigh quotient, to a few
     1e-9 (the rounding of the two LU factorisations decides where it stops) and the vector to
     about 1e-4."""
     o = lintest_case(case["re_d"], case["alphastar"])
@@ -62,7 +62,7 @@ def test_orrsomm_omega_output_lintest(case):
     lu, lv, lw, lom = L.solve_orrsomm(o.d, o.d2, o.ubar, o.ubarp, o.ubarpp, o.zpts, XNU, 1e-10, al, 0.0, g, 0)
     assert abs(lom - want) < tol, (lom, want)
     assert abs(lom - om) < 1e-8
-    # the mode's phase is whatever the iteration left (L1 normalisation only, orrsomm.f90:371-372)
+    # the mode's phase is whatever the iteration left (L1 normalisation only, orrsomm.f90:386-387)
     ph = np.vdot(u, lu)
     ph = ph / abs(ph)
     for a, b in ((lu, u), (lw, w)):
