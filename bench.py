@@ -173,7 +173,8 @@ def diagnostics(case, reps):
     """Wall time per call (host-synchronised; the records include their PCIe copy to
     host memory) of the per-step diagnostics, the I/O records and the two filters (mode-cut
     and TNS) on the FP64 resident state of `case`, after two steps; then one FP64 step
-    without and with the temperature / buoyancy path.  The kernels' own device times come from a
+    without and with the on-the-fly time average (and its qavg record), and with the
+    temperature / buoyancy path.  The kernels' own device times come from a
     rocprofv3 --kernel-trace run of this mode."""
     import numpy as np
     import lsbflow_amd as L
@@ -207,11 +208,15 @@ def diagnostics(case, reps):
         ctx.state_set_temperature(0.5 * ctx.state_get()["uf"])
 
     calls["timestep_fp64"] = step
+    calls["time_average_on"] = lambda: ctx.set_time_average(1, 0, 5)
+    calls["timestep_fp64_with_time_average"] = step
+    calls["state_snapshot_average_record"] = lambda: ctx._L.lsb_state_snapshot_average(ctx.h, None, ctx._fp(snap))
+    calls["time_average_off"] = lambda: ctx.set_time_average(0, 0, 1)
     calls["temperature_on"] = temperature_on
     calls["timestep_fp64_with_temperature"] = step
     out = {}
     for name, fn in calls.items():
-        if name == "temperature_on":
+        if name.endswith("_on") or name.endswith("_off"):   # set-up, not timed
             fn()
             continue
         fn()

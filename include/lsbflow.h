@@ -232,24 +232,38 @@ int lsb_synchronize(lsb_ctx* ctx);
 int lsb_snapshot_record(lsb_ctx* ctx, const double* uf, const double* vf, const double* wf,
                         const double* tempf, const double* pf, const double* tbar, int name,
                         int isum, double t, double dt, float* rec);
-/* The same record from the resident state (temp = 0; isum, t, dt of the context).
- * path != NULL writes the file (an existing one is renamed prev_<name>, mpi_io.f90:400-404;
- * the caller names it q%05d.dat / qavg%05d.dat / q_ref.dat as :396-402); rec may be NULL. */
+/* The same record from the resident state (temp from the temperature path, else 0; tbar
+ * of lsb_set_temperature; isum, t, dt of the context).  path != NULL writes the file (an
+ * existing one is renamed prev_<name>, mpi_io.f90:400-404; the caller names it q%05d.dat /
+ * qavg%05d.dat / q_ref.dat as :396-402); rec may be NULL. */
 int lsb_state_snapshot(lsb_ctx* ctx, int name, const char* path, float* rec);
+/* main.f90:69-76, 98-106 on-the-fly time averaging (runparam otfavg, istartavg, iomod):
+ * with otfavg = 1, after every step of lsb_timestep whose total step count exceeds
+ * istartavg, q_tavg = q_tavg + q/iomod for uf, vf, wf, tempf, pf, on the device.  The
+ * averages start at zero here, on lsb_state_set and on lsb_read_restart (init.f90:112-119);
+ * otfavg = 0 releases them. */
+int lsb_set_time_average(lsb_ctx* ctx, int otfavg, int istartavg, int iomod);
+/* mpi_snapshot(uf_tavg, vf_tavg, wf_tavg, tempf_tavg, pf_tavg, 1) then the averages back to
+ * zero (main.f90:98-106): the qavg record (u + ubar, temp + tbar), path / rec as
+ * lsb_state_snapshot.  Averaging off -> LSB_E_STATE. */
+int lsb_state_snapshot_average(lsb_ctx* ctx, const char* path, float* rec);
+/* The current averages, spectral (nxpl, ny, nzp) each; any pointer may be NULL. */
+int lsb_state_get_time_average(lsb_ctx* ctx, double* uf, double* vf, double* wf, double* tempf, double* pf);
 /* mpi_io.f90:500-650 mpi_read_snapshot: q(nx, ny, nzp, 5) physical (Fortran order) from a
  * snapshot file; dimension mismatch -> LSB_E_ARG.  Host-only. */
 int lsb_read_snapshot(lsb_ctx* ctx, const char* path, double* q, int* isum, double* t, double* dt);
 /* mpi_io.f90:9-160 mpi_write_restart_file: 9 floats [isum, t, dt, dt2, dt3, nx, ny, nzp,
  * nxpl] then 20 float32 spectral fields (uf, vf, wf, tempf, unif(3), unmif(3), nlnif(3),
  * nlnmif(3), tnf, tnmf, nltnf, nltnmf), z fastest then y then x.  Slots 14-16 hold
- * nlnmif (the reference writes nlnif twice there, quirk Q12).  Temperature slots are 0.
- * From the resident state; path and/or rec (9 + 20*nx*ny*nzp floats) may be NULL. */
+ * nlnmif (the reference writes nlnif twice there, quirk Q12).  Temperature slots hold
+ * the temperature path's fields (0 while it is off).  From the resident state; path and/or
+ * rec (9 + 20*nx*ny*nzp floats) may be NULL. */
 int lsb_write_restart(lsb_ctx* ctx, const char* path, float* rec);
 /* mpi_io.f90:163-336 mpi_read_restart_file into the resident state (rec, else the file):
  * the total step count = nint(isum), t, dt, dt2, dt3 from the header; the run's own step
  * counter starts again at 0 (main.f90:61), so the next step follows the nstart start-up
  * rule.  pf/pnf keep their values (zero on a fresh context).  Nonzero temperature
- * slots -> LSB_E_UNSUPPORTED. */
+ * slots switch the temperature path on (FP64 form; in the mixed form -> LSB_E_UNSUPPORTED). */
 int lsb_read_restart(lsb_ctx* ctx, const char* path, const float* rec);
 
 /* navierstokes.f90:358-511 maxcalc(uf, vf, wf, tempf, dtmax, dt, dt2, dt3) with the hidden

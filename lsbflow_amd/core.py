@@ -454,6 +454,25 @@ class Context:
               "state_snapshot")
         return rec
 
+    def set_time_average(self, otfavg=1, istartavg=0, iomod=1):
+        """main.f90:69-76: on-the-fly averages of uf, vf, wf, tempf, pf over the steps of
+        timestep() once the total step count exceeds istartavg (q/iomod per step)."""
+        check(self._L.lsb_set_time_average(self.h, otfavg, istartavg, iomod), "set_time_average")
+
+    def state_snapshot_average(self, path=None, record=True):
+        """main.f90:98-106: the qavg snapshot record of the averages, which then restart at zero."""
+        n = self.nx * self.ny * self.nzp
+        rec = np.zeros(7 + 5 * n, dtype=np.float32) if record else None
+        check(self._L.lsb_state_snapshot_average(self.h, None if path is None else os.fsencode(path),
+                                                 self._fp(rec)), "state_snapshot_average")
+        return rec
+
+    def state_get_time_average(self):
+        """(uf, vf, wf, tempf, pf) averages, spectral."""
+        out = [np.zeros(self.sshape) for _ in range(5)]
+        check(self._L.lsb_state_get_time_average(self.h, *[_p(x) for x in out]), "state_get_time_average")
+        return tuple(out)
+
     def read_snapshot(self, path):
         """(q (5, nzp, ny, nx), isum, t, dt) of a snapshot file."""
         q = np.zeros((5,) + self.sshape)

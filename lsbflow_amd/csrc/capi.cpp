@@ -268,7 +268,7 @@ extern "C" int lsb_destroy(lsb_ctx* c) {
     step_free(c);
     temp_free(c);
     double* ds[] = {c->d_wavx, c->d_wavy, c->d_zpts, c->d_mdz, c->d_md2, c->d_md2b, c->d_d, c->d_smat,
-                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red, c->d_lam};
+                    c->d_simat, c->d_diag, c->d_lift, c->d_paral, c->d_delta, c->d_fz, c->d_smag, c->d_red, c->d_ftab, c->d_diag_red, c->d_lam, c->d_tavg};
     for (double* p : ds)
         if (p) (void)hipFree(p);
     if (c->d_walls) (void)hipFree(c->d_walls);
@@ -425,6 +425,12 @@ struct Stage {
 
 int set_device(lsb_ctx* c) {
     LSB_HIP(hipSetDevice(c->device));
+    return LSB_OK;
+}
+
+// the on-the-fly time averages back to zero (no-op while averaging is off)
+int tavg_zero(lsb_ctx* c) {
+    if (c->d_tavg) LSB_HIP(hipMemsetAsync(c->d_tavg, 0, 5 * c->nspec * sizeof(double), c->stream));
     return LSB_OK;
 }
 
@@ -1033,6 +1039,7 @@ extern "C" int lsb_state_set(lsb_ctx* c, const double* uf, const double* vf, con
     }
     LSB_TRY(put(15, pf));
     LSB_TRY(put(16, pnf));
+    LSB_TRY(tavg_zero(c));  // a new run starts its averages at zero (init.f90:112-119)
     c->istep = istep;
     c->isum = istep;
     c->coef_valid = false;
@@ -1512,6 +1519,7 @@ extern "C" int lsb_read_restart(lsb_ctx* c, const char* path, const float* rec) 
     }
     // isum = nint(prebuf(1)) (mpi_io.f90:231); the restarted run's own step counter
     // starts again at 1 (main.f90:61), so the start-up rule of nstart applies
+    LSB_TRY(tavg_zero(c));  // the restarted run starts its averages at zero (init.f90:112-119)
     c->isum = (int)std::lround(hv[0]);
     c->istep = 0;
     c->coef_valid = false;
@@ -1618,7 +1626,87 @@ extern "C" int lsb_set_adaptive_dt(lsb_ctx* c, int dtadapt, int iomod, double fs
     return LSB_OK;
 }
 
+extern "C" int lsb_set_time_average(lsb_ctx* c, int otfavg, int istartavg, int iomod) {
+    LSB_ENTER(c);
+    if ((otfavg != 0 && otfavg != 1) || iomod < 1) {
+        set_error("set_time_average: otfavg 0/1, iomod >= 1");
+        return LSB_E_ARG;
+    }
+    c->otfavg = otfavg;
+    c->istartavg = istartavg;
+    c->avg_iomod = iomod;
+    if (otfavg == 1) {  // init.f90:112-119: allocated and zeroed
+        if (!c->d_tavg) LSB_TRY(dev_alloc((void**)&c->d_tavg, 5 * c->nspec * sizeof(double)));
+        LSB_TRY(tavg_zero(c));
+    } else if (c->d_tavg) {
+        LSB_HIP(hipStreamSynchronize(c->stream));
+        LSB_HIP(hipFree(c->d_tavg));
+        c->d_tavg = nullptr;
+    }
+    return st.sync();
+}
+
+extern "C" int lsb_state_get_time_average(lsb_ctx* c, double* uf, double* vf, double* wf, double* tempf, double* pf) {
+    LSB_ENTER(c);
+    if (!c->d_tavg) {
+        set_error("state_get_time_average: time averaging is off (lsb_set_time_average)");
+        return LSB_E_STATE;
+    }
+    double* hs[5] = {uf, vf, wf, tempf, pf};
+    for (int l = 0; l < 5; ++l)
+        if (hs[l])
+            LSB_HIP(hipMemcpyAsync(hs[l], c->d_tavg + (size_t)l * c->nspec, c->nspec * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+    return st.sync();
+}
+
+extern "C" int lsb_state_snapshot_average(lsb_ctx* c, const char* path, float* rec) {
+    LSB_ENTER(c);
+    LSB_TRY(need_state(c, "state_snapshot_average"));
+    if (!c->d_tavg) {
+        set_error("state_snapshot_average: time averaging is off (lsb_set_time_average)");
+        return LSB_E_STATE;
+    }
+    const double* spec[5];
+    for (int l = 0; l < 5; ++l) spec[l] = c->d_tavg + (size_t)l * c->nspec;
+    float* body;
+    LSB_TRY(snapshot_body(c, st, spec, 1, temp_tbar_dev(c), &body));
+    const size_t nb = c->nspec * 5;
+    std::vector<float> own;
+    float* hb = rec ? rec + kSnapHead : nullptr;
+    if (!hb) {
+        own.resize(nb);
+        hb = own.data();
+    }
+    float head[kSnapHead];
+    snap_header(c, c->isum, c->t, c->dt, head);
+    LSB_HIP(hipMemcpyAsync(hb, body, nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LSB_TRY(tavg_zero(c));  // main.f90:104-106, after the averaged snapshot
+    LSB_TRY(st.sync());
+    if (rec) std::memcpy(rec, head, sizeof(head));
+    if (path) LSB_TRY(write_record(path, head, kSnapHead, hb, nb));
+    return LSB_OK;
+}
+
 namespace lsb {
+// main.f90:69-76, after timestep() (whose time update has counted the step in its):
+// q_tavg = q_tavg + q/iomod for uf, vf, wf, tempf, pf once its > istartavg.  tempf is
+// zero while the temperature path is off, so its average is left alone then.
+int state_time_average(lsb_ctx* c) {
+    if (c->otfavg != 1 || !c->d_tavg || c->isum <= c->istartavg) return LSB_OK;
+    const double* f[5];
+    double* a[5];
+    int n = 0;
+    const int which[5] = {0, 1, 2, -1, 15};
+    for (int l = 0; l < 5; ++l) {
+        if (which[l] < 0 && !temp_on(c)) continue;
+        f[n] = which[l] < 0 ? temp_field(c, 0) : step_field(c, which[l]);
+        a[n] = c->d_tavg + (size_t)l * c->nspec;
+        ++n;
+    }
+    return tavg_accum(c, f, a, n, c->avg_iomod, c->stream);
+}
+
 // navierstokes.f90:74-87, after the time update of step istep: every 5th (or iomod-th)
 // step a NaN check (abort) and maxcalc(..., dtmax = 1.0, dt, dt2, dt3).
 int state_adapt_dt(lsb_ctx* c) {

@@ -11,7 +11,8 @@
 //   k_restart_unpack  the inverse transpose with the FP32 -> FP64 widening;
 //   k_maxcalc, k_nan_scan  grid-stride reductions, one partial per block,
 //                     finished on the host in a fixed order (deterministic);
-//   k_wall_mean       spanwise mean of two physical planes, 64 x-points per block.
+//   k_wall_mean       spanwise mean of two physical planes, 64 x-points per block;
+//   k_tavg_accum      the on-the-fly time average, one streaming pass per field.
 
 #include <cfloat>
 
@@ -240,6 +241,22 @@ __global__ void k_zero_mean(FieldList<double> f, int nf, int nzp, long plane) {
     f.p[l][(long)(r >> 1) * plane + (r & 1)] = 0.0;
 }
 
+// main.f90:69-76: avg_l = avg_l + f_l / div over n2 complex pairs of field l =
+// blockIdx.y.  A division, as the reference's q/iomod (not a product with 1/div,
+// which rounds differently).
+__global__ void __launch_bounds__(256) k_tavg_accum(FieldList<const double> f, FieldList<double> avg, size_t n2,
+                                                    double div) {
+    const double2* __restrict__ src = reinterpret_cast<const double2*>(f.p[blockIdx.y]);
+    double2* __restrict__ dst = reinterpret_cast<double2*>(avg.p[blockIdx.y]);
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x) {
+        const double2 q = src[t];
+        double2 a = dst[t];
+        a.x = a.x + q.x / div;
+        a.y = a.y + q.y / div;
+        dst[t] = a;
+    }
+}
+
 int diag_scratch(lsb_ctx* c, double** p) {
     if (!c->d_diag_red) LSB_TRY(dev_alloc((void**)&c->d_diag_red, (size_t)kDiagDoubles * sizeof(double)));
     *p = c->d_diag_red;
@@ -358,6 +375,19 @@ int zero_mean_modes(lsb_ctx* c, double* const* spec, int nf, hipStream_t s) {
     const int n = nf * c->nzp * 2;
     hipLaunchKernelGGL(k_zero_mean, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, nf, c->nzp,
                        (long)c->plane_s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int tavg_accum(lsb_ctx* c, const double* const* f, double* const* avg, int nf, int div, hipStream_t s) {
+    if (nf < 1 || nf > kMaxFields || div < 1 || (c->nspec & 1)) return LSB_E_ARG;
+    FieldList<const double> in{};
+    FieldList<double> out{};
+    for (int l = 0; l < nf; ++l) {
+        in.p[l] = f[l];
+        out.p[l] = avg[l];
+    }
+    hipLaunchKernelGGL(k_tavg_accum, dim3(1024u, (unsigned)nf), dim3(256), 0, s, in, out, c->nspec / 2, (double)div);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }

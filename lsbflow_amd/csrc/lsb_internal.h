@@ -189,6 +189,10 @@ struct lsb_ctx {
     double fsize = 0.0;
     // filtering (navierstokes.f90:31-39; runparam ifilt, modfilt)
     int ifilt = 0, modfilt = 1;
+    // on-the-fly time average (main.f90:69-76; runparam otfavg, istartavg, iomod):
+    // uf, vf, wf, tempf, pf averages, 5 spectral FP64 fields in one block
+    int otfavg = 0, istartavg = 0, avg_iomod = 1;
+    double* d_tavg = nullptr;
     double abfac[3] = {0, 0, 0};
     double bdf[3] = {0, 0, 0};
     double bdfv = 0.0;
@@ -251,6 +255,10 @@ int plane_ke(lsb_ctx* c, const double* u, const double* v, const double* w, bool
              hipStream_t s);
 // zero the (kx, ky) = (0, 0) mode of nf spectral fields
 int zero_mean_modes(lsb_ctx* c, double* const* spec, int nf, hipStream_t s);
+// avg_l += f_l / div for nf spectral FP64 fields (the on-the-fly time average)
+int tavg_accum(lsb_ctx* c, const double* const* f, double* const* avg, int nf, int div, hipStream_t s);
+// time average after a step when otfavg is on and the total step count exceeds istartavg (capi.cpp)
+int state_time_average(lsb_ctx* c);
 // NaN check + maxcalc on the resident state after a step when dtadapt is on (capi.cpp)
 int state_adapt_dt(lsb_ctx* c);
 // filterall3pt / TNS filtering of the resident state before a step when ifilt >= 1 (capi.cpp)

@@ -1071,6 +1071,7 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
             c->dt3 = c->dt2;
             c->dt2 = c->dt;
             LSB_TRY(state_adapt_dt(c));
+            LSB_TRY(state_time_average(c));
         }
     }
     return LSB_OK;

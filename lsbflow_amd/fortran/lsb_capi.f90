@@ -377,6 +377,22 @@ module lsb_capi
       integer(c_int), value :: name
       type(c_ptr), value :: path, rec
     end function
+    integer(c_int) function lsb_set_time_average(ctx, otfavg, istartavg, iomod) bind(C, name="lsb_set_time_average")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: otfavg, istartavg, iomod
+    end function
+    integer(c_int) function lsb_state_snapshot_average(ctx, path, rec) bind(C, name="lsb_state_snapshot_average")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: path, rec
+    end function
+    integer(c_int) function lsb_state_get_time_average(ctx, uf, vf, wf, tempf, pf) &
+        bind(C, name="lsb_state_get_time_average")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: uf(*), vf(*), wf(*), tempf(*), pf(*)
+    end function
     integer(c_int) function lsb_read_snapshot(ctx, path, q, isum, t, dt) bind(C, name="lsb_read_snapshot")
       import :: c_int, c_ptr, c_double, c_char
       type(c_ptr), value :: ctx
@@ -486,12 +502,12 @@ module lsb_capi
     end function
     integer(c_int) function lsb_solve_orrsomm(nzp, d, d2, ubar, ubarp, ubarpp, zpts, xnu, eps, alphax, betax, &
         omega, iglb, uvec, vvec, wvec) bind(C, name="lsb_solve_orrsomm")
-      import :: c_int, c_double
+      import :: c_int, c_double, c_double_complex
       integer(c_int), value :: nzp, iglb
       real(c_double), intent(in) :: d(*), d2(*), ubar(*), ubarp(*), ubarpp(*), zpts(*)
       real(c_double), value :: xnu, eps, alphax, betax
-      real(c_double), intent(inout) :: omega(2)
-      real(c_double), intent(out) :: uvec(*), vvec(*), wvec(*)
+      complex(c_double_complex), intent(inout) :: omega            ! double[2] (Re, Im) on the C side
+      complex(c_double_complex), intent(out) :: uvec(*), vvec(*), wvec(*)
     end function
     integer(c_int) function lsb_set_temperature(ctx, tbar, tbarp, tbarpp, coex, gx, gy, gz) &
         bind(C, name="lsb_set_temperature")

@@ -34,6 +34,21 @@ Pinning (tests/test_oracle_*.py):
                           forms for the CFL planes, fringe range and the dt rule
   wall_shear / nancheck -- single-mode closed forms (cf and its zero crossings);
                           Inf vs NaN
+  time_average         -- window (its > istartavg) and scale (q/iomod) on fields that
+                          grow linearly with the step
+  sponge_lambda        -- the fringe's plateau, zero region and rise against the
+                          published Herbst-Henningson form (ftype 1)
+  modecut_fc / filterall3pt / tns_automatic_filtering -- one mode times a linear z
+                          profile: 1 - (1 - T)^6; the TNS energy ratio of one mode
+                          ((1 - T16)/(1 - T12))^12 and both decision branches
+  temperature path     -- first step in closed form: advection by a uniform stream
+                          (-U dT/dx, kappa = nu/Pr), buoyancy coex*g*T on the (0, 0)
+                          mode, base-profile diffusion with tbarpp
+  ener / lintest       -- one streamwise mode over ubar (cross term vanishes, mean
+                          removed); a scaled state gives (s^2 - 1)/(2 s^2 dt)
+  solve_orrsomm        -- run/output_lintest.txt omega (10 digits) and run/energy.dat
+                          omega_i of the reference's sweep on its recovered grid;
+                          the pencil residual and energy normalisation of the mode
 """
 
 from __future__ import annotations
@@ -1143,6 +1158,18 @@ class Oracle:
         st["dt3"] = st["dt2"]
         st["dt2"] = st["dt"]
         return st
+
+    AVG_FIELDS = ("uf", "vf", "wf", "tempf", "pf")
+
+    def time_average(self, st, avg, istartavg, iomod, its=None):
+        """main.f90:69-76, after timestep(): q_tavg = q_tavg + q/iomod for uf, vf, wf, tempf,
+        pf once its (the total step count, navierstokes.f90:63; the run's istep unless a
+        restart set it) exceeds istartavg.  avg: dict of the five averages, updated."""
+        its = st["istep"] if its is None else its
+        if its > istartavg:
+            for k in self.AVG_FIELDS:
+                avg[k] = avg[k] + st[k] / iomod
+        return avg
 
     def divcheck(self, uf, vf, wf):
         """derivatives.f90:348-392: mean |div u| in physical space (pads are 0 here)."""

@@ -12,7 +12,9 @@ import numpy as np
 import pytest
 
 from oracle import Oracle, OracleConfig
+import lsbflow_amd as L
 from lsbflow_amd import _lib
+from tests.test_oracle_linstab import lintest_case
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FC = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
@@ -41,12 +43,18 @@ def test_fortran_program_through_the_binding(tmp_path):
     o = Oracle(OracleConfig(nx=8, ny=4, nz=40, mapping=3, zlen=2.0))
     nzp, n1, n2 = o.nzp, 3, 2
     y = np.random.default_rng(5).standard_normal((nzp, n2, n1))     # Fortran y(n1, n2, nzp)
+    ls = lintest_case(1805.0, 0.5)                                  # an Orr-Sommerfeld case (nzp = 65)
+    guess = ls.alpha * 0.4
     inp = tmp_path / "in.bin"
     with open(inp, "wb") as fh:
         np.array([nzp, n1, n2], dtype=np.int32).tofile(fh)
         np.array([o.xl], dtype=np.float64).tofile(fh)
         o.gf.astype(np.float64).tofile(fh)
         y.astype(np.float64).tofile(fh)
+        np.array([ls.nzp], dtype=np.int32).tofile(fh)
+        for a in (ls.d.T, ls.d2.T, ls.ubar, ls.ubarp, ls.ubarpp, ls.zpts):   # column-major matrices
+            np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
+        np.array([1e-3, 1e-10, ls.alpha, guess, 0.0], dtype=np.float64).tofile(fh)
     out = tmp_path / "out.bin"
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -55,10 +63,20 @@ def test_fortran_program_through_the_binding(tmp_path):
     assert list(st[:3]) == [0, 0, 0]
     assert st[3] == -1 and "nx" in r.stdout             # LSB_E_ARG with the library's message
     assert st[4] == 14 * 4 + 17 * 8                       # sizeof(lsb_config) seen from Fortran
-    vals = np.frombuffer(raw[20:], dtype=np.float64)
     nzm = nzp - 2
     sizes = [nzp * nzp] * 3 + [n1 * n2 * nzp, nzm, nzm * nzm, nzm * nzm]
+    nb = 20 + 8 * sum(sizes)
+    vals = np.frombuffer(raw[20:nb], dtype=np.float64)
     parts = np.split(vals, np.cumsum(sizes)[:-1])
+    # the Orr-Sommerfeld mode through complex Fortran arguments equals the C / Python call's
+    assert np.frombuffer(raw[nb:nb + 4], dtype=np.int32)[0] == 0
+    osv = np.frombuffer(raw[nb + 4:], dtype=np.complex128)
+    assert osv.size == 1 + 3 * ls.nzp
+    lu, lv, lw, lom = L.solve_orrsomm(ls.d, ls.d2, ls.ubar, ls.ubarp, ls.ubarpp, ls.zpts, 1e-3, 1e-10, ls.alpha,
+                                      0.0, guess, 0)
+    assert osv[0] == lom
+    assert np.array_equal(osv[1:1 + ls.nzp], lu) and np.array_equal(osv[1 + 2 * ls.nzp:], lw)
+    assert not np.any(osv[1 + ls.nzp:1 + 2 * ls.nzp])
     dor, d, d2 = (p.reshape(nzp, nzp, order="F") for p in parts[:3])
     assert np.array_equal(dor, o.dor)
     assert np.max(np.abs(d2 - o.d2)) < 1e-14 * np.max(np.abs(o.d2))

@@ -16,6 +16,7 @@ import pytest
 
 from oracle import Oracle, OracleConfig
 import lsbflow_amd as L
+from lsbflow_amd._lib import LsbError
 from lsbflow_amd.core import setup_case
 from tests.conftest import rel_l2
 
@@ -317,5 +318,76 @@ def test_ener_and_lintest_match_oracle(case, rng):
         for k in (2, 3):
             assert arrs[k][0] == 0.0
             assert np.max(np.abs(arrs[k][1:] - oa[k][1:])) < 1e-6 * np.max(np.abs(oa[k][1:])) + 1e-12
+    finally:
+        c.close()
+
+
+# --------------------------------------------------------- on-the-fly averaging
+@pytest.mark.parametrize("temperature", [False, True])
+def test_time_average_and_qavg_snapshot(case, temperature):
+    """otfavg = 1, istartavg = 3, iomod = 4 (main.f90:69-76, 98-106): the device averages
+    after 9 steps against the oracle's, the qavg record of lsb_state_snapshot_average
+    equal bit for bit to the record of the downloaded averages (name 1: u + ubar,
+    temp + tbar), the averages zero after it, 4 more steps against the oracle again,
+    and zero after a restart read.  With the temperature path on (FP64, buoyancy), the
+    temperature average too.  Bar: the step tests' 1e-10, or 1000 x the oracle's own
+    response to a 1e-15 perturbation of the initial condition where that is larger."""
+    o = oracle_for(case)
+    u, v, w = L.cases.initial_condition(case)
+    T0 = None
+    if temperature:
+        o.cfg.coex, o.cfg.gz = 1.0, -1.0
+        th = np.tanh(o.zpts / 0.2)
+        o.tbar, o.tbarp, o.tbarpp = 1.0 + 0.1 * th, 0.5 * (1 - th ** 2), -2 * 0.5 / 0.2 * th * (1 - th ** 2)
+        T0 = 0.5 * u
+
+    def oracle_run(fields, nsteps, avg, st=None):
+        st = o.new_state(*fields) if st is None else st
+        for _ in range(nsteps):
+            o.timestep(st)
+            o.time_average(st, avg, 3, 4)
+        return st
+
+    zeros = lambda: {k: np.zeros((o.nzp, o.ny, o.nx)) for k in Oracle.AVG_FIELDS}  # noqa: E731
+    ic = [u, v, w] + ([T0] if temperature else [])
+    avg = zeros()
+    st = oracle_run(ic, 9, avg)
+    rng2 = np.random.default_rng(99)
+    avg2 = zeros()
+    st2 = oracle_run([f * (1.0 + 1e-15 * rng2.standard_normal(f.shape)) for f in ic], 9, avg2)
+    keys = Oracle.AVG_FIELDS if temperature else ("uf", "vf", "wf", "pf")
+    c = setup_case(case)
+    try:
+        if temperature:
+            c.set_temperature(o.tbar, o.tbarp, o.tbarpp, 1.0, 0.0, 0.0, -1.0)
+        c.set_time_average(1, 3, 4)
+        c.state_set(u, v, w)
+        if temperature:
+            c.state_set_temperature(T0)
+        c.timestep(9)
+        ga = dict(zip(Oracle.AVG_FIELDS, c.state_get_time_average()))
+        for k in keys:
+            bound = max(1e-10, 1000.0 * rel_l2(avg2[k], avg[k]))
+            assert rel_l2(ga[k], avg[k]) < bound, (k, rel_l2(ga[k], avg[k]), bound)
+        if not temperature:
+            assert not np.any(ga["tempf"])
+        rec = c.state_snapshot_average()
+        want = c.snapshot_record(*[ga[k] for k in Oracle.AVG_FIELDS], name=1, isum=9, t=st["t"], dt=case.dt,
+                                 tbar=o.tbar if temperature else None)
+        assert np.array_equal(rec, want)
+        assert all(not np.any(x) for x in c.state_get_time_average())
+        avg, avg2 = zeros(), zeros()
+        oracle_run(None, 4, avg, st)
+        oracle_run(None, 4, avg2, st2)
+        c.timestep(4)
+        ga = dict(zip(Oracle.AVG_FIELDS, c.state_get_time_average()))
+        for k in keys:
+            bound = max(1e-10, 1000.0 * rel_l2(avg2[k], avg[k]))
+            assert rel_l2(ga[k], avg[k]) < bound, (k, rel_l2(ga[k], avg[k]), bound)
+        c.read_restart(None, c.write_restart(None))
+        assert all(not np.any(x) for x in c.state_get_time_average())
+        c.set_time_average(0, 0, 1)
+        with pytest.raises(LsbError):
+            c.state_snapshot_average()
     finally:
         c.close()

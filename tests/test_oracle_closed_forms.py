@@ -387,3 +387,84 @@ def test_step_projection_is_divergence_free():
     for _ in range(3):
         o.timestep(st)
     assert o.divcheck(st["uf"], st["vf"], st["wf"]) < 1e-10
+
+
+def _interior_solve(o, k2, sigma, rhs):
+    """u with u = 0 at both walls and (d2 - k2 - sigma) u = rhs on the interior rows."""
+    A = o.d2[1:o.nz, 1:o.nz] - (k2 + sigma) * np.eye(o.nzm)
+    u = np.zeros(o.nzp, dtype=np.result_type(rhs, float))
+    u[1:o.nz] = np.linalg.solve(A, rhs[1:o.nz])
+    return u
+
+
+def test_temperature_advection_closed_form():
+    """First (Euler / backward-Euler) step of the temperature equation under a uniform
+    stream U (ubar = U, no perturbation, tbar = 0): the skew-symmetric form
+    -1/2 u.grad T - 1/2 div(u T) (navierstokes.f90:800-845) is -U dT/dx, and
+    (T1 - T0)/dt = kappa (d2 - kx^2) T1 - i kx U T0 with kappa = nu/Pr and T1 = 0 at the
+    walls (navierstokes.f90:1251-1275)."""
+    o = small(nx=16, ny=4, nz=24, bctype=0, xnu=2e-2, dt=5e-3, dealiasflag=0, prtl=0.7)
+    U = 1.3
+    o.ubar = np.full(o.nzp, U)
+    o.ubarp = np.zeros(o.nzp)
+    o.ubarpp = np.zeros(o.nzp)
+    z = o.zpts
+    g = (1 - z ** 2) * (0.4 + z)
+    T0 = o.horfft_fwd(phys(o, lambda x, y, zz: np.cos(2 * x) * ((1 - zz ** 2) * (0.4 + zz))))
+    zero = np.zeros((o.nzp, o.ny, o.nx))
+    st = o.new_state(zero, zero, zero, tempf=T0)
+    o.timestep(st)
+    kx = o.wavx[4]
+    assert kx == 2.0 * 2 * math.pi / o.cfg.xlen
+    c0 = T0[:, 0, 4] + 1j * T0[:, 0, 5]                 # the cos(2x) coefficient
+    assert np.max(np.abs(c0 - 0.5 * g)) < 1e-14
+    kap, dt = o.cfg.xnu / o.cfg.prtl, o.cfg.dt
+    rhs = -(c0 / dt - 1j * kx * U * c0) / kap
+    want = _interior_solve(o, kx * kx, 1.0 / (kap * dt), rhs)
+    got = st["tempf"][:, 0, 4] + 1j * st["tempf"][:, 0, 5]
+    assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    mask = np.ones(st["tempf"].shape, bool)
+    mask[:, 0, 4:6] = False
+    assert np.max(np.abs(st["tempf"][mask])) < 1e-15
+    assert np.max(np.abs(st["uf"])) < 1e-15 and np.max(np.abs(st["wf"])) < 1e-15
+
+
+def test_buoyancy_closed_form():
+    """No flow, a (0, 0)-mode temperature T0(z) and gravity coex*g: after the first step
+    T1 solves (T1 - T0)/dt = kappa d2 T1, and the streamwise velocity responds to the
+    buoyancy force coex*gx*T1: u1/dt = nu d2 u1 + coex gx T1 (navierstokes.f90:1296-1303).
+    The wall-normal force is balanced by the pressure on the (0, 0) mode (w stays 0)."""
+    o = small(nx=8, ny=4, nz=30, bctype=0, xnu=1e-2, dt=1e-2, dealiasflag=0, prtl=0.9,
+              coex=3e-3, gx=2.0, gz=-9.81)
+    z = o.zpts
+    prof = (1 - z ** 2) * np.exp(z)
+    T0 = o.horfft_fwd(phys(o, lambda x, y, zz: (1 - zz ** 2) * np.exp(zz) + 0 * x + 0 * y))
+    assert np.max(np.abs(T0[:, 0, 0] - prof)) < 1e-14
+    zero = np.zeros((o.nzp, o.ny, o.nx))
+    st = o.new_state(zero, zero, zero, tempf=T0)
+    o.timestep(st)
+    kap, nu, dt = o.cfg.xnu / o.cfg.prtl, o.cfg.xnu, o.cfg.dt
+    T1 = _interior_solve(o, 0.0, 1.0 / (kap * dt), -prof / (kap * dt))
+    assert np.max(np.abs(st["tempf"][:, 0, 0] - T1)) < 1e-12 * np.max(np.abs(T1))
+    u1 = _interior_solve(o, 0.0, 1.0 / (nu * dt), -o.cfg.coex * o.cfg.gx * T1 / nu)
+    assert np.max(np.abs(st["uf"][:, 0, 0] - u1)) < 1e-12 * np.max(np.abs(u1))
+    assert np.max(np.abs(st["wf"])) < 1e-12 * np.max(np.abs(u1)) and np.max(np.abs(st["vf"])) == 0
+
+
+def test_temperature_base_profile_source():
+    """A base temperature profile with tbarpp != 0 and no flow: the first step adds the
+    base-profile diffusion, (T1 - 0)/dt = kappa (d2 T1 + tbarpp) (navierstokes.f90:1266-1275,
+    the analogue of ubarpp for the velocity)."""
+    o = small(nx=8, ny=4, nz=30, bctype=0, xnu=1e-2, dt=1e-2, dealiasflag=0, prtl=0.8)
+    z = o.zpts
+    o.tbar = 1.0 + 0.5 * np.tanh(z / 0.3)
+    o.tbarp = 0.5 * (1 - np.tanh(z / 0.3) ** 2) / 0.3
+    o.tbarpp = -2 * np.tanh(z / 0.3) * 0.5 * (1 - np.tanh(z / 0.3) ** 2) / 0.09
+    zero = np.zeros((o.nzp, o.ny, o.nx))
+    st = o.new_state(zero, zero, zero)
+    o.timestep(st)
+    kap, dt = o.cfg.xnu / o.cfg.prtl, o.cfg.dt
+    T1 = _interior_solve(o, 0.0, 1.0 / (kap * dt), -o.tbarpp)
+    got = st["tempf"][:, 0, 0]
+    assert np.max(np.abs(got[1:o.nz] - T1[1:o.nz])) < 1e-12 * np.max(np.abs(T1))
+    assert got[0] == 0 and got[-1] == 0

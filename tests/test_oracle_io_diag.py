@@ -381,3 +381,33 @@ def test_ener_and_lintest_closed_form():
     s = 1.01
     o.lintest(s * uf, zero, zero, 2, dt, enr1, enr2, gr1, gr2)
     assert abs(gr2[1] - (s * s - 1) / (2 * s * s * dt)) < 1e-9 * abs(gr2[1])
+
+
+# --------------------------------------------------------- on-the-fly averaging
+def test_time_average_window_and_scale():
+    """main.f90:69-76: steps whose total count exceeds istartavg add q/iomod, earlier ones
+    add nothing.  Fields that grow linearly with the step (q_n = n a + b) averaged over
+    steps 4..7 with iomod = 4 give 5.5 a + b, the mean of the window; a zero temperature
+    stays zero."""
+    o = Oracle(OracleConfig(nx=8, ny=4, nz=8))
+    rng = np.random.default_rng(11)
+    a = {k: rng.standard_normal((o.nzp, o.ny, o.nx)) for k in Oracle.AVG_FIELDS}
+    b = {k: rng.standard_normal((o.nzp, o.ny, o.nx)) for k in Oracle.AVG_FIELDS}
+    a["tempf"][:] = 0.0
+    b["tempf"][:] = 0.0
+    avg = {k: np.zeros((o.nzp, o.ny, o.nx)) for k in Oracle.AVG_FIELDS}
+    st = {"istep": 0}
+    for n in range(1, 8):
+        st["istep"] = n
+        st.update({k: n * a[k] + b[k] for k in Oracle.AVG_FIELDS})
+        o.time_average(st, avg, istartavg=3, iomod=4)
+    for k in Oracle.AVG_FIELDS:
+        want = 5.5 * a[k] + b[k]
+        assert np.max(np.abs(avg[k] - want)) < 1e-14 * (1 + np.max(np.abs(want))), k
+    assert not np.any(avg["tempf"])
+    # the count that decides is its (the total), not the run's step: a restarted run at
+    # isum = 100 with istartavg = 99 averages from its first step
+    avg2 = {k: np.zeros((o.nzp, o.ny, o.nx)) for k in Oracle.AVG_FIELDS}
+    st["istep"] = 1
+    o.time_average(st, avg2, istartavg=99, iomod=2, its=100)
+    assert np.array_equal(avg2["uf"], st["uf"] / 2)
