@@ -402,8 +402,10 @@ int zcontract_ex(lsb_ctx* c, const double* M, int ldm, int nout, int nin, const 
                  double* const* out, int nf, long P, const double* const* aux, hipStream_t s);
 // Helmholtz solves in place: rhs_f (nzp planes); bc_f may be null (homogeneous).
 // sigma = dtnui (constant over fields).
+// split32: the mixed form's velocity solves (FP64 leading term, FP32 correction) when
+// sigma dominates the eigenvalues; otherwise the FP64 projections
 int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top,
-              const double* const* bc_bot, int nf, double sigma, hipStream_t s);
+              const double* const* bc_bot, int nf, double sigma, hipStream_t s, bool split32 = false);
 
 // spectral pointwise (spectral.hip)
 int sp_dfdx(lsb_ctx* c, const double* in, double* out, hipStream_t s);

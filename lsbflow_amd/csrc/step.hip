@@ -975,7 +975,8 @@ static int update_once(lsb_ctx* c) {
         double* rh[3] = {L.spec[S->rhs[0]], L.spec[S->rhs[1]], L.spec[S->rhs[2]]};
         const double* bt[3] = {S->bct[0], S->bct[1], S->bct[2]};
         const double* bb[3] = {S->bcb[0], S->bcb[1], S->bcb[2]};
-        LSB_TRY(helmholtz(c, rh, vel_nhom ? bt : nullptr, vel_nhom ? bb : nullptr, 3, dtnui / c->bdfv, s));
+        LSB_TRY(helmholtz(c, rh, vel_nhom ? bt : nullptr, vel_nhom ? bb : nullptr, 3, dtnui / c->bdfv, s,
+                          mixed_form(c)));
     }
     // pressure step (navierstokes.f90:1411-1464)
     double* uN = L.spec[S->rhs[0]];

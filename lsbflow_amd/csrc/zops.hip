@@ -70,6 +70,9 @@ struct ZArgs {
     double pdt, pbdfv;  // ZM_PROJ: dt and bdfv
     const double* M3;   // k_zhelm: stage-2 P and Q (stage 1 in M, M2)
     const double* M4;
+    const float* M32b;  // k_zhelm32: FP32 stage-2 P, Q at M32b + q32b (stage 1 in M32, q32)
+    long q32b;
+    const double* eel;  // k_zhelm32: E applied to the four Einv-lifting vectors (4*nzm, physical rows)
 };
 
 constexpr int kZRB = 3;   // row blocks (of 16) per wave
@@ -1159,6 +1162,273 @@ k_zhelm(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double>
 // 49 KB LDS tile per block leaves room for three blocks per CU)
 constexpr int kZHRB = 2, kZHCB = 2;
 
+// The velocity solves of the mixed form: with s = k^2 + sigma much larger than every
+// |lam| (sigma = 1/(nu dt bdfv)),
+//   1/(lam - s) = -1/s + c(lam),   c(lam) = lam / (s (lam - s)),   0 <= s c(lam) < 1,
+// so with q = Einv r - L (L the lifting terms of the mode values) the solution is
+//   u = E diag(1/(lam - s)) q + h bc = (r - E L)/(-s) + E diag(c) q + h bc.
+// The first term is formed in FP64 from the right-hand side and the precomputed E L
+// vectors (E Einv = I); only the correction E diag(c) q, a fraction max|lam|/s of the
+// solution, goes through the two FP32 parity-split projections (v_mfma_f32_16x16x4f32,
+// twice the FP64 rate), so their rounding enters the solution scaled by that fraction.
+// Stage 1 leaves c q in an FP32 LDS tile; stage 2 re-reads each right-hand-side value
+// just before overwriting it (the block owns its pencils; stage 1's reads are behind the
+// barrier).
+template <int RB, int CB, int PD, int WPE>
+__global__ void __launch_bounds__(64 * kZMaxWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zhelm32(ZArgs a, FieldList<double> fld, FieldList<double> bct, FieldList<double> bcb) {
+    extern __shared__ float Tf[];  // n rows x W pencils
+    constexpr int W = 16 * CB;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int f = blockIdx.z;
+    const int me = a.nout, n = a.nsp, mo = n - me;
+    const int nrb = (me + 15) >> 4;
+    const int base = nrb / nw, extra = nrb % nw;
+    const int myn = base + (wave < extra ? 1 : 0);
+    const int rb0 = wave * base + min(wave, extra);
+    const long P = a.P;
+    const long p0 = (long)blockIdx.x * W;
+    long cj[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) cj[j] = min(p0 + 16 * j + li, P - 1);
+    double* __restrict__ Fq = fld.p[f];
+    double* __restrict__ X = Fq + P;  // interior rows 1..nz: read in stage 1, overwritten in stage 2
+    const unsigned xstep = 4u * (unsigned)P, mstep = 4u * (unsigned)a.ldm;
+    unsigned ot[CB], ob[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        ot[j] = (unsigned)lk * (unsigned)P + (unsigned)cj[j];
+        ob[j] = (unsigned)(n - 1 - lk) * (unsigned)P + (unsigned)cj[j];
+    }
+    const unsigned om = (unsigned)(rb0 * 16 + li + lk * a.ldm);
+    const int nks = (me + 3) >> 2;
+    const int nkp = (nks + PD - 1) / PD * PD;
+    const int kmax = (n - 4) >> 2;
+    double dfac[CB], vbt[CB], vbb[CB];
+    const bool nhom = bct.p[f] != nullptr;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        const int c = (int)cj[j];
+        const double wx = a.wavx[c % a.nx], wy = a.wavy[c / a.nx];
+        dfac[j] = -(wx * wx) - wy * wy - a.sigma;
+        vbt[j] = nhom ? bct.p[f][c] : 0.0;
+        vbb[j] = nhom ? bcb.p[f][c] : 0.0;
+    }
+    double rdf1[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) rdf1[j] = 1.0 / dfac[j];
+    v4f accE[RB][CB], accO[RB][CB];
+    float fa[PD][RB], fq[PD][RB];
+    // ---------------------------------------------------------------- stage 1
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+            accO[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        }
+    {
+        const float* __restrict__ Mp = a.M32;
+        const float* __restrict__ Mq = a.M32 + a.q32;
+        double fx[PD][CB], fy[PD][CB];
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+            const unsigned xk = (unsigned)min(k, kmax) * xstep;
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(Mp, om + mk + 16u * i);
+                fq[s][i] = ldo(Mq, om + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = ldo((const double*)X, ot[j] + xk);
+                fy[s][j] = ldo((const double*)X, ob[j] - xk);
+            }
+            asm volatile("" ::: "memory");  // keep the prefetch in place (see k_zsplit)
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+                float sv[CB], dv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    sv[j] = (float)(fx[s][j] + fy[s][j]);
+                    dv[j] = (float)(fx[s][j] - fy[s][j]);
+                }
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        accE[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[s][i], sv[j], accE[i][j], 0, 0, 0);
+                        accO[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fq[s][i], dv[j], accO[i][j], 0, 0, 0);
+                    }
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // stage-1 epilogue into LDS: c(lam) (Einv r - L); even mode r -> row r, odd mode r -> row n-1-r
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + Z32_ROW(lk, v);
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+                        const double lam = a.lam[row];
+                        double e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
+                        if (nhom) {
+                            e1 = a.elift[row];
+                            e2 = a.elift[n + row];
+                            e3 = a.elift[2 * n + row];
+                            e4 = a.elift[3 * n + row];
+                        }
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            double x = (double)(half ? accO[i][j][v] : accE[i][j][v]);
+                            if (nhom) {
+                                x = x - vbb[j] * (e3 + dfac[j] * e1);
+                                x = x - vbt[j] * (e4 + dfac[j] * e2);
+                            }
+                            // c(lam) with s = -dfac: lam / (s (lam - s)) = -(t / dfac) / (1 + t),
+                            // t = lam / dfac in [0, 1/16]; it scales an FP32 value, so FP32 suffices
+                            const float t = (float)(lam * rdf1[j]);
+                            const float cl = -((float)rdf1[j] * t) / (1.0f + t);
+                            Tf[row * W + 16 * j + li] = (float)x * cl;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---------------------------------------------------------------- stage 2
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            accE[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+            accO[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        }
+    {
+        const float* __restrict__ Mp = a.M32b;
+        const float* __restrict__ Mq = a.M32b + a.q32b;
+        float fx[PD][CB], fy[PD][CB];
+        auto load = [&](int s, int k) {
+            const unsigned mk = (unsigned)min(k, nkp - 1) * mstep;
+            const int kr = min(4 * k + lk, me - 1);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                fa[s][i] = ldo(Mp, om + mk + 16u * i);
+                fq[s][i] = ldo(Mq, om + mk + 16u * i);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                fx[s][j] = Tf[kr * W + 16 * j + li];
+                fy[s][j] = Tf[(n - 1 - kr) * W + 16 * j + li];
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+        };
+#pragma unroll
+        for (int s = 0; s < PD; ++s) load(s, s);
+        for (int k0 = 0; k0 < nkp; k0 += PD) {
+#pragma unroll
+            for (int s = 0; s < PD; ++s) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) {
+                        accE[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[s][i], fx[s][j], accE[i][j], 0, 0, 0);
+                        accO[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fq[s][i], fy[s][j], accO[i][j], 0, 0, 0);
+                    }
+                load(s, k0 + s + PD);
+            }
+        }
+    }
+    // stage-2 epilogue: physical rows r and n-1-r; the FP64 part (r - E L)/dfac + h bc
+    // plus the FP32 correction.  The right-hand-side values of one row block (both
+    // halves) are loaded together before any of them is overwritten, so the loads are
+    // in flight at once rather than each waiting behind the previous store.
+    const double* rdf = rdf1;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        if (i < myn) {
+            double rv[2][4][CB];
+#pragma unroll
+            for (int half = 0; half < 2; ++half)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min((rb0 + i) * 16 + Z32_ROW(lk, v), me - 1);
+                    const int row = half ? n - 1 - r : r;
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) rv[half][v][j] = X[(long)row * P + cj[j]];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (rb0 + i) * 16 + Z32_ROW(lk, v);
+                    if (r < me && (half == 0 || r < mo)) {
+                        const int row = half ? n - 1 - r : r;
+                        double h0 = 0.0, h1 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0;
+                        if (nhom) {
+                            h0 = a.lift[row + 1];
+                            h1 = a.lift[a.nzp + row + 1];
+                            g1 = a.eel[row];
+                            g2 = a.eel[n + row];
+                            g3 = a.eel[2 * n + row];
+                            g4 = a.eel[3 * n + row];
+                        }
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const long col = p0 + 16 * j + li;
+                            const float e = accE[i][j][v], o = accO[i][j][v];
+                            const double corr = (double)(half ? (e - o) : e + o);
+                            double y = rv[half][v][j];
+                            if (nhom) {
+                                y = y - vbb[j] * (g3 + dfac[j] * g1);
+                                y = y - vbt[j] * (g4 + dfac[j] * g2);
+                            }
+                            double x = y * rdf[j] + corr;
+                            if (nhom) x = x + h0 * vbb[j] + h1 * vbt[j];
+                            if (col < P) X[(long)row * P + col] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (wave == 0 && lk == 0) {
+        // boundary planes 0 and nz
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const long col = p0 + 16 * j + li;
+            double v0 = 0.0, vn = 0.0;
+            if (nhom) {
+                v0 = 0.0 + a.lift[0] * vbb[j] + a.lift[a.nzp + 0] * vbt[j];
+                vn = 0.0 + a.lift[a.nzp - 1] * vbb[j] + a.lift[a.nzp + a.nzp - 1] * vbt[j];
+            }
+            if (col < P) {
+                Fq[col] = v0;
+                Fq[(long)(a.nzp - 1) * P + col] = vn;
+            }
+        }
+    }
+}
+// two row blocks x three column blocks per wave set (measured against 2-4 x 2-3 on the
+// 512 x 193 x 256 step: 2 x 3 fastest; its 37 KB FP32 tile leaves room for four blocks per CU)
+constexpr int kZH32RB = 2, kZH32CB = 3;
+
 // The pressure step of navierstokes.f90:1411-1488 for a block's 16 CB pencils in
 // one kernel, the intermediate field of each stage in one LDS tile (nzp rows):
 //   A  r = -d(i kx u + i ky v)/dz - k^2 w          (dfdz split, div_h in the loads)
@@ -1781,8 +2051,15 @@ int zsplit_setup(lsb_ctx* c, const std::vector<double>& elift) {
             }
         if (err > 1e-9) return LSB_OK;
     }
-    // eigenvalues and Einv-lifting data in parity-ordered rows
-    std::vector<double> hs((size_t)5 * n, 0.0);
+    // eigenvalues and Einv-lifting data in parity-ordered rows, then E applied to the
+    // four Einv-lifting vectors in physical rows (the FP64 part of the mixed form's solve)
+    std::vector<double> hs((size_t)9 * n, 0.0);
+    for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += cm(E, n, i, k) * elift[(size_t)q * n + k];
+            hs[(size_t)5 * n + (size_t)q * n + i] = acc;
+        }
     for (int a = 0; a < me; ++a) {
         hs[a] = c->diag[ev[a]];
         for (int q = 0; q < 4; ++q) hs[(size_t)n + (size_t)q * n + a] = elift[(size_t)q * n + ev[a]];
@@ -2063,7 +2340,7 @@ int zpress_step(lsb_ctx* c, double* u, double* v, const double* w, double* wout,
 // In-place Helmholtz/Poisson solves of nf fields (interior rows 1..nz-1), with
 // optional non-homogeneous Dirichlet data bc_top (plane 0) / bc_bot (plane nzp-1).
 int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const double* const* bc_bot, int nf,
-              double sigma, hipStream_t s) {
+              double sigma, hipStream_t s, bool split32) {
     if (nf < 1 || nf > kMaxFields) {
         set_error("helmholtz: 1..16 fields per launch");
         return LSB_E_ARG;
@@ -2101,7 +2378,7 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
     const double bytes = 2.0 * c->nzp * (double)a.P * nf * sizeof(double);
     if (zs_fits(c->zs_h1, a.P) && zs_fits(c->zs_h2, a.P) && c->zs_h1.ldm == c->zs_h2.ldm) {
         // both parity-split projections in one kernel, the mode values in LDS
-        prof_mark_begin(c, PC_HELM, s, &tok, 2.0 * zsplit_flops(c->zs_h1, (double)a.P) * nf, bytes);
+        const double hflops = 2.0 * zsplit_flops(c->zs_h1, (double)a.P) * nf;
         a.lam = c->d_helm_s;
         a.elift = c->d_helm_s + nzm;
         int gy = 1, nw = 1;
@@ -2112,6 +2389,30 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
         dim3 grid((unsigned)((a.P + 16 * kZHCB - 1) / (16 * kZHCB)), 1, (unsigned)nf);
         const char* off = std::getenv("LSB_NO_ZHELM");  // the two-kernel form, for tests
         const bool ok = gy == 1 && lds <= 64 * 1024 && !(off && off[0] && std::strcmp(off, "0") != 0);
+        if (ok && split32) {
+            // the FP32-correction form needs sigma to dominate every eigenvalue: then
+            // s c(lam) <= max|lam| / sigma <= 1/16 scales the FP32 rounding
+            double lmax = 0.0;
+            for (int k = 0; k < nzm; ++k) lmax = std::max(lmax, std::fabs(c->diag[k]));
+            int gy32 = 1, nw32 = 1;
+            ZArgs b = a;
+            zsplit_shape(b, c->zs_h1, false, kZH32RB, &gy32, &nw32);
+            b.M32b = c->zs_h2.d_m32;
+            b.q32b = (long)c->zs_h2.qoff;
+            b.eel = c->d_helm_s + 5 * nzm;
+            const size_t lds32 = (size_t)nzm * 16 * kZH32CB * sizeof(float);
+            if (sigma >= 16.0 * lmax && gy32 == 1 && lds32 <= 64 * 1024) {
+                // profiled with the FP32 contractions (its projections run on the FP32 matrix cores)
+                prof_mark_begin(c, PC_Z, s, &tok, hflops, bytes);
+                dim3 grid32((unsigned)((a.P + 16 * kZH32CB - 1) / (16 * kZH32CB)), 1, (unsigned)nf);
+                hipLaunchKernelGGL((k_zhelm32<kZH32RB, kZH32CB, 3, 2>), grid32, dim3(64 * nw32), lds32, s, b, fr,
+                                   bt, bb);
+                prof_mark_end(c, tok, s);
+                LSB_HIP(hipGetLastError());
+                return LSB_OK;
+            }
+        }
+        prof_mark_begin(c, PC_HELM, s, &tok, hflops, bytes);
         if (ok) hipLaunchKernelGGL((k_zhelm<kZHRB, kZHCB, 3, 3>), grid, dim3(64 * nw), lds, s, a, fr, bt, bb);
         if (ok) {
             prof_mark_end(c, tok, s);
