@@ -6,6 +6,7 @@
 // step passes aliased pointers for symmetric tensors (same values, fewer
 // fields).
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 
@@ -115,27 +116,35 @@ __device__ __forceinline__ double sigma_nu(const double g[3][3], double delta) {
 // navierstokes.f90:725-769: base flow added, nlt_l = -1/2 sum_m u_m du_l/dx_m,
 // uu_lm = u_l u_m, tau from the SGS model, h_lm = 1/2 uu_lm + tau_lm.
 // Arithmetic in FP64 whatever the storage type T.
-template <class T, int MODE>
-__global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
-    PH_LOOP(d) {
-        int i, j, k;
-        long off;
-        ph_index(d, e, i, j, k, off);
+// Element access by byte offset: with a 32-bit offset the loads and stores take the
+// scalar-base + vector-offset form (no 64-bit address arithmetic per field).
+template <class T, class Off>
+__device__ __forceinline__ T ldb(const T* p, Off boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
+}
+template <class T, class Off>
+__device__ __forceinline__ void stb(T* p, Off boff, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + boff) = v;
+}
+
+template <class T, int MODE, class Off>
+__device__ __forceinline__ void nl_point(const NlArgsT<T>& a, int i, int k, Off bo) {
+    {
         double u[3], g[3][3];
-        const double up0 = (double)a.ui[0][off];
+        const double up0 = (double)ldb(a.ui[0], bo);
         u[0] = up0 + a.ubar[k];
-        u[1] = (double)a.ui[1][off];
-        u[2] = (double)a.ui[2][off];
+        u[1] = (double)ldb(a.ui[1], bo);
+        u[2] = (double)ldb(a.ui[2], bo);
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) g[l][m] = (double)a.g[l + 3 * m][off];
+            for (int m = 0; m < 3; ++m) g[l][m] = (double)ldb(a.g[l + 3 * m], bo);
         g[0][2] = g[0][2] + a.ubarp[k];
         if (a.write_base) {
-            a.uo[0][off] = (T)u[0];
-            a.uo[1][off] = (T)u[1];
-            a.uo[2][off] = (T)u[2];
-            a.go[0 + 3 * 2][off] = (T)g[0][2];
+            stb(a.uo[0], bo, (T)u[0]);
+            stb(a.uo[1], bo, (T)u[1]);
+            stb(a.uo[2], bo, (T)u[2]);
+            stb(a.go[0 + 3 * 2], bo, (T)g[0][2]);
         }
         // sponge (navierstokes.f90:190-200, 750-756): F = lambda(x) (ubar - u, -v, -w)
         const double lam = a.lam ? a.lam[i] : 0.0;
@@ -146,7 +155,7 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * g[l][m];
             if (a.lam) s = s + lam * (tgt[l] - u[l]);
-            a.nlt[l][off] = (T)s;
+            stb(a.nlt[l], bo, (T)s);
         }
         double tau[3][3];
         if (MODE == NL_SIGMA) {
@@ -174,7 +183,7 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
 #pragma unroll
-                for (int m = 0; m < 3; ++m) tau[l][m] = a.tau[l + 3 * m] ? (double)a.tau[l + 3 * m][off] : 0.0;
+                for (int m = 0; m < 3; ++m) tau[l][m] = a.tau[l + 3 * m] ? (double)ldb(a.tau[l + 3 * m], bo) : 0.0;
         } else {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
@@ -187,7 +196,7 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
             for (int l = 0; l < 3; ++l)
 #pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    if (a.h[l + 3 * m]) a.h[l + 3 * m][off] = (T)(u[l] * u[m]);
+                    if (a.h[l + 3 * m]) stb(a.h[l + 3 * m], bo, (T)(u[l] * u[m]));
         } else {
 #pragma unroll
             for (int l = 0; l < 3; ++l)
@@ -197,9 +206,34 @@ __global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
                         double hv = 0.5 * (u[l] * u[m]) + tau[l][m];
                         // u0^2 - ubar^2 = up0 (u0 + ubar): the perturbation part without cancellation
                         if (l == 0 && m == 0 && a.sub_base) hv = 0.5 * (up0 * (u[0] + a.ubar[k])) + tau[0][0];
-                        a.h[l + 3 * m][off] = (T)hv;
+                        stb(a.h[l + 3 * m], bo, (T)hv);
                     }
         }
+    }
+}
+
+// One x-row per block row: blockIdx.y strides over the rows k*ny + j, so k and the row
+// offset are uniform (scalar) and each thread computes one 32-bit byte offset.
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_nonlinear(PhDims d, NlArgsT<T> a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const unsigned bo = ((unsigned)row * (unsigned)d.nxpp + (unsigned)i) * (unsigned)sizeof(T);
+        nl_point<T, MODE>(a, i, k, bo);
+    }
+}
+
+// the same for fields of 4 GiB and more (64-bit offsets)
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_nonlinear_big(PhDims d, NlArgsT<T> a) {
+    PH_LOOP(d) {
+        int i, j, k;
+        long off;
+        ph_index(d, e, i, j, k, off);
+        nl_point<T, MODE>(a, i, k, (size_t)off * sizeof(T));
     }
 }
 
@@ -609,15 +643,23 @@ static int launch_nonlinear(lsb_ctx* c, int mode, const NlArgsT<T>& a, hipStream
     int nh = 0;
     for (int q = 0; q < 9; ++q) nh += a.h[q] ? 1 : 0;
     prof_mark_begin(c, PC_NL, s, &tok, 0.0, (double)(15 + nh) * d.npts * sizeof(T));
-    const dim3 g = ph_grid(d.npts);
+    const bool small = (size_t)d.nxpp * d.ny * d.nzp * sizeof(T) < ((size_t)1 << 32);
+    const int nrows = d.ny * d.nzp;
+    const dim3 g = small ? dim3((unsigned)((d.nx + 255) / 256), (unsigned)std::min(nrows, 65535)) : ph_grid(d.npts);
+#define LSB_NL_LAUNCH(M)                                                                   \
+    if (small)                                                                             \
+        hipLaunchKernelGGL((k_nonlinear<T, M>), g, dim3(256), 0, s, d, a);                 \
+    else                                                                                   \
+        hipLaunchKernelGGL((k_nonlinear_big<T, M>), g, dim3(256), 0, s, d, a);
     switch (mode) {
-        case NL_NONE: hipLaunchKernelGGL((k_nonlinear<T, NL_NONE>), g, dim3(256), 0, s, d, a); break;
-        case NL_SIGMA: hipLaunchKernelGGL((k_nonlinear<T, NL_SIGMA>), g, dim3(256), 0, s, d, a); break;
-        case NL_STATIC: hipLaunchKernelGGL((k_nonlinear<T, NL_STATIC>), g, dim3(256), 0, s, d, a); break;
-        case NL_EXTTAU: hipLaunchKernelGGL((k_nonlinear<T, NL_EXTTAU>), g, dim3(256), 0, s, d, a); break;
-        case NL_PREP: hipLaunchKernelGGL((k_nonlinear<T, NL_PREP>), g, dim3(256), 0, s, d, a); break;
+        case NL_NONE: LSB_NL_LAUNCH(NL_NONE) break;
+        case NL_SIGMA: LSB_NL_LAUNCH(NL_SIGMA) break;
+        case NL_STATIC: LSB_NL_LAUNCH(NL_STATIC) break;
+        case NL_EXTTAU: LSB_NL_LAUNCH(NL_EXTTAU) break;
+        case NL_PREP: LSB_NL_LAUNCH(NL_PREP) break;
         default: set_error("nonlinear: bad mode"); return LSB_E_ARG;
     }
+#undef LSB_NL_LAUNCH
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
