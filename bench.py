@@ -123,6 +123,72 @@ def measured_traffic(case, cls, precision, p, steps):
     return float(f"{d['bytes_per_step'] * steps / p['n']:.4g}")
 
 
+def traffic_class(name):
+    """Profiling class of a kernel of the step (the classes lsb_profile_end reports)."""
+    if "k_yfwd_rhs" in name or "k_rhs_vel" in name:
+        return "rhs"
+    if "_fast<" in name:
+        return "fft"
+    if "k_zhelm32" in name:       # the mixed form's velocity solves: FP32 matrix work
+        return "zcontract"
+    if any(t in name for t in ("k_zcontract<1>", "k_zcontract<2>", "k_zsplit<1,", "k_zsplit<2,", "k_zhelm",
+                               "k_zpress")):
+        return "helmholtz"
+    if "zcontract" in name or "zsplit" in name:
+        return "zcontract"
+    if "k_nonlinear" in name:
+        return "nonlinear"
+    if name.startswith("lsb::"):
+        return "pointwise"
+    return None
+
+
+def pmc_summarize(fetch_csv, write_csv, out_path=TRAFFIC_FILE, steps_per_form=2):
+    """HBM bytes per kernel and per class and step from the FETCH_SIZE and WRITE_SIZE runs of
+    --pmc-probe: each counter scaled by what it reads for the 1 GiB copy; the FP64 form's
+    kernels are those up to its last step's final k_norm_nyq, the mixed form's follow."""
+    import csv
+
+    def load(fn, counter):
+        rows = [r for r in csv.DictReader(open(fn)) if r["Counter_Name"] == counter]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return [(r["Kernel_Name"].split("(")[0].replace("void ", ""), float(r["Counter_Value"])) for r in rows]
+
+    F, W = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    if len(F) != len(W) or any(a[0] != b[0] for a, b in zip(F, W)):
+        raise RuntimeError("pmc_summarize: the two runs dispatched different kernels")
+    ci = next(i for i, (n, v) in enumerate(F) if "copyBuffer" in n and v > 1e5)
+    fetch_cal = COPY_BYTES / (F[ci][1] * 1024.0)
+    write_cal = COPY_BYTES / (W[ci][1] * 1024.0)
+    ends = [i for i, (n, _) in enumerate(F) if "k_norm_nyq" in n]
+    last_fp64 = ends[steps_per_form - 1]
+    out = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": F[ci][1], "write_kib": W[ci][1],
+                           "fetch_factor": fetch_cal, "write_factor": write_cal},
+           "probe_steps_per_form": steps_per_form, "classes": {"fp64": {}, "mixed": {}}, "kernels": {}}
+    for i, ((n, f), (_, w)) in enumerate(zip(F, W)):
+        if i <= ci:
+            continue
+        form = "fp64" if i <= last_fp64 else "mixed"
+        rb, wb = f * 1024 * fetch_cal, w * 1024 * write_cal
+        c = traffic_class(n)
+        if c:
+            d = out["classes"][form].setdefault(c, {"read_bytes_per_step": 0.0, "write_bytes_per_step": 0.0})
+            d["read_bytes_per_step"] += rb / steps_per_form
+            d["write_bytes_per_step"] += wb / steps_per_form
+        k = out["kernels"].setdefault(form + ":" + n, {"dispatches": 0, "read_bytes": 0.0, "write_bytes": 0.0})
+        k["dispatches"] += 1
+        k["read_bytes"] += rb
+        k["write_bytes"] += wb
+    for form in ("fp64", "mixed"):
+        for d in out["classes"][form].values():
+            d["bytes_per_step"] = d["read_bytes_per_step"] + d["write_bytes_per_step"]
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps({"pmc_summary": out_path, "classes": {f: {c: f"{d['bytes_per_step']:.4g}" for c, d in
+                                                                out["classes"][f].items()} for f in out["classes"]}}))
+    return 0
+
+
 def roofline(prof, precision, case=None, steps=1):
     """Dominant class by device time -> achieved vs peak (algorithmic work / measured time)."""
     cls = max(prof, key=lambda k: prof[k]["ms"])
@@ -249,9 +315,14 @@ def main(argv=None):
     ap.add_argument("--diagnostics", action="store_true",
                     help="time the snapshot / restart records and the per-step diagnostics (maxcalc, NaN "
                          "scan, wall shear) on the resident state of the workload instead of the step")
+    ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the HBM-traffic table bench.py reads (profiles/..._hbm_traffic.json) from the "
+                         "counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE runs of --pmc-probe")
     ap.add_argument("--cpu-sample-ny", type=int, default=16)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     args = ap.parse_args(argv)
+    if args.pmc_summarize:
+        return pmc_summarize(*args.pmc_summarize)
     if args.gpus != 1:
         print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
               "(use --gpus 1)", file=sys.stderr)

@@ -177,6 +177,19 @@ int lsb_setup_dealiasing(lsb_ctx* ctx, int* nx_lg, int* ny_lg);
 /* dealiasing.f90:262-447 dealiasing_2d_fftw(a, b, ab): padded product of two
  * physical fields (ref_compat placement per lsb_config.dealias_ref_compat). */
 int lsb_dealiasing_2d_fftw(lsb_ctx* ctx, const double* a, const double* b, double* ab);
+/* dealiasing.f90:449-618 dealiasf_2d_fftw(af, bf, abf): the same padded product of two
+ * spectral fields (nxpl, ny, nzp), spectral out; the reference's ky placement as above. */
+int lsb_dealiasf_2d_fftw(lsb_ctx* ctx, const double* af, const double* bf, double* abf);
+/* horfft.f90:7-58 test_fft(u, avgerror): forward and inverse 2-D transform of the physical
+ * field u (nxpp, ny, nzp) on the device; avgerror = mean |u - q| over the nx physical
+ * points of every row. */
+int lsb_test_fft(lsb_ctx* ctx, const double* u, double* avgerror);
+/* derivatives.f90:9-70 zinteg(tmp, tint, n1, n2, mapping, per, rev): cumulative
+ * wall-normal integral of every pencil of tmp (n1, n2, nzp) on the device -- per = 0 the
+ * trapezoid rule on zpts (rev = 0 from the top plane nzp down, rev = 1 from plane 1 up),
+ * per = 1 the interior quadrature weights with the change of variable 0.5 xl / gf;
+ * mapping = 1 copies plane 2 to plane 1 (rev = 0 / per = 1). */
+int lsb_zinteg(lsb_ctx* ctx, const double* tmp, double* tint, int n1, int n2, int mapping, int per, int rev);
 
 /* ------------------------------------------------- tier A: step routines */
 

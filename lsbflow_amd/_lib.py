@@ -90,6 +90,9 @@ _SIGS = {
     "lsb_snapshot_record": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
                                       _fp]),
     "lsb_state_snapshot": (C.c_int, [_vp, C.c_int, C.c_char_p, _fp]),
+    "lsb_dealiasf_2d_fftw": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "lsb_test_fft": (C.c_int, [_vp, _dp, _dp]),
+    "lsb_zinteg": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "lsb_set_time_average": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "lsb_state_snapshot_average": (C.c_int, [_vp, C.c_char_p, _fp]),
     "lsb_state_get_time_average": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),

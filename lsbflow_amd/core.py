@@ -454,6 +454,26 @@ class Context:
               "state_snapshot")
         return rec
 
+    def dealiasf_2d_fftw(self, af, bf):
+        """dealiasing.f90:449-618: padded product of two spectral fields, spectral out."""
+        out = np.zeros(self.sshape)
+        check(self._L.lsb_dealiasf_2d_fftw(self.h, _p(_f64(af)), _p(_f64(bf)), _p(out)), "dealiasf_2d_fftw")
+        return out
+
+    def test_fft(self, u):
+        """horfft.f90:7-58: mean |u - inverse(forward(u))| over the physical points."""
+        e = C.c_double()
+        check(self._L.lsb_test_fft(self.h, _p(_f64(u)), C.byref(e)), "test_fft")
+        return e.value
+
+    def zinteg(self, tmp, mapping=None, per=0, rev=0):
+        """derivatives.f90:9-70 on tmp (nzp, n2, n1) (numpy order of Fortran (n1, n2, nzp))."""
+        t = _f64(tmp)
+        out = np.zeros(t.shape)
+        mp = self.cfg.mapping if mapping is None else mapping
+        check(self._L.lsb_zinteg(self.h, _p(t), _p(out), t.shape[2], t.shape[1], mp, per, rev), "zinteg")
+        return out
+
     def set_time_average(self, otfavg=1, istartavg=0, iomod=1):
         """main.f90:69-76: on-the-fly averages of uf, vf, wf, tempf, pf over the steps of
         timestep() once the total step count exceeds istartavg (q/iomod per step)."""

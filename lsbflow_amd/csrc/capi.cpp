@@ -983,6 +983,94 @@ extern "C" int lsb_dealiasing_2d_fftw(lsb_ctx* c, const double* a, const double*
     return st.sync();
 }
 
+extern "C" int lsb_dealiasf_2d_fftw(lsb_ctx* c, const double* af, const double* bf, double* abf) {
+    LSB_ENTER(c);
+    if (!af || !bf || !abf) {
+        set_error("dealiasf_2d_fftw: null array");
+        return LSB_E_ARG;
+    }
+    LSB_TRY(dealias_setup(c));
+    double *da, *db, *dab;
+    LSB_TRY(st.get(0, c->nspec, &da));
+    LSB_TRY(st.get(1, c->nspec, &db));
+    LSB_TRY(st.get(2, c->nspec, &dab));
+    LSB_TRY(st.in(da, af, c->nspec));
+    LSB_TRY(st.in(db, bf, c->nspec));
+    LSB_TRY(dealias_product_spec(c, da, db, dab, c->stream));
+    LSB_TRY(st.out(abf, dab, c->nspec));
+    return st.sync();
+}
+
+extern "C" int lsb_test_fft(lsb_ctx* c, const double* u, double* avgerror) {
+    LSB_ENTER(c);
+    if (!u || !avgerror) {
+        set_error("test_fft: null argument");
+        return LSB_E_ARG;
+    }
+    double *dq, *dqf;
+    LSB_TRY(st.get(0, c->nphys, &dq));
+    LSB_TRY(st.get(1, c->nspec, &dqf));
+    LSB_TRY(st.in(dq, u, c->nphys));
+    {
+        const double* in[1] = {dq};
+        double* out[1] = {dqf};
+        LSB_TRY(horfft_fwd(c, c->hp, in, out, 1, c->nzp, c->stream));
+    }
+    {
+        const double* in[1] = {dqf};
+        double* out[1] = {dq};
+        LSB_TRY(horfft_inv(c, c->hp, in, out, 1, c->nzp, c->stream));
+    }
+    std::vector<double> q(c->nphys);
+    LSB_TRY(st.out(q.data(), dq, c->nphys));
+    LSB_TRY(st.sync());
+    // horfft.f90:26-33: the mean absolute round-trip error over the nx physical points
+    const double npts = (double)c->nx * c->ny * c->nzp;
+    double e = 0.0;
+    for (int k = 0; k < c->nzp; ++k)
+        for (int j = 0; j < c->ny; ++j)
+            for (int i = 0; i < c->nx; ++i) {
+                const size_t o = ((size_t)k * c->ny + j) * c->nxpp + i;
+                e = e + std::fabs(u[o] - q[o]) / npts;
+            }
+    *avgerror = e;
+    return LSB_OK;
+}
+
+extern "C" int lsb_zinteg(lsb_ctx* c, const double* tmp, double* tint, int n1, int n2, int mapping, int per,
+                          int rev) {
+    LSB_ENTER(c);
+    if (!tmp || !tint || n1 < 1 || n2 < 1 || (per != 0 && per != 1) || (rev != 0 && rev != 1)) {
+        set_error("zinteg: need arrays (n1, n2, nzp), n1, n2 >= 1, per and rev 0 or 1");
+        return LSB_E_ARG;
+    }
+    const size_t n = (size_t)n1 * n2 * c->nzp;
+    double *dq, *dt, *dw = nullptr;
+    LSB_TRY(st.get(0, n, &dq));
+    LSB_TRY(st.get(1, n, &dt));
+    LSB_TRY(st.in(dq, tmp, n));
+    std::vector<double> w;
+    if (per == 1) {
+        // derivatives.f90:25-35: interior weights wts(j), j = 1..nzm, times the change of
+        // variable 0.5 xl / gf(k) at point k = j + 1
+        const int nz = c->nz, nzm = c->nzm;
+        const double pi = 4.0 * std::atan(1.0);
+        w.assign(c->nzp, 0.0);
+        for (int j = 1; j <= nzm; ++j) {
+            const double tj = pi * (double)j / (double)nz;
+            double s = 0.0;
+            for (int m = 1; m <= nzm; ++m) s = s + std::sin((double)m * tj) * (1.0 - std::cos(pi * (double)m)) / (double)m;
+            const double wts = 2.00 / (double)nz * std::sin(tj) * s;
+            w[j] = wts * 0.5 * c->xl / c->gf[j];
+        }
+        LSB_TRY(st.get(2, c->nzp, &dw));
+        LSB_TRY(st.in(dw, w.data(), c->nzp));
+    }
+    LSB_TRY(zinteg_dev(c, dq, dt, n1, n2, mapping, per, rev, dw, c->stream));
+    LSB_TRY(st.out(tint, dt, n));
+    return st.sync();
+}
+
 // ------------------------------------------------------------ step API
 extern "C" int lsb_nlterm_skewsym(lsb_ctx* c, const double* uif, double* nltermif) {
     LSB_ENTER(c);

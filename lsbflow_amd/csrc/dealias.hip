@@ -79,25 +79,21 @@ int dealias_setup(lsb_ctx* c) {
     return LSB_OK;
 }
 
-int dealias_product(lsb_ctx* c, const double* a, const double* b, double* ab, hipStream_t s) {
+// dealiasing.f90:449-618 dealiasf_2d_fftw: the padded product of two spectral fields,
+// spectral in and out -- embed in the 3/2 (or larger) spectrum, inverse transforms,
+// product, forward transform (scaled 1/(nx_lg ny_lg)), truncation.
+int dealias_product_spec(lsb_ctx* c, const double* af, const double* bf, double* abf, hipStream_t s) {
     LSB_TRY(dealias_setup(c));
     const int nzp = c->nzp, lh = c->nx / 2, ny = c->ny;
     const int nxl = c->nx_lg, nyl = c->ny_lg, lhl = nxl / 2;
     const size_t nspec_lg = (size_t)nxl * nyl * nzp;
     const size_t nphys_lg = (size_t)(nxl + 2) * nyl * nzp;
-    double *af, *bf, *al, *bl, *pa, *pb;
-    LSB_TRY(stage_get(c, 10, c->nspec * sizeof(double), (void**)&af));
-    LSB_TRY(stage_get(c, 11, c->nspec * sizeof(double), (void**)&bf));
+    double *al, *bl, *pa, *pb;
     LSB_TRY(stage_get(c, 12, nspec_lg * sizeof(double), (void**)&al));
     LSB_TRY(stage_get(c, 13, nspec_lg * sizeof(double), (void**)&bl));
     LSB_TRY(stage_get(c, 14, nphys_lg * sizeof(double), (void**)&pa));
     LSB_TRY(stage_get(c, 15, nphys_lg * sizeof(double), (void**)&pb));
     const int ref = c->cfg.dealias_ref_compat ? 1 : 0;
-    {
-        const double* in[2] = {a, b};
-        double* out[2] = {af, bf};
-        LSB_TRY(horfft_fwd(c, c->hp, in, out, 2, nzp, s));
-    }
     LSB_HIP(hipMemsetAsync(al, 0, nspec_lg * sizeof(double), s));
     LSB_HIP(hipMemsetAsync(bl, 0, nspec_lg * sizeof(double), s));
     const long ne = (long)nzp * ny * lh;
@@ -118,13 +114,29 @@ int dealias_product(lsb_ctx* c, const double* a, const double* b, double* ab, hi
         double* out[1] = {al};
         LSB_TRY(horfft_fwd(c, c->hp_lg, in, out, 1, nzp, s));
     }
-    hipLaunchKernelGGL(k_truncate, grid_for(ne), dim3(256), 0, s, (const double2*)al, (double2*)af, lh, ny, lhl, nyl,
+    hipLaunchKernelGGL(k_truncate, grid_for(ne), dim3(256), 0, s, (const double2*)al, (double2*)abf, lh, ny, lhl, nyl,
                        nzp, ref);
     LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+// dealiasing.f90:262-447 dealiasing_2d_fftw: physical in and out, the spectral product
+// between a forward and an inverse 2-D transform.
+int dealias_product(lsb_ctx* c, const double* a, const double* b, double* ab, hipStream_t s) {
+    LSB_TRY(dealias_setup(c));
+    double *af, *bf;
+    LSB_TRY(stage_get(c, 10, c->nspec * sizeof(double), (void**)&af));
+    LSB_TRY(stage_get(c, 11, c->nspec * sizeof(double), (void**)&bf));
+    {
+        const double* in[2] = {a, b};
+        double* out[2] = {af, bf};
+        LSB_TRY(horfft_fwd(c, c->hp, in, out, 2, c->nzp, s));
+    }
+    LSB_TRY(dealias_product_spec(c, af, bf, af, s));
     {
         const double* in[1] = {af};
         double* out[1] = {ab};
-        LSB_TRY(horfft_inv(c, c->hp, in, out, 1, nzp, s));
+        LSB_TRY(horfft_inv(c, c->hp, in, out, 1, c->nzp, s));
     }
     return LSB_OK;
 }

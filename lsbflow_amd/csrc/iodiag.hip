@@ -257,6 +257,40 @@ __global__ void __launch_bounds__(256) k_tavg_accum(FieldList<const double> f, F
     }
 }
 
+// derivatives.f90:9-70 zinteg: cumulative z integral of each of np pencils (Fortran
+// (n1, n2, nzp): a pencil's values np apart, so consecutive threads read consecutive
+// memory).  w != null: per = 1, tint(k) = sum over k' = k..nz of w(k') tmp(k') with the
+// per-point weights w; otherwise the trapezoid rule on zpts, from the top (rev = 0) or
+// from the wall (rev = 1).  0-based k here.
+__global__ void __launch_bounds__(256) k_zinteg(const double* __restrict__ q, double* __restrict__ out, long np,
+                                                int nzp, const double* __restrict__ zp,
+                                                const double* __restrict__ w, int mapping, int rev) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    double integ = 0.0;
+    if (w) {
+        out[(long)(nzp - 1) * np + t] = 0.0;
+        for (int k = nzp - 2; k >= 1; --k) {
+            integ = integ + w[k] * q[(long)k * np + t];
+            out[(long)k * np + t] = integ;
+        }
+        out[t] = mapping == 1 ? out[np + t] : 0.0;
+    } else if (rev == 0) {
+        out[(long)(nzp - 1) * np + t] = 0.0;
+        for (int k = nzp - 1; k >= 1; --k) {
+            integ = integ + 0.5 * (q[(long)(k - 1) * np + t] + q[(long)k * np + t]) * (zp[k - 1] - zp[k]);
+            out[(long)(k - 1) * np + t] = integ;
+        }
+        if (mapping == 1) out[t] = out[np + t];
+    } else {
+        out[t] = 0.0;
+        for (int k = 1; k < nzp; ++k) {
+            integ = integ + 0.5 * (q[(long)(k - 1) * np + t] + q[(long)k * np + t]) * (zp[k - 1] - zp[k]);
+            out[(long)k * np + t] = integ;
+        }
+    }
+}
+
 int diag_scratch(lsb_ctx* c, double** p) {
     if (!c->d_diag_red) LSB_TRY(dev_alloc((void**)&c->d_diag_red, (size_t)kDiagDoubles * sizeof(double)));
     *p = c->d_diag_red;
@@ -388,6 +422,15 @@ int tavg_accum(lsb_ctx* c, const double* const* f, double* const* avg, int nf, i
         out.p[l] = avg[l];
     }
     hipLaunchKernelGGL(k_tavg_accum, dim3(1024u, (unsigned)nf), dim3(256), 0, s, in, out, c->nspec / 2, (double)div);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+int zinteg_dev(lsb_ctx* c, const double* tmp, double* tint, int n1, int n2, int mapping, int per, int rev,
+               const double* w, hipStream_t s) {
+    const long np = (long)n1 * n2;
+    hipLaunchKernelGGL(k_zinteg, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, tmp, tint, np, c->nzp, c->d_zpts,
+                       per == 1 ? w : nullptr, mapping, rev);
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }

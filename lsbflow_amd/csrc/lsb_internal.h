@@ -438,5 +438,11 @@ int isc_full(lsb_ctx* c, double* ui, double* tau, double* work, hipStream_t s);
 // dealiasing (dealias.hip)
 int dealias_setup(lsb_ctx* c);
 int dealias_product(lsb_ctx* c, const double* a, const double* b, double* ab, hipStream_t s);
+// the same product of two spectral fields, spectral in and out (abf may alias af or bf)
+int dealias_product_spec(lsb_ctx* c, const double* af, const double* bf, double* abf, hipStream_t s);
+// zinteg on device: n1 x n2 pencils of nzp values (Fortran (n1, n2, nzp)); w: nzp
+// per-point weights (per = 1) or null (trapezoid on zpts)
+int zinteg_dev(lsb_ctx* c, const double* tmp, double* tint, int n1, int n2, int mapping, int per, int rev,
+               const double* w, hipStream_t s);
 
 }  // namespace lsb

@@ -377,6 +377,25 @@ module lsb_capi
       integer(c_int), value :: name
       type(c_ptr), value :: path, rec
     end function
+    integer(c_int) function lsb_dealiasf_2d_fftw(ctx, af, bf, abf) bind(C, name="lsb_dealiasf_2d_fftw")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: af(*), bf(*)
+      real(c_double), intent(out) :: abf(*)
+    end function
+    integer(c_int) function lsb_test_fft(ctx, u, avgerror) bind(C, name="lsb_test_fft")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: u(*)
+      real(c_double), intent(out) :: avgerror
+    end function
+    integer(c_int) function lsb_zinteg(ctx, tmp, tint, n1, n2, mapping, per, rev) bind(C, name="lsb_zinteg")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tmp(*)
+      real(c_double), intent(out) :: tint(*)
+      integer(c_int), value :: n1, n2, mapping, per, rev
+    end function
     integer(c_int) function lsb_set_time_average(ctx, otfavg, istartavg, iomod) bind(C, name="lsb_set_time_average")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

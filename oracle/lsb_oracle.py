@@ -34,6 +34,11 @@ Pinning (tests/test_oracle_*.py):
                           forms for the CFL planes, fringe range and the dt rule
   wall_shear / nancheck -- single-mode closed forms (cf and its zero crossings);
                           Inf vs NaN
+  dealiasf_2d_fftw     -- the composition with dealiasing_2d_fftw; two single modes'
+                          product coefficients
+  test_fft             -- a kx-Nyquist field returns its mean |u| = 1
+  zinteg               -- trapezoid exact on linear profiles, top + wall sums = total;
+                          per = 1 is Fejer's second rule (exact on a degree-7 polynomial)
   time_average         -- window (its > istartavg) and scale (q/iomod) on fields that
                           grow linearly with the step
   sponge_lambda        -- the fringe's plateau, zero region and rise against the
@@ -834,10 +839,16 @@ class Oracle:
         """dealiasing.f90:262-447.  ref_compat reproduces the reference's ky placement
         (ky slots copied to the first ny of ny_lg, Q1); otherwise negative ky go to the
         top of the padded spectrum (textbook 3/2 rule)."""
+        abf = self.dealiasf_2d_fftw(self.horfft_fwd(a), self.horfft_fwd(b), ref_compat)
+        return self.horfft_inv(abf)
+
+    def dealiasf_2d_fftw(self, af, bf, ref_compat=True):
+        """dealiasing.f90:449-618: the padded product of two spectral fields (nzp, ny, nx),
+        spectral out (the middle of dealiasing_2d_fftw)."""
         nx, ny = self.nx, self.ny
         nxl, nyl = self.dealias_sizes()
-        af = self.horfft_fwd(a).view(np.complex128)
-        bf = self.horfft_fwd(b).view(np.complex128)
+        af = np.ascontiguousarray(af, dtype=np.float64).view(np.complex128)
+        bf = np.ascontiguousarray(bf, dtype=np.float64).view(np.complex128)
 
         def pad(xf):
             big = np.zeros(xf.shape[:-2] + (nyl, nxl // 2 + 1), dtype=np.complex128)
@@ -861,7 +872,51 @@ class Oracle:
             h = ny // 2
             out[..., :h, :] = P[..., :h, : nx // 2]
             out[..., h:, :] = P[..., nyl - (ny - h):, : nx // 2]
-        return self.horfft_inv(np.ascontiguousarray(out).view(np.float64))
+        return np.ascontiguousarray(out).view(np.float64)
+
+    def test_fft(self, u):
+        """horfft.f90:7-58: mean |u - q| over the nx physical points, q the inverse of the
+        forward transform of u."""
+        q = self.horfft_inv(self.horfft_fwd(u))
+        n = self.nx * self.ny * self.nzp
+        return float(np.sum(np.abs(u[..., : self.nx] - q[..., : self.nx]) / n))
+
+    def zinteg(self, tmp, mapping, per, rev):
+        """derivatives.f90:9-70 on tmp (nzp, n2, n1): per = 1 the interior quadrature
+        weights wts(j) = 2/nz sin(t_j) sum_m sin(m t_j)(1 - cos(pi m))/m, t_j = pi j/nz,
+        with the change of variable 0.5 xl/gf(k), summed from plane nz down; per = 0 the
+        trapezoid rule on zpts from the top (rev = 0) or from plane 1 (rev = 1)."""
+        nz, nzm, nzp = self.nz, self.nzm, self.nzp
+        z = self.zpts
+        tint = np.zeros(tmp.shape)
+        if per == 1:
+            pi = 4.0 * math.atan(1.0)
+            wts = np.zeros(nzm + 1)
+            for j in range(1, nzm + 1):
+                tj = pi * float(j) / float(nz)
+                acc = 0.0
+                for m in range(1, nzm + 1):
+                    acc = acc + math.sin(float(m) * tj) * (1.0 - math.cos(pi * float(m))) / float(m)
+                wts[j] = 2.00 / float(nz) * math.sin(tj) * acc
+            integ = np.zeros(tmp.shape[1:])
+            for k in range(nz, 1, -1):               # 1-based k = nz..2
+                integ = integ + wts[k - 1] * 0.5 * self.xl / self.gf[k - 1] * tmp[k - 1]
+                tint[k - 1] = integ
+            if mapping == 1:
+                tint[0] = tint[1]
+        elif rev == 0:
+            integ = np.zeros(tmp.shape[1:])
+            for k in range(nzp, 1, -1):              # 1-based k = nzp..2
+                integ = integ + 0.5 * (tmp[k - 2] + tmp[k - 1]) * (z[k - 2] - z[k - 1])
+                tint[k - 2] = integ
+            if mapping == 1:
+                tint[0] = tint[1]
+        else:
+            integ = np.zeros(tmp.shape[1:])
+            for k in range(2, nzp + 1):
+                integ = integ + 0.5 * (tmp[k - 2] + tmp[k - 1]) * (z[k - 2] - z[k - 1])
+                tint[k - 1] = integ
+        return tint
 
     # ---------------------------------------------------------------- boundary
     def bound(self, t):

@@ -229,6 +229,45 @@ def test_dealiasing(ref_compat, rng):
     assert rel_l2(got[..., : o.nx], ref[..., : o.nx]) < TOL
 
 
+@pytest.mark.parametrize("ref_compat", [1, 0])
+def test_dealiasf(ref_compat, rng):
+    """dealiasing.f90:449-618, spectral in and out."""
+    o, c = make(32, 16, 40, dealiasflag=1, dealias_ref_compat=ref_compat)
+    af, bf = spec_field(o, rng), spec_field(o, rng)
+    got = c.dealiasf_2d_fftw(af, bf)
+    ref = o.dealiasf_2d_fftw(af, bf, bool(ref_compat))
+    assert rel_l2(got, ref) < TOL
+    c.close()
+
+
+def test_test_fft(rng):
+    """horfft.f90:7-58: a kx-Nyquist field (dropped by the transform) gives its mean |u| = 1
+    (to the transforms' rounding, 1e-12); a random resolved field comes back to rounding,
+    as in the oracle."""
+    o, c = make(32, 16, 40)
+    alt = np.zeros((o.nzp, o.ny, o.nxpp))
+    alt[..., : o.nx] = (-1.0) ** np.arange(o.nx)
+    assert abs(c.test_fft(alt) - 1.0) < 1e-12
+    f = o.horfft_inv(spec_field(o, rng) * (np.arange(o.ny)[None, :, None] != o.ny // 2))
+    e, eo = c.test_fft(f), o.test_fft(f)
+    assert e < 1e-15 * np.max(np.abs(f)) * 10 and eo < 1e-15 * np.max(np.abs(f)) * 10
+    c.close()
+
+
+@pytest.mark.parametrize("grid,mapping,per,rev", [(3, 3, 0, 0), (3, 3, 0, 1), (3, 3, 1, 0), (2, 2, 1, 0),
+                                                 (3, 1, 0, 0), (2, 1, 1, 0)])
+def test_zinteg(grid, mapping, per, rev, rng):
+    """derivatives.f90:9-70 on (n1, n2) = (nxpp, ny) pencils; mapping is the routine's
+    argument (1: plane 1 copies plane 2), the grid's own mapping 3 or 2."""
+    o, c = make(32, 16, 40, mapping=grid, zlen=2.0 if grid == 3 else 20.0)
+    tmp = phys_field(o, rng)
+    got = c.zinteg(tmp, mapping, per, rev)
+    ref = o.zinteg(tmp, mapping, per, rev)
+    assert rel_l2(got, ref) < 1e-14
+    assert np.array_equal(got == 0, ref == 0)
+    c.close()
+
+
 @pytest.mark.parametrize("sgs", [(0, 0), (1, 2), (1, 0), (1, 1), (1, 3)])
 def test_nlterm_skewsym(sgs, rng):
     case = L.cases.CASES["c1_dns"].with_(nx=32, ny=16, nz=40)

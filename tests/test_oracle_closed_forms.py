@@ -468,3 +468,60 @@ def test_temperature_base_profile_source():
     got = st["tempf"][:, 0, 0]
     assert np.max(np.abs(got[1:o.nz] - T1[1:o.nz])) < 1e-12 * np.max(np.abs(T1))
     assert got[0] == 0 and got[-1] == 0
+
+
+# ------------------------------------------- boundary test / integration routines
+def test_dealiasf_is_the_spectral_core():
+    """dealiasing_2d_fftw (dealiasing.f90:262-447) is horfft(+1) of dealiasf_2d_fftw
+    (dealiasing.f90:449-618) of the forward transforms; on two single modes (textbook
+    placement) dealiasf gives the coefficients of their product,
+    cos(2x)cos(2y) cos(x)sin(4y) (four sine modes of amplitude 1/8)."""
+    o = small(nx=16, ny=8, dealiasflag=1)
+    rng = np.random.default_rng(12)
+    a = phys(o, lambda x, y, z: rng.standard_normal(x.shape[:2] + (x.shape[2],)) + 0 * y + 0 * z)
+    b = phys(o, lambda x, y, z: np.cos(x + 3 * y) + 0 * z)
+    for rc in (True, False):
+        got = o.horfft_inv(o.dealiasf_2d_fftw(o.horfft_fwd(a), o.horfft_fwd(b), rc))
+        want = o.dealiasing_2d_fftw(a, b, rc)
+        assert np.max(np.abs(got - want)[..., : o.nx]) < 1e-14
+    a1 = phys(o, lambda x, y, z: np.cos(2 * x) * np.cos(2 * y) + 0 * z)
+    b1 = phys(o, lambda x, y, z: np.cos(x) * np.sin(4 * y) + 0 * z)
+    got = o.dealiasf_2d_fftw(o.horfft_fwd(a1), o.horfft_fwd(b1), False)
+    assert np.max(np.abs(got - o.horfft_fwd(phys(o, lambda x, y, z: (np.cos(2 * x) * np.cos(2 * y) * np.cos(x)
+                                                                        * np.sin(4 * y)) + 0 * z)))) < 1e-15
+
+
+def test_test_fft_nyquist_closed_form():
+    """test_fft (horfft.f90:7-58): a kx-Nyquist field is dropped by the forward transform, so
+    the round trip returns 0 and the mean error is the mean |u| = 1; a resolved field
+    comes back to rounding."""
+    o = small(nx=16, ny=8, nz=8)
+    alt = phys(o, lambda x, y, z: np.cos(math.pi * np.round(x * o.nx / (2 * math.pi))) + 0 * y + 0 * z)
+    assert abs(o.test_fft(alt) - 1.0) < 1e-15
+    ok = phys(o, lambda x, y, z: np.sin(3 * x - 2 * y) * (1 + z) + 0.5)
+    assert o.test_fft(ok) < 1e-15
+
+
+@pytest.mark.parametrize("mapping,zlen", [(0, 1.0), (3, 2.0)])
+def test_zinteg_closed_forms(mapping, zlen):
+    """zinteg (derivatives.f90:9-70): the trapezoid rule integrates a linear profile exactly
+    from the top (rev = 0) and from plane 1 (rev = 1), the two adding to the full integral;
+    per = 1 is Fejer's second rule on the interior Chebyshev points t_j = cos(pi j/nz),
+    exact for polynomials of degree <= nz - 2: with tmp = p(t) gf / (0.5 xl) the value at
+    plane 2 is the integral of p over [-1, 1]."""
+    o = small(nx=4, ny=2, nz=24, mapping=mapping, zlen=zlen)
+    z = o.zpts
+    f = (0.7 + 1.3 * z)[:, None, None] * np.ones((1, o.ny, o.nx))
+    F = 0.7 * z + 0.65 * z ** 2
+    top = o.zinteg(f, mapping, 0, 0)
+    bot = o.zinteg(f, mapping, 0, 1)
+    assert np.max(np.abs(top[:, 0, 0] - (F - F[-1]))) < 1e-13
+    assert np.max(np.abs(bot[:, 0, 0] - (F[0] - F))) < 1e-13
+    assert np.max(np.abs(top + bot - top[0])) < 1e-13
+    t = np.cos(np.pi * np.arange(o.nzp) / o.nz)
+    p = t ** 4 + 0.3 * t - 0.2 * t ** 7
+    tmp = np.zeros((o.nzp, o.ny, o.nx))
+    tmp[1:o.nz] = (p * o.gf / (0.5 * o.xl))[1:o.nz, None, None]
+    q = o.zinteg(tmp, mapping, 1, 0)
+    assert np.max(np.abs(q[1] - 2.0 / 5.0)) < 1e-13          # int t^4 = 2/5, odd parts vanish
+    assert np.all(q[-1] == 0) and np.all(q[0] == 0)
