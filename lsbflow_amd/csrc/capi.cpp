@@ -64,7 +64,11 @@ int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token, double flops
     ev.cls = cls;
     ev.flops = flops;
     ev.bytes = bytes;
-    if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return LSB_OK;
+    // timing-only events: no system-scope fence (cache write-back and invalidation) at
+    // each record; lsb_profile_end synchronises the stream before reading them
+    if (hipEventCreateWithFlags(&ev.a, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&ev.b, hipEventDisableSystemFence) != hipSuccess)
+        return LSB_OK;
     (void)hipEventRecord(ev.a, s);
     c->prof.push_back(ev);
     *token = (int)c->prof.size() - 1;
