@@ -2389,7 +2389,8 @@ int helmholtz(lsb_ctx* c, double* const* rhs, const double* const* bc_top, const
         dim3 grid((unsigned)((a.P + 16 * kZHCB - 1) / (16 * kZHCB)), 1, (unsigned)nf);
         const char* off = std::getenv("LSB_NO_ZHELM");  // the two-kernel form, for tests
         const bool ok = gy == 1 && lds <= 64 * 1024 && !(off && off[0] && std::strcmp(off, "0") != 0);
-        if (ok && split32) {
+        const char* off32 = std::getenv("LSB_NO_ZHELM32");  // FP64 velocity solves in the mixed form, for tests
+        if (ok && split32 && !(off32 && off32[0] && std::strcmp(off32, "0") != 0)) {
             // the FP32-correction form needs sigma to dominate every eigenvalue: then
             // s c(lam) <= max|lam| / sigma <= 1/16 scales the FP32 rounding
             double lmax = 0.0;
