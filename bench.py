@@ -324,8 +324,8 @@ def main(argv=None):
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
     if args.gpus != 1:
-        print("bench.py: this build runs the whole domain on one GPU; multi-GPU runs are not part of it "
-              "(use --gpus 1)", file=sys.stderr)
+        print("bench.py: the library runs the whole domain on one GPU; runs on several GPUs are not "
+              "supported (use --gpus 1)", file=sys.stderr)
         return 2
     import numpy as np
     import lsbflow_amd as L

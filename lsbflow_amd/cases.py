@@ -81,8 +81,8 @@ class Case:
         return asdict(self)
 
 
-# Names in BASELINE order: x = reference x (nx), y_wall = reference z (nzp points),
-# z_span = reference y (ny).  "512x193x256" is nx=512, nz=192, ny=256.
+# Workload names list streamwise x planes (nx), wall-normal points (nzp = nz + 1) and
+# spanwise points (ny): "512x193x256" is nx=512, nz=192, ny=256.
 CASES = {
     "c1_dns": Case("c1_dns", nx=64, ny=32, nz=64, xnu=1.0e-3, dt=1.0e-3, nsteps=10),
     "c2_dyn_les": Case("c2_dyn_les", nx=256, ny=128, nz=128, xnu=0.2 / 5.0e4, samp=0.3,

@@ -923,7 +923,7 @@ class Oracle:
         """navierstokes.f90:214-355 -> spectral bc field (nzp, ny, nx).
         bctype 1: Spalart ceiling suction (+ fringe blowing rescaled for mass balance);
         bctype 3: steady ceiling profile V(x) = samp*(-G(0.4 Lx) + G(0.7 Lx)) with
-        Gaussian width 0.05 Lx, no-slip wall (the configs' driver patch)."""
+        Gaussian width 0.05 Lx, no-slip wall (the workloads' driver-side patch)."""
         c = self.cfg
         nx, nzp = self.nx, self.nzp
         bc = np.zeros((nzp, self.ny, self.nxpp))
