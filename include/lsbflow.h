@@ -369,6 +369,10 @@ int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields
  * algorithmic flops and algorithmic HBM bytes (arrays of LSB_PROF_CLASSES). */
 #define LSB_PROF_CLASSES 6
 int lsb_profile_begin(lsb_ctx* ctx);
+/* The same, bracketing only the launch groups of the classes whose bit is set in
+ * class_mask (bit c for class c); the others run without events.  No counterpart in the
+ * reference: it keeps the timing events of a measured run to the class being measured. */
+int lsb_profile_begin_classes(lsb_ctx* ctx, unsigned class_mask);
 int lsb_profile_end(lsb_ctx* ctx, double* ms, int* count, double* flops, double* bytes);
 
 #ifdef __cplusplus

@@ -85,6 +85,7 @@ _SIGS = {
     "lsb_dfdz_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "lsb_solve_helmholtz_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp]),
     "lsb_profile_begin": (C.c_int, [_vp]),
+    "lsb_profile_begin_classes": (C.c_int, [_vp, C.c_uint]),
     "lsb_profile_end": (C.c_int, [_vp, _dp, _ip, _dp, _dp]),
     "lsb_synchronize": (C.c_int, [_vp]),
     "lsb_snapshot_record": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double,

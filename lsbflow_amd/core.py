@@ -420,6 +420,14 @@ class Context:
     def profile_begin(self):
         check(self._L.lsb_profile_begin(self.h), "profile_begin")
 
+    def profile_begin_classes(self, classes):
+        """Profile only the launch groups of the named classes (see profile_end)."""
+        keys = ("fft", "zcontract", "helmholtz", "pointwise", "rhs", "nonlinear")
+        mask = 0
+        for k in classes:
+            mask |= 1 << keys.index(k)
+        check(self._L.lsb_profile_begin_classes(self.h, mask), "profile_begin_classes")
+
     def profile_end(self):
         """{class: dict(ms=total device ms, n=launch groups, flops=..., bytes=...)}."""
         ms = np.zeros(PROF_CLASSES)

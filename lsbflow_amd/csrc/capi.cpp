@@ -59,7 +59,7 @@ int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p) {
 
 int prof_mark_begin(lsb_ctx* c, int cls, hipStream_t s, int* token, double flops, double bytes) {
     *token = -1;
-    if (!c->profiling) return LSB_OK;
+    if (!c->profiling || !((c->prof_mask >> cls) & 1u)) return LSB_OK;
     lsb_ctx::ProfEv ev;
     ev.cls = cls;
     ev.flops = flops;
@@ -1203,7 +1203,7 @@ extern "C" int lsb_state_device_ptrs(lsb_ctx* c, void** uf, void** vf, void** wf
 
 extern "C" void* lsb_stream(lsb_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
-extern "C" int lsb_profile_begin(lsb_ctx* c) {
+extern "C" int lsb_profile_begin_classes(lsb_ctx* c, unsigned class_mask) {
     LSB_TRY(check_ctx(c));
     for (auto& ev : c->prof) {
         (void)hipEventDestroy(ev.a);
@@ -1211,8 +1211,11 @@ extern "C" int lsb_profile_begin(lsb_ctx* c) {
     }
     c->prof.clear();
     c->profiling = true;
+    c->prof_mask = class_mask;
     return LSB_OK;
 }
+
+extern "C" int lsb_profile_begin(lsb_ctx* c) { return lsb_profile_begin_classes(c, ~0u); }
 
 extern "C" int lsb_synchronize(lsb_ctx* c) {
     LSB_TRY(check_ctx(c));

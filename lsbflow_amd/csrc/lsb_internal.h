@@ -202,6 +202,7 @@ struct lsb_ctx {
 
     // profiling
     bool profiling = false;
+    unsigned prof_mask = ~0u;  // classes bracketed while profiling
     struct ProfEv {
         hipEvent_t a, b;
         int cls;

@@ -351,6 +351,11 @@ module lsb_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function lsb_profile_begin_classes(ctx, class_mask) bind(C, name="lsb_profile_begin_classes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: class_mask
+    end function
     integer(c_int) function lsb_profile_end(ctx, ms, count, flops, bytes) bind(C, name="lsb_profile_end")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
