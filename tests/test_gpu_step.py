@@ -221,3 +221,27 @@ def test_c1_dns_temperature_buoyancy():
     for k in ("tempf", "tnf", "nltnf"):
         bound = max(TOL, 1000.0 * rel_l2(st2[k], st[k]))
         assert rel_l2(gt[k], st[k]) < bound, (k, rel_l2(gt[k], st[k]), bound)
+
+
+def test_profile_selected_classes():
+    """lsb_profile_begin_classes: only the selected class's launch groups carry events;
+    its measured groups and algorithmic work equal those of the all-class profile."""
+    case = L.cases.CASES["c1_dns"]
+    c = setup_case(case)
+    try:
+        c.state_set(*L.cases.initial_condition(case))
+        c.timestep(3)
+        c.profile_begin()
+        c.timestep(2)
+        full = c.profile_end()
+        c.profile_begin_classes(["helmholtz"])
+        c.timestep(2)
+        sel = c.profile_end()
+    finally:
+        c.close()
+    assert sel["helmholtz"]["n"] == full["helmholtz"]["n"] > 0
+    assert sel["helmholtz"]["flops"] == full["helmholtz"]["flops"]
+    assert sel["helmholtz"]["ms"] > 0
+    for k, v in sel.items():
+        if k != "helmholtz":
+            assert v["n"] == 0 and v["ms"] == 0.0, (k, v)
