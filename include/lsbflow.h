@@ -59,6 +59,14 @@ typedef struct lsb_config {
     int sgsflag, sgsmodel;          /* sgsterm.f90:21-35 (4: TNS, no SGS stress) */
     int dealiasflag;                /* 0 none, 1 3/2-rule, 2 spectral filter     */
     int dealias_ref_compat;         /* 1: reference ky placement (dealiasing.f90:329,435) */
+    int dct_exact;                  /* 0: dct_diff as chebyshev.f90:242 (alpha(n) not
+                                       scaled); 1: alpha(n)/xl, the exact Gauss-Lobatto
+                                       derivative (identical when xl = 1)           */
+    int top_bc;                     /* 0: u, v at the ceiling from D^2(u,v) -> 0
+                                       (navierstokes.f90:1335-1347); 1: the wall's
+                                       Zang-Hussaini values mirrored at the ceiling;
+                                       2 (bctype 3): (1 - m(x)) free + m(x) Dirichlet,
+                                       m = the profile's blowing Gaussian G(0.4 Lx) */
     int ifrz;                       /* navierstokes.f90:53-59                    */
     int nstart;                     /* 0: Euler/BDF2 start (navierstokes.f90:1507) */
     int precision;                  /* LSB_FP64 or LSB_MIXED                     */

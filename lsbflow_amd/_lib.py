@@ -25,7 +25,7 @@ class LsbConfig(C.Structure):
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("mapping", C.c_int), ("bctype", C.c_int), ("forcing", C.c_int),
         ("sgsflag", C.c_int), ("sgsmodel", C.c_int), ("dealiasflag", C.c_int),
-        ("dealias_ref_compat", C.c_int), ("ifrz", C.c_int), ("nstart", C.c_int),
+        ("dealias_ref_compat", C.c_int), ("dct_exact", C.c_int), ("top_bc", C.c_int), ("ifrz", C.c_int), ("nstart", C.c_int),
         ("precision", C.c_int), ("device", C.c_int),
         ("xlen", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double), ("x0", C.c_double),
         ("u0", C.c_double), ("xnu", C.c_double), ("blfac", C.c_double), ("prtl", C.c_double),

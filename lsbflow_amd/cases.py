@@ -18,6 +18,30 @@ library and for the CPU oracle:
 * ceiling profile (bctype 3): ``w(x, zlen) = samp*(-G(0.4 Lx) + G(0.7 Lx))``,
   ``G(c) = exp(-((x - x0 - c)/(0.05 Lx))^2)`` on the reference's ``xpts``.
 
+The LES workloads (suction amplitude 0.3 at Re 5e4-1e5) run with two switches
+on, applied identically in the library and the oracle:
+
+* ``dct_exact = 1``: the Chebyshev derivative scales the top DCT coefficient
+  like the others (chebyshev.f90:242 leaves it unscaled, which is exact only for
+  xl = 1; the channel has xl = 2).  With the reference operator the
+  non-homogeneous step has a growing mode of factor 1.585 per step at these
+  viscosities (single-pencil step map), whatever the ceiling condition;
+* ``top_bc = 2``: the tangential ceiling values are ``(1 - m(x)) f + m(x) g``
+  with ``m`` the profile's blowing Gaussian G(0.4 Lx), ``f`` the reference's
+  free-flow value from D^2(u,v) -> 0 (navierstokes.f90:1335-1347) and ``g`` the
+  wall's Zang-Hussaini value (:1329-1331, 1365-1367), blended in physical space.
+  The free-flow value advances the ceiling with the interior's wall-normal
+  derivative, an outflow treatment: where the profile blows fluid in it
+  extrapolates against the incoming characteristic and the run diverges within
+  ~60 steps.  A Dirichlet ceiling everywhere (``top_bc = 1``) instead grows a
+  suction boundary layer of thickness nu/V ~ 1e-5, twenty times finer than the
+  first Chebyshev spacing, which diverges after ~200 steps.  The blend
+  prescribes the incoming fluid's tangential velocity (u' = 0, i.e. the free
+  stream ubar(zlen)) where fluid enters and keeps the outflow condition
+  elsewhere; it stays bounded (2000 steps at reduced horizontal size).
+
+The DNS workload keeps the reference's operator and ceiling condition.
+
 The random numbers come from SplitMix64 + Box-Muller so that every consumer
 reproduces the same bits from the same seed.
 """
@@ -59,6 +83,8 @@ class Case:
     filt_order: float = 1.0
     filt_amp: float = 20.0
     dealiasflag: int = 2
+    dct_exact: int = 0
+    top_bc: int = 0
     dt: float = 1.0e-3
     nsteps: int = 10
     delta: float = 0.2
@@ -86,11 +112,11 @@ class Case:
 CASES = {
     "c1_dns": Case("c1_dns", nx=64, ny=32, nz=64, xnu=1.0e-3, dt=1.0e-3, nsteps=10),
     "c2_dyn_les": Case("c2_dyn_les", nx=256, ny=128, nz=128, xnu=0.2 / 5.0e4, samp=0.3,
-                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=5.0e-4, nsteps=100),
+                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=5.0e-4, nsteps=100, dct_exact=1, top_bc=2),
     "c4_sigma_les": Case("c4_sigma_les", nx=512, ny=256, nz=192, xnu=0.2 / 1.0e5, samp=0.3,
-                         sgsflag=1, sgsmodel=2, dt=2.5e-4, nsteps=100),
+                         sgsflag=1, sgsmodel=2, dt=2.5e-4, nsteps=100, dct_exact=1, top_bc=2),
     "c4_dyn_les": Case("c4_dyn_les", nx=512, ny=256, nz=192, xnu=0.2 / 1.0e5, samp=0.3,
-                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=2.5e-4, nsteps=100),
+                       sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=2.5e-4, nsteps=100, dct_exact=1, top_bc=2),
 }
 
 

@@ -59,6 +59,8 @@ class Config:
     sgsmodel: int = 0
     dealiasflag: int = 2
     dealias_ref_compat: int = 1
+    dct_exact: int = 0
+    top_bc: int = 0
     ifrz: int = 0
     nstart: int = 0
     precision: int = 0

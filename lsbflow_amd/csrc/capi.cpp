@@ -221,6 +221,14 @@ extern "C" int lsb_create(const lsb_config* cfg, lsb_ctx** out) {
         set_error("lsb_create: bctype must be 0, 1 or 3");
         return LSB_E_UNSUPPORTED;
     }
+    if ((g.dct_exact != 0 && g.dct_exact != 1) || g.top_bc < 0 || g.top_bc > 2) {
+        set_error("lsb_create: dct_exact must be 0 or 1, top_bc 0, 1 or 2");
+        return LSB_E_ARG;
+    }
+    if (g.top_bc == 2 && g.bctype == 1) {
+        set_error("lsb_create: top_bc 2 weights the ceiling by the bctype 3 blowing profile (not bctype 1)");
+        return LSB_E_ARG;
+    }
     lsb_ctx* c = new lsb_ctx();
     c->cfg = g;
     c->nx = g.nx;

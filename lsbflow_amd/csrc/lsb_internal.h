@@ -220,7 +220,7 @@ int build_grid(lsb_ctx* c);
 int build_operators(lsb_ctx* c);
 int build_derived(lsb_ctx* c);
 void cheb_matrix(int nzp, double xl, const double* gf, double* dor, double* d, double* d2);
-void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m);
+void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m, int exact = 0);
 int eigen_decompose(const double* a, int n, double* lam, double* e, double* ei);
 void lifting_functions(lsb_ctx* c);
 void sponge_lambda(int nx, const double* xpts, double xlen, double u0, double fsize, double lfsize, double sfsize,

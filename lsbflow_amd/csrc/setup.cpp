@@ -151,8 +151,12 @@ void cheb_matrix(int nzp, double xl, const double* gf, double* dor, double* d, d
 // chebyshev.f90:204-270 dct_diff applied to the unit vectors: column j of the
 // returned (column-major n x n) matrix is dct_diff(e_j).  The DCT-I
 // (FFTW REDFT00: Y_k = x_0 + (-1)^k x_{n-1} + 2 sum x_j cos(pi j k/(n-1))) is
-// evaluated directly in long double.
-void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m) {
+// evaluated directly in long double.  The reference scales only alpha(2:n-1) by
+// the half interval (chebyshev.f90:242); the top coefficient needs 1/(2 half) =
+// 1/xl (its DCT-I weight is half the interior one), so the reference operator is
+// exact for xl = 1 and doubles the derivative of T_N for xl = 2.  exact = 1
+// applies 1/xl to alpha(n): the exact Gauss-Lobatto derivative on every mapping.
+void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m, int exact) {
     const ld half = 0.5L * (ld)xl;
     const ld nlog = 2.0L * (n - 1);
     std::vector<ld> cs((size_t)n * n);
@@ -175,6 +179,7 @@ void dct_diff_matrix(int n, double xl, const double* gf, int p, double* m) {
         for (int l = 0; l < p; ++l) {
             redft00(beta, alpha);
             for (int i = 1; i < n - 1; ++i) alpha[i] /= half;  // alpha(2:n-1) only (Q2)
+            if (exact) alpha[n - 1] /= (ld)xl;
             for (int i = 0; i < n; ++i) beta[i] = 0;
             beta[n - 2] = 2.0L * (n - 1.0L) * alpha[n - 1];
             for (int k = n - 1; k >= 2; --k) beta[k - 2] = beta[k] + 2.0L * (k - 1.0L) * alpha[k - 1];
@@ -627,8 +632,8 @@ int build_derived(lsb_ctx* c) {
     c->mdz.assign((size_t)nzp * nzp, 0.0);
     c->md2.assign((size_t)nzp * nzp, 0.0);
     if (nz > 31) {  // derivatives.f90:122,175
-        dct_diff_matrix(nzp, c->xl, c->gf.data(), 1, c->mdz.data());
-        dct_diff_matrix(nzp, c->xl, c->gf.data(), 2, c->md2.data());
+        dct_diff_matrix(nzp, c->xl, c->gf.data(), 1, c->mdz.data(), c->cfg.dct_exact);
+        dct_diff_matrix(nzp, c->xl, c->gf.data(), 2, c->md2.data(), c->cfg.dct_exact);
     } else {
         c->mdz = c->d;
         c->md2 = c->d2;

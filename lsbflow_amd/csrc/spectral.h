@@ -31,6 +31,8 @@ struct StepBcArgs {
     const double* rhs1;    // plane 0 of rhs_v
     const double* p_wall;  // plane nzp-1 of pf
     const double* pn_wall; // plane nzp-1 of pnf
+    const double* p_top;   // plane 0 of pf  (top_bc = 1)
+    const double* pn_top;  // plane 0 of pnf (top_bc = 1)
     const double* d2w;     // 2 planes: rows 0 and nzp-1 of d2fdz2(w^n)
     const double* bnd_top; // bound() spectral plane 0 (nullable)
     const double* bnd_bot; // bound() spectral plane nzp-1 (nullable)
@@ -38,7 +40,13 @@ struct StepBcArgs {
     double* bcb[3];
     double dtnui, bdfv, dt, xnu, pfac0, pfac1;
     int nhom;
+    int top_bc;
+    double* cdiff[2];      // top_bc 2: Dirichlet minus free ceiling values of u, v
 };
+
+// top_bc 2: bct[l] += F[m(x) F^-1(cdiff[l])] for l = 0, 1 (physical planes as scratch)
+int sp_ceiling_blend(lsb_ctx* c, double* const* bct, double* const* cdiff, double* const* phys, const double* m,
+                     hipStream_t s);
 
 int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s);
 int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s);

@@ -301,17 +301,36 @@ __global__ void k_bc_vel(SpDims d, const double* __restrict__ wavx, const double
         const double wx = wavx[2 * m], wy = wavy[j];
         const double wav2 = wx * wx + wy * wy;
         const double den = wav2 + a.dtnui / a.bdfv;
-        // top (plane 0): -rhs/(wav2 + dtnui/bdfv) for u and v
+        const double dtb = a.dt * a.bdfv;
+        // top (plane 0): -rhs/(wav2 + dtnui/bdfv) for u and v (free flow), or the wall's
+        // Zang-Hussaini values (Dirichlet ceiling), or the free values plus the difference
+        // that the ceiling weight blends in physical space (top_bc 2, finished by the caller)
         const c2 r0 = reinterpret_cast<const c2*>(a.rhs0)[pr];
         const c2 r1 = reinterpret_cast<const c2*>(a.rhs1)[pr];
-        reinterpret_cast<c2*>(a.bct[0])[pr] = make_double2(-r0.x / den, -r0.y / den);
-        reinterpret_cast<c2*>(a.bct[1])[pr] = make_double2(-r1.x / den, -r1.y / den);
+        c2 fu = make_double2(-r0.x / den, -r0.y / den);
+        c2 fv = make_double2(-r1.x / den, -r1.y / den);
+        if (a.top_bc) {
+            const c2 q0 = reinterpret_cast<const c2*>(a.p_top)[pr];
+            const c2 q1 = reinterpret_cast<const c2*>(a.pn_top)[pr];
+            const double qr = a.pfac0 * q0.x + a.pfac1 * q1.x;
+            const double qi = a.pfac0 * q0.y + a.pfac1 * q1.y;
+            const c2 du = make_double2(dtb * (-wx * qi), dtb * (wx * qr));
+            const c2 dv = make_double2(dtb * (-wy * qi), dtb * (wy * qr));
+            if (a.top_bc == 1) {
+                fu = du;
+                fv = dv;
+            } else {
+                reinterpret_cast<c2*>(a.cdiff[0])[pr] = make_double2(du.x - fu.x, du.y - fu.y);
+                reinterpret_cast<c2*>(a.cdiff[1])[pr] = make_double2(dv.x - fv.x, dv.y - fv.y);
+            }
+        }
+        reinterpret_cast<c2*>(a.bct[0])[pr] = fu;
+        reinterpret_cast<c2*>(a.bct[1])[pr] = fv;
         // wall (plane nzp-1): dt*bdfv*d(p*)/dx, dt*bdfv*d(p*)/dy, p* = pfac0 p + pfac1 pn
         const c2 p0 = reinterpret_cast<const c2*>(a.p_wall)[pr];
         const c2 p1 = reinterpret_cast<const c2*>(a.pn_wall)[pr];
         const double psr = a.pfac0 * p0.x + a.pfac1 * p1.x;
         const double psi = a.pfac0 * p0.y + a.pfac1 * p1.y;
-        const double dtb = a.dt * a.bdfv;
         reinterpret_cast<c2*>(a.bcb[0])[pr] = make_double2(dtb * (-wx * psi), dtb * (wx * psr));
         reinterpret_cast<c2*>(a.bcb[1])[pr] = make_double2(dtb * (-wy * psi), dtb * (wy * psr));
         // w: imposed field (bound) plus the Zang-Hussaini correction xnu*dt*bdfv*d2w/dz2
@@ -388,6 +407,41 @@ int sp_bc_vel(lsb_ctx* c, const StepBcArgs& a, hipStream_t s) {
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
+__global__ void k_row_weight(double* __restrict__ a, double* __restrict__ b, const double* __restrict__ m, int nx,
+                             int nxpp, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % nxpp);
+    const double w = (i < nx) ? m[i] : 0.0;
+    a[t] *= w;
+    b[t] *= w;
+}
+
+__global__ void k_add_planes(double* __restrict__ y0, double* __restrict__ y1, const double* __restrict__ x0,
+                             const double* __restrict__ x1, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    y0[t] += x0[t];
+    y1[t] += x1[t];
+}
+
+int sp_ceiling_blend(lsb_ctx* c, double* const* bct, double* const* cdiff, double* const* phys, const double* m,
+                     hipStream_t s) {
+    const double* in[2] = {cdiff[0], cdiff[1]};
+    LSB_TRY(horfft_inv(c, c->hp, in, phys, 2, 1, s));
+    const long np = (long)c->plane_p;
+    hipLaunchKernelGGL(k_row_weight, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, phys[0], phys[1], m, c->nx,
+                       c->nxpp, np);
+    LSB_HIP(hipGetLastError());
+    const double* pin[2] = {phys[0], phys[1]};
+    LSB_TRY(horfft_fwd(c, c->hp, pin, cdiff, 2, 1, s));
+    const long ns = (long)c->plane_s;
+    hipLaunchKernelGGL(k_add_planes, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, bct[0], bct[1], cdiff[0],
+                       cdiff[1], ns);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
 int sp_project(lsb_ctx* c, const double* dwdz, const double* divf, double* p, double* u, double* v, double dt,
                double bdfv, hipStream_t s) {
     SpDims d = sp_dims(c, c->nzp);

@@ -77,6 +77,8 @@ struct StepState {
     // bc planes: bct[3], bcb[3], d2w (2), bnd_top, bnd_bot
     double *bct[3], *bcb[3], *d2w, *bnd_top, *bnd_bot;
     double bnd00_top = 0.0, bnd00_bot = 0.0;  // (0,0) Re of the unit-amplitude bound planes
+    // top_bc 2: the two difference planes, two physical scratch planes and the weight m(x)
+    double *cdiff[2], *cphys[2], *cweight;
 };
 
 static std::vector<std::pair<const lsb_ctx*, StepState*>> g_states;
@@ -113,8 +115,9 @@ int step_alloc(lsb_ctx* c) {
     StepState* S = new StepState();
     const size_t ns = c->nspec, np = c->nphys, pl = c->plane_s;
     const int nsgs = sgs_fields(c);
-    const size_t nplanes = 6 + 2 + 2;
-    const size_t bytes = (NSPEC_SLOTS * ns + (NPHYS_SLOTS + nsgs) * np + nplanes * pl) * sizeof(double);
+    const size_t nplanes = 6 + 2 + 2 + 2;
+    const size_t bytes =
+        (NSPEC_SLOTS * ns + (NPHYS_SLOTS + nsgs) * np + nplanes * pl + 2 * c->plane_p + c->nx) * sizeof(double);
     void* blk = nullptr;
     int rc = dev_alloc(&blk, bytes);
     if (rc != LSB_OK) {
@@ -138,6 +141,24 @@ int step_alloc(lsb_ctx* c) {
     S->d2w = p + 6 * pl;
     S->bnd_top = p + 8 * pl;
     S->bnd_bot = p + 9 * pl;
+    S->cdiff[0] = p + 10 * pl;
+    S->cdiff[1] = p + 11 * pl;
+    S->cphys[0] = p + 12 * pl;
+    S->cphys[1] = S->cphys[0] + c->plane_p;
+    S->cweight = S->cphys[1] + c->plane_p;
+    {   // inflow weight of the ceiling: the blowing Gaussian of the bctype 3 profile
+        std::vector<double> m(c->nx);
+        const double L = c->cfg.xlen;
+        for (int i = 0; i < c->nx; ++i) {
+            const double e = (c->xpts[i] - c->cfg.x0 - 0.4 * L) / (0.05 * L);
+            m[i] = std::exp(-(e * e));
+        }
+        int ru = upload(S->cweight, m.data(), m.size() * sizeof(double), c->stream);
+        if (ru != LSB_OK) {
+            delete S;
+            return ru;
+        }
+    }
     for (int l = 0; l < 3; ++l) {
         S->u[l] = S_U0 + l;
         S->un[l] = S_UN0 + l;
@@ -954,6 +975,11 @@ static int update_once(lsb_ctx* c) {
     b.rhs1 = L.spec[S->rhs[1]];
     b.p_wall = L.spec[S->p] + (size_t)(c->nzp - 1) * c->plane_s;
     b.pn_wall = L.spec[S->pn] + (size_t)(c->nzp - 1) * c->plane_s;
+    b.p_top = L.spec[S->p];
+    b.pn_top = L.spec[S->pn];
+    b.top_bc = g.top_bc;
+    b.cdiff[0] = S->cdiff[0];
+    b.cdiff[1] = S->cdiff[1];
     b.d2w = S->d2w;
     b.bnd_top = nhom ? sc_top : nullptr;
     b.bnd_bot = nhom ? sc_bot : nullptr;
@@ -969,6 +995,7 @@ static int update_once(lsb_ctx* c) {
     b.pfac1 = pfac1;
     b.nhom = nhom ? 1 : 0;
     LSB_TRY(sp_bc_vel(c, b, s));
+    if (g.top_bc == 2 && vel_nhom) LSB_TRY(sp_ceiling_blend(c, S->bct, S->cdiff, S->cphys, S->cweight, s));
     // pnf <- pf (the new pressure is written into the old pnf buffer below)
     // viscous Helmholtz solves (navierstokes.f90:1388-1394)
     {
@@ -1060,8 +1087,11 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
         if (with_norm) LSB_TRY(state_filter_hook(c));
         LSB_TRY(update_once(c));
         if (with_norm) {
-            double* f[3] = {S->L.spec[S->u[0]], S->L.spec[S->u[1]], S->L.spec[S->u[2]]};
-            LSB_TRY(sp_norm(c, f, 3, c->stream));
+            // norm(uf, vf, wf, tempf) (navierstokes.f90:49, horfft.f90:153-224)
+            double* f[4] = {S->L.spec[S->u[0]], S->L.spec[S->u[1]], S->L.spec[S->u[2]], nullptr};
+            const bool with_t = temp_on(c);
+            if (with_t) f[3] = temp_field(c, 0);
+            LSB_TRY(sp_norm(c, f, with_t ? 4 : 3, c->stream));
             if (c->cfg.ifrz == 1 && c->wavx[0] == 0.0) {
                 for (int l = 0; l < 3; ++l)
                     LSB_HIP(hipMemset2DAsync(f[l], (size_t)c->plane_s * sizeof(double), 0, 2 * sizeof(double),

@@ -24,6 +24,7 @@ module lsb_capi
     integer(c_int) :: mapping, bctype, forcing
     integer(c_int) :: sgsflag, sgsmodel
     integer(c_int) :: dealiasflag, dealias_ref_compat
+    integer(c_int) :: dct_exact, top_bc
     integer(c_int) :: ifrz, nstart, precision, device
     real(c_double) :: xlen, ylen, zlen, x0, u0, xnu, blfac, prtl
     real(c_double) :: a, b, cs, kc, filt_order, filt_amp

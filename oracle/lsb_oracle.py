@@ -101,6 +101,8 @@ class OracleConfig:
     filt_amp: float = 20.0
     dealiasflag: int = 2
     dealias_ref_compat: int = 1
+    dct_exact: int = 0
+    top_bc: int = 0
     ifrz: int = 0
     nstart: int = 0
     dt: float = 1.0e-3
@@ -414,7 +416,13 @@ class Oracle:
         return out
 
     def dct_diff(self, y, p):
-        """chebyshev.f90:204-270 along axis 0 of y (n, ...): DCT-I, recurrence, DCT-I, *gf/2(n-1)."""
+        """chebyshev.f90:204-270 along axis 0 of y (n, ...): DCT-I, recurrence, DCT-I, *gf/2(n-1).
+
+        The reference divides only alpha(2:n-1) by the half interval (chebyshev.f90:242), so
+        the top coefficient alpha(n) misses the factor 1/(c_N * half) = 1/xl it needs; that
+        is exact for xl = 1 and doubles the derivative of T_N for the channel's xl = 2.
+        ``dct_exact = 1`` divides alpha(n) by xl, which makes the operator the exact
+        Gauss-Lobatto derivative on every mapping (and changes nothing when xl = 1)."""
         n = y.shape[0]
         half = 0.5 * self.xl
         nlog = 2 * (n - 1)
@@ -423,6 +431,8 @@ class Oracle:
         for _ in range(p):
             alpha = sfft.dct(beta, type=1, axis=0)
             alpha[1 : n - 1] = alpha[1 : n - 1] / half
+            if self.cfg.dct_exact:
+                alpha[n - 1] = alpha[n - 1] / self.xl
             beta = np.zeros_like(alpha)
             beta[n - 2] = 2.0 * (n - 1.0) * alpha[n - 1]
             for k in range(n - 1, 1, -1):               # 1-based k = n-1 .. 2
@@ -961,6 +971,21 @@ class Oracle:
             bcf[nzp - 1] = bfac * bcf[nzp - 1]
         return bcf
 
+    def ceiling_weight(self):
+        """Inflow weight m(x) of the bctype-3 ceiling: the blowing Gaussian G(0.4 Lx)."""
+        c = self.cfg
+        L = c.xlen
+        xr = self.xpts - c.x0
+        return np.exp(-(((xr - 0.4 * L) / (0.05 * L)) ** 2))
+
+    def ceiling_blend(self, free, dirichlet):
+        """Spectral ceiling plane (1 - m(x)) free + m(x) dirichlet, the product in physical space."""
+        m = np.zeros(self.nxpp)
+        m[: self.nx] = self.ceiling_weight()
+        a = self.horfft_inv(free[None])
+        b = self.horfft_inv(dirichlet[None])
+        return self.horfft_fwd((1.0 - m) * a + m * b)[0]
+
     # ------------------------------------------------------------ time coeffs
     @staticmethod
     def abcoef(nstart, istep):
@@ -1152,8 +1177,16 @@ class Oracle:
         dpf = self.dfdx(psf)
         uif[0, nzp - 1] = dt * bdfv * dpf[nzp - 1]
         wav2 = self.wavxx[None, :] + self.wavyy[:, None]
-        uif[0, 0] = -rhsf[0, 0] / (wav2 + dtnui / bdfv)
-        uif[1, 0] = -rhsf[1, 0] / (wav2 + dtnui / bdfv)
+        if c.top_bc == 1:
+            # Dirichlet ceiling: the wall's Zang-Hussaini value mirrored at the top
+            uif[0, 0] = dt * bdfv * dpf[0]
+        else:
+            uif[0, 0] = -rhsf[0, 0] / (wav2 + dtnui / bdfv)
+            uif[1, 0] = -rhsf[1, 0] / (wav2 + dtnui / bdfv)
+        if c.top_bc == 2:
+            dpy = self.dfdy(psf)
+            uif[0, 0] = self.ceiling_blend(uif[0, 0], dt * bdfv * dpf[0])
+            uif[1, 0] = self.ceiling_blend(uif[1, 0], dt * bdfv * dpy[0])
         dwdzf = self.d2fdz2(uif[2])
         nhom = (c.bctype > 0 and st["t"] >= c.istartbound) or c.forcing == 1
         if c.bctype > 0 and st["t"] >= c.istartbound:
@@ -1164,6 +1197,8 @@ class Oracle:
             uif[2, nzp - 1] = c.xnu * dt * bdfv * dwdzf[nzp - 1]
         dpf = self.dfdy(psf)
         uif[1, nzp - 1] = dt * bdfv * dpf[nzp - 1]
+        if c.top_bc == 1:
+            uif[1, 0] = dt * bdfv * dpf[0]
         for i in range(3):
             if nhom:
                 rhsf[i] = self.solve_nhom_helmholtz(rhsf[i], uif[i], dtnui / bdfv)
