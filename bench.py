@@ -10,10 +10,10 @@ The metric compares the two forms of the step: with --precision mixed (the
 default) the FP64 form runs first from the same initial condition, timed the
 same way, and the line reports the mixed form's throughput (value) and the FP64
 form's.  The accuracy leg then integrates both forms for --accuracy-steps (100)
-steps on the same grid, model and time step with no-slip walls (the configured
-ceiling's non-homogeneous boundary path diverges in the FP64 form itself within
-~30 steps at this Reynolds number) and reports the relative L2 difference of the
-final states per field.
+steps of the same workload (same ceiling, grid, model and time step) and
+reports the relative L2 difference of the final states per field, of the wall
+skin friction, and the separation and reattachment points.  A run whose state
+is not finite prints a null value and exits with status 1.
 
 Besides the metric the JSON line carries
   roofline      the dominant kernel class -- the largest device time in an untimed
@@ -75,22 +75,36 @@ def gpu_run(case, steps, warmup, precision, ic):
 
 
 def accuracy_leg(case, steps, ic):
-    """Relative L2 of the mixed form against the FP64 form after `steps` steps."""
+    """Relative L2 of the mixed form against the FP64 form after `steps` steps: the four
+    state fields, the wall skin friction Cf(x) (spanwise mean), and the separation and
+    reattachment points (first sign changes of Cf, reported as absolute differences in x)."""
     import numpy as np
     from lsbflow_amd.core import setup_case
 
-    states = {}
+    states, wall = {}, {}
     for prec in (0, 1):
         ctx = setup_case(case, precision=prec)
         ctx.state_set(*ic)
         ctx.timestep(steps)
         g = ctx.state_get()
+        wall[prec] = ctx.wall_shear()
         ctx.close()
         states[prec] = g
-    errs = {k: float(f"{np.linalg.norm(states[1][k] - states[0][k]) / np.linalg.norm(states[0][k]):.3e}")
-            for k in ("uf", "vf", "wf", "pf")}
-    finite = all(bool(np.all(np.isfinite(states[p][k]))) for p in (0, 1) for k in errs)
-    return errs, finite
+
+    def rel(a, b):
+        nb = np.linalg.norm(b)
+        return float(f"{np.linalg.norm(a - b) / nb:.3e}") if nb > 0 else float(np.linalg.norm(a - b))
+
+    errs = {k: rel(states[1][k], states[0][k]) for k in ("uf", "vf", "wf", "pf")}
+    errs["wall_shear"] = rel(wall[1][0], wall[0][0])
+    xs = {p: (wall[p][1], wall[p][2]) for p in (0, 1)}
+    finite = all(bool(np.all(np.isfinite(states[p][k]))) for p in (0, 1) for k in ("uf", "vf", "wf", "pf"))
+    def num(x):            # no sign change of Cf: no such point (JSON null)
+        return float(f"{x:.6g}") if np.isfinite(x) else None
+
+    sep = {"x_sep_fp64": num(xs[0][0]), "x_reatt_fp64": num(xs[0][1]),
+           "x_sep_abs_diff": num(abs(xs[1][0] - xs[0][0])), "x_reatt_abs_diff": num(abs(xs[1][1] - xs[0][1]))}
+    return errs, sep, finite
 
 
 COPY_BYTES = 1 << 30
@@ -121,13 +135,27 @@ def pmc_probe(case):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c4_sigma_les_512x193x256_hbm_traffic.json")
 
 
+def sources_digest():
+    """SHA-256 over the library's kernel and host sources (the build the counters measured)."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "lsbflow_amd", "csrc")
+    for f in sorted(os.listdir(d)):
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(case, cls, precision, p, steps):
     """HBM bytes per launch group of class `cls`, from the committed rocprofv3 --pmc
-    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), when they cover this workload."""
+    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), when they cover this workload and were
+    measured on the current library sources (the table carries their digest)."""
     if case is None or case.name != "c4_sigma_les" or not os.path.exists(TRAFFIC_FILE) or p["n"] == 0:
         return None
     with open(TRAFFIC_FILE) as fh:
         t = json.load(fh)
+    if t.get("sources_sha") != sources_digest():
+        return None
     d = t.get("classes", {}).get("mixed" if precision == 1 else "fp64", {}).get(cls)
     if d is None:
         return None
@@ -173,7 +201,8 @@ def pmc_summarize(fetch_csv, write_csv, out_path=TRAFFIC_FILE, steps_per_form=2)
     write_cal = COPY_BYTES / (W[ci][1] * 1024.0)
     ends = [i for i, (n, _) in enumerate(F) if "k_norm_nyq" in n]
     last_fp64 = ends[steps_per_form - 1]
-    out = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": F[ci][1], "write_kib": W[ci][1],
+    out = {"sources_sha": sources_digest(),
+           "calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": F[ci][1], "write_kib": W[ci][1],
                            "fetch_factor": fetch_cal, "write_factor": write_cal},
            "probe_steps_per_form": steps_per_form, "classes": {"fp64": {}, "mixed": {}}, "kernels": {}}
     for i, ((n, f), (_, w)) in enumerate(zip(F, W)):
@@ -222,13 +251,15 @@ def roofline(prof, precision, case=None, steps=1):
                 launches=p["n"], device_ms=round(p["ms"], 3))
 
 
-def cpu_baseline(case, sample_ny, sample_steps):
-    """The CPU oracle on a spanwise-reduced sample of the same workload, one thread."""
+def cpu_worker(case_name, sample_ny, sample_steps):
+    """One oracle sample (one thread): prints its wall time as JSON.  Runs in a child process
+    that never touches the GPU."""
     import numpy as np
     from threadpoolctl import threadpool_limits
     import lsbflow_amd as L
     from oracle import Oracle, OracleConfig
 
+    case = L.cases.CASES[case_name]
     sc = case.with_(ny=sample_ny)
     keys = OracleConfig.__dataclass_fields__
     with threadpool_limits(limits=1):
@@ -240,12 +271,47 @@ def cpu_baseline(case, sample_ny, sample_steps):
         for _ in range(sample_steps):
             o.timestep(st)
         t1 = time.perf_counter()
-    pts = sc.nx * sc.nzp * sc.ny
-    val = pts * sample_steps / (t1 - t0)
-    ok = bool(np.all(np.isfinite(st["uf"])))
-    return dict(value=float(f"{val:.4g}"), unit="grid-point-steps/s", cores=1, kind="port",
-                sample=f"{sample_steps} oracle steps of {sc.nx}x{sc.nzp}x{sc.ny} (nx x nzp x ny; spanwise "
-                       f"extent {sample_ny}/{case.ny} of the workload), {t1 - t0:.1f} s, finite={ok}")
+    print(json.dumps({"seconds": t1 - t0, "points": sc.nx * sc.nzp * sc.ny,
+                      "finite": bool(np.all(np.isfinite(st["uf"])))}), flush=True)
+    return 0
+
+
+def host_cores():
+    """CPU threads this process may use, capped at 16 (one GPU's share of a GPU node)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(case, sample_ny, sample_steps):
+    """The CPU oracle on every host core: one spanwise-reduced sample of the same workload per
+    core, run concurrently in child processes (one thread each); aggregate grid-point-steps/s
+    over the slowest worker's time."""
+    import subprocess
+
+    cores = host_cores()
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", case.name, str(sample_ny),
+           str(sample_steps)]
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+             for _ in range(cores)]
+    res = []
+    for p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu worker failed: {err[-400:]}")
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    wall = time.perf_counter() - t0
+    slowest = max(r["seconds"] for r in res)
+    val = sum(r["points"] for r in res) * sample_steps / slowest
+    return dict(value=float(f"{val:.4g}"), unit="grid-point-steps/s", cores=cores, kind="port",
+                sample=f"{cores} concurrent processes, one per core, each {sample_steps} oracle steps of "
+                       f"{case.nx}x{case.nzp}x{sample_ny} (nx x nzp x ny; spanwise extent {sample_ny}/{case.ny} "
+                       f"of the workload), slowest {slowest:.1f} s, wall {wall:.1f} s, "
+                       f"finite={all(r['finite'] for r in res)}")
 
 
 def diagnostics(case, reps):
@@ -331,7 +397,10 @@ def main(argv=None):
                          "counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE runs of --pmc-probe")
     ap.add_argument("--cpu-sample-ny", type=int, default=16)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    ap.add_argument("--cpu-worker", nargs=3, metavar=("CASE", "NY", "STEPS"), help=argparse.SUPPRESS)
     args = ap.parse_args(argv)
+    if args.cpu_worker:
+        return cpu_worker(args.cpu_worker[0], int(args.cpu_worker[1]), int(args.cpu_worker[2]))
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
     if args.gpus != 1:
@@ -356,12 +425,11 @@ def main(argv=None):
                  "f32 MFMA), f64 state, right-hand sides and pressure step; velocity solves as an f64 leading "
                  "term (-rhs/(k^2+sigma)) plus an f32-MFMA eigen-projection correction")
         precision = 1
-        acc_case = case.with_(bctype=0)
-        acc_errs, acc_finite = accuracy_leg(acc_case, args.accuracy_steps, ic)
-        accuracy = {"workload": f"{case.name} grid/model/dt with no-slip walls (bctype 0), "
-                                f"{args.accuracy_steps} steps from the same seed",
-                    "rel_l2": acc_errs, "finite": acc_finite,
-                    "velocity_within_1e-5": bool(max(acc_errs[k] for k in ("uf", "vf", "wf")) <= 1e-5)}
+        acc_errs, acc_sep, acc_finite = accuracy_leg(case, args.accuracy_steps, ic)
+        accuracy = {"workload": f"{case.name}, the timed workload, {args.accuracy_steps} steps from the same seed",
+                    "rel_l2": acc_errs, "separation": acc_sep, "finite": acc_finite,
+                    "fields_within_1e-5": {k: bool(v <= 1e-5) for k, v in acc_errs.items()},
+                    "all_within_1e-5": bool(max(acc_errs.values()) <= 1e-5)}
     else:
         secs, prof, dom, finite = secs64, prof64, dom64, finite64
         dtype = "f64"
@@ -381,10 +449,11 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic: seeded tanh base flow + curl of a random vector potential (rms 1e-3), "
-                "ceiling suction/blowing profile",
+                "ceiling suction/blowing profile (amplitude 0.3)",
         "config": {"workload": f"{case.name}: {case.nx}x{case.nzp}x{case.ny} (nx x nzp x ny), sgsmodel="
                                f"{case.sgsmodel if case.sgsflag else 'off'}, dt={case.dt}, xnu={case.xnu:.3g}",
                    "nx": case.nx, "nzp": case.nzp, "ny": case.ny, "precision": args.precision,
+                   "ceiling": f"bctype {case.bctype}, top_bc {case.top_bc}, dct_exact {case.dct_exact}",
                    "finite_after_run": finite},
         "fp64": fp64,
         "speedup_vs_fp64": round(secs64 / secs, 3),
@@ -395,8 +464,12 @@ def main(argv=None):
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_ny, args.cpu_sample_steps)
+    ok = finite and finite64 and (accuracy is None or accuracy["finite"])
+    if not ok:
+        out["value"] = None
+        out["error"] = "non-finite state after the timed steps or the accuracy leg"
     print(json.dumps(out), flush=True)
-    return 0
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":
