@@ -369,6 +369,17 @@ int lsb_horfft_dev(lsb_ctx* ctx, double* ain, double* ainf, int is, int nfields,
 int lsb_dfdz_dev(lsb_ctx* ctx, const double* uf, double* dudzf, int nfields, void* stream);
 int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields, void* stream);
 
+/* lsb_horfft_dev without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * lsb_horfft_dev; the inverse transform's staging buffer is a library-owned device
+ * array of 256 MiB per GPU (2^25 doubles; resolved by lsb_create, held until the
+ * process ends) instead of a buffer allocated on first use, and the call makes no
+ * hipMalloc/hipFree.  LSB_E_ARG when one field's staging (nzp*ny*nx doubles) exceeds
+ * the array; fields are transformed in batches that fit it.  All calls on one GPU
+ * share the array: calls that overlap on a GPU (two streams, two threads) race. */
+int lsb_horfft_dev_noalloc(lsb_ctx* ctx, double* ain, double* ainf, int is, int nfields, void* stream);
+
 /* Kernel timing for roofline accounting.  Between lsb_profile_begin and
  * lsb_profile_end every launch group is bracketed by HIP events on the launch
  * stream and tagged with its algorithmic work.  Per class c (0 FFT, 1 Chebyshev

@@ -82,6 +82,7 @@ _SIGS = {
     "lsb_state_device_ptrs": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "lsb_stream": (_vp, [_vp]),
     "lsb_horfft_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "lsb_horfft_dev_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lsb_dfdz_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "lsb_solve_helmholtz_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp]),
     "lsb_profile_begin": (C.c_int, [_vp]),

@@ -265,6 +265,7 @@ extern "C" int lsb_create(const lsb_config* cfg, lsb_ctx** out) {
     }
     rc = hor_plan_create(&c->hp, c->nx, c->ny);
     if (rc == LSB_OK) rc = upload_context(c);
+    if (rc == LSB_OK) rc = noalloc_resolve(g.device);
     if (rc != LSB_OK) {
         lsb_destroy(c);
         return rc;

@@ -462,7 +462,7 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
 bool horfft_has_ops(const HorPlan& hp) { return fast_fft_len(hp.ny) && fast_fft_len(hp.lh); }
 
 int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
-                   int nplanes, const int* ops, hipStream_t s) {
+                   int nplanes, const int* ops, hipStream_t s, double* scratch) {
     if (nf < 1 || nf > kMaxFields) {
         set_error("horfft: 1..16 fields per launch");
         return LSB_E_ARG;
@@ -487,9 +487,9 @@ int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, dou
     if (horfft_has_ops(hp)) {
         // y pass into a compact scratch (rows of lh complex, 128-byte aligned), then the
         // x pass out of place into the padded physical rows
-        void* scr = nullptr;
+        void* scr = scratch;
         const size_t per = (size_t)nplanes * ny * nx;
-        LSB_TRY(stage_get(c, 8, per * nf * sizeof(double), &scr));
+        if (!scr) LSB_TRY(stage_get(c, 8, per * nf * sizeof(double), &scr));
         FieldList<double> mid{};
         for (int f = 0; f < nf; ++f) mid.p[f] = (double*)scr + per * f;
         LSB_TRY((fast_yinv<double, double, double>(in, mid, nplanes, nf, lh, ny, lh, (const cpx<double>*)hp.tw_y, fo,

@@ -301,8 +301,13 @@ int horfft_xfwd(lsb_ctx* c, const HorPlan& hp, const T* const* phys, T* const* r
 struct YRhsArgs;
 template <class R, class TH>
 int horfft_yfwd_rhs(lsb_ctx* c, const HorPlan& hp, R* const* pieces, int nplanes, const YRhsArgs& a, hipStream_t s);
+// scratch: the y pass's staging buffer (nf * nplanes * ny * nx doubles) supplied by the
+// caller; null takes the context's staging slot 8 (allocated on first use)
 int horfft_inv_ops(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double* const* phys, int nf,
-                   int nplanes, const int* ops, hipStream_t s);
+                   int nplanes, const int* ops, hipStream_t s, double* scratch = nullptr);
+// allocation-free entry points (noalloc.hip): resolve the device address of the
+// file-scope scratch array on `device` (the current device); called by lsb_create
+int noalloc_resolve(int device);
 // u, du/dx, du/dy of 3 spectra (one load each); RT = physical storage and arithmetic type
 template <class RT>
 int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
