@@ -45,11 +45,13 @@ PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 vector = FP64 matrix
 PEAK_FP32_TFLOPS = 157.3       # MI355X dense FP32 matrix
 
 
-def gpu_run(case, steps, warmup, precision, ic):
+def gpu_run(case, steps, warmup, precision, ic, tune=True):
     import numpy as np
     from lsbflow_amd.core import setup_case
 
     ctx = setup_case(case, precision=precision)
+    # launch shapes from the autotuner, timed on this grid before the warm-up (untimed)
+    table = ctx.autotune() if tune else ctx.tune_table()
     ctx.state_set(*ic)
     if warmup > 0:
         ctx.timestep(warmup)
@@ -71,7 +73,7 @@ def gpu_run(case, steps, warmup, precision, ic):
     g = ctx.state_get()
     finite = all(bool(np.all(np.isfinite(g[k]))) for k in ("uf", "vf", "wf", "pf"))
     ctx.close()
-    return t1 - t0, (prof_all, prof_dom), finite, {k: g[k] for k in ("uf", "vf", "wf", "pf")}
+    return t1 - t0, (prof_all, prof_dom), finite, table
 
 
 def accuracy_leg(case, steps, ic):
@@ -385,6 +387,7 @@ def main(argv=None):
     ap.add_argument("--case", default="c4_sigma_les")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes (no lsb_autotune)")
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="the workload of the HBM-traffic counter runs: a 1 GiB device copy (calibration), "
@@ -416,11 +419,13 @@ def main(argv=None):
         return diagnostics(case, args.steps)
     ic = L.cases.initial_condition(case)
     pts = case.points
-    secs64, (prof64, dom64), finite64, st64 = gpu_run(case, args.steps, args.warmup, 0, ic)
+    tune = not args.no_autotune
+    secs64, (prof64, dom64), finite64, tune64 = gpu_run(case, args.steps, args.warmup, 0, ic, tune)
     fp64 = {"value": float(f"{pts * args.steps / secs64:.6g}"), "ms_per_step": round(1e3 * secs64 / args.steps, 4),
             "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof64.items()}}
+    table = tune64
     if args.precision == "mixed":
-        secs, (prof, dom), finite, st = gpu_run(case, args.steps, args.warmup, 1, ic)
+        secs, (prof, dom), finite, table = gpu_run(case, args.steps, args.warmup, 1, ic, tune)
         dtype = ("mixed: f32 nonlinear chain (FFTs, products/SGS storage, its two Chebyshev contractions on "
                  "f32 MFMA), f64 state, right-hand sides and pressure step; velocity solves as an f64 leading "
                  "term (-rhs/(k^2+sigma)) plus an f32-MFMA eigen-projection correction")
@@ -461,6 +466,7 @@ def main(argv=None):
         # the dominant class of the untimed all-class pass, measured again in the timed steps
         "roofline": roofline(dom, precision, case, args.steps),
         "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
+        "autotune": table,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_ny, args.cpu_sample_steps)

@@ -67,6 +67,8 @@ typedef struct lsb_config {
                                        Zang-Hussaini values mirrored at the ceiling;
                                        2 (bctype 3): (1 - m(x)) free + m(x) Dirichlet,
                                        m = the profile's blowing Gaussian G(0.4 Lx) */
+    int dyn_avg;                    /* dynamic model <M:L>/<M:M>: 0 the local 3x3x3 filter
+                                       (sgsterm.f90:243-244), 1 the x-y plane means      */
     int ifrz;                       /* navierstokes.f90:53-59                    */
     int nstart;                     /* 0: Euler/BDF2 start (navierstokes.f90:1507) */
     int precision;                  /* LSB_FP64 or LSB_MIXED                     */
@@ -242,6 +244,17 @@ int lsb_state_device_ptrs(lsb_ctx* ctx, void** uf, void** vf, void** wf, void** 
 void* lsb_stream(lsb_ctx* ctx);
 /* Wait for all work queued on the context stream (lsb_timestep only enqueues). */
 int lsb_synchronize(lsb_ctx* ctx);
+
+/* Launch-shape autotuner (no counterpart in the reference, whose FFTW plans are
+ * chosen by FFTW_MEASURE, modfftw.f90:93-132): times every launch variant of each
+ * tunable kernel family on this context's grid and keeps the fastest; scratch is
+ * allocated for the duration of the call.  Families (slot: meaning): 0 the 3-point
+ * filter on FP64 fields, 1 on FP32 fields; -1 in a slot means the built-in default.
+ * lsb_autotune_get reads n slots (variant index, measured ms per launch),
+ * lsb_autotune_set installs a saved table. */
+int lsb_autotune(lsb_ctx* ctx);
+int lsb_autotune_get(const lsb_ctx* ctx, int* variant, float* ms, int n);
+int lsb_autotune_set(lsb_ctx* ctx, const int* variant, int n);
 
 /* ------------------------- snapshots, restart files, per-step diagnostics */
 

@@ -85,6 +85,7 @@ class Case:
     dealiasflag: int = 2
     dct_exact: int = 0
     top_bc: int = 0
+    dyn_avg: int = 0
     dt: float = 1.0e-3
     nsteps: int = 10
     delta: float = 0.2

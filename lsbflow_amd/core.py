@@ -61,6 +61,7 @@ class Config:
     dealias_ref_compat: int = 1
     dct_exact: int = 0
     top_bc: int = 0
+    dyn_avg: int = 0
     ifrz: int = 0
     nstart: int = 0
     precision: int = 0
@@ -443,6 +444,25 @@ class Context:
 
     def synchronize(self):
         check(self._L.lsb_synchronize(self.h), "synchronize")
+
+    TUNE_FAMILIES = ("filter3_f64", "filter3_f32")
+
+    def autotune(self):
+        """Time the launch variants of the tunable kernels on this grid; returns the table."""
+        check(self._L.lsb_autotune(self.h), "autotune")
+        return self.tune_table()
+
+    def tune_table(self):
+        n = len(self.TUNE_FAMILIES)
+        v = (C.c_int * n)()
+        ms = (C.c_float * n)()
+        check(self._L.lsb_autotune_get(self.h, v, ms, n), "autotune_get")
+        return {f: {"variant": v[i], "ms": round(float(ms[i]), 4)} for i, f in enumerate(self.TUNE_FAMILIES)}
+
+    def set_tune_table(self, table):
+        n = len(self.TUNE_FAMILIES)
+        v = (C.c_int * n)(*[int(table[f]["variant"]) for f in self.TUNE_FAMILIES])
+        check(self._L.lsb_autotune_set(self.h, v, n), "autotune_set")
 
     # ------------------------------------------ snapshots, restart, diagnostics
     def _fp(self, a):

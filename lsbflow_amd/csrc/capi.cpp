@@ -1232,6 +1232,39 @@ extern "C" int lsb_synchronize(lsb_ctx* c) {
     return LSB_OK;
 }
 
+// Launch-shape autotuner: times every variant of each tunable kernel family on this
+// context's grid (scratch fields allocated for the duration of the call) and keeps the
+// fastest in the context; later launches of the family use it.
+extern "C" int lsb_autotune(lsb_ctx* c) {
+    LSB_TRY(check_ctx(c));
+    LSB_HIP(hipSetDevice(c->device));
+    constexpr int nf = 8;
+    void* work = nullptr;
+    const size_t bytes = 2 * (size_t)nf * c->nphys * sizeof(double);
+    LSB_TRY(dev_alloc(&work, bytes));
+    int rc = LSB_OK;
+    if (hipMemsetAsync(work, 0, bytes, c->stream) != hipSuccess) rc = LSB_E_HIP;
+    if (rc == LSB_OK) rc = ph_tune_filt3(c, work, nf, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(work);
+    return rc;
+}
+
+extern "C" int lsb_autotune_get(const lsb_ctx* c, int* variant, float* ms, int n) {
+    LSB_TRY(check_ctx(c));
+    for (int i = 0; i < n && i < lsb::TUNE_SLOTS; ++i) {
+        if (variant) variant[i] = c->tune[i];
+        if (ms) ms[i] = c->tune_ms[i];
+    }
+    return LSB_OK;
+}
+
+extern "C" int lsb_autotune_set(lsb_ctx* c, const int* variant, int n) {
+    LSB_TRY(check_ctx(c));
+    for (int i = 0; i < n && i < lsb::TUNE_SLOTS; ++i) c->tune[i] = variant[i];
+    return LSB_OK;
+}
+
 extern "C" int lsb_profile_end(lsb_ctx* c, double* ms, int* count, double* flops, double* bytes) {
     LSB_TRY(check_ctx(c));
     LSB_HIP(hipStreamSynchronize(c->stream));

@@ -1,0 +1,277 @@
+// Dynamic Smagorinsky model of the mixed form (sgsterm.f90:123-280) and the plane means of
+// the plane-averaged coefficient (both forms).
+//
+// Mixed form: every work field is stored in FP32 and every expression is evaluated in FP64
+// registers.  The fields hold perturbation quantities only; the base flow U = (ubar(z), 0, 0)
+// enters the kernels as FP64 profiles, so no FP32 field carries an O(1) part that later
+// cancels:
+//   stage 1  from u', g' (perturbation velocity and gradient): the advective term, the
+//            perturbation products p_lm = u_l u_m - U_l U_m, the perturbation strain
+//            S'_lm = (g'_lm + g'_ml)/2, |S| and L1_lm = delta^2 |S| S_lm of the full strain;
+//   filter   S', L1, u', p with the (a, b) test filter;
+//   stage 2  M_lm = 2 (L1^ - (2 delta)^2 |S^| S^_lm) with S^ = S'^ + (S^U)^ and
+//            L_lm = (u_l u_m)^ - u^_l u^_m = p^_lm - (U^_l u'^_m + u'^_l U^_m + u'^_l u'^_m)
+//                   + [(U^2)^ - (U^)^2] (0,0 only),
+//            numerator M:L and denominator M:M;
+//   <.>      (1, 1) filter of both, or their plane means;
+//   stage 3  C = <M:L>/<M:M> clipped to [0, 0.04], tau = -2 C delta^2 |S| S, and
+//            h_lm = p_lm/2 + tau_lm written over p (h_00 without its plane-constant ubar^2/2,
+//            as the mixed form's other models store it).
+// The z-filtered base profiles U^, (S^U)^_02 and (U^2)^ - (U^)^2 are precomputed in FP64.
+// Symmetric tensors are stored once per distinct component in the order
+// (0,0), (1,0), (2,0), (1,1), (2,1), (2,2).
+
+#include <cmath>
+
+#include "lsb_internal.h"
+#include "physical.h"
+
+namespace lsb {
+
+namespace {
+
+struct DDims {
+    int nx, ny, nxpp, nzp;
+};
+
+DDims ddims(const lsb_ctx* c) { return DDims{c->nx, c->ny, c->nxpp, c->nzp}; }
+
+constexpr int kL[6] = {0, 1, 2, 1, 2, 2};
+constexpr int kM[6] = {0, 0, 0, 1, 1, 2};
+
+// grid: x in blocks of 256 points, one (j, k) row per blockIdx.y (strided)
+dim3 row_grid(const DDims& d) {
+    const unsigned gx = (unsigned)((d.nx + 255) / 256);
+    const long rows = (long)d.ny * d.nzp;
+    return dim3(gx, (unsigned)(rows < 65535 ? rows : 65535));
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const long o = (long)row * d.nxpp + i;
+        const double U = a.ubar[k];
+        double up[3], u[3], g[3][3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) up[l] = (double)a.ui[l][o];
+        u[0] = up[0] + U;
+        u[1] = up[1];
+        u[2] = up[2];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) g[l][m] = (double)a.g[l + 3 * m][o];
+        // advective term -1/2 u_m du_l/dx_m with the full gradient (g'_02 + ubarp), sponge
+        const double g02 = g[0][2] + a.ubarp[k];
+        const double lam = a.lam ? a.lam[i] : 0.0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * ((l == 0 && m == 2) ? g02 : g[l][m]);
+            if (a.lam) s = s - lam * up[l];  // lambda (target - u): target = ubar for u, 0 for v, w
+            a.nlt[l][o] = (float)s;
+        }
+        // perturbation products: u_l u_m - U_l U_m (only U_0 = ubar is nonzero)
+        a.p[0][o] = (float)(up[0] * (u[0] + U));
+        a.p[1][o] = (float)(up[1] * u[0]);
+        a.p[2][o] = (float)(up[2] * u[0]);
+        a.p[3][o] = (float)(up[1] * up[1]);
+        a.p[4][o] = (float)(up[2] * up[1]);
+        a.p[5][o] = (float)(up[2] * up[2]);
+        // strain of the perturbation and of the full field
+        double sp[6], sn = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sp[q] = 0.5 * (g[kL[q]][kM[q]] + g[kM[q]][kL[q]]);
+        const double su = 0.5 * a.ubarp[k];  // base strain S^U_20 = S^U_02
+        double sf[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            sf[q] = sp[q] + (q == 2 ? su : 0.0);
+            const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;  // off-diagonals appear twice
+            sn = sn + w * 2.0 * sf[q] * sf[q];
+        }
+        sn = sqrt(sn);
+        const double dl = a.delta[k];
+        a.snorm[o] = (float)sn;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            a.sp[q][o] = (float)sp[q];
+            a.l1[q][o] = (float)(dl * dl * sn * sf[q]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const long o = (long)row * d.nxpp + i;
+        const double Ub = a.base[k];
+        const double SUh = a.base[d.nzp + k];
+        const double LU = a.base[2 * d.nzp + k];
+        double sh[6], snh = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            sh[q] = (double)a.sph[q][o] + (q == 2 ? SUh : 0.0);
+            const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;
+            snh = snh + w * 2.0 * sh[q] * sh[q];
+        }
+        snh = sqrt(snh);
+        const double cd = 2.0 * a.delta[k];
+        double uh[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) uh[l] = (double)a.uh[l][o];
+        const double Uh[3] = {Ub, 0.0, 0.0};
+        double num = 0.0, den = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int l = kL[q], m = kM[q];
+            const double mij = 2.0 * ((double)a.l1h[q][o] - cd * cd * snh * sh[q]);
+            double lij = (double)a.ph[q][o] - (Uh[l] * uh[m] + uh[l] * Uh[m] + uh[l] * uh[m]);
+            if (q == 0) lij = lij + LU;
+            const double w = (l == m) ? 1.0 : 2.0;
+            num = num + w * mij * lij;
+            den = den + w * mij * mij;
+        }
+        a.num[o] = (float)num;
+        a.den[o] = (float)den;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dynm_stage3(DDims d, DynMixArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const long o = (long)row * d.nxpp + i;
+        double ct = a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : (double)a.numf[o] / (double)a.denf[o];
+        if (ct < 0.0 || isnan(ct)) ct = 0.0;
+        if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
+        const double dl = a.delta[k];
+        const double nu = ct * dl * dl * (double)a.snorm[o];
+        const double su = 0.5 * a.ubarp[k];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double s = (double)a.sp[q][o] + (q == 2 ? su : 0.0);
+            const double tau = -2.0 * nu * s;
+            a.h[q][o] = (float)(0.5 * (double)a.h[q][o] + tau);  // h holds p on entry
+        }
+    }
+}
+
+// Plane means of two fields over the nx physical points of every row: B partial sums per
+// plane (one block each), then the B partials of each plane summed in a fixed order.
+constexpr int kPlaneBlocks = 16;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_plane_partial2(DDims d, const T* __restrict__ f0, const T* __restrict__ f1,
+                                                        double* __restrict__ part) {
+    const int k = blockIdx.y;
+    const int b = blockIdx.x;
+    const long n = (long)d.nx * d.ny;
+    double s0 = 0.0, s1 = 0.0;
+    for (long t = (long)b * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % d.nx), j = (int)(t / d.nx);
+        const long o = ((long)k * d.ny + j) * d.nxpp + i;
+        s0 += (double)f0[o];
+        s1 += (double)f1[o];
+    }
+    __shared__ double sh0[256], sh1[256];
+    sh0[threadIdx.x] = s0;
+    sh1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + h];
+            sh1[threadIdx.x] += sh1[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[((long)k * kPlaneBlocks + b) * 2 + 0] = sh0[0];
+        part[((long)k * kPlaneBlocks + b) * 2 + 1] = sh1[0];
+    }
+}
+
+__global__ void k_plane_finish2(DDims d, const double* __restrict__ part, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.nzp) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = 0; b < kPlaneBlocks; ++b) {
+        s0 += part[((long)k * kPlaneBlocks + b) * 2 + 0];
+        s1 += part[((long)k * kPlaneBlocks + b) * 2 + 1];
+    }
+    const double n = (double)d.nx * (double)d.ny;
+    out[k] = s0 / n;
+    out[d.nzp + k] = s1 / n;
+}
+
+template <class T>
+int ph_plane_mean2(lsb_ctx* c, const T* f0, const T* f1, double* scratch, double* out, hipStream_t s) {
+    const DDims d = ddims(c);
+    hipLaunchKernelGGL((k_plane_partial2<T>), dim3(kPlaneBlocks, (unsigned)d.nzp), dim3(256), 0, s, d, f0, f1,
+                       scratch);
+    LSB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_plane_finish2, dim3((unsigned)((d.nzp + 63) / 64)), dim3(64), 0, s, d, scratch, out);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+template int ph_plane_mean2<double>(lsb_ctx*, const double*, const double*, double*, double*, hipStream_t);
+template int ph_plane_mean2<float>(lsb_ctx*, const float*, const float*, double*, double*, hipStream_t);
+
+size_t ph_plane_mean2_scratch(const lsb_ctx* c) { return (size_t)c->nzp * kPlaneBlocks * 2; }
+
+int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s) {
+    const DDims d = ddims(c);
+    const dim3 g = row_grid(d);
+    int tok = -1;
+    // FP32 fields read + written per point by each stage
+    const double per = (stage == 1) ? 12.0 + 22.0 : (stage == 2 ? 21.0 + 2.0 : 2.0 + 1.0 + 6.0 + 12.0);
+    prof_mark_begin(c, PC_NL, s, &tok, 0.0, per * 4.0 * (double)d.nx * d.ny * d.nzp);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL(k_dynm_stage1, g, dim3(256), 0, s, d, a); break;
+        case 2: hipLaunchKernelGGL(k_dynm_stage2, g, dim3(256), 0, s, d, a); break;
+        default: hipLaunchKernelGGL(k_dynm_stage3, g, dim3(256), 0, s, d, a); break;
+    }
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+// z-filtered base profiles of the mixed dynamic model (FP64, host): U^ = filter(ubar),
+// (S^U)^_02 = filter(ubarp / 2), (U^2)^ - (U^)^2.  The x-y passes leave a function of z
+// unchanged; the z pass uses the interpolating weights of interior planes (the boundary
+// planes keep their value, filters.f90:450-453).
+void dyn_base_profiles(const lsb_ctx* c, std::vector<double>& out) {
+    const int n = c->nzp;
+    const double s = 2.0 * c->cfg.a + c->cfg.b;
+    const std::vector<double>& w = c->fz_ab;
+    auto zf = [&](const std::vector<double>& v, int k) {
+        if (k == 0 || k == n - 1) return v[k];
+        return (w[k] * v[k - 1] + w[n + k] * v[k] + w[2 * n + k] * v[k + 1]) / s;
+    };
+    std::vector<double> u(n), su(n), u2(n);
+    for (int k = 0; k < n; ++k) {
+        u[k] = c->ubar[k];
+        su[k] = 0.5 * c->ubarp[k];
+        u2[k] = c->ubar[k] * c->ubar[k];
+    }
+    out.assign(3 * (size_t)n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        const double ub = zf(u, k);
+        out[k] = ub;
+        out[n + k] = zf(su, k);
+        out[2 * n + k] = zf(u2, k) - ub * ub;
+    }
+}
+
+}  // namespace lsb

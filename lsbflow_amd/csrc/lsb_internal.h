@@ -92,6 +92,8 @@ struct ZSplitOp {
 };
 
 constexpr int kMaxFields = 16;
+// launch-shape tuning table of a context (lsb_autotune): the variant index per kernel family
+enum { TUNE_FILT3_F64 = 0, TUNE_FILT3_F32 = 1, TUNE_SLOTS = 16 };
 // leading dimension of the two-row boundary operator d_md2b (rows 0 and nzp-1 of d2fdz2)
 constexpr int kLdmBnd = 64;
 // doubles in the reduction / small-table scratch d_red
@@ -200,6 +202,9 @@ struct lsb_ctx {
     int s_u[3], s_un[3], s_unm[3], s_nl[3], s_nlm[3], s_p, s_pn;
     double* bc_spec = nullptr;  // spectral bound() field, planes 0 and nzp-1 (2*plane_s)
 
+    // launch-shape variants chosen by lsb_autotune (-1: the built-in default)
+    int tune[lsb::TUNE_SLOTS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    float tune_ms[lsb::TUNE_SLOTS] = {};
     // profiling
     bool profiling = false;
     unsigned prof_mask = ~0u;  // classes bracketed while profiling

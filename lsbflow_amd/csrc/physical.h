@@ -41,10 +41,51 @@ struct DynArgs {
     double* den;
     const double* numf;
     const double* denf;
+    const double* cplane;  // plane means <M:L>(k), <M:M>(k) (2 nzp) instead of numf/denf, or null
     double* tau[9];
+    double* h[9];          // non-null: h = uu/2 + tau written over uu (the distinct components)
+    const double* uu[9];
+    int hsym;              // h/uu are the symmetric view (6 distinct fields)
+};
+
+// mixed-form dynamic model (dynamic.hip): FP32 fields, symmetric tensors as 6 fields
+// in the order (0,0), (1,0), (2,0), (1,1), (2,1), (2,2)
+struct DynMixArgs {
+    const float* ui[3];    // perturbation velocity
+    const float* g[9];     // perturbation gradient at l + 3m
+    const double* ubar;    // nzp
+    const double* ubarp;   // nzp
+    const double* delta;   // nzp (dynamic widths)
+    const double* lam;     // sponge lambda(x) or null
+    float* nlt[3];         // stage 1 outputs
+    float* p[6];
+    float* sp[6];
+    float* l1[6];
+    float* snorm;
+    const float* sph[6];   // stage 2 inputs (filtered)
+    const float* l1h[6];
+    const float* uh[3];
+    const float* ph[6];
+    const double* base;    // 3 nzp: U^, (S^U)^_02, (U^2)^ - (U^)^2
+    float* num;
+    float* den;
+    const float* numf;     // stage 3 inputs
+    const float* denf;
+    const double* cplane;  // or the plane means (2 nzp)
+    float* h[6];           // holds p on entry, h = p/2 + tau on exit
 };
 
 int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s);
+int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s);
+// out[k] = mean of f0 over plane k, out[nzp + k] = mean of f1 (nx physical points per row);
+// scratch: ph_plane_mean2_scratch(c) doubles
+template <class T>
+int ph_plane_mean2(lsb_ctx* c, const T* f0, const T* f1, double* scratch, double* out, hipStream_t s);
+size_t ph_plane_mean2_scratch(const lsb_ctx* c);
+void dyn_base_profiles(const lsb_ctx* c, std::vector<double>& out);
+int ph_filter3_f32(lsb_ctx* c, const float* const* in, float* const* out, int nf, double a, double b, hipStream_t s);
+// time the fused filter's launch variants (FP64 and FP32) on nf scratch fields and record them
+int ph_tune_filt3(lsb_ctx* c, void* work, int nf, hipStream_t s);
 // qb = (first ? 0 : qb) + coef*q on the physical points of nf fields
 int ph_axpy(lsb_ctx* c, const double* const* q, double* const* qb, int nf, double coef, bool first, hipStream_t s);
 // e_host[k] = sum over the nf fields and the physical points of plane k of (q - f)^2 (synchronises)

@@ -353,13 +353,16 @@ __global__ void k_dyn2(PhDims d, DynArgs a) {
         a.den[off] = den;
     }
 }
-// stage 3 (sgsterm.f90:248-267): C = <ML>/<MM> clipped to [0, 0.04], tau = -2 C delta^2 |S| S_ij
+// stage 3 (sgsterm.f90:248-267): C = <ML>/<MM> clipped to [0, 0.04], tau = -2 C delta^2 |S| S_ij;
+// <.> the local filter (numf, denf) or the plane means (cplane).  With h set, h = uu/2 + tau is
+// written over uu instead of tau (the step's h_lm = 1/2 u_l u_m + tau_lm, navierstokes.f90:769),
+// once per distinct component.
 __global__ void k_dyn3(PhDims d, DynArgs a) {
     PH_LOOP(d) {
         int i, j, k;
         long off;
         ph_index(d, e, i, j, k, off);
-        double ct = a.numf[off] / a.denf[off];
+        double ct = a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : a.numf[off] / a.denf[off];
         if (ct < 0.0 || isnan(ct)) ct = 0.0;
         if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
         const double dl = a.delta[k];
@@ -369,7 +372,13 @@ __global__ void k_dyn3(PhDims d, DynArgs a) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
                 const int q = l + 3 * m;
-                if (a.tau[q]) a.tau[q][off] = -2.0 * nu * a.sij[q][off];
+                const double tau = -2.0 * nu * a.sij[q][off];
+                if (a.h[q]) {
+                    if (a.hsym && l < m) continue;  // (l, m) and (m, l) share one field
+                    a.h[q][off] = 0.5 * a.uu[q][off] + tau;
+                } else if (a.tau[q]) {
+                    a.tau[q][off] = tau;
+                }
             }
     }
 }
@@ -459,15 +468,16 @@ __global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, 
 // xy-filtered values of planes k-1, k, k+1 in registers (threads over x, NXT points each), so
 // every input value is read from HBM about once (halo rows and planes aside) and the
 // xy-filtered field never goes through HBM.  Same per-element expressions as the two passes.
-template <int R, int NXT>
-__global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double> in, FieldList<double> out, double a,
+// T is the storage type (double; float in the mixed form); the arithmetic is FP64.
+template <class T, int R, int NXT>
+__global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, FieldList<T> out, double a,
                                                      double b, const double* __restrict__ w, int KC) {
     const int f = blockIdx.z;
     const int J0 = blockIdx.x * R;
     const int k0 = blockIdx.y * KC;
     const int k1 = min(k0 + KC, d.nzp);
-    const double* q = in.p[f];
-    double* o = out.p[f];
+    const T* q = in.p[f];
+    T* o = out.p[f];
     const double s = 2. * a + b;
     const double s2 = 2.00 * a + b;
     const long plane = (long)d.ny * d.nxpp;
@@ -482,14 +492,14 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double>
         double X[R + 2][NXT];
 #pragma unroll
         for (int rr = 0; rr < R + 2; ++rr) {
-            const double* row = q + (long)k * plane + (long)jr[rr] * d.nxpp;
+            const T* row = q + (long)k * plane + (long)jr[rr] * d.nxpp;
 #pragma unroll
             for (int t = 0; t < NXT; ++t) {
                 const int i = threadIdx.x + 256 * t;
                 if (i < d.nx) {
                     const int im = (i == 0) ? d.nx - 1 : i - 1;
                     const int ip = (i == d.nx - 1) ? 0 : i + 1;
-                    X[rr][t] = (a * row[im] + b * row[i] + a * row[ip]) / s;
+                    X[rr][t] = (a * (double)row[im] + b * (double)row[i] + a * (double)row[ip]) / s;
                 } else {
                     X[rr][t] = 0.0;
                 }
@@ -511,7 +521,7 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double>
         for (int r = 0; r < R; ++r) {
             const int j = J0 + r;
             if (j >= d.ny) continue;
-            double* orow = o + (long)k * plane + (long)j * d.nxpp;
+            T* orow = o + (long)k * plane + (long)j * d.nxpp;
 #pragma unroll
             for (int t = 0; t < NXT; ++t) {
                 const int i = threadIdx.x + 256 * t;
@@ -523,7 +533,7 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double>
                 } else {
                     v = Yc[r][t];
                 }
-                orow[i] = v;
+                orow[i] = (T)v;
             }
         }
 #pragma unroll
@@ -536,12 +546,180 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<double>
     }
 }
 
-template <int R, int NXT>
-static void launch_filt3_fused(const PhDims& d, const FieldList<double>& fi, const FieldList<double>& fo, int nb,
-                               double a, double b, const double* w, hipStream_t s) {
-    constexpr int KC = 16;
+// The same filter through LDS: a block owns an x-tile of TX points by R rows of j and
+// slides over KC planes of k.  Each plane's (R + 2) x (TX + 2) input tile (periodic halos)
+// goes global -> registers (issued one plane ahead) -> LDS; the x pass fills a second LDS
+// array, the y pass gives the xy-filtered value of the thread's points, and the z pass
+// combines the three planes held in registers.  Same per-element expressions and order
+// as the two-pass kernels, so the results are bit-identical to them.
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, FieldList<T> out, double a, double b,
+                                                   const double* __restrict__ w, int KC) {
+    constexpr int TX = 64;
+    constexpr int NT = (R + 2) * (TX + 2);          // tile elements
+    constexpr int NL = (NT + 255) / 256;            // tile loads per thread
+    constexpr int NP = (R * TX) / 256;              // output points per thread
+    static_assert((R * TX) % 256 == 0, "R * 64 must be a multiple of 256");
+    __shared__ double tile[NT];
+    __shared__ double xs[(R + 2) * TX];
+    const int f = blockIdx.z;
+    const int ntx = (d.nx + TX - 1) / TX;
+    const int i0 = (blockIdx.x % ntx) * TX;
+    const int J0 = (blockIdx.x / ntx) * R;
+    const int k0 = blockIdx.y * KC;
+    const int k1 = min(k0 + KC, d.nzp);
+    const T* q = in.p[f];
+    T* o = out.p[f];
+    const double s = 2. * a + b;
+    const double s2 = 2.00 * a + b;
+    const long plane = (long)d.ny * d.nxpp;
+    // this thread's tile elements (row offsets within a plane) and output points
+    int toff[NL];
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        int off = -1;
+        if (e < NT) {
+            const int rr = e / (TX + 2), cc = e - rr * (TX + 2);
+            int j = (J0 - 1 + rr) % d.ny;
+            j = (j < 0) ? j + d.ny : j;
+            int i = (i0 - 1 + cc) % d.nx;
+            i = (i < 0) ? i + d.nx : i;
+            off = j * d.nxpp + i;
+        }
+        toff[m] = off;
+    }
+    T pre[NL];
+    auto load = [&](int k) {
+        const T* qk = q + (long)k * plane;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) pre[m] = (toff[m] >= 0) ? qk[toff[m]] : (T)0;
+    };
+    // xy-filtered values of plane k (the tile in `pre`) at the thread's points
+    auto xy = [&](double (&y)[NP]) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int e = threadIdx.x + 256 * m;
+            if (e < NT) tile[e] = (double)pre[m];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < (R + 2) * TX; e += 256) {
+            const int rr = e / TX, c = e - rr * TX;
+            const double* t = tile + rr * (TX + 2) + c;
+            xs[e] = (a * t[0] + b * t[1] + a * t[2]) / s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int p = threadIdx.x + 256 * m;
+            const int r = p / TX, c = p - r * TX;
+            y[m] = (a * xs[r * TX + c] + b * xs[(r + 1) * TX + c] + a * xs[(r + 2) * TX + c]) / s2;
+        }
+    };
+    double Yp[NP], Yc[NP], Yn[NP];
+    if (k0 > 0) {
+        load(k0 - 1);
+        xy(Yp);
+        __syncthreads();
+    }
+    load(k0);
+    xy(Yc);
+    for (int k = k0; k < k1; ++k) {
+        if (k + 1 < d.nzp) {
+            __syncthreads();
+            load(k + 1);
+            xy(Yn);
+        }
+        const bool interior = (k > 0 && k < d.nzp - 1);
+        const double w0 = w[k], w1 = w[d.nzp + k], w2 = w[2 * d.nzp + k];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int p = threadIdx.x + 256 * m;
+            const int r = p / TX, c = p - r * TX;
+            const int j = J0 + r, i = i0 + c;
+            if (j < d.ny && i < d.nx) {
+                double v;
+                if (interior) {
+                    v = w0 * Yp[m] + w1 * Yc[m] + w2 * Yn[m];
+                    v = v / s;
+                } else {
+                    v = Yc[m];
+                }
+                o[(long)k * plane + (long)j * d.nxpp + i] = (T)v;
+            }
+            Yp[m] = Yc[m];
+            Yc[m] = Yn[m];
+        }
+    }
+}
+
+template <class T, int R>
+static void launch_filt3_lds(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
+                             double b, const double* w, int KC, hipStream_t s) {
+    const int ntx = (d.nx + 63) / 64;
+    const dim3 grid((unsigned)(ntx * ((d.ny + R - 1) / R)), (unsigned)((d.nzp + KC - 1) / KC), (unsigned)nb);
+    hipLaunchKernelGGL((k_filt3_lds<T, R>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
+}
+
+template <class T, int R, int NXT>
+static void launch_filt3_fused(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
+                               double b, const double* w, int KC, hipStream_t s) {
     const dim3 grid((unsigned)((d.ny + R - 1) / R), (unsigned)((d.nzp + KC - 1) / KC), (unsigned)nb);
-    hipLaunchKernelGGL((k_filt3_fused<R, NXT>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
+    hipLaunchKernelGGL((k_filt3_fused<T, R, NXT>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
+}
+
+// Launch shapes of the fused filter: R rows of j per block, KC planes of k per block
+// (the x extent is the whole row, NXT points per thread).  The variant comes from the
+// context's tuning table (lsb_autotune), or the default.
+static const int kFiltR[3] = {2, 4, 8};
+static const int kFiltKC[3] = {8, 16, 32};
+// variants 0..8: the register-sliding kernel (R, KC) above; 9..14: the LDS kernel with
+// R in {4, 8, 16} and KC in {16, 32}
+static const int kLdsR[3] = {4, 8, 16};
+static const int kLdsKC[2] = {16, 32};
+constexpr int kFilt3Variants = 15;
+constexpr int kFilt3Default = 13;  // LDS, R = 16, KC = 16
+
+template <class T, int NXT>
+static void filt3_launch_r(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
+                           double b, const double* w, int R, int KC, hipStream_t s) {
+    switch (R) {
+        case 2: launch_filt3_fused<T, 2, NXT>(d, fi, fo, nb, a, b, w, KC, s); break;
+        case 4: launch_filt3_fused<T, 4, NXT>(d, fi, fo, nb, a, b, w, KC, s); break;
+        default: launch_filt3_fused<T, 8, NXT>(d, fi, fo, nb, a, b, w, KC, s); break;
+    }
+}
+
+template <class T>
+static int filt3_variant(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a, double b,
+                         const double* w, int v, hipStream_t s) {
+    if (v >= 9) {
+        const int R = kLdsR[(v - 9) / 2], KC = kLdsKC[(v - 9) % 2];
+        switch (R) {
+            case 4: launch_filt3_lds<T, 4>(d, fi, fo, nb, a, b, w, KC, s); break;
+            case 8: launch_filt3_lds<T, 8>(d, fi, fo, nb, a, b, w, KC, s); break;
+            default: launch_filt3_lds<T, 16>(d, fi, fo, nb, a, b, w, KC, s); break;
+        }
+        LSB_HIP(hipGetLastError());
+        return LSB_OK;
+    }
+    const int R = kFiltR[v / 3], KC = kFiltKC[v % 3];
+    if (d.nx <= 256)
+        filt3_launch_r<T, 1>(d, fi, fo, nb, a, b, w, R, KC, s);
+    else if (d.nx <= 512)
+        filt3_launch_r<T, 2>(d, fi, fo, nb, a, b, w, R, KC, s);
+    else
+        filt3_launch_r<T, 4>(d, fi, fo, nb, a, b, w, R, KC, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
+}
+
+template <class T>
+static int filt3_dispatch(lsb_ctx* c, const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb,
+                          double a, double b, const double* w, hipStream_t s) {
+    int v = c->tune[sizeof(T) == 4 ? TUNE_FILT3_F32 : TUNE_FILT3_F64];
+    if (v < 0 || v >= kFilt3Variants) v = kFilt3Default;
+    return filt3_variant<T>(d, fi, fo, nb, a, b, w, v, s);
 }
 
 // ------------------------------------------------------ mode-cut accumulation
@@ -705,6 +883,15 @@ int ph_smag_factor_f32(lsb_ctx* c, const float* g02, const double* ubarp, double
     return smag_factor<float>(c, g02, ubarp, cs, s);
 }
 
+// The z weights on the device for the pairs the step uses, or null.
+static const double* filt_weights(const lsb_ctx* c, double a, double b) {
+    if (a == c->cfg.a && b == c->cfg.b) return c->d_fz;
+    if (a == 1.0 && b == 1.0) return c->d_fz + 3 * c->nzp;
+    if (a == 1.0 && b == 6.0) return c->d_fz + 6 * c->nzp;
+    if (a == 1.0 && b == 2.0) return c->d_fz + 9 * c->nzp;
+    return nullptr;
+}
+
 // LSB_FILT3_TWO_PASS=1 selects the two-pass kernels (x-y, then z through tmp)
 static bool filt3_two_pass() {
     static const bool v = [] {
@@ -720,17 +907,7 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
     int tok = -1;
     // x-y pass and z pass, each one read and one write per field
     prof_mark_begin(c, PC_PW, s, &tok, 0.0, 4.0 * nf * (double)d.npts * sizeof(double));
-    const double* w;
-    if (a == c->cfg.a && b == c->cfg.b)
-        w = c->d_fz;
-    else if (a == 1.0 && b == 1.0)
-        w = c->d_fz + 3 * c->nzp;
-    else if (a == 1.0 && b == 6.0)
-        w = c->d_fz + 6 * c->nzp;
-    else if (a == 1.0 && b == 2.0)
-        w = c->d_fz + 9 * c->nzp;
-    else
-        w = nullptr;
+    const double* w = filt_weights(c, a, b);
     std::vector<double> hw;
     double* wtmp = nullptr;
     if (interpflag != 1 || w == nullptr) {
@@ -758,12 +935,8 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
             ft.p[q] = tmp + (size_t)q * c->nphys;
             fo.p[q] = out[f0 + q];
         }
-        if (d.nx <= 256 && !filt3_two_pass()) {
-            launch_filt3_fused<8, 1>(d, fi, fo, nb, a, b, w, s);
-        } else if (d.nx <= 512 && !filt3_two_pass()) {
-            launch_filt3_fused<8, 2>(d, fi, fo, nb, a, b, w, s);
-        } else if (d.nx <= 1024 && !filt3_two_pass()) {
-            launch_filt3_fused<4, 4>(d, fi, fo, nb, a, b, w, s);
+        if (d.nx <= 1024 && !filt3_two_pass()) {
+            LSB_TRY(filt3_dispatch<double>(c, d, fi, fo, nb, a, b, w, s));
         } else {
             hipLaunchKernelGGL(k_filt_xy, ph_grid(d.npts * nb), dim3(256), 0, s, d, fi, ft, nb, a, b);
             LSB_HIP(hipGetLastError());
@@ -774,6 +947,72 @@ int ph_filter3(lsb_ctx* c, const double* const* in, double* const* out, int nf, 
     prof_mark_end(c, tok, s);
     if (wtmp) LSB_HIP(hipStreamSynchronize(s));
     return LSB_OK;
+}
+
+// FP32 storage, FP64 arithmetic (the mixed form's dynamic model); interpolating z weights.
+int ph_filter3_f32(lsb_ctx* c, const float* const* in, float* const* out, int nf, double a, double b, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    const double* w = filt_weights(c, a, b);
+    if (!w || d.nx > 1024) {
+        set_error("filter (FP32): needs nx <= 1024 and the configured (a, b) or (1, 1)");
+        return LSB_E_UNSUPPORTED;
+    }
+    int tok = -1;
+    prof_mark_begin(c, PC_PW, s, &tok, 0.0, 2.0 * nf * (double)d.npts * sizeof(float));
+    for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+        const int nb = (nf - f0 < kMaxFields) ? nf - f0 : kMaxFields;
+        FieldList<float> fi{}, fo{};
+        for (int q = 0; q < nb; ++q) {
+            fi.p[q] = const_cast<float*>(in[f0 + q]);
+            fo.p[q] = out[f0 + q];
+        }
+        LSB_TRY(filt3_dispatch<float>(c, d, fi, fo, nb, a, b, w, s));
+    }
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+
+// Times every launch variant of the fused filter on nf scratch fields of storage type T
+// (the first 2 nf physical fields of `work`) and records the fastest in the context.
+template <class T>
+static int tune_filt3_t(lsb_ctx* c, void* work, int nf, hipStream_t s) {
+    PhDims d = ph_dims(c);
+    if (d.nx > 1024) return LSB_OK;
+    const double* w = c->d_fz;
+    FieldList<T> fi{}, fo{};
+    for (int q = 0; q < nf; ++q) {
+        fi.p[q] = (T*)work + (size_t)q * c->nphys;
+        fo.p[q] = (T*)work + (size_t)(nf + q) * c->nphys;
+    }
+    hipEvent_t e0, e1;
+    LSB_HIP(hipEventCreate(&e0));
+    LSB_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bv = kFilt3Default;
+    for (int v = 0; v < kFilt3Variants; ++v) {
+        LSB_TRY(filt3_variant<T>(d, fi, fo, nf, c->cfg.a, c->cfg.b, w, v, s));  // warm-up
+        LSB_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) LSB_TRY(filt3_variant<T>(d, fi, fo, nf, c->cfg.a, c->cfg.b, w, v, s));
+        LSB_HIP(hipEventRecord(e1, s));
+        LSB_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LSB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int slot = sizeof(T) == 4 ? TUNE_FILT3_F32 : TUNE_FILT3_F64;
+    c->tune[slot] = bv;
+    c->tune_ms[slot] = best / 3.0f;
+    return LSB_OK;
+}
+
+int ph_tune_filt3(lsb_ctx* c, void* work, int nf, hipStream_t s) {
+    LSB_TRY(tune_filt3_t<double>(c, work, nf, s));
+    return tune_filt3_t<float>(c, work, nf, s);
 }
 
 int ph_dyn(lsb_ctx* c, int stage, const DynArgs& a, hipStream_t s) {

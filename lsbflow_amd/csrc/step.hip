@@ -79,6 +79,9 @@ struct StepState {
     double bnd00_top = 0.0, bnd00_bot = 0.0;  // (0,0) Re of the unit-amplitude bound planes
     // top_bc 2: the two difference planes, two physical scratch planes and the weight m(x)
     double *cdiff[2], *cphys[2], *cweight;
+    // mixed dynamic model: z-filtered base profiles (3 nzp) and their host copy
+    double* dynbase = nullptr;
+    std::vector<double> dynbase_h;
 };
 
 static std::vector<std::pair<const lsb_ctx*, StepState*>> g_states;
@@ -117,7 +120,8 @@ int step_alloc(lsb_ctx* c) {
     const int nsgs = sgs_fields(c);
     const size_t nplanes = 6 + 2 + 2 + 2;
     const size_t bytes =
-        (NSPEC_SLOTS * ns + (NPHYS_SLOTS + nsgs) * np + nplanes * pl + 2 * c->plane_p + c->nx) * sizeof(double);
+        (NSPEC_SLOTS * ns + (NPHYS_SLOTS + nsgs) * np + nplanes * pl + 2 * c->plane_p + c->nx + 3 * c->nzp) *
+        sizeof(double);
     void* blk = nullptr;
     int rc = dev_alloc(&blk, bytes);
     if (rc != LSB_OK) {
@@ -146,6 +150,7 @@ int step_alloc(lsb_ctx* c) {
     S->cphys[0] = p + 12 * pl;
     S->cphys[1] = S->cphys[0] + c->plane_p;
     S->cweight = S->cphys[1] + c->plane_p;
+    S->dynbase = S->cweight + c->nx;
     {   // inflow weight of the ceiling: the blowing Gaussian of the bctype 3 profile
         std::vector<double> m(c->nx);
         const double L = c->cfg.xlen;
@@ -350,7 +355,7 @@ static int filt_view(lsb_ctx* c, const TView& in, const TView& out, double a, do
 
 // sgsterm.f90:123-280 with work fields carved from `w` (physical fields).
 int sgs_dynamic(lsb_ctx* c, double* const* ui, const TView& uu, const TView& g, const TView& tau, bool sym,
-                double* w, hipStream_t s) {
+                double* w, hipStream_t s, bool fuse_h = false) {
     const size_t np = c->nphys;
     auto F = [&](int i) { return w + (size_t)i * np; };
     int k = 0;
@@ -391,6 +396,11 @@ int sgs_dynamic(lsb_ctx* c, double* const* ui, const TView& uu, const TView& g, 
     a.den = den;
     a.numf = numf;
     a.denf = denf;
+    for (int q = 0; q < 9; ++q) {
+        a.h[q] = fuse_h ? uu.p[q] : nullptr;
+        a.uu[q] = uu.p[q];
+    }
+    a.hsym = sym ? 1 : 0;
     const double fa = c->cfg.a, fb = c->cfg.b;
     LSB_TRY(ph_dyn(c, 1, a, s));
     LSB_TRY(filt_view(c, sij, sijh, fa, fb, tmp, s));
@@ -401,7 +411,13 @@ int sgs_dynamic(lsb_ctx* c, double* const* ui, const TView& uu, const TView& g, 
     }
     LSB_TRY(filt_view(c, uu, uuh, fa, fb, tmp, s));
     LSB_TRY(ph_dyn(c, 2, a, s));
-    {
+    if (c->cfg.dyn_avg) {
+        // plane-averaged coefficient: <.> the mean over each x-y plane
+        double* cp = tmp + ph_plane_mean2_scratch(c);
+        LSB_TRY(ph_plane_mean2<double>(c, num, den, tmp, cp, s));
+        a.cplane = cp;
+    } else {
+        // the local 3x3x3 average (sgsterm.f90:243-244)
         const double* fi[2] = {num, den};
         double* fo[2] = {numf, denf};
         LSB_TRY(ph_filter3(c, fi, fo, 2, 1.0, 1.0, 1, tmp, s));
@@ -549,10 +565,8 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         for (int q = 0; q < 6; ++q) tb[q] = L.sgs + (size_t)(kDynWork + 6 + q) * c->nphys;  // after work + tmp
         TView tau = sym_view(tb);
         double* ui[3] = {L.phys[P_UI0], L.phys[P_UI0 + 1], L.phys[P_UI0 + 2]};
-        LSB_TRY(sgs_dynamic(c, ui, uu, gv, tau, true, L.sgs, s));
-        const double* uu6[6] = {six[0], six[1], six[2], six[3], six[4], six[5]};
-        const double* t6[6] = {tb[0], tb[1], tb[2], tb[3], tb[4], tb[5]};
-        LSB_TRY(ph_hcombine(c, uu6, t6, six, 6, s));
+        // h = 1/2 uu + tau written over uu by the model's last stage
+        LSB_TRY(sgs_dynamic(c, ui, uu, gv, tau, true, L.sgs, s, true));
     } else if (model == 3) {
         // interscale: tau from the filtered velocity, then h = 1/2 uu + tau (9 fields)
         double* ui[3] = {L.phys[P_UI0], L.phys[P_UI0 + 1], L.phys[P_UI0 + 2]};
@@ -607,6 +621,72 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
 
 static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED; }
 
+// Mixed form, dynamic model (sgsterm.f90:123-280): products, strain and L1 (stage 1), the
+// (a, b) test filter of S', L1, u', p in FP32, M:L and M:M (stage 2), their local (1, 1)
+// average or plane means, and h = p/2 + tau over the products (stage 3).  The FP32 work
+// fields follow each other in the SGS block.
+static int dyn_mixed(lsb_ctx* c, StepState* S, hipStream_t s) {
+    StepLayout& L = S->L;
+    auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
+    auto W = [&](int i) { return reinterpret_cast<float*>(L.sgs) + (size_t)i * c->nphys; };
+    DynMixArgs a{};
+    for (int l = 0; l < 3; ++l) {
+        a.ui[l] = ph(P_UI0 + l);
+        a.nlt[l] = ph(P_NLT0 + l);
+        a.uh[l] = W(25 + l);
+    }
+    for (int q = 0; q < 9; ++q) a.g[q] = ph(P_G0 + q);
+    a.ubar = c->d_paral;
+    a.ubarp = c->d_paral + c->nzp;
+    a.delta = c->d_delta + c->nzp;
+    a.lam = (c->cfg.forcing == 1) ? c->d_lam : nullptr;
+    for (int q = 0; q < 6; ++q) {
+        a.p[q] = ph(P_H0 + q);
+        a.h[q] = ph(P_H0 + q);
+        a.sp[q] = W(q);
+        a.l1[q] = W(6 + q);
+        a.sph[q] = W(13 + q);
+        a.l1h[q] = W(19 + q);
+        a.ph[q] = W(28 + q);
+    }
+    a.snorm = W(12);
+    a.num = W(34);
+    a.den = W(35);
+    a.numf = W(36);
+    a.denf = W(37);
+    a.base = S->dynbase;
+    LSB_TRY(ph_dyn_mixed(c, 1, a, s));
+    {
+        const float* fi[21];
+        float* fo[21];
+        for (int q = 0; q < 6; ++q) {
+            fi[q] = a.sp[q];
+            fo[q] = W(13 + q);
+            fi[6 + q] = a.l1[q];
+            fo[6 + q] = W(19 + q);
+            fi[15 + q] = a.p[q];
+            fo[15 + q] = W(28 + q);
+        }
+        for (int l = 0; l < 3; ++l) {
+            fi[12 + l] = a.ui[l];
+            fo[12 + l] = W(25 + l);
+        }
+        LSB_TRY(ph_filter3_f32(c, fi, fo, 21, c->cfg.a, c->cfg.b, s));
+    }
+    LSB_TRY(ph_dyn_mixed(c, 2, a, s));
+    if (c->cfg.dyn_avg) {
+        double* scr = reinterpret_cast<double*>(W(38));
+        double* cp = scr + ph_plane_mean2_scratch(c);
+        LSB_TRY(ph_plane_mean2<float>(c, a.num, a.den, scr, cp, s));
+        a.cplane = cp;
+    } else {
+        const float* fi[2] = {a.num, a.den};
+        float* fo[2] = {W(36), W(37)};
+        LSB_TRY(ph_filter3_f32(c, fi, fo, 2, 1.0, 1.0, s));
+    }
+    return ph_dyn_mixed(c, 3, a, s);
+}
+
 // FP32 chain of the mixed form (sigma, static or no SGS model): the 9 + 3 inverse
 // transforms, the products and the SGS model in one pass (FP64 arithmetic on FP32
 // storage), 3 + 6 forward transforms to FP32 spectra (nltf_out and hf hold FP32).
@@ -619,8 +699,8 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
     // sgsmodel 4 (TNS) has no SGS stress: tauij stays 0 (navierstokes.f90:696, sgsterm.f90:21-35)
     const bool sgs = c->cfg.sgsflag == 1 && c->cfg.sgsmodel != 4;
     const int model = c->cfg.sgsmodel;
-    if (sgs && model != 0 && model != 2) {
-        set_error("mixed precision: the dynamic and interscale models run in the FP64 form only");
+    if (sgs && model != 0 && model != 1 && model != 2) {
+        set_error("mixed precision: the interscale model runs in the FP64 form only");
         return LSB_E_UNSUPPORTED;
     }
     auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
@@ -661,6 +741,8 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
         LSB_TRY(ph_nonlinear_f32(c, NL_NONE, a, s));
     } else if (model == 2) {
         LSB_TRY(ph_nonlinear_f32(c, NL_SIGMA, a, s));
+    } else if (model == 1) {
+        LSB_TRY(dyn_mixed(c, S, s));
     } else {
         LSB_TRY(ph_smag_factor_f32(c, ph(P_G0 + 6), c->d_paral + c->nzp, c->cfg.cs, s));
         LSB_TRY(ph_nonlinear_f32(c, NL_STATIC, a, s));
@@ -1081,6 +1163,12 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
     if (!S || !c->state_ready) {
         set_error("timestep: call lsb_state_set first");
         return LSB_E_STATE;
+    }
+    if (mixed_form(c) && c->cfg.sgsflag == 1 && c->cfg.sgsmodel == 1) {
+        // the mixed dynamic model's z-filtered base profiles (the base flow may have changed)
+        dyn_base_profiles(c, S->dynbase_h);
+        LSB_HIP(hipMemcpyAsync(S->dynbase, S->dynbase_h.data(), S->dynbase_h.size() * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
     }
     for (int n = 0; n < nsteps; ++n) {
         c->istep += 1;

@@ -24,7 +24,7 @@ module lsb_capi
     integer(c_int) :: mapping, bctype, forcing
     integer(c_int) :: sgsflag, sgsmodel
     integer(c_int) :: dealiasflag, dealias_ref_compat
-    integer(c_int) :: dct_exact, top_bc
+    integer(c_int) :: dct_exact, top_bc, dyn_avg
     integer(c_int) :: ifrz, nstart, precision, device
     real(c_double) :: xlen, ylen, zlen, x0, u0, xnu, blfac, prtl
     real(c_double) :: a, b, cs, kc, filt_order, filt_amp
@@ -327,6 +327,32 @@ module lsb_capi
 
     ! ------------------------------------------- device-pointer entry points
     integer(c_int) function lsb_horfft_dev(ctx, ain, ainf, is, nfields, stream) bind(C, name="lsb_horfft_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ain, ainf
+      integer(c_int), value :: is, nfields
+      type(c_ptr), value :: stream
+    end function
+    ! launch-shape autotuner
+    integer(c_int) function lsb_autotune(ctx) bind(C, name="lsb_autotune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lsb_autotune_get(ctx, variant, ms, n) bind(C, name="lsb_autotune_get")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: variant(*)
+      real(c_float), intent(out) :: ms(*)
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function lsb_autotune_set(ctx, variant, n) bind(C, name="lsb_autotune_set")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: variant(*)
+      integer(c_int), value :: n
+    end function
+    ! lsb_horfft_dev without device allocation (callers that cannot allocate)
+    integer(c_int) function lsb_horfft_dev_noalloc(ctx, ain, ainf, is, nfields, stream) &
+        bind(C, name="lsb_horfft_dev_noalloc")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, ain, ainf
       integer(c_int), value :: is, nfields

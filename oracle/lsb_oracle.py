@@ -103,6 +103,7 @@ class OracleConfig:
     dealias_ref_compat: int = 1
     dct_exact: int = 0
     top_bc: int = 0
+    dyn_avg: int = 0
     ifrz: int = 0
     nstart: int = 0
     dt: float = 1.0e-3
@@ -692,7 +693,11 @@ class Oracle:
         return tau
 
     def dynamic_smagorinsky(self, ui, uiuj, duidxj):
-        """sgsterm.f90:123-280 (local 3-pt average of MijLij / MijMij, Q9)."""
+        """sgsterm.f90:123-280 (local 3-pt average of MijLij / MijMij, Q9).
+
+        ``dyn_avg = 1``: <.> is the mean over each x-y plane (the nx physical points of
+        every row) instead of the local filter of sgsterm.f90:243-244, so the coefficient
+        is a function of z only (the plane-averaged dynamic procedure)."""
         c = self.cfg
         a, b = c.a, c.b
         z = self.zpts
@@ -737,8 +742,14 @@ class Oracle:
             for l in range(3):
                 numc = numc + mij[l, m] * lij[l, m]
                 denomc = denomc + mij[l, m] * mij[l, m]
-        ctot = self.filter_q_3pt(numc, 1.0, 1.0, 1)
-        cdd = self.filter_q_3pt(denomc, 1.0, 1.0, 1)
+        if c.dyn_avg == 1:
+            ctot = np.zeros_like(numc)
+            cdd = np.zeros_like(denomc)
+            ctot[..., :nx] = numc[..., :nx].mean(axis=(1, 2))[:, None, None]
+            cdd[..., :nx] = denomc[..., :nx].mean(axis=(1, 2))[:, None, None]
+        else:
+            ctot = self.filter_q_3pt(numc, 1.0, 1.0, 1)
+            cdd = self.filter_q_3pt(denomc, 1.0, 1.0, 1)
         with np.errstate(invalid="ignore", divide="ignore"):
             cc = ctot[..., :nx] / cdd[..., :nx]
         cc = np.where((cc < 0.0) | np.isnan(cc), 0.0, cc)

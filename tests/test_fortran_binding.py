@@ -62,7 +62,7 @@ def test_fortran_program_through_the_binding(tmp_path):
     st = np.frombuffer(raw[:20], dtype=np.int32)
     assert list(st[:3]) == [0, 0, 0]
     assert st[3] == -1 and "nx" in r.stdout             # LSB_E_ARG with the library's message
-    assert st[4] == 16 * 4 + 17 * 8                       # sizeof(lsb_config) seen from Fortran
+    assert st[4] == 18 * 4 + 17 * 8                       # sizeof(lsb_config) seen from Fortran
     nzm = nzp - 2
     sizes = [nzp * nzp] * 3 + [n1 * n2 * nzp, nzm, nzm * nzm, nzm * nzm]
     nb = 20 + 8 * sum(sizes)

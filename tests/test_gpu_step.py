@@ -92,6 +92,15 @@ def test_les_models_small(sgsmodel):
     check_fields(st, g)
 
 
+def test_les_dynamic_plane_average():
+    """The plane-averaged dynamic coefficient (<M:L>, <M:M> as x-y plane means) inside the
+    step against the oracle's, 6 steps on a 32 x 41 x 16 grid with the ceiling profile."""
+    case = L.cases.CASES["c1_dns"].with_(nx=32, ny=16, nz=40, xnu=2e-4, samp=0.3, sgsflag=1, sgsmodel=1,
+                                        a=1.0, b=2.0, dt=5e-4, pert_rms=0.05, dyn_avg=1)
+    st, g = run_both(case, 6)
+    check_fields(st, g)
+
+
 @pytest.mark.parametrize("dealias", [0, 2])
 def test_dealias_modes_and_homogeneous_bc(dealias):
     case = L.cases.CASES["c1_dns"].with_(nx=32, ny=16, nz=36, dealiasflag=dealias, bctype=0, pert_rms=0.02)

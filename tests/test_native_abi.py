@@ -31,13 +31,14 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_layout():
-    """lsb_config as seen by ctypes has the header's field order (16 ints then 17 doubles)."""
+    """lsb_config as seen by ctypes has the header's field order (17 ints, 4 bytes of
+    alignment padding, then 17 doubles)."""
     names = [n for n, _ in _lib.LsbConfig._fields_]
     txt = open(os.path.join(ROOT, "include", "lsbflow.h")).read()
     body = txt.split("typedef struct lsb_config {")[1].split("} lsb_config;")[0]
     hdr = re.findall(r"\b([a-z_0-9]+)\s*[,;]", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
     assert names == hdr
-    assert ctypes.sizeof(_lib.LsbConfig) == 16 * 4 + 17 * 8
+    assert ctypes.sizeof(_lib.LsbConfig) == 18 * 4 + 17 * 8
 
 
 def test_host_setup_matches_oracle():
