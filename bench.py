@@ -379,6 +379,60 @@ def diagnostics(case, reps):
     return 0
 
 
+def c3_helmholtz(reps):
+    """Batched Helmholtz solve alone: 256 x 256 pencils x 4 fields on 193 points (the
+    512 x 193 x 256 spectral grid) per call, FP64 / FP32 / FP32 + one FP64 refinement, for
+    sigma = c / (nu dt), c in {1, 1.5}, and sigma = 0.  Device-resident arrays; wall time
+    around `reps` calls with the stream synchronised on both sides."""
+    import ctypes as C
+    import numpy as np
+    import lsbflow_amd as L
+    from lsbflow_amd import _lib
+    from lsbflow_amd.core import setup_case
+
+    case = L.cases.CASES["c4_sigma_les"]
+    ctx = setup_case(case)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    nf, nzp, P = 4, ctx.nzp, ctx.nx * ctx.ny
+    rng = np.random.default_rng(20261203)
+    n = np.arange(nzp)
+    T = np.polynomial.chebyshev.chebvander(np.cos(np.pi * n / (nzp - 1)), nzp - 1)
+    rhs = np.stack([T @ (rng.standard_normal((nzp, P)) * np.exp(-0.1 * n)[:, None]) for _ in range(nf)])
+    d = C.c_void_p()
+    assert hip.hipMalloc(C.byref(d), rhs.nbytes) == 0
+    lib = _lib.lib()
+    nzm = nzp - 2
+    out = {"c3": "batched Helmholtz, 256x256 pencils x 4 fields x 193 points per call (nx=512, ny=256, nzp=193)",
+           "systems_per_call": (P // 2) * nf, "reps": reps, "sigma": {}}
+    try:
+        for cval in (1.0, 1.5, 0.0):
+            sigma = cval / (case.xnu * case.dt)
+            row = {}
+            for mode, name in ((0, "fp64"), (1, "fp32"), (2, "fp32_refined")):
+                assert hip.hipMemcpy(d, rhs.ctypes.data_as(C.c_void_p), rhs.nbytes, 1) == 0
+                lib.lsb_solve_helmholtz_mixed_dev(ctx.h, d, sigma, nf, mode, None)     # warm-up
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    rc = lib.lsb_solve_helmholtz_mixed_dev(ctx.h, d, sigma, nf, mode, None)
+                    if rc != 0:
+                        raise RuntimeError(lib.lsb_last_error())
+                ctx.synchronize()
+                ms = 1e3 * (time.perf_counter() - t0) / reps
+                flops = 2 * 2.0 * nzm * nzm * P * nf * (2 if mode == 2 else 1)   # dense count of the projections
+                row[name] = {"ms_per_call": round(ms, 4), "systems_per_s": float(f"{(P // 2) * nf / (ms * 1e-3):.4g}"),
+                             "dense_equiv_tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
+            out["sigma"][f"c={cval}"] = row
+    finally:
+        hip.hipFree(d)
+        ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,6 +442,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes (no lsb_autotune)")
+    ap.add_argument("--c3", action="store_true",
+                    help="time the batched Helmholtz solve alone (FP64, FP32, FP32 + one FP64 refinement)")
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="the workload of the HBM-traffic counter runs: a 1 GiB device copy (calibration), "
@@ -417,6 +473,8 @@ def main(argv=None):
         return pmc_probe(case)
     if args.diagnostics:
         return diagnostics(case, args.steps)
+    if args.c3:
+        return c3_helmholtz(args.steps)
     ic = L.cases.initial_condition(case)
     pts = case.points
     tune = not args.no_autotune

@@ -84,6 +84,7 @@ _SIGS = {
     "lsb_horfft_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lsb_horfft_dev_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lsb_autotune": (C.c_int, [_vp]),
+    "lsb_solve_helmholtz_mixed_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp]),
     "lsb_autotune_get": (C.c_int, [_vp, _ip, _fp, C.c_int]),
     "lsb_autotune_set": (C.c_int, [_vp, _ip, C.c_int]),
     "lsb_dfdz_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 #include "lsb_internal.h"
@@ -464,6 +465,14 @@ __global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, 
     }
 }
 
+// x / s of the filter passes.  When s = 2a + b is a power of two, x * (1/s) is the same
+// correctly rounded value (1/s is exact), without the FP64 division sequence.
+__device__ __forceinline__ double fdiv(double x, double s, double inv, bool pow2) { return pow2 ? x * inv : x / s; }
+__host__ __device__ inline bool pow2_exact(double s) {
+    int e = 0;
+    return s > 0.0 && std::frexp(s, &e) == 0.5;
+}
+
 // The same filter in one pass: block (R rows j, KC planes k, field) slides over k with the
 // xy-filtered values of planes k-1, k, k+1 in registers (threads over x, NXT points each), so
 // every input value is read from HBM about once (halo rows and planes aside) and the
@@ -480,6 +489,8 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
     T* o = out.p[f];
     const double s = 2. * a + b;
     const double s2 = 2.00 * a + b;
+    const bool p2 = pow2_exact(s);
+    const double is = 1.0 / s;
     const long plane = (long)d.ny * d.nxpp;
     int jr[R + 2];
 #pragma unroll
@@ -499,7 +510,7 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
                 if (i < d.nx) {
                     const int im = (i == 0) ? d.nx - 1 : i - 1;
                     const int ip = (i == d.nx - 1) ? 0 : i + 1;
-                    X[rr][t] = (a * (double)row[im] + b * (double)row[i] + a * (double)row[ip]) / s;
+                    X[rr][t] = fdiv(a * (double)row[im] + b * (double)row[i] + a * (double)row[ip], s, is, p2);
                 } else {
                     X[rr][t] = 0.0;
                 }
@@ -508,7 +519,7 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int t = 0; t < NXT; ++t) y[r][t] = (a * X[r][t] + b * X[r + 1][t] + a * X[r + 2][t]) / s2;
+            for (int t = 0; t < NXT; ++t) y[r][t] = fdiv(a * X[r][t] + b * X[r + 1][t] + a * X[r + 2][t], s2, is, p2);
     };
     double Yp[R][NXT], Yc[R][NXT], Yn[R][NXT];
     xy(k0, Yc);
@@ -529,7 +540,7 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
                 double v;
                 if (interior) {
                     v = w0 * Yp[r][t] + w1 * Yc[r][t] + w2 * Yn[r][t];
-                    v = v / s;
+                    v = fdiv(v, s, is, p2);
                 } else {
                     v = Yc[r][t];
                 }
@@ -548,10 +559,11 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
 
 // The same filter through LDS: a block owns an x-tile of TX points by R rows of j and
 // slides over KC planes of k.  Each plane's (R + 2) x (TX + 2) input tile (periodic halos)
-// goes global -> registers (issued one plane ahead) -> LDS; the x pass fills a second LDS
-// array, the y pass gives the xy-filtered value of the thread's points, and the z pass
-// combines the three planes held in registers.  Same per-element expressions and order
-// as the two-pass kernels, so the results are bit-identical to them.
+// goes global -> registers -> LDS, the loads of the plane after next in flight while the
+// next plane is filtered; the x pass fills a second LDS array, the y pass gives the
+// xy-filtered value of the thread's points, and the z pass combines the three planes held
+// in registers.  Same per-element expressions and order as the two-pass kernels, so the
+// results are bit-identical to them.
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, FieldList<T> out, double a, double b,
                                                    const double* __restrict__ w, int KC) {
@@ -560,7 +572,7 @@ __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, Fi
     constexpr int NL = (NT + 255) / 256;            // tile loads per thread
     constexpr int NP = (R * TX) / 256;              // output points per thread
     static_assert((R * TX) % 256 == 0, "R * 64 must be a multiple of 256");
-    __shared__ double tile[NT];
+    __shared__ T tile[NT];
     __shared__ double xs[(R + 2) * TX];
     const int f = blockIdx.z;
     const int ntx = (d.nx + TX - 1) / TX;
@@ -572,8 +584,10 @@ __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, Fi
     T* o = out.p[f];
     const double s = 2. * a + b;
     const double s2 = 2.00 * a + b;
+    const bool p2 = pow2_exact(s);
+    const double is = 1.0 / s;
     const long plane = (long)d.ny * d.nxpp;
-    // this thread's tile elements (row offsets within a plane) and output points
+    // this thread's tile elements (offsets within a plane; -1 past the tile)
     int toff[NL];
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
@@ -595,41 +609,41 @@ __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, Fi
 #pragma unroll
         for (int m = 0; m < NL; ++m) pre[m] = (toff[m] >= 0) ? qk[toff[m]] : (T)0;
     };
-    // xy-filtered values of plane k (the tile in `pre`) at the thread's points
-    auto xy = [&](double (&y)[NP]) {
+    // xy-filtered values (y) of the plane whose tile is in `pre`; then the loads of plane
+    // knext (>= 0) into `pre`, in flight during the x and y passes
+    auto stage = [&](double (&y)[NP], int knext) {
 #pragma unroll
         for (int m = 0; m < NL; ++m) {
             const int e = threadIdx.x + 256 * m;
-            if (e < NT) tile[e] = (double)pre[m];
+            if (e < NT) tile[e] = pre[m];
         }
         __syncthreads();
+        if (knext >= 0) load(knext);
         for (int e = threadIdx.x; e < (R + 2) * TX; e += 256) {
             const int rr = e / TX, c = e - rr * TX;
-            const double* t = tile + rr * (TX + 2) + c;
-            xs[e] = (a * t[0] + b * t[1] + a * t[2]) / s;
+            const T* t = tile + rr * (TX + 2) + c;
+            xs[e] = fdiv(a * (double)t[0] + b * (double)t[1] + a * (double)t[2], s, is, p2);
         }
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < NP; ++m) {
             const int p = threadIdx.x + 256 * m;
             const int r = p / TX, c = p - r * TX;
-            y[m] = (a * xs[r * TX + c] + b * xs[(r + 1) * TX + c] + a * xs[(r + 2) * TX + c]) / s2;
+            y[m] = fdiv(a * xs[r * TX + c] + b * xs[(r + 1) * TX + c] + a * xs[(r + 2) * TX + c], s2, is, p2);
         }
     };
+    // planes needed: k0 - 1 (if any) .. min(k1, nzp - 1)
+    const int klast = min(k1, d.nzp - 1);
     double Yp[NP], Yc[NP], Yn[NP];
     if (k0 > 0) {
         load(k0 - 1);
-        xy(Yp);
-        __syncthreads();
+        stage(Yp, k0);
+    } else {
+        load(k0);
     }
-    load(k0);
-    xy(Yc);
+    stage(Yc, (k0 + 1 <= klast) ? k0 + 1 : -1);
     for (int k = k0; k < k1; ++k) {
-        if (k + 1 < d.nzp) {
-            __syncthreads();
-            load(k + 1);
-            xy(Yn);
-        }
+        if (k + 1 <= klast) stage(Yn, (k + 2 <= klast) ? k + 2 : -1);
         const bool interior = (k > 0 && k < d.nzp - 1);
         const double w0 = w[k], w1 = w[d.nzp + k], w2 = w[2 * d.nzp + k];
 #pragma unroll
@@ -641,7 +655,7 @@ __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, Fi
                 double v;
                 if (interior) {
                     v = w0 * Yp[m] + w1 * Yc[m] + w2 * Yn[m];
-                    v = v / s;
+                    v = fdiv(v, s, is, p2);
                 } else {
                     v = Yc[m];
                 }
@@ -673,12 +687,15 @@ static void launch_filt3_fused(const PhDims& d, const FieldList<T>& fi, const Fi
 // context's tuning table (lsb_autotune), or the default.
 static const int kFiltR[3] = {2, 4, 8};
 static const int kFiltKC[3] = {8, 16, 32};
-// variants 0..8: the register-sliding kernel (R, KC) above; 9..14: the LDS kernel with
-// R in {4, 8, 16} and KC in {16, 32}
+// variants 0..8: the register-sliding kernel (R, KC) above; 9..17: the LDS kernel with
+// R in {4, 8, 16} and KC in {16, 32, 64}
 static const int kLdsR[3] = {4, 8, 16};
-static const int kLdsKC[2] = {16, 32};
-constexpr int kFilt3Variants = 15;
-constexpr int kFilt3Default = 13;  // LDS, R = 16, KC = 16
+static const int kLdsKC[3] = {16, 32, 64};
+constexpr int kFilt3Variants = 18;
+// defaults (measured on 512 x 193 x 256): LDS kernel, FP64 storage R = 4, KC = 16;
+// FP32 storage R = 8, KC = 16
+constexpr int kFilt3Default64 = 9;
+constexpr int kFilt3Default32 = 12;
 
 template <class T, int NXT>
 static void filt3_launch_r(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
@@ -694,7 +711,7 @@ template <class T>
 static int filt3_variant(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a, double b,
                          const double* w, int v, hipStream_t s) {
     if (v >= 9) {
-        const int R = kLdsR[(v - 9) / 2], KC = kLdsKC[(v - 9) % 2];
+        const int R = kLdsR[(v - 9) / 3], KC = kLdsKC[(v - 9) % 3];
         switch (R) {
             case 4: launch_filt3_lds<T, 4>(d, fi, fo, nb, a, b, w, KC, s); break;
             case 8: launch_filt3_lds<T, 8>(d, fi, fo, nb, a, b, w, KC, s); break;
@@ -718,7 +735,7 @@ template <class T>
 static int filt3_dispatch(lsb_ctx* c, const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb,
                           double a, double b, const double* w, hipStream_t s) {
     int v = c->tune[sizeof(T) == 4 ? TUNE_FILT3_F32 : TUNE_FILT3_F64];
-    if (v < 0 || v >= kFilt3Variants) v = kFilt3Default;
+    if (v < 0 || v >= kFilt3Variants) v = (sizeof(T) == 4) ? kFilt3Default32 : kFilt3Default64;
     return filt3_variant<T>(d, fi, fo, nb, a, b, w, v, s);
 }
 
@@ -988,7 +1005,7 @@ static int tune_filt3_t(lsb_ctx* c, void* work, int nf, hipStream_t s) {
     LSB_HIP(hipEventCreate(&e0));
     LSB_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int bv = kFilt3Default;
+    int bv = (sizeof(T) == 4) ? kFilt3Default32 : kFilt3Default64;
     for (int v = 0; v < kFilt3Variants; ++v) {
         LSB_TRY(filt3_variant<T>(d, fi, fo, nf, c->cfg.a, c->cfg.b, w, v, s));  // warm-up
         LSB_HIP(hipEventRecord(e0, s));
@@ -997,6 +1014,9 @@ static int tune_filt3_t(lsb_ctx* c, void* work, int nf, hipStream_t s) {
         LSB_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         LSB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (std::getenv("LSB_TUNE_VERBOSE"))
+            std::fprintf(stderr, "lsb_autotune filter3 %s variant %d: %.4f ms per launch of %d fields\n",
+                         sizeof(T) == 4 ? "f32" : "f64", v, ms / 3.0f, nf);
         if (ms < best) {
             best = ms;
             bv = v;

@@ -2134,6 +2134,8 @@ template int zcontract_f32<double, float>(lsb_ctx*, const double*, int, int, int
                                           float* const*, int, hipStream_t, long, const float* const*);
 template int zcontract_f32<double, double>(lsb_ctx*, const double*, int, int, int, const double* const*,
                                            double* const*, int, hipStream_t, long, const double* const*);
+template int zcontract_f32<float, double>(lsb_ctx*, const double*, int, int, int, const float* const*,
+                                          double* const*, int, hipStream_t, long, const double* const*);
 template int zcontract_f32<float, float>(lsb_ctx*, const double*, int, int, int, const float* const*,
                                          float* const*, int, hipStream_t, long, const float* const*);
 

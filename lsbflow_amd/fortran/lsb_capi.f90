@@ -332,6 +332,15 @@ module lsb_capi
       integer(c_int), value :: is, nfields
       type(c_ptr), value :: stream
     end function
+    ! batched Helmholtz solves in FP64, FP32 or FP32 + one FP64 refinement (mode 0, 1, 2)
+    integer(c_int) function lsb_solve_helmholtz_mixed_dev(ctx, rhs, dtnui, nfields, mode, stream) &
+        bind(C, name="lsb_solve_helmholtz_mixed_dev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, rhs
+      real(c_double), value :: dtnui
+      integer(c_int), value :: nfields, mode
+      type(c_ptr), value :: stream
+    end function
     ! launch-shape autotuner
     integer(c_int) function lsb_autotune(ctx) bind(C, name="lsb_autotune")
       import :: c_int, c_ptr
