@@ -433,6 +433,68 @@ def c3_helmholtz(reps):
     return 0
 
 
+def c5_ensemble(steps, warmup):
+    """The uneven ensemble: 16 static-Smagorinsky LES members of different grids stepped
+    together from captured graphs (one graph launch per step for all members), against
+    the same members stepped eagerly one after another on their own streams.  Members
+    whose nx/2 and ny are register-resident FFT lengths run the mixed form, the others
+    the FP64 form."""
+    import numpy as np
+    import lsbflow_amd as L
+    from lsbflow_amd.core import Ensemble, setup_case
+
+    fast = (16, 32, 64, 128, 256)
+    members = L.cases.ensemble_members()
+    ctxs, prec = [], []
+    for m in members:
+        p = 1 if (m.nx // 2 in fast and m.ny in fast) else 0
+        c = setup_case(m, precision=p)
+        c.autotune()
+        c.state_set(*L.cases.initial_condition(m))
+        c.timestep(3)
+        ctxs.append(c)
+        prec.append(p)
+    pts = sum(m.points for m in members)
+    for c in ctxs:
+        c.synchronize()
+    # eager: every member's steps enqueued in turn, each on its own stream
+    for _ in range(warmup):
+        for c in ctxs:
+            c.timestep(1)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in ctxs:
+            c.timestep(1)
+    for c in ctxs:
+        c.synchronize()
+    t_eager = time.perf_counter() - t0
+    ens = Ensemble(ctxs)
+    try:
+        ens.timestep(warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.timestep(steps)
+        ens.synchronize()
+        t_graph = time.perf_counter() - t0
+        info = ens.info()
+        finite = all(bool(np.isfinite(c.state_get()["uf"]).all()) for c in ctxs)
+    finally:
+        ens.close()
+        for c in ctxs:
+            c.close()
+    out = {"c5": "uneven ensemble of 16 static-Smagorinsky LES members (Cs 0.1), one graph per step",
+           "members": [f"{m.nx}x{m.nzp}x{m.ny} Re {0.2 / m.xnu:.3g} samp {m.samp:.3f} "
+                       f"{'mixed' if p else 'fp64'}" for m, p in zip(members, prec)],
+           "points": pts, "steps": steps, "graphs": info["graphs"],
+           "graph": {"value": float(f"{pts * steps / t_graph:.4g}"), "ms_per_step": round(1e3 * t_graph / steps, 4)},
+           "eager": {"value": float(f"{pts * steps / t_eager:.4g}"), "ms_per_step": round(1e3 * t_eager / steps, 4)},
+           "unit": "grid-point-steps/s (all members)", "finite_after_run": finite}
+    print(json.dumps(out), flush=True)
+    return 0 if finite else 1
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,6 +504,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes (no lsb_autotune)")
+    ap.add_argument("--c5", action="store_true",
+                    help="time the uneven 16-member ensemble stepped from captured graphs (and eagerly)")
     ap.add_argument("--c3", action="store_true",
                     help="time the batched Helmholtz solve alone (FP64, FP32, FP32 + one FP64 refinement)")
     ap.add_argument("--accuracy-steps", type=int, default=100)
@@ -475,6 +539,8 @@ def main(argv=None):
         return diagnostics(case, args.steps)
     if args.c3:
         return c3_helmholtz(args.steps)
+    if args.c5:
+        return c5_ensemble(args.steps, args.warmup)
     ic = L.cases.initial_condition(case)
     pts = case.points
     tune = not args.no_autotune

@@ -69,7 +69,11 @@ typedef struct lsb_config {
                                        m = the profile's blowing Gaussian G(0.4 Lx) */
     int dyn_avg;                    /* dynamic model <M:L>/<M:M>: 0 the local 3x3x3 filter
                                        (sgsterm.f90:243-244), 1 the x-y plane means      */
-    int ifrz;                       /* navierstokes.f90:53-59                    */
+    int nut_ceil0;                  /* 1: zero filter width (so zero eddy viscosity) on the
+                                       ceiling row for the static (sgsterm.f90:95-103) and
+                                       sigma (:368-376) models, as the wall row has (Q8);
+                                       0: the reference's widths                         */
+    int ifrz;                      /* navierstokes.f90:53-59                    */
     int nstart;                     /* 0: Euler/BDF2 start (navierstokes.f90:1507) */
     int precision;                  /* LSB_FP64 or LSB_MIXED                     */
     int device;                     /* HIP device ordinal                        */
@@ -253,6 +257,23 @@ int lsb_synchronize(lsb_ctx* ctx);
  * lsb_autotune_get reads n slots (variant index, measured ms per launch),
  * lsb_autotune_set installs a saved table. */
 int lsb_autotune(lsb_ctx* ctx);
+
+/* Ensembles of independent contexts stepped from captured HIP graphs (no counterpart in
+ * the reference, whose driver steps one domain, navierstokes.f90:10-90).  The members
+ * (contexts on one device, each with its state set and past its three start-up steps,
+ * constant dt, no adaptive dt / filtering / time averaging / profiling / temperature,
+ * bctype 0 or 3) are captured together: one graph per phase of the history rotation
+ * (60 steps), each graph one time step of every member on its own stream, the members
+ * forked from and joined into the ensemble's stream.  lsb_ensemble_timestep launches
+ * one graph per step and advances every member's step counter, time and history slots;
+ * member state is read with lsb_state_get after lsb_ensemble_synchronize.  While in an
+ * ensemble a member refuses lsb_timestep and may not grow its staging buffers. */
+typedef struct lsb_ensemble lsb_ensemble;
+int lsb_ensemble_create(lsb_ctx** members, int n, lsb_ensemble** out);
+int lsb_ensemble_timestep(lsb_ensemble* ens, int nsteps);
+int lsb_ensemble_synchronize(lsb_ensemble* ens);
+int lsb_ensemble_info(const lsb_ensemble* ens, int* nmembers, int* ngraphs, int* phase);
+int lsb_ensemble_destroy(lsb_ensemble* ens);
 int lsb_autotune_get(const lsb_ctx* ctx, int* variant, float* ms, int n);
 int lsb_autotune_set(lsb_ctx* ctx, const int* variant, int n);
 

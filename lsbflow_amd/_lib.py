@@ -25,7 +25,8 @@ class LsbConfig(C.Structure):
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("mapping", C.c_int), ("bctype", C.c_int), ("forcing", C.c_int),
         ("sgsflag", C.c_int), ("sgsmodel", C.c_int), ("dealiasflag", C.c_int),
-        ("dealias_ref_compat", C.c_int), ("dct_exact", C.c_int), ("top_bc", C.c_int), ("dyn_avg", C.c_int), ("ifrz", C.c_int), ("nstart", C.c_int),
+        ("dealias_ref_compat", C.c_int), ("dct_exact", C.c_int), ("top_bc", C.c_int), ("dyn_avg", C.c_int), ("nut_ceil0", C.c_int),
+        ("ifrz", C.c_int), ("nstart", C.c_int),
         ("precision", C.c_int), ("device", C.c_int),
         ("xlen", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double), ("x0", C.c_double),
         ("u0", C.c_double), ("xnu", C.c_double), ("blfac", C.c_double), ("prtl", C.c_double),
@@ -84,6 +85,11 @@ _SIGS = {
     "lsb_horfft_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lsb_horfft_dev_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lsb_autotune": (C.c_int, [_vp]),
+    "lsb_ensemble_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "lsb_ensemble_timestep": (C.c_int, [_vp, C.c_int]),
+    "lsb_ensemble_synchronize": (C.c_int, [_vp]),
+    "lsb_ensemble_info": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "lsb_ensemble_destroy": (C.c_int, [_vp]),
     "lsb_solve_helmholtz_mixed_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp]),
     "lsb_autotune_get": (C.c_int, [_vp, _ip, _fp, C.c_int]),
     "lsb_autotune_set": (C.c_int, [_vp, _ip, C.c_int]),

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblsbflow_amd.so")
 SOURCES = ["setup.cpp", "capi.cpp", "fft.hip", "fft_fast.hip", "zops.hip", "spectral.hip", "physical.hip", "step.hip",
-           "dealias.hip", "iodiag.hip", "temperature.hip", "noalloc.hip", "dynamic.hip", "helmrefine.hip"]
+           "dealias.hip", "iodiag.hip", "temperature.hip", "noalloc.hip", "dynamic.hip", "helmrefine.hip", "ensemble.hip"]
 ARCH = "gfx950"
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -40,6 +40,16 @@ on, applied identically in the library and the oracle:
   stream ubar(zlen)) where fluid enters and keeps the outflow condition
   elsewhere; it stays bounded (2000 steps at reduced horizontal size).
 
+The uneven static-Smagorinsky ensemble adds a third switch:
+
+* ``nut_ceil0 = 1``: the static model's filter width is zero on the ceiling row, as
+  it is on the wall row (sgsterm.f90:95-103 never sets delta(nzp)).  Where the blend
+  prescribes the tangential velocity, the impulsively started blowing leaves a
+  Stokes layer thinner than the first Chebyshev spacing; its strain (~samp/dz) times
+  the undamped (Cs delta)^2 of the coarse horizontal grids gives the ceiling row an
+  explicit eddy viscosity whose diffusion grows without bound at t ~ 2e-3 whatever
+  the time step (dt from 2.5e-4 down to 2e-6, same blow-up time).
+
 The DNS workload keeps the reference's operator and ceiling condition.
 
 The random numbers come from SplitMix64 + Box-Muller so that every consumer
@@ -86,6 +96,7 @@ class Case:
     dct_exact: int = 0
     top_bc: int = 0
     dyn_avg: int = 0
+    nut_ceil0: int = 0
     dt: float = 1.0e-3
     nsteps: int = 10
     delta: float = 0.2
@@ -119,6 +130,35 @@ CASES = {
     "c4_dyn_les": Case("c4_dyn_les", nx=512, ny=256, nz=192, xnu=0.2 / 1.0e5, samp=0.3,
                        sgsflag=1, sgsmodel=1, a=1.0, b=2.0, dt=2.5e-4, nsteps=100, dct_exact=1, top_bc=2),
 }
+
+
+def ensemble_members(n: int = 16, seed: int = 5150, nsteps: int = 200):
+    """The uneven ensemble: n independent static-Smagorinsky (Cs = 0.1) LES instances whose
+    grids are drawn from nx in {64, 96, 128, 192, 256}, nzp in {65, 97, 129} and ny in
+    {32, 64, 128}, Re log-uniform in [1e4, 1e5] on the delta = 0.2 tanh thickness, ceiling
+    amplitude uniform in [0.1, 0.4], each with its own seed for the initial condition.  The
+    draws come from SplitMix64 (uniforms in (0, 1]) so that every consumer gets the same
+    members.  Each member's dt is the smaller of 2.5e-4 and 0.15 dz_top / samp, a
+    wall-normal Courant number of 0.15 for the blowing speed over the first Chebyshev
+    spacing (members at 0.27 and 0.33 diverge within 12 steps, 0.20 stays bounded);
+    the ceiling row carries no eddy viscosity (``nut_ceil0``)."""
+    r = splitmix64(seed, 6 * n)
+    uni = ((r >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+    nxs, nzps, nys = (64, 96, 128, 192, 256), (65, 97, 129), (32, 64, 128)
+    out = []
+    for i in range(n):
+        u = uni[6 * i: 6 * i + 6]
+        nx = nxs[min(int(u[0] * len(nxs)), len(nxs) - 1)]
+        nzp = nzps[min(int(u[1] * len(nzps)), len(nzps) - 1)]
+        ny = nys[min(int(u[2] * len(nys)), len(nys) - 1)]
+        re = 10.0 ** (4.0 + u[3])
+        samp = 0.1 + 0.3 * u[4]
+        dz_top = 0.5 * Case.zlen * (1.0 - math.cos(PI / (nzp - 1)))
+        dt = min(2.5e-4, 0.15 * dz_top / samp)
+        out.append(Case(f"c5_m{i:02d}", nx=nx, ny=ny, nz=nzp - 1, xnu=0.2 / re, samp=samp, sgsflag=1, sgsmodel=0,
+                        cs=0.1, dt=dt, nsteps=nsteps, dct_exact=1, top_bc=2, nut_ceil0=1,
+                        seed=seed + 7919 * (i + 1)))
+    return out
 
 
 # ---------------------------------------------------------------- random numbers

@@ -62,6 +62,7 @@ class Config:
     dct_exact: int = 0
     top_bc: int = 0
     dyn_avg: int = 0
+    nut_ceil0: int = 0
     ifrz: int = 0
     nstart: int = 0
     precision: int = 0
@@ -648,6 +649,40 @@ class Context:
         check(self._L.lsb_set_filtering(self.h, ifilt, modfilt), "set_filtering")
 
 
+class Ensemble:
+    """Contexts stepped together from captured HIP graphs (lsb_ensemble_*): each member must
+    have its state set and have run its three start-up steps."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        self.h = C.c_void_p()
+        self._L = _lib.lib()
+        check(self._L.lsb_ensemble_create(arr, len(self.members), C.byref(self.h)), "ensemble_create")
+
+    def timestep(self, nsteps=1):
+        check(self._L.lsb_ensemble_timestep(self.h, nsteps), "ensemble_timestep")
+
+    def synchronize(self):
+        check(self._L.lsb_ensemble_synchronize(self.h), "ensemble_synchronize")
+
+    def info(self):
+        a, b, p = C.c_int(), C.c_int(), C.c_int()
+        check(self._L.lsb_ensemble_info(self.h, C.byref(a), C.byref(b), C.byref(p)), "ensemble_info")
+        return {"members": a.value, "graphs": b.value, "phase": p.value}
+
+    def close(self):
+        if self.h:
+            check(self._L.lsb_ensemble_destroy(self.h), "ensemble_destroy")
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:
     """Context for a configuration of lsbflow_amd.cases with its base flow installed."""
     from . import cases
@@ -658,4 +693,4 @@ def setup_case(case, precision: int = 0, device: int = 0, **kw) -> Context:
     return ctx
 
 
-__all__ = ["Config", "Context", "build_cheb_zder_matrix", "dct_diff", "diagonalize_matrix", "setup_case", "field"]
+__all__ = ["Config", "Context", "Ensemble", "build_cheb_zder_matrix", "dct_diff", "diagonalize_matrix", "setup_case", "field"]

@@ -42,7 +42,13 @@ int stage_get(lsb_ctx* c, int slot, size_t bytes, void** p) {
         c->stage.resize(slot + 1, nullptr);
         c->stage_bytes.resize(slot + 1, 0);
     }
+    if ((int)c->stage_in_graph.size() <= slot) c->stage_in_graph.resize(slot + 1, 0);
+    if (c->capturing) c->stage_in_graph[slot] = 1;
     if (c->stage_bytes[slot] < bytes) {
+        if (c->stage_locked && c->stage_in_graph[slot]) {
+            set_error("staging buffers are locked while the context's steps are captured (lsb_ensemble)");
+            return LSB_E_STATE;
+        }
         if (c->stage[slot]) {
             // the old buffer may still be in use on a caller's stream
             LSB_HIP(hipDeviceSynchronize());
@@ -223,6 +229,15 @@ extern "C" int lsb_create(const lsb_config* cfg, lsb_ctx** out) {
     }
     if ((g.dct_exact != 0 && g.dct_exact != 1) || g.top_bc < 0 || g.top_bc > 2) {
         set_error("lsb_create: dct_exact must be 0 or 1, top_bc 0, 1 or 2");
+        return LSB_E_ARG;
+    }
+    if (g.nut_ceil0 != 0 && g.nut_ceil0 != 1) {
+        set_error("lsb_create: nut_ceil0 must be 0 or 1");
+        return LSB_E_ARG;
+    }
+    if (g.nut_ceil0 == 1 && g.sgsflag == 1 && (g.sgsmodel == 1 || g.sgsmodel == 3)) {
+        set_error("lsb_create: nut_ceil0 applies to the static and sigma models' filter widths "
+                  "(the dynamic and interscale models compute their coefficients from theirs)");
         return LSB_E_ARG;
     }
     if (g.top_bc == 2 && g.bctype == 1) {

@@ -202,6 +202,12 @@ struct lsb_ctx {
     int s_u[3], s_un[3], s_unm[3], s_nl[3], s_nlm[3], s_p, s_pn;
     double* bc_spec = nullptr;  // spectral bound() field, planes 0 and nzp-1 (2*plane_s)
 
+    // member of a captured ensemble (lsb_ensemble_create): its steps run from graphs and
+    // the staging buffers the graphs read may not be reallocated
+    bool graphs_live = false;
+    bool stage_locked = false;
+    bool capturing = false;                 // stage slots used while set are the graphs'
+    std::vector<char> stage_in_graph;
     // launch-shape variants chosen by lsb_autotune (-1: the built-in default)
     int tune[lsb::TUNE_SLOTS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     float tune_ms[lsb::TUNE_SLOTS] = {};
@@ -435,6 +441,12 @@ int ph_sgs(lsb_ctx* c, int model, const double* ui, const double* uiuj, const do
 // step (step.hip)
 int step_alloc(lsb_ctx* c);
 int step_run(lsb_ctx* c, int nsteps, bool with_norm);
+// graph capture of the step (ensemble.hip): one step enqueued with its host bookkeeping,
+// the bookkeeping alone, the slot-rotation period, and whether a step can be captured
+int step_enqueue(lsb_ctx* c);
+void step_host_advance(lsb_ctx* c);
+int step_rotation_period(lsb_ctx* c);
+bool step_capturable(const lsb_ctx* c, std::string& why);
 int step_prepare(lsb_ctx* c);
 void step_free(lsb_ctx* c);
 double* step_field(lsb_ctx* c, int which);

@@ -685,6 +685,10 @@ int build_derived(lsb_ctx* c) {
             c->delta_sigma[k] = std::pow(dx * dy * dz, 1. / 3.);
         }
     }
+    if (c->cfg.nut_ceil0) {   // no eddy viscosity on the ceiling row, as on the wall row
+        c->delta_static[0] = 0.0;
+        c->delta_sigma[0] = 0.0;
+    }
     return LSB_OK;
 }
 

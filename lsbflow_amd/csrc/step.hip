@@ -902,6 +902,8 @@ static void bdfcoef(int nstart, int istep, double dt1, double dt2, double dt3, d
     }
 }
 
+static void rotate_slots(StepState* S);
+
 static int update_once(lsb_ctx* c) {
     StepState* S = state_of(c);
     if (!S) return LSB_E_STATE;
@@ -1122,7 +1124,18 @@ static int update_once(lsb_ctx* c) {
         LSB_TRY(sp_project(c, L.spec[S_DWDZ], L.spec[S_DIVF], pnew, uN, vN, dt, c->bdfv, s));
     }
     }
-    // rotate (navierstokes.f90:1308-1312, 1332, 1399-1401, 1464)
+    if (g.dealiasflag > 2) {
+        // u^n history is the filtered field: keep a copy in the free slot
+        for (int l = 0; l < 3; ++l)
+            LSB_HIP(hipMemcpyAsync(L.spec[S->u[l]], uif[l], c->nspec * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    rotate_slots(S);
+    return LSB_OK;
+}
+
+// The history slots rotate by index (navierstokes.f90:1308-1312, 1332, 1399-1401, 1464):
+// host bookkeeping only.
+static void rotate_slots(StepState* S) {
     int ou[3], oun[3], ounm[3], onl[3], onlm[3], onn[3], orhs[4];
     for (int l = 0; l < 3; ++l) {
         ou[l] = S->u[l];
@@ -1133,11 +1146,6 @@ static int update_once(lsb_ctx* c) {
         onn[l] = S->nnew[l];
     }
     for (int l = 0; l < 4; ++l) orhs[l] = S->rhs[l];
-    if (g.dealiasflag > 2) {
-        // u^n history is the filtered field: keep a copy in the free slot
-        for (int l = 0; l < 3; ++l)
-            LSB_HIP(hipMemcpyAsync(L.spec[ou[l]], uif[l], c->nspec * sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
     for (int l = 0; l < 3; ++l) {
         S->unm[l] = oun[l];
         S->un[l] = ou[l];
@@ -1155,7 +1163,6 @@ static int update_once(lsb_ctx* c) {
     const int op = S->p;
     S->p = S->pn;
     S->pn = op;
-    return LSB_OK;
 }
 
 int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
@@ -1170,30 +1177,94 @@ int step_run(lsb_ctx* c, int nsteps, bool with_norm) {
         LSB_HIP(hipMemcpyAsync(S->dynbase, S->dynbase_h.data(), S->dynbase_h.size() * sizeof(double),
                                hipMemcpyHostToDevice, c->stream));
     }
+    if (c->graphs_live) {
+        set_error("timestep: the context belongs to a captured ensemble; step it with lsb_ensemble_timestep");
+        return LSB_E_STATE;
+    }
     for (int n = 0; n < nsteps; ++n) {
-        c->istep += 1;
-        if (with_norm) LSB_TRY(state_filter_hook(c));
-        LSB_TRY(update_once(c));
         if (with_norm) {
-            // norm(uf, vf, wf, tempf) (navierstokes.f90:49, horfft.f90:153-224)
-            double* f[4] = {S->L.spec[S->u[0]], S->L.spec[S->u[1]], S->L.spec[S->u[2]], nullptr};
-            const bool with_t = temp_on(c);
-            if (with_t) f[3] = temp_field(c, 0);
-            LSB_TRY(sp_norm(c, f, with_t ? 4 : 3, c->stream));
-            if (c->cfg.ifrz == 1 && c->wavx[0] == 0.0) {
-                for (int l = 0; l < 3; ++l)
-                    LSB_HIP(hipMemset2DAsync(f[l], (size_t)c->plane_s * sizeof(double), 0, 2 * sizeof(double),
-                                             (size_t)c->nzp, c->stream));
-            }
-            c->t = c->t + c->dt;
-            c->isum += 1;
-            c->dt3 = c->dt2;
-            c->dt2 = c->dt;
-            LSB_TRY(state_adapt_dt(c));
-            LSB_TRY(state_time_average(c));
+            LSB_TRY(step_enqueue(c));
+        } else {
+            c->istep += 1;
+            LSB_TRY(update_once(c));
         }
     }
     return LSB_OK;
+}
+
+// One time step (navierstokes.f90:10-90): the filtering hook, navierstokes_update, norm,
+// the time update and the per-step diagnostics, enqueued on the context's stream.
+int step_enqueue(lsb_ctx* c) {
+    StepState* S = state_of(c);
+    c->istep += 1;
+    LSB_TRY(state_filter_hook(c));
+    LSB_TRY(update_once(c));
+    // norm(uf, vf, wf, tempf) (navierstokes.f90:49, horfft.f90:153-224)
+    double* f[4] = {S->L.spec[S->u[0]], S->L.spec[S->u[1]], S->L.spec[S->u[2]], nullptr};
+    const bool with_t = temp_on(c);
+    if (with_t) f[3] = temp_field(c, 0);
+    LSB_TRY(sp_norm(c, f, with_t ? 4 : 3, c->stream));
+    if (c->cfg.ifrz == 1 && c->wavx[0] == 0.0) {
+        for (int l = 0; l < 3; ++l)
+            LSB_HIP(hipMemset2DAsync(f[l], (size_t)c->plane_s * sizeof(double), 0, 2 * sizeof(double),
+                                     (size_t)c->nzp, c->stream));
+    }
+    c->t = c->t + c->dt;
+    c->isum += 1;
+    c->dt3 = c->dt2;
+    c->dt2 = c->dt;
+    LSB_TRY(state_adapt_dt(c));
+    LSB_TRY(state_time_average(c));
+    return LSB_OK;
+}
+
+// The host side of one step whose kernels ran from a captured graph: the step counter,
+// the slot rotation and the time update (what step_enqueue does besides enqueueing).
+void step_host_advance(lsb_ctx* c) {
+    StepState* S = state_of(c);
+    c->istep += 1;
+    rotate_slots(S);
+    c->t = c->t + c->dt;
+    c->isum += 1;
+    c->dt3 = c->dt2;
+    c->dt2 = c->dt;
+}
+
+// Steps after which the slot assignment repeats (the lcm of the rotation's cycles).
+int step_rotation_period(lsb_ctx* c) {
+    StepState* S = state_of(c);
+    if (!S) return 0;
+    StepState T = *S;
+    for (int n = 1; n <= 1000; ++n) {
+        rotate_slots(&T);
+        bool same = T.p == S->p && T.pn == S->pn;
+        for (int l = 0; l < 3 && same; ++l)
+            same = T.u[l] == S->u[l] && T.un[l] == S->un[l] && T.unm[l] == S->unm[l] && T.nl[l] == S->nl[l] &&
+                   T.nlm[l] == S->nlm[l] && T.nnew[l] == S->nnew[l];
+        for (int l = 0; l < 4 && same; ++l) same = T.rhs[l] == S->rhs[l];
+        if (same) return n;
+    }
+    return 0;
+}
+
+// A step can be captured when everything it enqueues repeats: the steady EX3/BDF3
+// coefficients (past the start-up steps, constant dt), no per-step host work (adaptive
+// dt, filtering, time averages, profiling), boundary data that does not change with t.
+bool step_capturable(const lsb_ctx* c, std::string& why) {
+    const lsb_config& g = c->cfg;
+    if (!state_of(c) || !c->state_ready) why = "no state (lsb_state_set)";
+    else if (c->istep < 3 && g.nstart == 0) why = "the Euler/BDF2 start-up steps must run first (3 steps)";
+    else if (c->dt != c->dt2 || c->dt2 != c->dt3) why = "the time step must be constant";
+    else if (c->dtadapt == 1) why = "adaptive time step on";
+    else if (c->ifilt >= 1) why = "per-step filtering on";
+    else if (c->otfavg == 1) why = "time averaging on";
+    else if (c->profiling) why = "profiling on";
+    else if (g.dealiasflag > 2 || g.ifrz == 1) why = "dealiasflag > 2 / ifrz";
+    else if (g.bctype == 1) why = "the time-ramped bctype 1 boundary data";
+    else if (g.bctype > 0 && c->t < g.istartbound) why = "before istartbound";
+    else if (temp_on(c)) why = "the temperature path (its host-side flags)";
+    else return true;
+    return false;
 }
 
 }  // namespace lsb

@@ -24,7 +24,7 @@ module lsb_capi
     integer(c_int) :: mapping, bctype, forcing
     integer(c_int) :: sgsflag, sgsmodel
     integer(c_int) :: dealiasflag, dealias_ref_compat
-    integer(c_int) :: dct_exact, top_bc, dyn_avg
+    integer(c_int) :: dct_exact, top_bc, dyn_avg, nut_ceil0
     integer(c_int) :: ifrz, nstart, precision, device
     real(c_double) :: xlen, ylen, zlen, x0, u0, xnu, blfac, prtl
     real(c_double) :: a, b, cs, kc, filt_order, filt_amp
@@ -340,6 +340,31 @@ module lsb_capi
       real(c_double), value :: dtnui
       integer(c_int), value :: nfields, mode
       type(c_ptr), value :: stream
+    end function
+    ! ensembles of contexts stepped from captured graphs
+    integer(c_int) function lsb_ensemble_create(members, n, ens) bind(C, name="lsb_ensemble_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: ens
+    end function
+    integer(c_int) function lsb_ensemble_timestep(ens, nsteps) bind(C, name="lsb_ensemble_timestep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function lsb_ensemble_synchronize(ens) bind(C, name="lsb_ensemble_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function
+    integer(c_int) function lsb_ensemble_info(ens, nmembers, ngraphs, phase) bind(C, name="lsb_ensemble_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), intent(out) :: nmembers, ngraphs, phase
+    end function
+    integer(c_int) function lsb_ensemble_destroy(ens) bind(C, name="lsb_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
     end function
     ! launch-shape autotuner
     integer(c_int) function lsb_autotune(ctx) bind(C, name="lsb_autotune")

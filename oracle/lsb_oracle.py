@@ -104,6 +104,7 @@ class OracleConfig:
     dct_exact: int = 0
     top_bc: int = 0
     dyn_avg: int = 0
+    nut_ceil0: int = 0
     ifrz: int = 0
     nstart: int = 0
     dt: float = 1.0e-3
@@ -648,7 +649,9 @@ class Oracle:
 
     # ---------------------------------------------------------------------- SGS
     def sgs_delta_static(self):
-        """sgsterm.f90:95-103 (Q8: delta(nzp) is never set; the Van Driest factor is 0 there)."""
+        """sgsterm.f90:95-103 (Q8: delta(nzp) is never set; the Van Driest factor is 0 there).
+
+        ``nut_ceil0 = 1`` also zeroes the ceiling row's width (no eddy viscosity there)."""
         z = self.zpts
         nzp, nz = self.nzp, self.nz
         dx, dy = self.dx, self.dy
@@ -661,6 +664,8 @@ class Oracle:
         dz = abs(z[nzp - 1] - z[nzp - 2])
         delta[0] = (dx * dy * dz) ** (1.0 / 3.0)
         delta[nzp - 1] = 0.0
+        if self.cfg.nut_ceil0:
+            delta[0] = 0.0
         return delta
 
     def static_smagorinsky(self, cs, duidxj):
@@ -795,6 +800,8 @@ class Oracle:
         for k in range(1, nzp - 1):
             dz[k] = 0.5 * (z[k - 1] - z[k + 1])
         delta = (self.dx * self.dy * dz) ** (1.0 / 3.0)
+        if self.cfg.nut_ceil0:
+            delta[0] = 0.0
         nu = np.zeros(duidxj.shape[2:])
         nu[..., :nx] = -((1.35 * delta) ** 2)[:, None, None] * ds
         tau = np.empty_like(duidxj)

@@ -44,7 +44,7 @@ program binding_check
                           uvec, vvec, wvec)
 
   ! an invalid configuration (odd nx) is refused before any device call
-  cfg = lsb_config(3, 4, 8, 3, 0, 0, 0, 0, 2, 0, 0, 0, 0, 1, 0, LSB_FP64, 0, &
+  cfg = lsb_config(3, 4, 8, 3, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0, 1, 0, LSB_FP64, 0, &
                    6.283185307179586d0, 3.141592653589793d0, 2.d0, 0.d0, 1.d0, 1.d-3, 1.d0, 1.d0, &
                    1.d0, 2.d0, 0.1d0, 0.6d0, 1.d0, 20.d0, 0.d0, 0.d0, 1.d-3)
   st(4) = lsb_create(cfg, ctx)

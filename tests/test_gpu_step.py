@@ -19,6 +19,7 @@ import pytest
 
 from oracle import Oracle, OracleConfig
 import lsbflow_amd as L
+from lsbflow_amd import _lib
 from lsbflow_amd.core import setup_case
 from tests.conftest import rel_l2
 
@@ -99,6 +100,20 @@ def test_les_dynamic_plane_average():
                                         a=1.0, b=2.0, dt=5e-4, pert_rms=0.05, dyn_avg=1)
     st, g = run_both(case, 6)
     check_fields(st, g)
+
+
+@pytest.mark.parametrize("sgsmodel", [0, 2])
+def test_les_ceiling_row_without_eddy_viscosity(sgsmodel):
+    """nut_ceil0 = 1 (zero filter width on the ceiling row) for the static and sigma
+    models with the blended ceiling, 6 steps on a 32 x 41 x 16 grid, against the oracle;
+    the dynamic and interscale models refuse the switch at lsb_create."""
+    case = L.cases.CASES["c1_dns"].with_(nx=32, ny=16, nz=40, xnu=2e-4, samp=0.3, sgsflag=1, sgsmodel=sgsmodel,
+                                        dt=5e-4, pert_rms=0.05, dct_exact=1, top_bc=2, nut_ceil0=1)
+    st, g = run_both(case, 6)
+    check_fields(st, g)
+    for bad in (1, 3):
+        with pytest.raises(_lib.LsbError, match="nut_ceil0"):
+            setup_case(case.with_(sgsmodel=bad))
 
 
 @pytest.mark.parametrize("dealias", [0, 2])

@@ -250,6 +250,21 @@ def test_static_smagorinsky_brute_force():
                 assert np.allclose(tau[:, :, k, j, i], nu * S, rtol=1e-12, atol=1e-300)
 
 
+def test_ceiling_row_without_eddy_viscosity():
+    """nut_ceil0 = 1: the static and sigma stresses vanish on the ceiling row (row 0) and
+    are unchanged on every other row."""
+    for model in ("static", "sigma"):
+        out = {}
+        for flag in (0, 1):
+            o = small(nx=8, ny=4, nz=6, nut_ceil0=flag)
+            g = np.zeros((3, 3, o.nzp, o.ny, o.nxpp))
+            g[..., : o.nx] = np.random.default_rng(13).standard_normal((3, 3, o.nzp, o.ny, o.nx))
+            out[flag] = o.static_smagorinsky(0.1, g) if model == "static" else o.sigma_model(g)
+        assert np.max(np.abs(out[0][:, :, 0])) > 0.0, model
+        assert np.all(out[1][:, :, 0] == 0.0), model
+        assert np.array_equal(out[1][:, :, 1:], out[0][:, :, 1:]), model
+
+
 def test_dynamic_smagorinsky_pointwise_algebra():
     """Restate sgsterm.f90:171-267 with explicit per-point loops on top of the filter."""
     o = small(nx=6, ny=4, nz=6, a=1.0, b=2.0)
