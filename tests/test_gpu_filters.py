@@ -13,7 +13,7 @@ import pytest
 from oracle import Oracle, OracleConfig
 import lsbflow_amd as L
 from lsbflow_amd.core import setup_case
-from tests.conftest import rel_l2
+from tests.conftest import noisy_oracle_run, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -119,16 +119,14 @@ def test_timestep_with_filtering(variant):
         g = c.state_get()
     finally:
         c.close()
-    # the pressure (a divergence over k^2 dt) is held to 1000 x the oracle's own response to a
-    # 1e-15 relative perturbation of the initial condition when that exceeds 1e-10, as in
-    # the other step tests
-    rng2 = np.random.default_rng(99)
-    st2 = o.new_state(*[f * (1.0 + 1e-15 * rng2.standard_normal(f.shape)) for f in (u, v, w)])
-    for _ in range(nsteps):
-        o.timestep(st2)
+    # the step tests' bar: 1e-10, or 10 x the oracle's own drift under rounding noise where
+    # that is larger, and 1e-7 for the pressure (a divergence over k^2 dt)
+    st2 = noisy_oracle_run(oracle_for(CASE, **kw), (u, v, w), nsteps)
     for k in ("uf", "vf", "wf", "pf"):
-        bound = max(1e-10, 1000.0 * rel_l2(st2[k], st[k]))
-        assert rel_l2(g[k], st[k]) < bound, (k, rel_l2(g[k], st[k]), bound)
+        e, s = rel_l2(g[k], st[k]), rel_l2(st2[k], st[k])
+        bound = max(1e-10, 10.0 * s, 1e-7 if k == "pf" else 0.0)
+        print(f"{variant} {k} {e:.2e} (sens {s:.1e})", flush=True)
+        assert e < bound, (k, e, bound)
     # the filter changed the solution (or, for tns_not_applied, did not)
     o0 = oracle_for(CASE, **{k: v for k, v in kw.items() if k not in ("ifilt", "modfilt")})
     st0 = o0.new_state(u, v, w)

@@ -15,7 +15,7 @@ import pytest
 from oracle import Oracle, OracleConfig
 import lsbflow_amd as L
 from lsbflow_amd.core import Config, Context
-from tests.conftest import rel_l2
+from tests.conftest import rel_l2, wrap_noisy_ops
 
 pytestmark = pytest.mark.gpu
 
@@ -279,9 +279,15 @@ def test_nlterm_skewsym(sgs, rng):
     uif = np.stack([u, v, w])
     got = c.nlterm_skewsym(uif)
     ref, _ = o.nlterm_skewsym(uif, np.zeros_like(u), sgs[0])
-    # the oracle's response to a 1e-15 relative input perturbation bounds the attainable agreement
-    # (1.4e-7 for the sigma model on this nearly parallel flow, ~6e-15 otherwise)
-    pert = uif * (1.0 + 1e-15 * rng.standard_normal(uif.shape))
-    ref2, _ = o.nlterm_skewsym(pert, np.zeros_like(u), sgs[0])
+    # the oracle's response to 1e-16 relative noise on its input and on the output of every
+    # transform and derivative bounds the attainable agreement (the sigma model's
+    # invariant/acos evaluation amplifies it on this nearly parallel flow)
+    o2 = Oracle(OracleConfig(nx=32, ny=16, nz=40, mapping=3, sgsflag=sgs[0], sgsmodel=sgs[1], a=1.0, b=2.0))
+    o2.ubar, o2.ubarp, o2.ubarpp = ub, ubp, ubpp
+    wrap_noisy_ops(o2, rng)
+    pert = uif * (1.0 + 1e-16 * rng.standard_normal(uif.shape))
+    ref2, _ = o2.nlterm_skewsym(pert, np.zeros_like(u), sgs[0])
     sens = rel_l2(ref2, ref)
-    assert rel_l2(got, ref) < max(1e-11, 1000.0 * sens), (rel_l2(got, ref), sens)
+    err = rel_l2(got, ref)
+    print(f"nlterm sgs {sgs}: rel L2 {err:.2e} (sens {sens:.1e})", flush=True)
+    assert err < max(1e-11, 10.0 * sens), (err, sens)
