@@ -297,6 +297,132 @@ def test_dynamic_smagorinsky_pointwise_algebra():
     assert np.allclose(tau[..., :nx], ref[..., :nx], rtol=1e-10, atol=1e-14)
 
 
+def _zweights(z, a, b):
+    """filters.f90:396-448, channel mapping, interpflag 1 (rows top first)."""
+    n = len(z)
+    az, bz, cz = np.full(n, a), np.full(n, b), np.full(n, a)
+    mid = (n + 1) // 2
+    for k in range(2, mid):
+        m, p = z[k - 1] - z[k - 2], z[k] - z[k - 1]
+        az[k - 1], bz[k - 1], cz[k - 1] = 2 * a * m / (p + m), 2 * a * (1 - m / p) + b, 2 * a * m * m / (p * (p + m))
+    for k in range(mid + 1, n):
+        m, p = z[k - 1] - z[k - 2], z[k] - z[k - 1]
+        az[k - 1], bz[k - 1], cz[k - 1] = 2 * a * p * p / (m * (m + p)), 2 * a * (1 - p / m) + b, 2 * a * p / (m + p)
+    az[mid - 1], bz[mid - 1], cz[mid - 1] = a, b, a
+    az[[0, -1]], bz[[0, -1]], cz[[0, -1]] = 0.0, 2 * a + b, 0.0
+    return az, bz, cz
+
+
+def _zpass(v, w, s):
+    az, bz, cz = w
+    out = np.array(v, copy=True)
+    out[1:-1] = (az[1:-1, None] * v[:-2] + bz[1:-1, None] * v[1:-1] + cz[1:-1, None] * v[2:]) / s
+    return out
+
+
+def _dynamic_x_only(o, u1, du1, dyn_avg, a=1.0, b=2.0):
+    """sgsterm.f90:123-280 for u = u1(x), v = w = 0 (no y or z dependence), written
+    separably: every x pass of the 3-point filter through its exact transfer function
+    (b + 2a cos(2 pi m / nx)) / (2a + b) on the DFT, the z pass (filters.f90:396-448) acting
+    only on the row widths Delta(z) (its weights sum to 2a + b, so the z-independent factors
+    pass through).  Returns tau_11 (nzp, nx) and the coefficient c (nzp, nx)."""
+    nx, nzp = o.nx, o.nzp
+    m = np.fft.fftfreq(nx, 1.0 / nx)
+
+    def fx(f, a_=a, b_=b):
+        t = (b_ + 2 * a_ * np.cos(2 * math.pi * m / nx)) / (2 * a_ + b_)
+        return np.real(np.fft.ifft(t * np.fft.fft(f, axis=-1), axis=-1))
+
+    z = o.zpts
+    dz = np.empty(nzp)
+    dz[0], dz[-1], dz[1:-1] = z[0] - z[1], z[-2] - z[-1], 0.5 * (z[:-2] - z[2:])
+    D2 = ((o.dx * o.dy * dz) ** (1 / 3)) ** 2
+    S, Sh = du1, fx(du1)
+    L = fx(u1 * u1) - fx(u1) ** 2                                  # Leonard term L_11
+    P = fx(math.sqrt(2) * np.abs(S) * S)                           # filter of |S| S_11 (x part)
+    Q = math.sqrt(2) * np.abs(Sh) * Sh                             # |S^| S^_11
+    ZD2 = _zpass(D2[:, None], _zweights(z, a, b), 2 * a + b)[:, 0]
+    M = 2 * (ZD2[:, None] * P[None, :] - 4 * D2[:, None] * Q[None, :])
+    N, D = M * L[None, :], M * M
+    if dyn_avg:
+        cn, cd = N.mean(axis=1, keepdims=True), D.mean(axis=1, keepdims=True)
+    else:
+        w1 = _zweights(z, 1.0, 1.0)
+        cn, cd = _zpass(fx(N, 1.0, 1.0), w1, 3.0), _zpass(fx(D, 1.0, 1.0), w1, 3.0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        c = cn / cd
+    c = np.minimum(np.where((c < 0) | np.isnan(c), 0.0, c), 0.2 ** 2) * np.ones((1, nx))
+    return -2 * c * D2[:, None] * math.sqrt(2) * np.abs(S)[None, :] * S[None, :], c
+
+
+def _xonly_fields(o, k1, a1, k2, a2):
+    nx = o.nx
+    x = np.arange(nx) * (o.cfg.xlen / nx)
+    u1 = a1 * np.sin(k1 * x) + a2 * np.cos(k2 * x)
+    du1 = a1 * k1 * np.cos(k1 * x) - a2 * k2 * np.sin(k2 * x)
+    sh = (o.nzp, o.ny, o.nxpp)
+    ui = np.zeros((3,) + sh)
+    ui[0, ..., :nx] = u1
+    g = np.zeros((3, 3) + sh)
+    g[0, 0, ..., :nx] = du1
+    return ui, g, u1, du1
+
+
+@pytest.mark.parametrize("dyn_avg,modes", [(1, (2, 1.0, 1, 0.7)), (1, (6, 1.0, 3, 1.0)), (0, (1, 1.0, 2, 0.5))])
+def test_dynamic_smagorinsky_x_only_field(dyn_avg, modes):
+    """The oracle's dynamic model on u = a1 sin(k1 x) + a2 cos(k2 x) against the separable
+    spectral restatement (the Leonard term of a single mode is also checked in closed form:
+    L_11 = A^2/2 (1 - T(2k) cos 2kx) - T(k)^2 A^2 sin^2 kx).  The mode pairs give
+    coefficients inside (0, 0.04) on a good share of the points, so the test is not
+    settled by the clipping alone."""
+    o = small(nx=16, ny=8, nz=12, a=1.0, b=2.0, dyn_avg=dyn_avg)
+    ui, g, u1, du1 = _xonly_fields(o, *modes)
+    uu = np.einsum("l...,m...->lm...", ui, ui)
+    tau = o.dynamic_smagorinsky(ui, uu, g)
+    want, c = _dynamic_x_only(o, u1, du1, dyn_avg)
+    assert np.mean((c > 1e-6) & (c < 0.04)) > 0.02, c
+    assert np.max(np.abs(tau[0, 0, :, :, : o.nx] - want[:, None, :])) < 1e-13 * np.max(np.abs(want))
+    others = np.delete(tau.reshape(9, *tau.shape[2:]), 0, axis=0)
+    assert np.max(np.abs(others)) == 0.0
+    # single-mode Leonard term in closed form
+    A, k = 0.8, 3
+    nx = o.nx
+    x = np.arange(nx) * (o.cfg.xlen / nx)
+    T = lambda kk: (2.0 + 2 * 1.0 * math.cos(kk * o.cfg.xlen / nx)) / 4.0  # noqa: E731
+    u = A * np.sin(k * x)
+    uu1 = np.zeros((o.nzp, o.ny, o.nxpp))
+    uu1[..., :nx] = u * u
+    u1f = np.zeros((o.nzp, o.ny, o.nxpp))
+    u1f[..., :nx] = u
+    Lo = (o.filter_q_3pt(uu1, 1.0, 2.0, 1) - o.filter_q_3pt(u1f, 1.0, 2.0, 1) ** 2)[..., :nx]
+    Lc = A * A / 2 * (1 - T(2 * k) * np.cos(2 * k * x)) - T(k) ** 2 * A * A * np.sin(k * x) ** 2
+    assert np.max(np.abs(Lo - Lc[None, None, :])) < 1e-14
+
+
+def test_dynamic_smagorinsky_invariances():
+    """Galilean invariance (a uniform velocity added to u_i changes neither L_ij nor M_ij),
+    degree-2 homogeneity (c is scale free, tau ~ |u|^2), and zero stress when the test
+    filter is the identity (a = 0: L_ij = 0)."""
+    o = small(nx=8, ny=4, nz=10, a=1.0, b=2.0)
+    rng = np.random.default_rng(21)
+    sh = (o.nzp, o.ny, o.nxpp)
+    ui = np.zeros((3,) + sh)
+    ui[..., : o.nx] = rng.standard_normal((3, o.nzp, o.ny, o.nx))
+    g = np.zeros((3, 3) + sh)
+    g[..., : o.nx] = rng.standard_normal((3, 3, o.nzp, o.ny, o.nx))
+    uu = lambda q: np.einsum("l...,m...->lm...", q, q)  # noqa: E731
+    tau = o.dynamic_smagorinsky(ui, uu(ui), g)
+    assert np.max(np.abs(tau)) > 0.0
+    shifted = ui.copy()
+    shifted[:, ..., : o.nx] += np.array([0.7, -1.3, 0.4])[:, None, None, None]
+    tau_g = o.dynamic_smagorinsky(shifted, uu(shifted), g)
+    assert np.max(np.abs(tau_g - tau)) < 1e-12 * np.max(np.abs(tau))
+    tau_2 = o.dynamic_smagorinsky(2.0 * ui, uu(2.0 * ui), 2.0 * g)
+    assert np.max(np.abs(tau_2 - 4.0 * tau)) < 1e-12 * np.max(np.abs(tau))
+    o0 = small(nx=8, ny=4, nz=10, a=0.0, b=2.0)
+    assert np.max(np.abs(o0.dynamic_smagorinsky(ui, uu(ui), g))) == 0.0
+
+
 # ---------------------------------------------------------------- dealias
 def test_dealiasing_correct_mode_exact_product():
     o = small(nx=16, ny=8, dealiasflag=1)
