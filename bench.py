@@ -433,6 +433,60 @@ def c3_helmholtz(reps):
     return 0
 
 
+def dyn_companion(steps, warmup, accuracy_steps, tune):
+    """The dynamic-Smagorinsky LES at the same 512 x 193 x 256 size (c4_dyn_les): both forms
+    timed as the headline workload is, and the mixed form against the FP64 form after
+    `accuracy_steps` steps from the same seed."""
+    import lsbflow_amd as L
+    case = L.cases.CASES["c4_dyn_les"]
+    ic = L.cases.initial_condition(case)
+    s64, _, f64, _ = gpu_run(case, steps, warmup, 0, ic, tune)
+    s32, _, f32, _ = gpu_run(case, steps, warmup, 1, ic, tune)
+    errs, sep, fin = accuracy_leg(case, accuracy_steps, ic)
+    pts = case.points
+    return {"workload": f"c4_dyn_les: {case.nx}x{case.nzp}x{case.ny}, dynamic Smagorinsky, dt={case.dt}",
+            "value": float(f"{pts * steps / s32:.6g}"), "ms_per_step": round(1e3 * s32 / steps, 4),
+            "fp64": {"value": float(f"{pts * steps / s64:.6g}"), "ms_per_step": round(1e3 * s64 / steps, 4)},
+            "speedup_vs_fp64": round(s64 / s32, 3),
+            "mixed_vs_fp64": {"steps": accuracy_steps, "rel_l2": errs, "separation": sep,
+                              "all_within_1e-5": bool(max(errs.values()) <= 1e-5)},
+            "finite": bool(f64 and f32 and fin)}
+
+
+def c2_dynamic(steps, warmup):
+    """C2: the plain FP64 dynamic-Smagorinsky LES, 256 x 129 x 128, Re 5e4, dt 5e-4, with its
+    configured ceiling, `steps` timed steps after `warmup`; the state finite and the discrete
+    divergence after the run relative to the velocity scale."""
+    import numpy as np
+    import lsbflow_amd as L
+    from lsbflow_amd.core import setup_case
+    case = L.cases.CASES["c2_dyn_les"]
+    ic = L.cases.initial_condition(case)
+    out = {"c2": f"{case.name}: {case.nx}x{case.nzp}x{case.ny} (nx x nzp x ny), dynamic Smagorinsky, "
+                 f"Re 5e4, dt {case.dt}", "steps": steps, "warmup": warmup, "unit": "grid-point-steps/s"}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        ctx = setup_case(case, precision=prec)
+        ctx.autotune()
+        ctx.state_set(*ic)
+        ctx.timestep(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.timestep(steps)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        g = ctx.state_get()
+        finite = all(bool(np.all(np.isfinite(g[k]))) for k in ("uf", "vf", "wf", "pf"))
+        div = ctx.divcheck(g["uf"], g["vf"], g["wf"])
+        scale = float(np.sqrt(np.mean(ctx.inv(g["uf"])[..., : ctx.nx] ** 2)))
+        ctx.close()
+        out[name] = {"value": float(f"{case.points * steps / (t1 - t0):.6g}"),
+                     "ms_per_step": round(1e3 * (t1 - t0) / steps, 4), "finite_after_run": finite,
+                     "mean_abs_divergence_over_u_rms": float(f"{div / max(scale, 1e-300):.3g}")}
+    out["speedup_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    print(json.dumps(out), flush=True)
+    return 0 if out["fp64"]["finite_after_run"] and out["mixed"]["finite_after_run"] else 1
+
+
 def c5_ensemble(steps, warmup):
     """The uneven ensemble: 16 static-Smagorinsky LES members of different grids stepped
     together from captured graphs (one graph launch per step for all members), against
@@ -506,6 +560,10 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes (no lsb_autotune)")
     ap.add_argument("--c5", action="store_true",
                     help="time the uneven 16-member ensemble stepped from captured graphs (and eagerly)")
+    ap.add_argument("--c2", action="store_true",
+                    help="the 256x129x128 dynamic LES (C2) line instead of the headline")
+    ap.add_argument("--no-dyn", action="store_true",
+                    help="skip the 512x193x256 dynamic-Smagorinsky companion of the headline line")
     ap.add_argument("--c3", action="store_true",
                     help="time the batched Helmholtz solve alone (FP64, FP32, FP32 + one FP64 refinement)")
     ap.add_argument("--accuracy-steps", type=int, default=100)
@@ -537,6 +595,8 @@ def main(argv=None):
         return pmc_probe(case)
     if args.diagnostics:
         return diagnostics(case, args.steps)
+    if args.c2:
+        return c2_dynamic(100 if args.steps == 10 else args.steps, args.warmup)
     if args.c3:
         return c3_helmholtz(args.steps)
     if args.c5:
@@ -592,9 +652,13 @@ def main(argv=None):
         "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
         "autotune": table,
     }
+    if args.precision == "mixed" and case.name == "c4_sigma_les" and not args.no_dyn:
+        out["c4_dyn_les"] = dyn_companion(args.steps, args.warmup, args.accuracy_steps, tune)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_ny, args.cpu_sample_steps)
     ok = finite and finite64 and (accuracy is None or accuracy["finite"])
+    if "c4_dyn_les" in out:
+        ok = ok and out["c4_dyn_les"]["finite"]
     if not ok:
         out["value"] = None
         out["error"] = "non-finite state after the timed steps or the accuracy leg"
