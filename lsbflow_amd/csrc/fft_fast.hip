@@ -20,6 +20,8 @@
 // apply d/dx or d/dy (x i*kx, x i*ky, derivatives.f90:261-303) to a field as it
 // loads it, so spectral derivative fields never round-trip through HBM.
 
+#include <cstring>
+
 #include "lsb_internal.h"
 
 namespace lsb {
@@ -230,15 +232,19 @@ __global__ void __launch_bounds__(128) k_xfwd_fast(FieldList<TI> in, FieldList<T
 
 // Inverse c2r of rows holding L complex coefficients (input row stride istride
 // reals) into physical rows (stride nxpp, the two pad reals written as zero);
-// in place when in == out and istride == nxpp.
-template <class R, class TI, class TO, int N1, int N2>
+// in place when in == out and istride == nxpp.  STAGE: the block's LPB rows, pads
+// included, are one contiguous run of LPB*nxpp reals of one field (nrows a multiple of
+// LPB); they are assembled in the LDS tile and stored as 16-byte vectors, so the
+// stores fill whole 128-byte segments although the rows of stride nxpp = L*2 + 2 start
+// off the segment grid.
+template <class R, class TI, class TO, int N1, int N2, bool STAGE = false>
 __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<TI> in, FieldList<TO> out, int nrows, int nf,
                                                    int istride, int nxpp, const cpx<R>* __restrict__ tw,
                                                    const cpx<R>* __restrict__ twr) {
     constexpr int L = N1 * N2;
     constexpr int LPB = 128 / N2;
     constexpr int TS = N1 * (N2 + 1);
-    __shared__ cpx<R> T[LPB * TS];
+    __shared__ __align__(16) cpx<R> T[LPB * TS];  // 16-byte aligned for the staged vector copy
     const int tid = threadIdx.x;
     const int l = tid / N2, n2 = tid % N2;
     const long r = (long)blockIdx.x * LPB + l;
@@ -285,6 +291,48 @@ __global__ void __launch_bounds__(128) k_xinv_fast(FieldList<TI> in, FieldList<T
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) Tl[k1 * (N2 + 1) + n2] = v[k1];
     __syncthreads();
+    if constexpr (STAGE && sizeof(TO) <= sizeof(R)) {
+        static_assert(LPB * TS * sizeof(cpx<R>) >= LPB * (2 * L + 2) * sizeof(TO), "staging tile");
+        cpx<R> u[N1 / N2][N2];
+#pragma unroll
+        for (int q = 0; q < N1 / N2; ++q) {
+            const int k1 = n2 + N2 * q;
+#pragma unroll
+            for (int j = 0; j < N2; ++j) u[q][j] = Tl[k1 * (N2 + 1) + j];
+            f_dft<R, 1, N2>(u[q]);
+        }
+        __syncthreads();
+        // the block's rows in their memory layout (stride nxpp = 2L + 2 reals of TO)
+        TO* S = reinterpret_cast<TO*>(T);
+        TO* Sl = S + (long)l * (2 * L + 2);
+#pragma unroll
+        for (int q = 0; q < N1 / N2; ++q) {
+            const int k1 = n2 + N2 * q;
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                Sl[2 * (k1 + N1 * k2)] = (TO)u[q][k2].re;
+                Sl[2 * (k1 + N1 * k2) + 1] = (TO)u[q][k2].im;
+            }
+        }
+        if (n2 == 0) {
+            Sl[2 * L] = (TO)0;
+            Sl[2 * L + 1] = (TO)0;
+        }
+        __syncthreads();
+        // the valid rows of the block: one run of the field (the host guarantees it)
+        const long r0 = (long)blockIdx.x * LPB;
+        const long nv = min((long)LPB, rlast + 1 - r0);
+        const int f0 = (int)(r0 / nrows);
+        TO* g = out.p[f0] + (r0 - (long)f0 * nrows) * nxpp;
+        const long nreal = nv * nxpp;
+        constexpr int VW = 16 / (int)sizeof(TO);
+        using V = typename std::conditional<sizeof(TO) == 8, double2, float4>::type;
+        const long nvec = nreal / VW;
+        for (long e = tid; e < nvec; e += 128)
+            reinterpret_cast<V*>(g)[e] = reinterpret_cast<const V*>(S)[e];
+        for (long e = nvec * VW + tid; e < nreal; e += 128) g[e] = S[e];
+        return;
+    }
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < N1 / N2; ++q) {
@@ -530,11 +578,22 @@ int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int 
               const cpx<R>* tw, const cpx<R>* twr, hipStream_t s) {
     const long rows = (long)nrows * nf;
     if (rows <= 0) return LSB_OK;
+    // the staged store: whole blocks of rows inside one field, rows of exactly 2 lh + 2
+    // reals, fields 16-byte aligned
+    bool aligned = nxpp == 2 * lh + 2 && sizeof(TO) <= sizeof(R);
+    const char* off = std::getenv("LSB_NO_XSTAGE");  // the direct row stores, for tests
+    if (off && off[0] && std::strcmp(off, "0") != 0) aligned = false;
+    for (int f = 0; f < nf && aligned; ++f) aligned = ((uintptr_t)out.p[f] & 15) == 0;
     switch (lh) {
 #define X(LL, A, B)                                                                                                 \
     case LL:                                                                                                        \
-        hipLaunchKernelGGL((k_xinv_fast<R, TI, TO, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))),       \
-                           dim3(128), 0, s, in, out, nrows, nf, istride, nxpp, tw, twr);                            \
+        if (aligned && nrows % (128 / B) == 0)                                                                      \
+            hipLaunchKernelGGL((k_xinv_fast<R, TI, TO, A, B, true>),                                                \
+                               dim3((unsigned)((rows + 128 / B - 1) / (128 / B))), dim3(128), 0, s, in, out, nrows, \
+                               nf, istride, nxpp, tw, twr);                                                         \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_xinv_fast<R, TI, TO, A, B>), dim3((unsigned)((rows + 128 / B - 1) / (128 / B))),   \
+                               dim3(128), 0, s, in, out, nrows, nf, istride, nxpp, tw, twr);                        \
         break;
         LSB_FAST_CASES(X)
 #undef X

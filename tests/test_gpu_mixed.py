@@ -142,3 +142,21 @@ def test_mixed_state_roundtrip_history():
             assert np.array_equal(g[k], hist[k].astype(np.float32).astype(np.float64))
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_inverse_x_pass_staged_store_bit_identical(precision, monkeypatch):
+    """The inverse x pass assembling a block's rows in LDS and storing them as one aligned
+    run gives the same bits as its direct row stores (LSB_NO_XSTAGE=1), in both forms, on
+    a 512-point x grid with the sigma model (every physical field goes through it)."""
+    case = L.cases.CASES["c1_dns"].with_(nx=512, ny=32, nz=32, sgsflag=1, sgsmodel=2, xnu=2e-5, dt=2e-4)
+    out = {}
+    for off in ("0", "1"):
+        monkeypatch.setenv("LSB_NO_XSTAGE", off)
+        c = setup_case(case, precision=precision)
+        c.state_set(*L.cases.initial_condition(case))
+        c.timestep(3)
+        out[off] = c.state_get()
+        c.close()
+    for k in ("uf", "vf", "wf", "pf"):
+        assert np.array_equal(out["0"][k], out["1"][k]), k
