@@ -89,3 +89,14 @@ def test_fortran_program_through_the_binding(tmp_path):
     D = o.d2[1:o.nz, 1:o.nz]
     assert np.max(np.abs(e @ np.diag(lam) @ ei - D)) < 1e-13 * np.max(np.abs(D))
     assert np.max(np.abs(np.sort(lam) - np.sort(o.diag)) / np.abs(np.sort(o.diag))) < 1e-10
+
+
+def test_drop_in_modules_compile_and_link(tmp_path):
+    """The drop-in modules (lsbflow_amd/fortran/lsb_shims.f90) and a caller that uses them
+    as the reference's driver does compile with amdflang and link against the library
+    (the run is the GPU test tests/test_gpu_fortran_shims.py)."""
+    from tests.test_gpu_fortran_shims import build_shim_program
+    exe = build_shim_program(tmp_path)
+    assert os.path.exists(exe)
+    for mod in ("modhorfft", "derivatives", "helmholtz", "sgs_models", "filters", "dealias"):
+        assert os.path.exists(tmp_path / f"{mod}.mod"), mod
