@@ -564,14 +564,13 @@ __global__ void __launch_bounds__(256) k_filt3_fused(PhDims d, FieldList<T> in, 
 // xy-filtered value of the thread's points, and the z pass combines the three planes held
 // in registers.  Same per-element expressions and order as the two-pass kernels, so the
 // results are bit-identical to them.
-template <class T, int R>
+template <class T, int R, int TX = 64>
 __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, FieldList<T> out, double a, double b,
                                                    const double* __restrict__ w, int KC) {
-    constexpr int TX = 64;
     constexpr int NT = (R + 2) * (TX + 2);          // tile elements
     constexpr int NL = (NT + 255) / 256;            // tile loads per thread
     constexpr int NP = (R * TX) / 256;              // output points per thread
-    static_assert((R * TX) % 256 == 0, "R * 64 must be a multiple of 256");
+    static_assert((R * TX) % 256 == 0, "R * TX must be a multiple of 256");
     __shared__ T tile[NT];
     __shared__ double xs[(R + 2) * TX];
     const int f = blockIdx.z;
@@ -667,12 +666,12 @@ __global__ void __launch_bounds__(256) k_filt3_lds(PhDims d, FieldList<T> in, Fi
     }
 }
 
-template <class T, int R>
+template <class T, int R, int TX = 64>
 static void launch_filt3_lds(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
                              double b, const double* w, int KC, hipStream_t s) {
-    const int ntx = (d.nx + 63) / 64;
+    const int ntx = (d.nx + TX - 1) / TX;
     const dim3 grid((unsigned)(ntx * ((d.ny + R - 1) / R)), (unsigned)((d.nzp + KC - 1) / KC), (unsigned)nb);
-    hipLaunchKernelGGL((k_filt3_lds<T, R>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
+    hipLaunchKernelGGL((k_filt3_lds<T, R, TX>), grid, dim3(256), 0, s, d, fi, fo, a, b, w, KC);
 }
 
 template <class T, int R, int NXT>
@@ -688,14 +687,17 @@ static void launch_filt3_fused(const PhDims& d, const FieldList<T>& fi, const Fi
 static const int kFiltR[3] = {2, 4, 8};
 static const int kFiltKC[3] = {8, 16, 32};
 // variants 0..8: the register-sliding kernel (R, KC) above; 9..17: the LDS kernel with
-// R in {4, 8, 16} and KC in {16, 32, 64}
+// 64-point x tiles, R in {4, 8, 16} and KC in {16, 32, 64}; 18..23: 128-point x tiles
+// (twice the outputs per thread and barrier, half the relative x halo), R in {4, 8}
 static const int kLdsR[3] = {4, 8, 16};
 static const int kLdsKC[3] = {16, 32, 64};
-constexpr int kFilt3Variants = 18;
-// defaults (measured on 512 x 193 x 256): LDS kernel, FP64 storage R = 4, KC = 16;
-// FP32 storage R = 8, KC = 16
+static const int kLdsWideR[2] = {4, 8};
+constexpr int kFilt3Variants = 24;
+// defaults (measured on 512 x 193 x 256, 8 fields per launch): LDS kernel, FP64 storage
+// 64-point tiles, R = 4, KC = 16 (0.77 ms); FP32 storage 128-point tiles, R = 4, KC = 16
+// (0.43 ms against 0.46 with the best 64-point tile)
 constexpr int kFilt3Default64 = 9;
-constexpr int kFilt3Default32 = 12;
+constexpr int kFilt3Default32 = 18;
 
 template <class T, int NXT>
 static void filt3_launch_r(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a,
@@ -710,6 +712,15 @@ static void filt3_launch_r(const PhDims& d, const FieldList<T>& fi, const FieldL
 template <class T>
 static int filt3_variant(const PhDims& d, const FieldList<T>& fi, const FieldList<T>& fo, int nb, double a, double b,
                          const double* w, int v, hipStream_t s) {
+    if (v >= 18) {
+        const int R = kLdsWideR[(v - 18) / 3], KC = kLdsKC[(v - 18) % 3];
+        if (R == 4)
+            launch_filt3_lds<T, 4, 128>(d, fi, fo, nb, a, b, w, KC, s);
+        else
+            launch_filt3_lds<T, 8, 128>(d, fi, fo, nb, a, b, w, KC, s);
+        LSB_HIP(hipGetLastError());
+        return LSB_OK;
+    }
     if (v >= 9) {
         const int R = kLdsR[(v - 9) / 3], KC = kLdsKC[(v - 9) % 3];
         switch (R) {

@@ -14,7 +14,7 @@ from lsbflow_amd.core import setup_case
 
 pytestmark = pytest.mark.gpu
 
-N_VARIANTS = 18
+N_VARIANTS = 24
 
 
 def run(case, precision, variant, nsteps=2):
