@@ -487,21 +487,20 @@ def c2_dynamic(steps, warmup):
     return 0 if out["fp64"]["finite_after_run"] and out["mixed"]["finite_after_run"] else 1
 
 
-def c5_ensemble(steps, warmup):
+def c5_ensemble(steps, warmup, precision=1):
     """The uneven ensemble: 16 static-Smagorinsky LES members of different grids stepped
     together from captured graphs (one graph launch per step for all members), against
-    the same members stepped eagerly one after another on their own streams.  Members
-    whose nx/2 and ny are register-resident FFT lengths run the mixed form, the others
-    the FP64 form."""
+    the same members stepped eagerly one after another on their own streams.  Every member
+    runs the mixed form (the FP64 form with --precision fp64); members whose nx/2 is not a
+    register-resident FFT length (nx 96, 192) take the FP32 Stockham transforms."""
     import numpy as np
     import lsbflow_amd as L
     from lsbflow_amd.core import Ensemble, setup_case
 
-    fast = (16, 32, 64, 128, 256)
     members = L.cases.ensemble_members()
     ctxs, prec = [], []
     for m in members:
-        p = 1 if (m.nx // 2 in fast and m.ny in fast) else 0
+        p = precision
         c = setup_case(m, precision=p)
         c.autotune()
         c.state_set(*L.cases.initial_condition(m))
@@ -600,7 +599,7 @@ def main(argv=None):
     if args.c3:
         return c3_helmholtz(args.steps)
     if args.c5:
-        return c5_ensemble(args.steps, args.warmup)
+        return c5_ensemble(args.steps, args.warmup, 1 if args.precision == "mixed" else 0)
     ic = L.cases.initial_condition(case)
     pts = case.points
     tune = not args.no_autotune

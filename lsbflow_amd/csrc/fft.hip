@@ -531,15 +531,129 @@ int horfft_inv(lsb_ctx* c, const HorPlan& hp, const double* const* spec, double*
     return horfft_inv_ops(c, hp, spec, phys, nf, nplanes, nullptr, s);
 }
 
+// ------------------------------------------- mixed form, generic lengths
+// Lengths that are not register-resident (nx/2 or ny with a factor 3 or 5, or outside
+// 16..256) run the mixed form's transforms on the Stockham kernels in FP32.
+
+// u, du/dx, du/dy of 3 FP64 spectra rounded to 9 FP32 spectra; the derivatives are
+// taken in FP64 first (derivatives.f90:261-303: i kx f, i ky f).  One thread per
+// (field, pair); pairs are (plane, ky, kx) with kx fastest.
+__global__ void k_grad_spec32(FieldList<double> in, FieldList<float> out, long npairs, int lh, int ny,
+                              const double* __restrict__ wavx, const double* __restrict__ wavy) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * npairs) return;
+    const int l = (int)(t / npairs);
+    const long pr = t - (long)l * npairs;
+    const int m = (int)(pr % lh);
+    const int j = (int)((pr / lh) % ny);
+    const double2 v = reinterpret_cast<const double2*>(in.p[l])[pr];
+    const double wx = wavx[2 * m], wy = wavy[j];
+    reinterpret_cast<float2*>(out.p[3 * l])[pr] = make_float2((float)v.x, (float)v.y);
+    reinterpret_cast<float2*>(out.p[3 * l + 1])[pr] = make_float2((float)(-wx * v.y), (float)(wx * v.x));
+    reinterpret_cast<float2*>(out.p[3 * l + 2])[pr] = make_float2((float)(-wy * v.y), (float)(wy * v.x));
+}
+
+// FP32 spectra widened to FP64, n values per field
+__global__ void k_widen_f32(FieldList<float> in, FieldList<double> out, long n, int nf) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nf) return;
+    const int f = (int)(t / n);
+    const long i = t - (long)f * n;
+    out.p[f][i] = (double)in.p[f][i];
+}
+
+// inverse y pass then x pass of nf FP32 spectra into padded FP32 physical rows
+static int generic_inv32(const HorPlan& hp, const FieldList<float>& spec, const FieldList<float>& phys, int nf,
+                         int nplanes, hipStream_t s) {
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    {
+        const int cols = lines_for(ny, sizeof(cpx<float>));
+        const size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<float>);
+        const long nblk = (long)((lh + cols - 1) / cols) * nplanes * nf;
+        hipLaunchKernelGGL(k_yinv<float>, dim3((unsigned)nblk), dim3(256), shm, s, spec, phys, nplanes, nx, ny, nxpp,
+                           hp.py, (const cpx<float>*)hp.tw_y32, cols);
+        LSB_HIP(hipGetLastError());
+    }
+    {
+        const int lines = lines_for(lh, sizeof(cpx<float>));
+        const size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<float>);
+        const long nblk = ((long)nplanes * ny * nf + lines - 1) / lines;
+        hipLaunchKernelGGL(k_xinv<float>, dim3((unsigned)nblk), dim3(256), shm, s, phys, nplanes * ny, nf, nx, nxpp,
+                           hp.px, (const cpx<float>*)hp.tw_x32, (const cpx<float>*)hp.tw_r32, lines);
+        LSB_HIP(hipGetLastError());
+    }
+    return LSB_OK;
+}
+
+// forward x pass then y pass of nf FP32 physical fields into FP32 spectra
+static int generic_fwd32(const HorPlan& hp, const FieldList<float>& phys, const FieldList<float>& spec, int nf,
+                         int nplanes, hipStream_t s) {
+    const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
+    {
+        const int lines = lines_for(lh, sizeof(cpx<float>));
+        const size_t shm = 2 * (size_t)lines * lh * sizeof(cpx<float>);
+        const long nblk = ((long)nplanes * ny * nf + lines - 1) / lines;
+        hipLaunchKernelGGL(k_xfwd<float>, dim3((unsigned)nblk), dim3(256), shm, s, phys, spec, nplanes * ny, nf, nx,
+                           nxpp, hp.px, (const cpx<float>*)hp.tw_x32, (const cpx<float>*)hp.tw_r32, lines);
+        LSB_HIP(hipGetLastError());
+    }
+    {
+        const int cols = lines_for(ny, sizeof(cpx<float>));
+        const size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<float>);
+        const long nblk = (long)((lh + cols - 1) / cols) * nplanes * nf;
+        hipLaunchKernelGGL(k_yfwd<float>, dim3((unsigned)nblk), dim3(256), shm, s, spec, nplanes, nx, ny, hp.py,
+                           (const cpx<float>*)hp.tw_y32, cols, (float)(1.0 / ((double)nx * ny)));
+        LSB_HIP(hipGetLastError());
+    }
+    return LSB_OK;
+}
+
+// the gradient transform of the mixed form at generic lengths: 9 FP32 spectra in a
+// staging buffer (k_grad_spec32), then 9 FP32 inverse transforms
+static int inv_grad32_generic(lsb_ctx* c, const HorPlan& hp, const double* const* uif, float* const* u,
+                              float* const* dudx, float* const* dudy, int nplanes, hipStream_t s) {
+    const long per = (long)nplanes * hp.ny * hp.nx;  // floats of one spectrum
+    const long npairs = per / 2;
+    void* scr = nullptr;
+    LSB_TRY(stage_get(c, 8, (size_t)per * 9 * sizeof(float), &scr));
+    FieldList<double> in{};
+    FieldList<float> mid{}, out{};
+    for (int l = 0; l < 3; ++l) {
+        in.p[l] = const_cast<double*>(uif[l]);
+        out.p[3 * l] = u[l];
+        out.p[3 * l + 1] = dudx[l];
+        out.p[3 * l + 2] = dudy[l];
+    }
+    for (int f = 0; f < 9; ++f) mid.p[f] = (float*)scr + per * f;
+    int tok = -1;
+    {
+        const double planes = (double)nplanes;
+        const double bytes = planes * (3.0 * hp.nx * hp.ny * 8 + 9.0 * (hp.nx + 2) * hp.ny * sizeof(float));
+        prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, 9, nplanes), bytes);
+    }
+    hipLaunchKernelGGL(k_grad_spec32, dim3((unsigned)((3 * npairs + 255) / 256)), dim3(256), 0, s, in, mid, npairs,
+                       hp.lh, hp.ny, (const double*)c->d_wavx, (const double*)c->d_wavy);
+    LSB_HIP(hipGetLastError());
+    LSB_TRY(generic_inv32(hp, mid, out, 9, nplanes, s));
+    prof_mark_end(c, tok, s);
+    return LSB_OK;
+}
+
 // u, du/dx, du/dy of 3 velocity spectra in one pass over each spectrum: the inverse
 // y pass produces the three transforms from one load (fast_yinv3), the x pass
 // writes the padded physical rows.  RT is the physical storage type and the
-// arithmetic type (double: FP64 form; float: mixed form).
+// arithmetic type (double: FP64 form; float: mixed form).  At generic lengths only
+// the mixed form is served here (the FP64 form takes spectral derivatives and the
+// plain inverse transform), and the spectral filter is not applied on load.
 template <class RT>
 int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT* const* u, RT* const* dudx,
                     RT* const* dudy, int nplanes, const double* ftab, hipStream_t s) {
     if (!horfft_has_ops(hp)) {
-        set_error("horfft: the fused gradient transform needs power-of-two nx/2 and ny in 16..256");
+        if constexpr (sizeof(RT) == 4) {
+            if (!ftab) return inv_grad32_generic(c, hp, uif, u, dudx, dudy, nplanes, s);
+        }
+        set_error("horfft: the fused FP64 gradient transform and the filter on load need power-of-two nx/2 and ny "
+                  "in 16..256");
         return LSB_E_UNSUPPORTED;
     }
     const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
@@ -578,13 +692,34 @@ template int horfft_inv_grad<float>(lsb_ctx*, const HorPlan&, const double* cons
 template <class TO>
 int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO* const* spec, int nf, int nplanes,
                      hipStream_t s) {
-    if (nf < 1 || nf > kMaxFields || !horfft_has_ops(hp)) {
-        set_error("horfft (mixed): 1..16 fields, power-of-two nx/2 and ny in 16..256");
-        return LSB_E_UNSUPPORTED;
+    if (nf < 1 || nf > kMaxFields) {
+        set_error("horfft (mixed): 1..16 fields per launch");
+        return LSB_E_ARG;
     }
     const int nx = hp.nx, ny = hp.ny, lh = hp.lh, nxpp = nx + 2;
     void* scr = nullptr;
     const size_t per = (size_t)nplanes * ny * nx;
+    if (!horfft_has_ops(hp)) {
+        // generic lengths: Stockham passes in FP32 (into staging first when the spectra are FP64)
+        FieldList<float> in{}, mid{};
+        FieldList<double> wide{};
+        if (sizeof(TO) == 8) LSB_TRY(stage_get(c, 8, per * nf * sizeof(float), &scr));
+        for (int f = 0; f < nf; ++f) {
+            in.p[f] = const_cast<float*>(phys[f]);
+            mid.p[f] = sizeof(TO) == 8 ? (float*)scr + per * f : reinterpret_cast<float*>(spec[f]);
+            wide.p[f] = reinterpret_cast<double*>(spec[f]);
+        }
+        int tok = -1;
+        prof_mark_begin(c, PC_FFT, s, &tok, fft_flops(hp, nf, nplanes), fft_bytes(hp, nf, nplanes, 4, sizeof(TO)));
+        LSB_TRY(generic_fwd32(hp, in, mid, nf, nplanes, s));
+        if (sizeof(TO) == 8) {
+            hipLaunchKernelGGL(k_widen_f32, dim3((unsigned)(((long)per * nf + 255) / 256)), dim3(256), 0, s, mid, wide,
+                               (long)per, nf);
+            LSB_HIP(hipGetLastError());
+        }
+        prof_mark_end(c, tok, s);
+        return LSB_OK;
+    }
     LSB_TRY(stage_get(c, 8, per * nf * sizeof(float), &scr));
     FieldList<float> in{}, mid{};
     FieldList<TO> out{};

@@ -785,13 +785,11 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
                        const double* ftab, hipStream_t s, bool xonly = false) {
     StepLayout& L = S->L;
     // With the register-resident FFT, d/dx and d/dy are applied as the inverse transform
-    // loads uif and d/dz is taken in physical space; otherwise all three are spectral passes
+    // loads uif and d/dz is taken in physical space; otherwise all three are spectral
+    // passes in the FP64 form, while the mixed form's gradient transform takes d/dx and
+    // d/dy itself and d/dz in physical space at every length
     const bool ops_on_load = horfft_has_ops(c->hp);
-    if (!ops_on_load) {
-        if (mixed_form(c)) {
-            set_error("mixed precision: needs power-of-two nx/2 and ny in 16..256");
-            return LSB_E_UNSUPPORTED;
-        }
+    if (!ops_on_load && !mixed_form(c)) {
         for (int l = 0; l < 3; ++l) {
             LSB_TRY(sp_dfdx(c, uif[l], L.spec[S_DX0 + l], s));
             LSB_TRY(sp_dfdy(c, uif[l], L.spec[S_DY0 + l], s));

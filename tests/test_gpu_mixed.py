@@ -69,6 +69,22 @@ def test_mixed_les_models(sgsmodel):
     check(run_pair(case, 20))
 
 
+@pytest.mark.parametrize("nx,ny,sgsmodel", [(96, 48, 0), (192, 32, 2), (64, 48, 2)])
+def test_mixed_generic_lengths(nx, ny, sgsmodel):
+    """Lengths outside the register-resident set (nx/2 = 48 or 96, ny = 48: factors of 3):
+    the mixed form's transforms run as FP32 Stockham passes, d/dx and d/dy taken in FP64
+    before the rounding, and the right-hand side as its own pass over the FP32 N pieces."""
+    case = L.cases.CASES["c1_dns"].with_(name=f"gen{nx}x{ny}", nx=nx, ny=ny, nz=64, xnu=2e-5, bctype=0,
+                                         sgsflag=1, sgsmodel=sgsmodel, dt=5e-4, pert_rms=1e-2)
+    check(run_pair(case, 20))
+
+
+def test_mixed_dynamic_generic_lengths():
+    """The mixed dynamic model with its configured ceiling at nx = 96 (nx/2 = 48)."""
+    case = L.cases.CASES["c2_dyn_les"].with_(name="dyn_gen", nx=96, ny=64, nz=64, pert_rms=1e-2)
+    check(run_pair(case, 20))
+
+
 @pytest.mark.parametrize("bctype", [3])
 def test_mixed_nonhomogeneous_velocity_solves(bctype, monkeypatch):
     """The velocity solves with boundary data (the ceiling profile) in the
