@@ -253,6 +253,27 @@ def roofline(prof, precision, case=None, steps=1):
                 launches=p["n"], device_ms=round(p["ms"], 3))
 
 
+def step_roofline(prof, precision, steps, ms_per_step):
+    """Speed of light of the whole step: each class's algorithmic bytes at the HBM peak or
+    its flops at the matrix peak of its precision, whichever takes longer, summed over the
+    classes, against the measured time per step."""
+    per, sol = {}, 0.0
+    for cls, p in prof.items():
+        if p["n"] == 0:
+            continue
+        t_hbm = p["bytes"] / (PEAK_HBM_GBS * 1e9)
+        t_mfma = 0.0
+        if cls in ("zcontract", "helmholtz"):
+            peak = PEAK_FP32_TFLOPS if (cls == "zcontract" and precision == 1) else PEAK_FP64_TFLOPS
+            t_mfma = p["flops"] / (peak * 1e12)
+        t = max(t_hbm, t_mfma) / steps * 1e3
+        sol += t
+        per[cls] = {"sol_ms": round(t, 4), "device_ms": round(p["ms"] / steps, 4),
+                    "bound": "mfma" if t_mfma > t_hbm else "hbm"}
+    return {"sol_ms_per_step": round(sol, 4), "ms_per_step": round(ms_per_step, 4),
+            "frac": round(sol / ms_per_step, 4) if ms_per_step > 0 else None, "classes": per}
+
+
 def cpu_worker(case_name, sample_ny, sample_steps):
     """One oracle sample (one thread): prints its wall time as JSON.  Runs in a child process
     that never touches the GPU."""
@@ -648,6 +669,7 @@ def main(argv=None):
         "mixed_vs_fp64": accuracy,
         # the dominant class of the untimed all-class pass, measured again in the timed steps
         "roofline": roofline(dom, precision, case, args.steps),
+        "step_roofline": step_roofline(prof, precision, args.steps, 1e3 * secs / args.steps),
         "kernel_classes_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
         "autotune": table,
     }
