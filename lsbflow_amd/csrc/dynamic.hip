@@ -46,26 +46,38 @@ dim3 row_grid(const DDims& d) {
     return dim3(gx, (unsigned)(rows < 65535 ? rows : 65535));
 }
 
+// FP32 field element by byte offset: with a 32-bit offset the loads and stores take the
+// scalar-base + vector-offset form (no 64-bit address arithmetic per field)
+template <class Off>
+__device__ __forceinline__ float ldf(const float* p, Off bo) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + bo);
+}
+template <class Off>
+__device__ __forceinline__ void stf(float* p, Off bo, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + bo) = v;
+}
+
 }  // namespace
 
+template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nx) return;
     const int nrows = d.ny * d.nzp;
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
-        const long o = (long)row * d.nxpp + i;
+        const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
         const double U = a.ubar[k];
         double up[3], u[3], g[3][3];
 #pragma unroll
-        for (int l = 0; l < 3; ++l) up[l] = (double)a.ui[l][o];
+        for (int l = 0; l < 3; ++l) up[l] = (double)ldf(a.ui[l], o);
         u[0] = up[0] + U;
         u[1] = up[1];
         u[2] = up[2];
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) g[l][m] = (double)a.g[l + 3 * m][o];
+            for (int m = 0; m < 3; ++m) g[l][m] = (double)ldf(a.g[l + 3 * m], o);
         // advective term -1/2 u_m du_l/dx_m with the full gradient (g'_02 + ubarp), sponge
         const double g02 = g[0][2] + a.ubarp[k];
         const double lam = a.lam ? a.lam[i] : 0.0;
@@ -75,15 +87,15 @@ __global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * ((l == 0 && m == 2) ? g02 : g[l][m]);
             if (a.lam) s = s - lam * up[l];  // lambda (target - u): target = ubar for u, 0 for v, w
-            a.nlt[l][o] = (float)s;
+            stf(a.nlt[l], o, (float)s);
         }
         // perturbation products: u_l u_m - U_l U_m (only U_0 = ubar is nonzero)
-        a.p[0][o] = (float)(up[0] * (u[0] + U));
-        a.p[1][o] = (float)(up[1] * u[0]);
-        a.p[2][o] = (float)(up[2] * u[0]);
-        a.p[3][o] = (float)(up[1] * up[1]);
-        a.p[4][o] = (float)(up[2] * up[1]);
-        a.p[5][o] = (float)(up[2] * up[2]);
+        stf(a.p[0], o, (float)(up[0] * (u[0] + U)));
+        stf(a.p[1], o, (float)(up[1] * u[0]));
+        stf(a.p[2], o, (float)(up[2] * u[0]));
+        stf(a.p[3], o, (float)(up[1] * up[1]));
+        stf(a.p[4], o, (float)(up[2] * up[1]));
+        stf(a.p[5], o, (float)(up[2] * up[2]));
         // strain of the perturbation and of the full field
         double sp[6], sn = 0.0;
 #pragma unroll
@@ -98,29 +110,30 @@ __global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
         }
         sn = sqrt(sn);
         const double dl = a.delta[k];
-        a.snorm[o] = (float)sn;
+        stf(a.snorm, o, (float)sn);
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            a.sp[q][o] = (float)sp[q];
-            a.l1[q][o] = (float)(dl * dl * sn * sf[q]);
+            stf(a.sp[q], o, (float)sp[q]);
+            stf(a.l1[q], o, (float)(dl * dl * sn * sf[q]));
         }
     }
 }
 
+template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nx) return;
     const int nrows = d.ny * d.nzp;
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
-        const long o = (long)row * d.nxpp + i;
+        const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
         const double Ub = a.base[k];
         const double SUh = a.base[d.nzp + k];
         const double LU = a.base[2 * d.nzp + k];
         double sh[6], snh = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            sh[q] = (double)a.sph[q][o] + (q == 2 ? SUh : 0.0);
+            sh[q] = (double)ldf(a.sph[q], o) + (q == 2 ? SUh : 0.0);
             const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;
             snh = snh + w * 2.0 * sh[q] * sh[q];
         }
@@ -128,42 +141,43 @@ __global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
         const double cd = 2.0 * a.delta[k];
         double uh[3];
 #pragma unroll
-        for (int l = 0; l < 3; ++l) uh[l] = (double)a.uh[l][o];
+        for (int l = 0; l < 3; ++l) uh[l] = (double)ldf(a.uh[l], o);
         const double Uh[3] = {Ub, 0.0, 0.0};
         double num = 0.0, den = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int l = kL[q], m = kM[q];
-            const double mij = 2.0 * ((double)a.l1h[q][o] - cd * cd * snh * sh[q]);
-            double lij = (double)a.ph[q][o] - (Uh[l] * uh[m] + uh[l] * Uh[m] + uh[l] * uh[m]);
+            const double mij = 2.0 * ((double)ldf(a.l1h[q], o) - cd * cd * snh * sh[q]);
+            double lij = (double)ldf(a.ph[q], o) - (Uh[l] * uh[m] + uh[l] * Uh[m] + uh[l] * uh[m]);
             if (q == 0) lij = lij + LU;
             const double w = (l == m) ? 1.0 : 2.0;
             num = num + w * mij * lij;
             den = den + w * mij * mij;
         }
-        a.num[o] = (float)num;
-        a.den[o] = (float)den;
+        stf(a.num, o, (float)num);
+        stf(a.den, o, (float)den);
     }
 }
 
+template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage3(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nx) return;
     const int nrows = d.ny * d.nzp;
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
-        const long o = (long)row * d.nxpp + i;
-        double ct = a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : (double)a.numf[o] / (double)a.denf[o];
+        const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
+        double ct = a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : (double)ldf(a.numf, o) / (double)ldf(a.denf, o);
         if (ct < 0.0 || isnan(ct)) ct = 0.0;
         if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
         const double dl = a.delta[k];
-        const double nu = ct * dl * dl * (double)a.snorm[o];
+        const double nu = ct * dl * dl * (double)ldf(a.snorm, o);
         const double su = 0.5 * a.ubarp[k];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const double s = (double)a.sp[q][o] + (q == 2 ? su : 0.0);
+            const double s = (double)ldf(a.sp[q], o) + (q == 2 ? su : 0.0);
             const double tau = -2.0 * nu * s;
-            a.h[q][o] = (float)(0.5 * (double)a.h[q][o] + tau);  // h holds p on entry
+            stf(a.h[q], o, (float)(0.5 * (double)ldf(a.h[q], o) + tau));  // h holds p on entry
         }
     }
 }
@@ -237,10 +251,27 @@ int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s) {
     // FP32 fields read + written per point by each stage
     const double per = (stage == 1) ? 12.0 + 22.0 : (stage == 2 ? 21.0 + 2.0 : 2.0 + 1.0 + 6.0 + 12.0);
     prof_mark_begin(c, PC_NL, s, &tok, 0.0, per * 4.0 * (double)d.nx * d.ny * d.nzp);
+    // 32-bit byte offsets while a field is below 4 GiB
+    const bool small = (double)d.ny * d.nzp * d.nxpp * sizeof(float) < 4294967296.0;
     switch (stage) {
-        case 1: hipLaunchKernelGGL(k_dynm_stage1, g, dim3(256), 0, s, d, a); break;
-        case 2: hipLaunchKernelGGL(k_dynm_stage2, g, dim3(256), 0, s, d, a); break;
-        default: hipLaunchKernelGGL(k_dynm_stage3, g, dim3(256), 0, s, d, a); break;
+        case 1:
+            if (small)
+                hipLaunchKernelGGL(k_dynm_stage1<unsigned>, g, dim3(256), 0, s, d, a);
+            else
+                hipLaunchKernelGGL(k_dynm_stage1<size_t>, g, dim3(256), 0, s, d, a);
+            break;
+        case 2:
+            if (small)
+                hipLaunchKernelGGL(k_dynm_stage2<unsigned>, g, dim3(256), 0, s, d, a);
+            else
+                hipLaunchKernelGGL(k_dynm_stage2<size_t>, g, dim3(256), 0, s, d, a);
+            break;
+        default:
+            if (small)
+                hipLaunchKernelGGL(k_dynm_stage3<unsigned>, g, dim3(256), 0, s, d, a);
+            else
+                hipLaunchKernelGGL(k_dynm_stage3<size_t>, g, dim3(256), 0, s, d, a);
+            break;
     }
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
