@@ -1637,10 +1637,6 @@ extern "C" int lsb_read_restart(lsb_ctx* c, const char* path, const float* rec) 
         const float* b = r + kRestartHead + (size_t)f * c->nspec;
         for (size_t q = 0; q < c->nspec && !hot; ++q) hot = b[q] != 0.0f;
     }
-    if (hot && c->cfg.precision != LSB_FP64) {
-        set_error("read_restart: nonzero temperature fields need the FP64 form (the temperature path)");
-        return LSB_E_UNSUPPORTED;
-    }
     if (hot) LSB_TRY(temp_enable(c));
     const double hv[5] = {r[0], r[1], r[2], r[3], r[4]};
     LSB_TRY(step_alloc(c));
@@ -2151,20 +2147,12 @@ extern "C" int lsb_state_lintest(lsb_ctx* c, double* enr1, double* enr2, double*
 extern "C" int lsb_set_temperature(lsb_ctx* c, const double* tbar, const double* tbarp, const double* tbarpp,
                                    double coex, double gx, double gy, double gz) {
     LSB_ENTER(c);
-    if (c->cfg.precision != LSB_FP64) {
-        set_error("set_temperature: the temperature path runs in the FP64 form only");
-        return LSB_E_UNSUPPORTED;
-    }
     return temp_set_params(c, tbar, tbarp, tbarpp, coex, gx, gy, gz);
 }
 
 extern "C" int lsb_state_set_temperature(lsb_ctx* c, const double* tempf, const double* tnf, const double* tnmf,
                                          const double* nltnf, const double* nltnmf) {
     LSB_ENTER(c);
-    if (c->cfg.precision != LSB_FP64) {
-        set_error("state_set_temperature: the temperature path runs in the FP64 form only");
-        return LSB_E_UNSUPPORTED;
-    }
     LSB_TRY(temp_enable(c));
     const double* hs[5] = {tempf, tnf, tnmf, nltnf, nltnmf};
     for (int q = 0; q < 5; ++q) {

@@ -910,10 +910,6 @@ static int update_once(lsb_ctx* c) {
     const lsb_config& g = c->cfg;
     const int istep = c->istep;
     const double dt = c->dt;
-    if (temp_on(c) && mixed_form(c)) {
-        set_error("timestep: the temperature path runs in the FP64 form only");
-        return LSB_E_UNSUPPORTED;
-    }
     // velocities used for the nonlinear term (navierstokes.f90:1213-1229)
     // (filterqspec, navierstokes.f90:1206-1209): with the register-resident FFT the filter
     // is applied as the fused inverse transform loads u; otherwise as its own pass

@@ -1,4 +1,5 @@
-// Temperature / buoyancy path of the time step (FP64 form):
+// Temperature / buoyancy path of the time step, in FP64 in both forms (the mixed form runs
+// it from the FP64 velocity spectra beside its FP32 momentum chain):
 //   nlterm_skewsym's temperature part (navierstokes.f90:800-845): grad T of the filtered
 //   temperature, products with the full velocity, -1/2 u.grad T - 1/2 div(u T);
 //   the temperature update (navierstokes.f90:1251-1275): EX3/BDF3 right-hand side with

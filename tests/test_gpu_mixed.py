@@ -20,6 +20,7 @@ import lsbflow_amd as L
 from lsbflow_amd import _lib
 from lsbflow_amd.core import setup_case
 from tests.conftest import rel_l2
+from tests.test_gpu_step import oracle_for
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +122,36 @@ def test_mixed_dynamic_small(dyn_avg):
     case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyn{dyn_avg}", nx=128, ny=64, nz=64, dyn_avg=dyn_avg,
                                              pert_rms=1e-2)
     check(run_pair(case, 20))
+
+
+@pytest.mark.parametrize("dealias", [2, 0])
+def test_mixed_temperature_buoyancy(dealias):
+    """The temperature path in the mixed form: its chain (grad T, the products with the
+    velocity, the temperature solve) runs in FP64 from the FP64 velocity spectra, the
+    buoyancy enters the FP64 right-hand side.  C1 with a tanh temperature profile,
+    perturbation 0.5 u' and buoyancy (0, 0, -1), 10 steps, mixed vs FP64."""
+    case = L.cases.CASES["c1_dns"].with_(dealiasflag=dealias)
+    u, v, w = L.cases.initial_condition(case)
+    z = oracle_for(case).zpts
+    th = np.tanh(z / 0.2)
+    tb, tbp, tbpp = 1.0 + 0.1 * th, 0.5 * (1 - th ** 2), -2 * 0.5 / 0.2 * th * (1 - th ** 2)
+    out = {}
+    for prec in (0, 1):
+        c = setup_case(case, precision=prec)
+        try:
+            c.set_temperature(tb, tbp, tbpp, 1.0, 0.0, 0.0, -1.0)
+            c.state_set(u, v, w)
+            c.state_set_temperature(0.5 * u)
+            c.timestep(10)
+            out[prec] = dict(c.state_get(), tempf=c.state_get_temperature()["tempf"])
+        finally:
+            c.close()
+    errs = {k: rel_l2(out[1][k], out[0][k]) for k in VEL + ("tempf", "pf")}
+    print("temperature, dealiasflag", dealias, "mixed vs FP64:", {k: f"{e:.3e}" for k, e in errs.items()},
+          flush=True)
+    for k in VEL + ("tempf",):
+        assert errs[k] < VEL_BAR, errs
+    assert errs["pf"] < P_BAR, errs
 
 
 def test_mixed_refuses_interscale_model():
