@@ -253,7 +253,10 @@ int lsb_synchronize(lsb_ctx* ctx);
  * chosen by FFTW_MEASURE, modfftw.f90:93-132): times every launch variant of each
  * tunable kernel family on this context's grid and keeps the fastest; scratch is
  * allocated for the duration of the call.  Families (slot: meaning): 0 the 3-point
- * filter on FP64 fields, 1 on FP32 fields; -1 in a slot means the built-in default.
+ * filter on FP64 fields, 1 on FP32 fields (launch variant index); the kx columns per
+ * tile (8, 16, or 32 with FP32 arithmetic) of 2 the forward y pass in FP64, 3 in FP32,
+ * 4 the gradient transform's inverse y pass in FP64, 5 in FP32, 6 the mixed form's
+ * forward y pass fused with the right-hand side; -1 in a slot means the built-in default.
  * lsb_autotune_get reads n slots (variant index, measured ms per launch),
  * lsb_autotune_set installs a saved table. */
 int lsb_autotune(lsb_ctx* ctx);

@@ -446,7 +446,9 @@ class Context:
     def synchronize(self):
         check(self._L.lsb_synchronize(self.h), "synchronize")
 
-    TUNE_FAMILIES = ("filter3_f64", "filter3_f32")
+    # the filter families hold a launch variant, the y-pass families the kx columns per
+    # tile (-1: the built-in default)
+    TUNE_FAMILIES = ("filter3_f64", "filter3_f32", "yfwd_f64", "yfwd_f32", "yinv3_f64", "yinv3_f32", "yrhs_f32")
 
     def autotune(self):
         """Time the launch variants of the tunable kernels on this grid; returns the table."""
@@ -461,8 +463,10 @@ class Context:
         return {f: {"variant": v[i], "ms": round(float(ms[i]), 4)} for i, f in enumerate(self.TUNE_FAMILIES)}
 
     def set_tune_table(self, table):
+        """Set the given families (the others keep their current choice)."""
+        cur = self.tune_table()
         n = len(self.TUNE_FAMILIES)
-        v = (C.c_int * n)(*[int(table[f]["variant"]) for f in self.TUNE_FAMILIES])
+        v = (C.c_int * n)(*[int(table.get(f, cur[f])["variant"]) for f in self.TUNE_FAMILIES])
         check(self._L.lsb_autotune_set(self.h, v, n), "autotune_set")
 
     # ------------------------------------------ snapshots, restart, diagnostics

@@ -1255,11 +1255,13 @@ extern "C" int lsb_autotune(lsb_ctx* c) {
     LSB_HIP(hipSetDevice(c->device));
     constexpr int nf = 8;
     void* work = nullptr;
-    const size_t bytes = 2 * (size_t)nf * c->nphys * sizeof(double);
+    // the filter's 2 nf fields; the y passes' right-hand-side variant needs 21 (of 24)
+    const size_t bytes = 24 * (size_t)c->nphys * sizeof(double);
     LSB_TRY(dev_alloc(&work, bytes));
     int rc = LSB_OK;
     if (hipMemsetAsync(work, 0, bytes, c->stream) != hipSuccess) rc = LSB_E_HIP;
     if (rc == LSB_OK) rc = ph_tune_filt3(c, work, nf, c->stream);
+    if (rc == LSB_OK) rc = fft_tune_y(c, work, c->stream);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(work);
     return rc;

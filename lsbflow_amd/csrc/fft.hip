@@ -17,6 +17,9 @@
 
 #include <cmath>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "lsb_internal.h"
 
 namespace lsb {
@@ -445,7 +448,7 @@ int horfft_fwd(lsb_ctx* c, const HorPlan& hp, const double* const* phys, double*
     const double scale = 1.0 / (double)(nx * ny);
     if (fast_fft_len(ny)) {
         LSB_TRY((fast_yfwd<double, double, double>(out, out, nplanes, nf, lh, ny, (const cpx<double>*)hp.tw_y, scale,
-                                                   s)));
+                                                   s, c->tune[TUNE_YFWD_F64])));
     } else {
         int cols = lines_for(ny, sizeof(cpx<double>));
         size_t shm = 2 * (size_t)cols * ny * sizeof(cpx<double>);
@@ -678,7 +681,8 @@ int horfft_inv_grad(lsb_ctx* c, const HorPlan& hp, const double* const* uif, RT*
     const cpx<RT>* twy = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_y : (void*)hp.tw_y32);
     const cpx<RT>* twx = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_x : (void*)hp.tw_x32);
     const cpx<RT>* twr = (const cpx<RT>*)(sizeof(RT) == 8 ? (void*)hp.tw_r : (void*)hp.tw_r32);
-    LSB_TRY((fast_yinv3<RT, double, RT>(in, mid, nplanes, 3, lh, ny, lh, twy, c->d_wavx, c->d_wavy, ftab, s)));
+    LSB_TRY((fast_yinv3<RT, double, RT>(in, mid, nplanes, 3, lh, ny, lh, twy, c->d_wavx, c->d_wavy, ftab, s,
+                                        c->tune[sizeof(RT) == 8 ? TUNE_YINV3_F64 : TUNE_YINV3_F32])));
     LSB_TRY((fast_xinv<RT, RT, RT>(mid, out, nplanes * ny, 9, lh, nx, nxpp, twx, twr, s)));
     prof_mark_end(c, tok, s);
     return LSB_OK;
@@ -734,7 +738,7 @@ int horfft_fwd_mixed(lsb_ctx* c, const HorPlan& hp, const float* const* phys, TO
     LSB_TRY((fast_xfwd<float, float, float>(in, mid, nrows, nf, lh, nxpp, nx, (const cpx<float>*)hp.tw_x32,
                                             (const cpx<float>*)hp.tw_r32, s)));
     LSB_TRY((fast_yfwd<float, float, TO>(mid, out, nplanes, nf, lh, ny, (const cpx<float>*)hp.tw_y32,
-                                             1.0f / (float)(nx * ny), s)));
+                                             1.0f / (float)(nx * ny), s, c->tune[TUNE_YFWD_F32])));
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }
@@ -742,6 +746,120 @@ template int horfft_fwd_mixed<double>(lsb_ctx*, const HorPlan&, const float* con
                                       hipStream_t);
 template int horfft_fwd_mixed<float>(lsb_ctx*, const HorPlan&, const float* const*, float* const*, int, int,
                                      hipStream_t);
+
+// ------------------------------------------------ y-pass tile autotuning
+// Times the kx-column tiles of the y passes the step uses (forward y pass, the gradient
+// transform's inverse y pass, the mixed form's forward y pass fused with the right-hand
+// side) on this context's grid, FP64 and FP32 arithmetic, and keeps the fastest.  work:
+// zeroed scratch of at least 24 physical fields of doubles.
+int fft_tune_y(lsb_ctx* c, void* work, hipStream_t s) {
+    const HorPlan& hp = c->hp;
+    if (!horfft_has_ops(hp)) return LSB_OK;
+    const int np = c->nzp, lh = hp.lh, ny = hp.ny;
+    const size_t nsp = (size_t)np * ny * hp.nx;  // reals of one spectral field
+    char* w = static_cast<char*>(work);
+    auto f64 = [&](size_t i) { return reinterpret_cast<double*>(w) + i * nsp; };
+    auto f32 = [&](size_t i) { return reinterpret_cast<float*>(w) + i * nsp; };
+    hipEvent_t e0, e1;
+    LSB_HIP(hipEventCreate(&e0));
+    LSB_HIP(hipEventCreate(&e1));
+    const double scale = 1.0 / ((double)hp.nx * ny);
+    const bool verbose = std::getenv("LSB_TUNE_VERBOSE") != nullptr;
+    int rc = LSB_OK;
+    // one warm-up launch, then the mean of three
+    auto pick = [&](int slot, const char* name, const int* cand, int ncand, auto&& launch) {
+        float best = 1e30f;
+        int bc = -1;
+        for (int i = 0; i < ncand && rc == LSB_OK; ++i) {
+            rc = launch(cand[i]);
+            if (rc != LSB_OK) break;
+            if (hipEventRecord(e0, s) != hipSuccess) rc = LSB_E_HIP;
+            for (int r = 0; r < 3 && rc == LSB_OK; ++r) rc = launch(cand[i]);
+            if (rc == LSB_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                rc = LSB_E_HIP;
+            float ms = 0.f;
+            if (rc == LSB_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LSB_E_HIP;
+            if (rc != LSB_OK) break;
+            if (verbose) std::fprintf(stderr, "lsb_autotune %s: %d columns %.4f ms\n", name, cand[i], ms / 3.0f);
+            if (ms < best) {
+                best = ms;
+                bc = cand[i];
+            }
+        }
+        if (rc == LSB_OK && bc > 0) {
+            c->tune[slot] = bc;
+            c->tune_ms[slot] = best / 3.0f;
+        }
+    };
+    const int c64[2] = {8, 16}, c32[3] = {8, 16, 32};
+    {   // forward y pass: 3 fields
+        FieldList<double> in{}, out{};
+        FieldList<float> in32{}, out32{};
+        for (int f = 0; f < 3; ++f) {
+            in.p[f] = f64(f);
+            out.p[f] = f64(3 + f);
+            in32.p[f] = f32(f);
+            out32.p[f] = f32(3 + f);
+        }
+        pick(TUNE_YFWD_F64, "yfwd f64", c64, 2, [&](int cc) {
+            return fast_yfwd<double, double, double>(in, out, np, 3, lh, ny, (const cpx<double>*)hp.tw_y, scale, s,
+                                                     cc);
+        });
+        pick(TUNE_YFWD_F32, "yfwd f32", c32, 3, [&](int cc) {
+            return fast_yfwd<float, float, float>(in32, out32, np, 3, lh, ny, (const cpx<float>*)hp.tw_y32,
+                                                  (float)scale, s, cc);
+        });
+    }
+    {   // gradient transform's inverse y pass: 3 FP64 spectra -> 9 fields
+        FieldList<double> in{}, out{};
+        FieldList<float> out32{};
+        for (int f = 0; f < 3; ++f) in.p[f] = f64(f);
+        for (int f = 0; f < 9; ++f) {
+            out.p[f] = f64(3 + f);
+            out32.p[f] = f32(6 + f);  // after the 3 FP64 inputs (6 FP32 fields of room)
+        }
+        pick(TUNE_YINV3_F64, "yinv3 f64", c64, 2, [&](int cc) {
+            return fast_yinv3<double, double, double>(in, out, np, 3, lh, ny, lh, (const cpx<double>*)hp.tw_y,
+                                                      c->d_wavx, c->d_wavy, nullptr, s, cc);
+        });
+        pick(TUNE_YINV3_F32, "yinv3 f32", c32, 3, [&](int cc) {
+            return fast_yinv3<float, double, float>(in, out32, np, 3, lh, ny, lh, (const cpx<float>*)hp.tw_y32,
+                                                    c->d_wavx, c->d_wavy, nullptr, s, cc);
+        });
+    }
+    {   // mixed form's forward y pass with the right-hand side: 9 FP32 pieces, FP32 N
+        // history, FP64 velocity history and right-hand sides (168 bytes per spectral real)
+        FieldList<float> pc{};
+        YRhsArgs a{};
+        for (int f = 0; f < 9; ++f) pc.p[f] = f32(f);
+        for (int l = 0; l < 3; ++l) {
+            a.nout[l] = f32(9 + l);
+            a.nl[l] = f32(12 + l);
+            a.nlm[l] = f32(15 + l);
+            a.u[l] = f64(9 + l);
+            a.un[l] = f64(12 + l);
+            a.unm[l] = f64(15 + l);
+            a.rhs[l] = f64(18 + l);
+        }
+        a.wavx = c->d_wavx;
+        a.wavy = c->d_wavy;
+        a.ubarpp = c->d_paral + 2 * c->nzp;
+        a.ab0 = 3.0;
+        a.ab1 = -3.0;
+        a.ab2 = 1.0;
+        a.bdf0 = 3.0;
+        a.bdf1 = -1.5;
+        a.bdf2 = 1.0 / 3.0;
+        a.dt = c->dt;
+        a.dtnui = 1.0;
+        pick(TUNE_YRHS_F32, "yfwd+rhs f32", c32, 3, [&](int cc) {
+            return fast_yfwd_rhs<float, float>(pc, np, lh, ny, (const cpx<float>*)hp.tw_y32, (float)scale, a, s, cc);
+        });
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
 
 // x pass only of nf forward transforms: physical rows -> rows of lh complex (the
 // layout of a spectral field before its y pass), for a fused consumer of the y pass.
@@ -789,7 +907,8 @@ int horfft_yfwd_rhs(lsb_ctx* c, const HorPlan& hp, R* const* pieces, int nplanes
                         pairs * per);
     }
     const cpx<R>* twy = (const cpx<R>*)(sizeof(R) == 8 ? (void*)hp.tw_y : (void*)hp.tw_y32);
-    LSB_TRY((fast_yfwd_rhs<R, TH>(pc, nplanes, hp.lh, hp.ny, twy, (R)(1.0 / ((double)hp.nx * hp.ny)), a, s)));
+    LSB_TRY((fast_yfwd_rhs<R, TH>(pc, nplanes, hp.lh, hp.ny, twy, (R)(1.0 / ((double)hp.nx * hp.ny)), a, s,
+                                  sizeof(R) == 4 ? c->tune[TUNE_YRHS_F32] : 0)));
     prof_mark_end(c, tok, s);
     return LSB_OK;
 }

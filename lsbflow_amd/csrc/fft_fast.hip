@@ -613,14 +613,33 @@ template <class R, class TI, class TO, int C>
 int fast_yfwd_c(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
                 const cpx<R>* tw, R scale, hipStream_t s);
 
-// The forward y pass takes 16 kx-columns per tile in both precisions: 256-byte row
+// kx columns per tile of a y pass: 8, 16 or (FP32 arithmetic) 32, from the context's
+// tuning table; anything else gives the built-in default.  Each column is transformed
+// by the same lanes in the same order whatever the tile width, so every choice gives
+// the same bits.
+template <class R>
+static int ycols(int cols, int dflt) {
+    if (cols == 8 || cols == 16 || (cols == 32 && sizeof(R) == 4)) return cols;
+    return dflt;
+}
+#define LSB_YCOLS_DISPATCH(R, CALL, cols, dflt)     \
+    switch (ycols<R>(cols, dflt)) {                 \
+        case 8: return CALL(8);                     \
+        case 32:                                    \
+            if constexpr (sizeof(R) == 4) return CALL(32); \
+            return LSB_E_ARG;                       \
+        default: return CALL(16);                   \
+    }
+
+// The forward y pass defaults to 16 kx-columns per tile in both precisions: 256-byte row
 // segments in FP64 (measured 0.57 -> 0.62 of the HBM rate against 8-column tiles on the
-// 512 x 193 x 256 step, results unchanged), 128-byte ones in FP32 (32 columns measured
-// no faster).
+// 512 x 193 x 256 step, results unchanged), 128-byte ones in FP32.
 template <class R, class TI, class TO>
 int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
-              const cpx<R>* tw, R scale, hipStream_t s) {
-    return fast_yfwd_c<R, TI, TO, 16>(in, out, nplanes, nf, lh, ny, tw, scale, s);
+              const cpx<R>* tw, R scale, hipStream_t s, int cols) {
+#define CALL(C) fast_yfwd_c<R, TI, TO, C>(in, out, nplanes, nf, lh, ny, tw, scale, s)
+    LSB_YCOLS_DISPATCH(R, CALL, cols, 16)
+#undef CALL
 }
 
 template <class R, class TI, class TO, int C>
@@ -642,10 +661,10 @@ int fast_yfwd_c(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, 
     return LSB_OK;
 }
 
-template <class R, class TI, class TO>
-int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
-              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s) {
-    constexpr int C = ytile_cols<R>();
+template <class R, class TI, class TO, int C>
+static int fast_yinv_c(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
+                       int ph, const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy,
+                       hipStream_t s) {
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nf;
     if (nblk <= 0) return LSB_OK;
     switch (ny) {
@@ -663,9 +682,18 @@ int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, in
 }
 
 template <class R, class TI, class TO>
-int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
-               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s) {
-    constexpr int C = ytile_cols<R>();
+int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
+              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s,
+              int cols) {
+#define CALL(C) fast_yinv_c<R, TI, TO, C>(in, out, nplanes, nf, lh, ny, ph, tw, ops, wavx, wavy, s)
+    LSB_YCOLS_DISPATCH(R, CALL, cols, ytile_cols<R>())
+#undef CALL
+}
+
+template <class R, class TI, class TO, int C>
+static int fast_yinv3_c(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny,
+                        int ph, const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab,
+                        hipStream_t s) {
     const long nblk = (long)((lh + C - 1) / C) * nplanes * nsrc;
     if (nblk <= 0) return LSB_OK;
     switch (ny) {
@@ -680,6 +708,15 @@ int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, i
     }
     LSB_HIP(hipGetLastError());
     return LSB_OK;
+}
+
+template <class R, class TI, class TO>
+int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
+               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s,
+               int cols) {
+#define CALL(C) fast_yinv3_c<R, TI, TO, C>(in, out, nplanes, nsrc, lh, ny, ph, tw, wavx, wavy, ftab, s)
+    LSB_YCOLS_DISPATCH(R, CALL, cols, ytile_cols<R>())
+#undef CALL
 }
 
 // Forward y pass fused with the velocity right-hand side (one block: C kx-columns
@@ -805,10 +842,9 @@ __global__ void __launch_bounds__(C * N2) k_yfwd_rhs(FieldList<R> pc, int nplane
     }
 }
 
-template <class R, class TH>
-int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
-                  const YRhsArgs& a, hipStream_t s) {
-    constexpr int C = ytile_cols<R>();
+template <class R, class TH, int C>
+static int fast_yfwd_rhs_c(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
+                           const YRhsArgs& a, hipStream_t s) {
     const long nblk = (long)((lh + C - 1) / C) * nplanes * 3;
     if (nblk <= 0) return LSB_OK;
     switch (ny) {
@@ -824,10 +860,17 @@ int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const
     LSB_HIP(hipGetLastError());
     return LSB_OK;
 }
+template <class R, class TH>
+int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
+                  const YRhsArgs& a, hipStream_t s, int cols) {
+#define CALL(C) fast_yfwd_rhs_c<R, TH, C>(pieces, nplanes, lh, ny, tw, scale, a, s)
+    LSB_YCOLS_DISPATCH(R, CALL, cols, ytile_cols<R>())
+#undef CALL
+}
 template int fast_yfwd_rhs<double, double>(const FieldList<double>&, int, int, int, const cpx<double>*, double,
-                                           const YRhsArgs&, hipStream_t);
+                                           const YRhsArgs&, hipStream_t, int);
 template int fast_yfwd_rhs<float, float>(const FieldList<float>&, int, int, int, const cpx<float>*, float,
-                                         const YRhsArgs&, hipStream_t);
+                                         const YRhsArgs&, hipStream_t, int);
 
 #define LSB_INST_FAST(R, TI, TO)                                                                                  \
     template int fast_xfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
@@ -835,11 +878,13 @@ template int fast_yfwd_rhs<float, float>(const FieldList<float>&, int, int, int,
     template int fast_xinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
                                       const cpx<R>*, const cpx<R>*, hipStream_t);                                 \
     template int fast_yfwd<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int,             \
-                                      const cpx<R>*, R, hipStream_t);                                             \
+                                      const cpx<R>*, R, hipStream_t, int);                                        \
     template int fast_yinv<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,        \
-                                      const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t); \
+                                      const cpx<R>*, const FftOps&, const double*, const double*, hipStream_t,  \
+                                      int);                                                                       \
     template int fast_yinv3<R, TI, TO>(const FieldList<TI>&, const FieldList<TO>&, int, int, int, int, int,       \
-                                       const cpx<R>*, const double*, const double*, const double*, hipStream_t);
+                                       const cpx<R>*, const double*, const double*, const double*, hipStream_t,  \
+                                       int);
 
 LSB_INST_FAST(double, double, double)
 LSB_INST_FAST(float, float, float)    // mixed: x passes (physical <-> compact staging)

@@ -93,7 +93,17 @@ struct ZSplitOp {
 
 constexpr int kMaxFields = 16;
 // launch-shape tuning table of a context (lsb_autotune): the variant index per kernel family
-enum { TUNE_FILT3_F64 = 0, TUNE_FILT3_F32 = 1, TUNE_SLOTS = 16 };
+// (the y-pass families hold the kx columns per tile: 8, 16 or, FP32 only, 32)
+enum {
+    TUNE_FILT3_F64 = 0,
+    TUNE_FILT3_F32 = 1,
+    TUNE_YFWD_F64 = 2,
+    TUNE_YFWD_F32 = 3,
+    TUNE_YINV3_F64 = 4,
+    TUNE_YINV3_F32 = 5,
+    TUNE_YRHS_F32 = 6,
+    TUNE_SLOTS = 16
+};
 // leading dimension of the two-row boundary operator d_md2b (rows 0 and nzp-1 of d2fdz2)
 constexpr int kLdmBnd = 64;
 // doubles in the reduction / small-table scratch d_red
@@ -366,15 +376,17 @@ int fast_xinv(const FieldList<TI>& in, const FieldList<TO>& out, int nrows, int 
               const cpx<R>* tw, const cpx<R>* twr, hipStream_t s);
 template <class R, class TI, class TO>
 int fast_yfwd(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny,
-              const cpx<R>* tw, R scale, hipStream_t s);
+              const cpx<R>* tw, R scale, hipStream_t s, int cols = 0);
 template <class R, class TI, class TO>
 int fast_yinv(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nf, int lh, int ny, int ph,
-              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s);
+              const cpx<R>* tw, const FftOps& ops, const double* wavx, const double* wavy, hipStream_t s,
+              int cols = 0);
 // nsrc spectra -> 3 nsrc outputs (u, du/dx, du/dy per source), one load per source
 // ftab (lh*ny factors, index m + lh*ky) applies the spectral filter on load; null: none
 template <class R, class TI, class TO>
 int fast_yinv3(const FieldList<TI>& in, const FieldList<TO>& out, int nplanes, int nsrc, int lh, int ny, int ph,
-               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s);
+               const cpx<R>* tw, const double* wavx, const double* wavy, const double* ftab, hipStream_t s,
+               int cols = 0);
 
 // Forward y pass of the three pieces of N_l (nlt_l, h_(l,x), h_(l,y), as x-transformed
 // rows) fused with the EX3/BDF3 velocity right-hand side: N_l and rhs_l are written,
@@ -394,7 +406,10 @@ struct YRhsArgs {
 };
 template <class R, class TH>
 int fast_yfwd_rhs(const FieldList<R>& pieces, int nplanes, int lh, int ny, const cpx<R>* tw, R scale,
-                  const YRhsArgs& a, hipStream_t s);
+                  const YRhsArgs& a, hipStream_t s, int cols = 0);
+// times the kx-column tiles of the y passes on this context's grid (scratch `work` of
+// at least 24 physical fields of doubles) and records the fastest in the tuning table
+int fft_tune_y(lsb_ctx* c, void* work, hipStream_t s);
 
 // z-contractions (zops.hip)
 // out_f = M * in_f along z for nf fields; M column-major nout x nin (ld = ldm).
