@@ -402,8 +402,9 @@ def diagnostics(case, reps):
 
 def c3_helmholtz(reps):
     """Batched Helmholtz solve alone: 256 x 256 pencils x 4 fields on 193 points (the
-    512 x 193 x 256 spectral grid) per call, FP64 / FP32 / FP32 + one FP64 refinement, for
-    sigma = c / (nu dt), c in {1, 1.5}, and sigma = 0.  Device-resident arrays; wall time
+    512 x 193 x 256 spectral grid) per call, FP64 / FP32 / FP32 + one FP64 refinement / the
+    FP64 leading term with the FP32 correction (the mixed form's velocity solve; the FP64
+    solve at sigma = 0), for sigma = c / (nu dt), c in {1, 1.5}, and sigma = 0.  Device-resident arrays; wall time
     around `reps` calls with the stream synchronised on both sides."""
     import ctypes as C
     import numpy as np
@@ -432,7 +433,7 @@ def c3_helmholtz(reps):
         for cval in (1.0, 1.5, 0.0):
             sigma = cval / (case.xnu * case.dt)
             row = {}
-            for mode, name in ((0, "fp64"), (1, "fp32"), (2, "fp32_refined")):
+            for mode, name in ((0, "fp64"), (1, "fp32"), (2, "fp32_refined"), (3, "fp64_lead_fp32_corr")):
                 assert hip.hipMemcpy(d, rhs.ctypes.data_as(C.c_void_p), rhs.nbytes, 1) == 0
                 lib.lsb_solve_helmholtz_mixed_dev(ctx.h, d, sigma, nf, mode, None)     # warm-up
                 ctx.synchronize()

@@ -407,13 +407,17 @@ int lsb_dfdz_dev(lsb_ctx* ctx, const double* uf, double* dudzf, int nfields, voi
 int lsb_solve_helmholtz_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields, void* stream);
 
 /* helmholtz.f90:130-208 (homogeneous Dirichlet rows) on nfields device spectral fields in
- * one of three precisions: LSB_HELM_FP64 the FP64 eigen-projections (as
+ * one of four precisions: LSB_HELM_FP64 the FP64 eigen-projections (as
  * lsb_solve_helmholtz_dev); LSB_HELM_FP32 both projections on the FP32 matrix cores;
  * LSB_HELM_REFINE the FP32 solve followed by one FP64 refinement (residual of
- * (D2 - k^2 - sigma) u = rhs in FP64 with D2 = d.d, FP32 solve of it, u += du). */
+ * (D2 - k^2 - sigma) u = rhs in FP64 with D2 = d.d, FP32 solve of it, u += du);
+ * LSB_HELM_SPLIT32 the mixed form's velocity solve: the FP64 leading term -rhs/(k^2 + sigma)
+ * plus the FP32 eigen-projection correction, when sigma >= 16 max|lambda| (the FP64
+ * projections otherwise). */
 #define LSB_HELM_FP64 0
 #define LSB_HELM_FP32 1
 #define LSB_HELM_REFINE 2
+#define LSB_HELM_SPLIT32 3
 int lsb_solve_helmholtz_mixed_dev(lsb_ctx* ctx, double* rhs, double dtnui, int nfields, int mode, void* stream);
 
 /* lsb_horfft_dev without device allocation.  No counterpart in the reference: it is

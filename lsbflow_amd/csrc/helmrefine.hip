@@ -1,4 +1,4 @@
-// Batched Helmholtz solves in three precisions (helmholtz.f90:130-208, homogeneous
+// Batched Helmholtz solves in four precisions (helmholtz.f90:130-208, homogeneous
 // Dirichlet rows): every (kx, ky) pencil of nf spectral fields, interior rows 2..nz,
 //   (D2 - k^2 - sigma) u = rhs,   u = E diag(1/(lambda - k^2 - sigma)) E^-1 rhs,
 // u(1) = u(nzp) = 0, written over rhs.
@@ -7,7 +7,10 @@
 //                   the mode values in FP32), the result widened to FP64;
 //   LSB_HELM_REFINE the FP32 solve, then one FP64 refinement: the residual
 //                   r = rhs - (D2 u - (k^2 + sigma) u) in FP64 with the D2 = d.d the
-//                   eigen-decomposition was made of, a second FP32 solve of r, u += du.
+//                   eigen-decomposition was made of, a second FP32 solve of r, u += du;
+//   LSB_HELM_SPLIT32 the mixed form's velocity solve (zops.hip k_zhelm32): the FP64
+//                   leading term -rhs/(k^2 + sigma) and the FP32 projections of the
+//                   correction, used when sigma >= 16 max|lambda| (else the FP64 solve).
 // The mode values, the FP64 operator and the residual live in the context's staging
 // slots 20-22 (no other entry point uses them).
 
@@ -91,17 +94,18 @@ static int fp32_solve(lsb_ctx* c, const double* in, double* out, float* t, doubl
 }
 
 int helmholtz_precision(lsb_ctx* c, double* rhs, int nf, double sigma, int mode, hipStream_t s) {
-    if (mode == LSB_HELM_FP64) {
+    if (mode == LSB_HELM_FP64 || mode == LSB_HELM_SPLIT32) {
         for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
             const int nb = std::min(kMaxFields, nf - f0);
             double* r[kMaxFields];
             for (int q = 0; q < nb; ++q) r[q] = rhs + (size_t)(f0 + q) * c->nspec;
-            LSB_TRY(helmholtz(c, r, nullptr, nullptr, nb, sigma, s));
+            LSB_TRY(helmholtz(c, r, nullptr, nullptr, nb, sigma, s, mode == LSB_HELM_SPLIT32));
         }
         return LSB_OK;
     }
     if (mode != LSB_HELM_FP32 && mode != LSB_HELM_REFINE) {
-        set_error("solve_helmholtz_mixed: mode must be LSB_HELM_FP64, LSB_HELM_FP32 or LSB_HELM_REFINE");
+        set_error("solve_helmholtz_mixed: mode must be LSB_HELM_FP64, LSB_HELM_FP32, LSB_HELM_REFINE or "
+                  "LSB_HELM_SPLIT32");
         return LSB_E_ARG;
     }
     const int nzm = c->nzm, nzp = c->nzp;

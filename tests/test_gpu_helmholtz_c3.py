@@ -9,7 +9,10 @@ sigma = 0 (the pressure Poisson problem).  Checks:
     own eigen-decomposition of d2(2:nz, 2:nz) on 64 pencils; the residual of
     (D2 - k^2 - sigma) u = rhs on the interior rows of every pencil;
   * FP32 projections, and FP32 with one FP64 refinement: their error against the FP64
-    solution, stated, the refinement required to gain at least four digits.
+    solution, stated, the refinement required to gain at least four digits;
+  * the mixed form's velocity-solve form (FP64 leading term -rhs/(k^2 + sigma), FP32
+    correction; LSB_HELM_SPLIT32) against the FP64 solution at sigma = c/(nu dt), where
+    sigma is over 16 x max|lambda|; at sigma = 0 it is the FP64 solve itself.
 """
 
 import ctypes as C
@@ -98,6 +101,7 @@ def test_c3_batched_helmholtz(c3, c):
     u64 = solve_dev(ctx, hip, rhs, sigma, 0)
     u32 = solve_dev(ctx, hip, rhs, sigma, 1)
     ur = solve_dev(ctx, hip, rhs, sigma, 2)
+    us = solve_dev(ctx, hip, rhs, sigma, 3)
     k2 = (g["wavx"][None, :] ** 2 + g["wavy"][:, None] ** 2)          # (ny, nx) per real column
     # FP64 against numpy's eigen-decomposition on 64 pencils
     D2 = m["d2"]
@@ -123,8 +127,15 @@ def test_c3_batched_helmholtz(c3, c):
         res_w = max(res_w, np.abs(r).max() / scale)
     e32 = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
     eref = np.linalg.norm(ur - u64) / np.linalg.norm(u64)
+    espl = np.linalg.norm(us - u64) / np.linalg.norm(u64)
     print(f"c3 sigma={sigma:.3g}: FP64 vs numpy eigen-solution {worst:.2e}, FP64 residual {res_w:.2e}, "
-          f"FP32 {e32:.2e}, FP32 + 1 FP64 refinement {eref:.2e} (rel L2 vs FP64)", flush=True)
+          f"FP32 {e32:.2e}, FP32 + 1 FP64 refinement {eref:.2e}, FP64 leading + FP32 correction {espl:.2e} "
+          f"(rel L2 vs FP64)", flush=True)
+    if sigma == 0.0:
+        assert np.array_equal(us, u64)          # below the 16 x max|lambda| margin: the FP64 solve
+    else:
+        assert espl < 1e-8 and espl < 1e-2 * e32
+        assert np.all(us[:, 0] == 0.0) and np.all(us[:, nzp - 1] == 0.0)
     assert worst < 1e-11
     assert res_w < 1e-13
     assert np.all(u64[:, 0] == 0.0) and np.all(u64[:, nzp - 1] == 0.0)
