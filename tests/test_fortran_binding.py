@@ -98,5 +98,7 @@ def test_drop_in_modules_compile_and_link(tmp_path):
     from tests.test_gpu_fortran_shims import build_shim_program
     exe = build_shim_program(tmp_path)
     assert os.path.exists(exe)
+    exe2 = build_shim_program(tmp_path, "shim_check2")
+    assert os.path.exists(exe2)
     for mod in ("modhorfft", "derivatives", "helmholtz", "sgs_models", "filters", "dealias"):
         assert os.path.exists(tmp_path / f"{mod}.mod"), mod

@@ -1,9 +1,11 @@
-"""A Fortran caller on the device through the drop-in modules (lsbflow_amd/fortran/
-lsb_shims.f90: modhorfft, derivatives, helmholtz, ... with the reference's names and
-arguments): tests/fortran/shim_check.f90 creates a context, then calls horfft, dfdz and
-solve_helmholtz exactly as the reference's driver does (the Helmholtz operators taken
-from the context as the driver's mats module holds them) and the inverse horfft.  Each
-result against the oracle on the same inputs."""
+"""Fortran callers on the device through the drop-in modules (lsbflow_amd/fortran/
+lsb_shims.f90: modhorfft, derivatives, helmholtz, sgs_models, filters, dealias with the
+reference's names and arguments): tests/fortran/shim_check.f90 creates a context, then
+calls horfft, dfdz and solve_helmholtz exactly as the reference's driver does (the
+Helmholtz operators taken from the context as the driver's mats module holds them) and
+the inverse horfft; tests/fortran/shim_check2.f90 calls dfdx, dfdy, divuv,
+solve_nhom_helmholtz, sigma_model, filter_q_3pt, setup_dealiasing, dealiasing_2d_fftw and
+norm.  Each result against the oracle on the same inputs."""
 
 import os
 import shutil
@@ -22,7 +24,7 @@ FC = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/o
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(FC is None, reason="amdflang not available")]
 
 
-def build_shim_program(tmp_path):
+def build_shim_program(tmp_path, program="shim_check"):
     """Compile the interface module, the drop-in modules and the caller; link the library."""
     libdir = os.path.dirname(_lib.LIB_PATH)
     fdir = os.path.join(ROOT, "lsbflow_amd", "fortran")
@@ -32,8 +34,8 @@ def build_shim_program(tmp_path):
         subprocess.run([FC, "-c", os.path.join(fdir, f"{name}.f90"), "-o", obj, "-I", str(tmp_path),
                         "-module-dir", str(tmp_path)], check=True, cwd=tmp_path)
         objs.append(obj)
-    exe = str(tmp_path / "shim_check")
-    subprocess.run([FC, os.path.join(ROOT, "tests", "fortran", "shim_check.f90"), *objs, "-I", str(tmp_path),
+    exe = str(tmp_path / program)
+    subprocess.run([FC, os.path.join(ROOT, "tests", "fortran", program + ".f90"), *objs, "-I", str(tmp_path),
                     "-o", exe, "-L", libdir, "-llsbflow_amd", f"-Wl,-rpath,{libdir}"], check=True, cwd=tmp_path)
     return exe
 
@@ -73,3 +75,63 @@ def test_fortran_driver_calls_through_the_drop_in_modules(tmp_path):
     for k, e in errs.items():
         assert e < 1e-12, (k, e)
     assert np.all(back[..., nx:] == 0.0)
+
+
+def test_fortran_driver_sgs_filters_dealias_through_the_drop_in_modules(tmp_path):
+    exe = build_shim_program(tmp_path, "shim_check2")
+    nx, ny, nz = 32, 16, 40
+    o = Oracle(OracleConfig(nx=nx, ny=ny, nz=nz, mapping=3, zlen=2.0, xlen=2 * np.pi, ylen=np.pi, dealiasflag=1,
+                            dealias_ref_compat=1, sgsflag=1, sgsmodel=2, xnu=1e-3, dt=1e-3, a=1.0, b=2.0))
+    nzp, nxpp = o.nzp, o.nxpp
+    rng = np.random.default_rng(37)
+    spec = [rng.standard_normal((nzp, ny, nx)) for _ in range(4)]
+    uf, vf, rhs, bc = spec
+    g = rng.standard_normal((3, 3, nzp, ny, nxpp))                     # g[l][m] = du_l/dx_m
+    g[..., nx:] = 0.0
+    pa, pb = (np.zeros((nzp, ny, nxpp)) for _ in range(2))
+    pa[..., :nx] = rng.standard_normal((nzp, ny, nx))
+    pb[..., :nx] = rng.standard_normal((nzp, ny, nx))
+    sigma = 250.0
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        np.array([nx, ny, nz], dtype=np.int32).tofile(fh)
+        np.array([sigma], dtype=np.float64).tofile(fh)
+        for a in (uf, vf, rhs, bc):
+            a.tofile(fh)
+        np.ascontiguousarray(np.swapaxes(g, 0, 1)).tofile(fh)          # duidxj(:, :, :, l, m)
+        pa.tofile(fh)
+        pb.tofile(fh)
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    nlg = np.frombuffer(raw[:8], dtype=np.int32)
+    got = np.frombuffer(raw[8:], dtype=np.float64)
+    ns, npf = nzp * ny * nx, nzp * ny * nxpp
+    sizes = [ns, ns, ns, ns, 9 * npf, npf, npf, ns, ns, ns, ns]
+    parts, o0 = [], 0
+    for n in sizes:
+        parts.append(got[o0:o0 + n])
+        o0 += n
+    assert o0 == got.size
+    dx, dy, dv, sol = (parts[i].reshape(nzp, ny, nx) for i in range(4))
+    tau = np.swapaxes(parts[4].reshape(3, 3, nzp, ny, nxpp), 0, 1)
+    pf, pab = parts[5].reshape(nzp, ny, nxpp), parts[6].reshape(nzp, ny, nxpp)
+    normed = [parts[7 + i].reshape(nzp, ny, nx) for i in range(4)]
+    assert tuple(int(v) for v in nlg) == tuple(o.dealias_sizes())
+    assert np.array_equal(dx, o.dfdx(uf)) and np.array_equal(dy, o.dfdy(uf))
+    refn = [uf.copy(), vf.copy(), o.dfdx(uf), o.dfdy(uf)]
+    o.norm(*refn)
+    for a, b in zip(normed, refn):
+        assert np.array_equal(a, b)
+    errs = {"divuv": rel_l2(dv, o.divuv(uf, vf)),
+            "solve_nhom_helmholtz": rel_l2(sol, o.solve_nhom_helmholtz(rhs, bc, sigma)),
+            "sigma_model": rel_l2(tau[..., :nx], o.sigma_model(g)[..., :nx]),
+            "filter_q_3pt": rel_l2(pf[..., :nx], o.filter_q_3pt(pa, 1.0, 2.0, 1)[..., :nx]),
+            "dealiasing_2d_fftw": rel_l2(pab[..., :nx], o.dealiasing_2d_fftw(pa, pb, True)[..., :nx])}
+    print("Fortran caller (SGS, filters, dealiasing) through the drop-in modules vs oracle:",
+          {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bars = {"divuv": 1e-15, "solve_nhom_helmholtz": 1e-11, "sigma_model": 1e-11, "filter_q_3pt": 1e-14,
+            "dealiasing_2d_fftw": 1e-12}
+    for k, e in errs.items():
+        assert e < bars[k], (k, e)
