@@ -100,5 +100,6 @@ def test_drop_in_modules_compile_and_link(tmp_path):
     assert os.path.exists(exe)
     exe2 = build_shim_program(tmp_path, "shim_check2")
     assert os.path.exists(exe2)
+    assert os.path.exists(build_shim_program(tmp_path, "step_check"))
     for mod in ("modhorfft", "derivatives", "helmholtz", "sgs_models", "filters", "dealias"):
         assert os.path.exists(tmp_path / f"{mod}.mod"), mod

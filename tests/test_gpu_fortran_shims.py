@@ -5,14 +5,22 @@ calls horfft, dfdz and solve_helmholtz exactly as the reference's driver does (t
 Helmholtz operators taken from the context as the driver's mats module holds them) and
 the inverse horfft; tests/fortran/shim_check2.f90 calls dfdx, dfdy, divuv,
 solve_nhom_helmholtz, sigma_model, filter_q_3pt, setup_dealiasing, dealiasing_2d_fftw and
-norm.  Each result against the oracle on the same inputs."""
+norm.  Each result against the oracle on the same inputs.  tests/fortran/step_check.f90
+drives the device-resident step (tier B) from Fortran: its state after the steps equals,
+bit for bit, the same steps driven through the Python binding (which the step tests hold
+to the oracle)."""
 
 import os
 import shutil
 import subprocess
 
+import ctypes as C
+
 import numpy as np
 import pytest
+
+import lsbflow_amd as L
+from lsbflow_amd.core import Config, setup_case
 
 from oracle import Oracle, OracleConfig
 from lsbflow_amd import _lib
@@ -135,3 +143,45 @@ def test_fortran_driver_sgs_filters_dealias_through_the_drop_in_modules(tmp_path
             "dealiasing_2d_fftw": 1e-12}
     for k, e in errs.items():
         assert e < bars[k], (k, e)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fortran_driver_device_resident_steps(tmp_path, precision):
+    exe = build_shim_program(tmp_path, "step_check")
+    case = L.cases.CASES["c1_dns"].with_(nx=32, ny=16, nz=40, xnu=2e-4, samp=0.3, sgsflag=1, sgsmodel=2)
+    nsteps = 6
+    cfg = Config.from_case(case, precision=precision).to_c()
+    ub, ubp, ubpp = L.cases.base_flow(case)
+    u, v, w = L.cases.initial_condition(case)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        fh.write(bytes(cfg))
+        np.array([nsteps], dtype=np.int32).tofile(fh)
+        for a in (ub, ubp, ubpp, u, v, w):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    istep = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    t = float(np.frombuffer(raw[4:12], dtype=np.float64)[0])
+    got = np.frombuffer(raw[12:], dtype=np.float64)
+    c = setup_case(case, precision=precision)
+    try:
+        c.state_set(u, v, w)
+        c.timestep(nsteps)
+        ref = c.state_get()
+    finally:
+        c.close()
+    n = u.size
+    keys = [("uf", 1), ("vf", 1), ("wf", 1), ("pf", 1), ("pnf", 1), ("unif", 3), ("unmif", 3), ("nlnif", 3),
+            ("nlnmif", 3)]
+    o0 = 0
+    for k, m in keys:
+        a = got[o0:o0 + m * n].reshape(ref[k].shape)
+        o0 += m * n
+        assert np.array_equal(a, ref[k]), k
+    assert o0 == got.size
+    assert istep == nsteps and abs(t - nsteps * case.dt) < 1e-12
+    print(f"Fortran tier-B driver, precision {precision}: {nsteps} steps bit-identical to the Python-driven run",
+          flush=True)
