@@ -393,10 +393,9 @@ int sp_rhs_vel(lsb_ctx* c, const StepRhsArgs& a, hipStream_t s) {
     else if (!a.h32)
         hipLaunchKernelGGL((k_rhs_vel<double, double>), sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx,
                            c->d_wavy, a);
-    else {
-        set_error("rhs_vel: FP32 N history needs FP32 N pieces");
-        return LSB_E_ARG;
-    }
+    else  // FP64 pieces into the FP32 history (the mixed form with an FP64 nonlinear chain)
+        hipLaunchKernelGGL((k_rhs_vel<double, float>), sp_grid(d.npairs * 3), dim3(256), 0, s, d, c->d_wavx,
+                           c->d_wavy, a);
     prof_mark_end(c, tok, s);
     LSB_HIP(hipGetLastError());
     return LSB_OK;

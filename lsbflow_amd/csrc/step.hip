@@ -620,6 +620,11 @@ static int nlterm_phys(lsb_ctx* c, StepState* S, const double* const* uif, doubl
 }
 
 static bool mixed_form(const lsb_ctx* c) { return c->cfg.precision == LSB_MIXED; }
+// the FP32 nonlinear chain of the mixed form: every SGS model but the interscale one,
+// whose chain (products of filtered velocities) runs in FP64 inside the mixed form
+static bool mixed_chain(const lsb_ctx* c) {
+    return mixed_form(c) && !(c->cfg.sgsflag == 1 && c->cfg.sgsmodel == 3);
+}
 
 // Mixed form, dynamic model (sgsterm.f90:123-280): products, strain and L1 (stage 1), the
 // (a, b) test filter of S', L1, u', p in FP32, M:L and M:M (stage 2), their local (1, 1)
@@ -700,7 +705,7 @@ static int nlterm_phys_mixed(lsb_ctx* c, StepState* S, const double* const* uif,
     const bool sgs = c->cfg.sgsflag == 1 && c->cfg.sgsmodel != 4;
     const int model = c->cfg.sgsmodel;
     if (sgs && model != 0 && model != 1 && model != 2) {
-        set_error("mixed precision: the interscale model runs in the FP64 form only");
+        set_error("mixed precision: the interscale model takes the FP64 nonlinear chain");
         return LSB_E_UNSUPPORTED;
     }
     auto ph = [&](int slot) { return reinterpret_cast<float*>(L.phys[slot]); };
@@ -789,7 +794,7 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
     // passes in the FP64 form, while the mixed form's gradient transform takes d/dx and
     // d/dy itself and d/dz in physical space at every length
     const bool ops_on_load = horfft_has_ops(c->hp);
-    if (!ops_on_load && !mixed_form(c)) {
+    if (!ops_on_load && !mixed_chain(c)) {
         for (int l = 0; l < 3; ++l) {
             LSB_TRY(sp_dfdx(c, uif[l], L.spec[S_DX0 + l], s));
             LSB_TRY(sp_dfdy(c, uif[l], L.spec[S_DY0 + l], s));
@@ -801,7 +806,7 @@ static int nlterm_core(lsb_ctx* c, StepState* S, const double* const* uif, doubl
         set_error("nlterm: filter-on-load needs the register-resident FFT sizes");
         return LSB_E_ARG;
     }
-    if (mixed_form(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, ftab, xonly, s);
+    if (mixed_chain(c)) return nlterm_phys_mixed(c, S, uif, nltf_out, hf, ftab, xonly, s);
     if (xonly) {
         set_error("nlterm: the fused forward pass is built for the mixed form");
         return LSB_E_ARG;
@@ -845,7 +850,7 @@ int nlterm_dev(lsb_ctx* c, const double* const* uif, double* const* nlt, hipStre
         r.unm[l] = uif[l];
         r.rhs[l] = S->L.spec[S_DX0 + l];
     }
-    r.n32 = mixed_form(c) ? 1 : 0;
+    r.n32 = mixed_chain(c) ? 1 : 0;
     r.ubarpp = c->d_paral + 2 * c->nzp;
     r.ab0 = 1.0;
     r.dtnui = 0.0;
@@ -966,7 +971,7 @@ static int update_once(lsb_ctx* c) {
         r.unm[l] = L.spec[S->unm[l]];
         r.rhs[l] = L.spec[S->rhs[l]];
     }
-    r.n32 = mixed_form(c) ? 1 : 0;
+    r.n32 = mixed_chain(c) ? 1 : 0;
     r.h32 = mixed_form(c) ? 1 : 0;
     r.ubarpp = c->d_paral + 2 * c->nzp;
     r.ab0 = c->abfac[0];

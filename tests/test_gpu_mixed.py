@@ -3,7 +3,8 @@
 The mixed form keeps the state, the histories, the right-hand sides and the
 Helmholtz/Poisson solves in FP64 and runs the nonlinear term's chain (inverse
 transforms, products and SGS model, forward transforms, its two Chebyshev
-contractions) in FP32; the dynamic model's work fields are FP32 as well.
+contractions) in FP32; the dynamic model's work fields are FP32 as well, and the
+interscale model's chain stays FP64 (its N history and solve correction FP32).
 Acceptance: relative L2 <= 1e-5 per field (u, v, w, p and the wall skin friction)
 against the FP64 form from the same initial condition after 100 steps of the
 512 x 193 x 256 sigma and dynamic-Smagorinsky LES workloads with their configured
@@ -17,7 +18,6 @@ import numpy as np
 import pytest
 
 import lsbflow_amd as L
-from lsbflow_amd import _lib
 from lsbflow_amd.core import setup_case
 from tests.conftest import rel_l2
 from tests.test_gpu_step import oracle_for
@@ -154,16 +154,15 @@ def test_mixed_temperature_buoyancy(dealias):
     assert errs["pf"] < P_BAR, errs
 
 
-def test_mixed_refuses_interscale_model():
-    case = L.cases.CASES["c1_dns"].with_(sgsflag=1, sgsmodel=3)
-    u, v, w = L.cases.initial_condition(case)
-    c = setup_case(case, precision=1)
-    try:
-        c.state_set(u, v, w)
-        with pytest.raises(_lib.LsbError, match="FP64 form only"):
-            c.timestep(1)
-    finally:
-        c.close()
+@pytest.mark.parametrize("nx,ny", [(128, 64), (96, 48)])
+def test_mixed_interscale_model(nx, ny):
+    """The interscale model in the mixed form: its nonlinear chain (products of the
+    filtered velocities, the non-symmetric tau) runs in FP64, the N history is kept in
+    FP32 and the velocity solves take the FP32 correction, on a register-resident grid
+    and on a generic one (nx/2 = 48, ny = 48)."""
+    case = L.cases.CASES["c1_dns"].with_(name=f"isc{nx}x{ny}", nx=nx, ny=ny, nz=64, xnu=2e-5, bctype=0,
+                                         sgsflag=1, sgsmodel=3, dt=5e-4, pert_rms=1e-2)
+    check(run_pair(case, 20))
 
 
 def test_mixed_state_roundtrip_history():
