@@ -159,6 +159,22 @@ __global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
     }
 }
 
+// stage 3 at one point of plane k from the coefficient ratio ct = <M:L> / <M:M>
+template <class Off>
+__device__ __forceinline__ void dynm_stage3_point(const DynMixArgs& a, Off o, int k, double ct) {
+    if (ct < 0.0 || isnan(ct)) ct = 0.0;
+    if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
+    const double dl = a.delta[k];
+    const double nu = ct * dl * dl * (double)ldf(a.snorm, o);
+    const double su = 0.5 * a.ubarp[k];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double s = (double)ldf(a.sp[q], o) + (q == 2 ? su : 0.0);
+        const double tau = -2.0 * nu * s;
+        stf(a.h[q], o, (float)(0.5 * (double)ldf(a.h[q], o) + tau));  // h holds p on entry
+    }
+}
+
 template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage3(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,19 +183,153 @@ __global__ void __launch_bounds__(256) k_dynm_stage3(DDims d, DynMixArgs a) {
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
         const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
-        double ct = a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : (double)ldf(a.numf, o) / (double)ldf(a.denf, o);
-        if (ct < 0.0 || isnan(ct)) ct = 0.0;
-        if (ct > 0.2 * 0.2) ct = 0.2 * 0.2;
-        const double dl = a.delta[k];
-        const double nu = ct * dl * dl * (double)ldf(a.snorm, o);
-        const double su = 0.5 * a.ubarp[k];
+        const double ct =
+            a.cplane ? a.cplane[k] / a.cplane[d.nzp + k] : (double)ldf(a.numf, o) / (double)ldf(a.denf, o);
+        dynm_stage3_point(a, o, k, ct);
+    }
+}
+
+// The local (1, 1) average of M:L and M:M and stage 3 in one pass: k_filt3_lds's steps
+// (a TX x R tile with periodic halos global -> registers -> LDS, the x pass into a second
+// LDS array, the y pass, the z pass over three planes in registers, the next tile's loads
+// in flight) for num and den in turn on every plane, and at each output point the two
+// averages, rounded to FP32 as stored averages would be, go into stage 3.  Same
+// expressions and order as the filter followed by k_dynm_stage3: bit-identical to them.
+template <int R, int TX>
+__global__ void __launch_bounds__(256) k_dynm_avg3(DDims d, DynMixArgs A, const double* __restrict__ w, int KC) {
+    constexpr int NT = (R + 2) * (TX + 2);  // tile elements
+    constexpr int NL = (NT + 255) / 256;    // tile loads per thread
+    constexpr int NP = (R * TX) / 256;      // output points per thread
+    static_assert((R * TX) % 256 == 0, "R * TX must be a multiple of 256");
+    __shared__ float tile[NT];
+    __shared__ double xs[(R + 2) * TX];
+    const int ntx = (d.nx + TX - 1) / TX;
+    const int i0 = (blockIdx.x % ntx) * TX;
+    const int J0 = (blockIdx.x / ntx) * R;
+    const int k0 = blockIdx.y * KC;
+    const int k1 = min(k0 + KC, d.nzp);
+    const float* src[2] = {A.num, A.den};
+    const double a = 1.0, b = 1.0;
+    const double s = 2. * a + b;
+    const double s2 = 2.00 * a + b;
+    const bool p2 = pow2_exact(s);
+    const double is = 1.0 / s;
+    const long plane = (long)d.ny * d.nxpp;
+    int toff[NL];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double s = (double)ldf(a.sp[q], o) + (q == 2 ? su : 0.0);
-            const double tau = -2.0 * nu * s;
-            stf(a.h[q], o, (float)(0.5 * (double)ldf(a.h[q], o) + tau));  // h holds p on entry
+    for (int m = 0; m < NL; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        int off = -1;
+        if (e < NT) {
+            const int rr = e / (TX + 2), cc = e - rr * (TX + 2);
+            int j = (J0 - 1 + rr) % d.ny;
+            j = (j < 0) ? j + d.ny : j;
+            int i = (i0 - 1 + cc) % d.nx;
+            i = (i < 0) ? i + d.nx : i;
+            off = j * d.nxpp + i;
+        }
+        toff[m] = off;
+    }
+    float pre[NL];
+    auto load = [&](int f, int k) {
+        const float* qk = src[f] + (long)k * plane;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) pre[m] = (toff[m] >= 0) ? qk[toff[m]] : 0.f;
+    };
+    // xy-filtered values of the tile in `pre`; then the loads of (fn, kn) (kn >= 0) in flight
+    auto stage = [&](double (&y)[NP], int fn, int kn) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int e = threadIdx.x + 256 * m;
+            if (e < NT) tile[e] = pre[m];
+        }
+        __syncthreads();
+        if (kn >= 0) load(fn, kn);
+        for (int e = threadIdx.x; e < (R + 2) * TX; e += 256) {
+            const int rr = e / TX, c = e - rr * TX;
+            const float* t = tile + rr * (TX + 2) + c;
+            xs[e] = fdiv(a * (double)t[0] + b * (double)t[1] + a * (double)t[2], s, is, p2);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int p = threadIdx.x + 256 * m;
+            const int r = p / TX, c = p - r * TX;
+            y[m] = fdiv(a * xs[r * TX + c] + b * xs[(r + 1) * TX + c] + a * xs[(r + 2) * TX + c], s2, is, p2);
+        }
+    };
+    // planes k0 - 1 (if any) .. klast, num then den on each
+    const int klast = min(k1, d.nzp - 1);
+    const int kstart = (k0 > 0) ? k0 - 1 : k0;
+    double Yp[2][NP], Yc[2][NP], Yn[2][NP];
+    auto out = [&](int k, const double (&vn)[NP], const double (&vd)[NP]) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int p = threadIdx.x + 256 * m;
+            const int r = p / TX, c = p - r * TX;
+            const int j = J0 + r, i = i0 + c;
+            if (j < d.ny && i < d.nx) {
+                const unsigned o = (unsigned)(((k * d.ny + j) * d.nxpp + i) * (int)sizeof(float));
+                dynm_stage3_point(A, o, k, (double)(float)vn[m] / (double)(float)vd[m]);
+            }
+        }
+    };
+    load(0, kstart);
+    for (int kk = kstart; kk <= klast; ++kk) {
+        double y[2][NP];
+        stage(y[0], 1, kk);
+        stage(y[1], 0, (kk + 1 <= klast) ? kk + 1 : -1);
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                if (kk < k0)
+                    Yp[f][m] = y[f][m];
+                else if (kk == k0)
+                    Yc[f][m] = y[f][m];
+                else
+                    Yn[f][m] = y[f][m];
+            }
+        if (kk > k0) {
+            const int k = kk - 1;
+            const bool interior = (k > 0 && k < d.nzp - 1);
+            const double w0 = w[k], w1 = w[d.nzp + k], w2 = w[2 * d.nzp + k];
+            double v[2][NP];
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int m = 0; m < NP; ++m) {
+                    if (interior) {
+                        double t = w0 * Yp[f][m] + w1 * Yc[f][m] + w2 * Yn[f][m];
+                        v[f][m] = fdiv(t, s, is, p2);
+                    } else {
+                        v[f][m] = Yc[f][m];
+                    }
+                    Yp[f][m] = Yc[f][m];
+                    Yc[f][m] = Yn[f][m];
+                }
+            out(k, v[0], v[1]);
         }
     }
+    // the last plane of the grid keeps its xy-filtered value (no z pass)
+    if (k1 == d.nzp) out(d.nzp - 1, Yc[0], Yc[1]);
+}
+
+int ph_dyn_mixed_avg3(lsb_ctx* c, const DynMixArgs& a, hipStream_t s) {
+    const DDims d = ddims(c);
+    // 32-bit element offsets; the (1, 1) weights on the device
+    if ((double)d.ny * d.nzp * d.nxpp * sizeof(float) >= 2147483648.0 || a.cplane) return LSB_E_UNSUPPORTED;
+    const double* w = c->d_fz + 3 * c->nzp;  // the (1, 1) interpolating weights
+    constexpr int R = 4, TX = 128, KC = 16;
+    const int ntx = (d.nx + TX - 1) / TX;
+    const dim3 grid((unsigned)(ntx * ((d.ny + R - 1) / R)), (unsigned)((d.nzp + KC - 1) / KC));
+    int tok = -1;
+    // num, den read; snorm, sp (6), h (6) read; h (6) written
+    prof_mark_begin(c, PC_NL, s, &tok, 0.0, 21.0 * 4.0 * (double)d.nx * d.ny * d.nzp);
+    hipLaunchKernelGGL((k_dynm_avg3<R, TX>), grid, dim3(256), 0, s, d, a, w, KC);
+    prof_mark_end(c, tok, s);
+    LSB_HIP(hipGetLastError());
+    return LSB_OK;
 }
 
 // Plane means of two fields over the nx physical points of every row: B partial sums per

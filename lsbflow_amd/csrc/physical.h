@@ -1,9 +1,19 @@
 // Argument blocks and launchers of the physical-space kernels.
 #pragma once
 
+#include <cmath>
+
 #include "lsb_internal.h"
 
 namespace lsb {
+
+// x / s of the 3-point filter passes.  When s = 2a + b is a power of two, x * (1/s) is the
+// same correctly rounded value (1/s is exact), without the FP64 division sequence.
+__device__ __forceinline__ double fdiv(double x, double s, double inv, bool pow2) { return pow2 ? x * inv : x / s; }
+__host__ __device__ inline bool pow2_exact(double s) {
+    int e = 0;
+    return s > 0.0 && std::frexp(s, &e) == 0.5;
+}
 
 enum { NL_NONE = 0, NL_SIGMA = 1, NL_STATIC = 2, NL_EXTTAU = 3, NL_PREP = 4 };
 
@@ -77,6 +87,9 @@ struct DynMixArgs {
 
 int ph_zero_pads(lsb_ctx* c, double* const* f, int nf, hipStream_t s);
 int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s);
+// the (1, 1) average of num and den and stage 3 in one pass (numf, denf not written);
+// LSB_E_UNSUPPORTED where that pass does not apply (the caller runs the two steps)
+int ph_dyn_mixed_avg3(lsb_ctx* c, const DynMixArgs& a, hipStream_t s);
 // out[k] = mean of f0 over plane k, out[nzp + k] = mean of f1 (nx physical points per row);
 // scratch: ph_plane_mean2_scratch(c) doubles
 template <class T>

@@ -465,13 +465,6 @@ __global__ void k_filt_z(PhDims d, FieldList<double> in, FieldList<double> out, 
     }
 }
 
-// x / s of the filter passes.  When s = 2a + b is a power of two, x * (1/s) is the same
-// correctly rounded value (1/s is exact), without the FP64 division sequence.
-__device__ __forceinline__ double fdiv(double x, double s, double inv, bool pow2) { return pow2 ? x * inv : x / s; }
-__host__ __device__ inline bool pow2_exact(double s) {
-    int e = 0;
-    return s > 0.0 && std::frexp(s, &e) == 0.5;
-}
 
 // The same filter in one pass: block (R rows j, KC planes k, field) slides over k with the
 // xy-filtered values of planes k-1, k, k+1 in registers (threads over x, NXT points each), so

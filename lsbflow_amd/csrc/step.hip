@@ -20,6 +20,7 @@
 //   once and rescaled per step (its time dependence is an amplitude ramp).
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "lsb_internal.h"
@@ -685,6 +686,11 @@ static int dyn_mixed(lsb_ctx* c, StepState* S, hipStream_t s) {
         LSB_TRY(ph_plane_mean2<float>(c, a.num, a.den, scr, cp, s));
         a.cplane = cp;
     } else {
+        // the (1, 1) average and stage 3 in one pass; LSB_NO_DYNFUSE=1 (for tests) or
+        // fields above 2 GiB: the averages through HBM, then stage 3
+        const char* e = std::getenv("LSB_NO_DYNFUSE");
+        const int rf = (e && e[0] == '1') ? LSB_E_UNSUPPORTED : ph_dyn_mixed_avg3(c, a, s);
+        if (rf != LSB_E_UNSUPPORTED) return rf;
         const float* fi[2] = {a.num, a.den};
         float* fo[2] = {W(36), W(37)};
         LSB_TRY(ph_filter3_f32(c, fi, fo, 2, 1.0, 1.0, s));

@@ -124,6 +124,30 @@ def test_mixed_dynamic_small(dyn_avg):
     check(run_pair(case, 20))
 
 
+@pytest.mark.parametrize("nx,ny", [(128, 64), (96, 30), (512, 32)])
+def test_mixed_dynamic_fused_average_bit_identical(nx, ny, monkeypatch):
+    """The local (1, 1) average of M:L and M:M and stage 3 in one pass give the same bits as
+    the averages through HBM followed by stage 3 (LSB_NO_DYNFUSE=1): a register-resident
+    grid, a generic one whose tiles are partial in x and y (nx = 96, ny = 30), a grid of
+    four x tiles; 65 planes, so the last block of planes holds one."""
+    case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyna{nx}x{ny}", nx=nx, ny=ny, nz=64, dyn_avg=0, pert_rms=1e-2)
+    u, v, w = L.cases.initial_condition(case)
+    out = {}
+    for sep in ("0", "1"):
+        monkeypatch.setenv("LSB_NO_DYNFUSE", sep)
+        c = setup_case(case, precision=1)
+        try:
+            c.state_set(u, v, w)
+            c.timestep(10)
+            out[sep] = c.state_get()
+        finally:
+            c.close()
+    for k in VEL + ("pf", "nlnif", "nlnmif"):
+        assert np.all(np.isfinite(out["0"][k])), k
+        assert np.array_equal(out["0"][k], out["1"][k]), k
+    print("fused (1, 1) average + stage 3 bit-identical:", case.name, flush=True)
+
+
 @pytest.mark.parametrize("dealias", [2, 0])
 def test_mixed_temperature_buoyancy(dealias):
     """The temperature path in the mixed form: its chain (grad T, the products with the
