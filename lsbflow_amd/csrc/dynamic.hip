@@ -320,9 +320,14 @@ int ph_dyn_mixed_avg3(lsb_ctx* c, const DynMixArgs& a, hipStream_t s) {
     // 32-bit element offsets; the (1, 1) weights on the device
     if ((double)d.ny * d.nzp * d.nxpp * sizeof(float) >= 2147483648.0 || a.cplane) return LSB_E_UNSUPPORTED;
     const double* w = c->d_fz + 3 * c->nzp;  // the (1, 1) interpolating weights
-    constexpr int R = 4, TX = 128, KC = 16;
+    constexpr int R = 4, TX = 128;
     const int ntx = (d.nx + TX - 1) / TX;
-    const dim3 grid((unsigned)(ntx * ((d.ny + R - 1) / R)), (unsigned)((d.nzp + KC - 1) / KC));
+    const long tiles = (long)ntx * ((d.ny + R - 1) / R);
+    // 16 planes per block, fewer (each block recomputes two halo planes) until the grid
+    // holds some 2048 blocks (8 per CU)
+    int KC = 16;
+    while (KC > 4 && tiles * ((d.nzp + KC - 1) / KC) < 2048) KC /= 2;
+    const dim3 grid((unsigned)tiles, (unsigned)((d.nzp + KC - 1) / KC));
     int tok = -1;
     // num, den read; snorm, sp (6), h (6) read; h (6) written
     prof_mark_begin(c, PC_NL, s, &tok, 0.0, 21.0 * 4.0 * (double)d.nx * d.ny * d.nzp);

@@ -124,13 +124,14 @@ def test_mixed_dynamic_small(dyn_avg):
     check(run_pair(case, 20))
 
 
-@pytest.mark.parametrize("nx,ny", [(128, 64), (96, 30), (512, 32)])
-def test_mixed_dynamic_fused_average_bit_identical(nx, ny, monkeypatch):
+@pytest.mark.parametrize("nx,ny,nz", [(128, 64, 64), (96, 30, 64), (512, 32, 64), (512, 256, 128)])
+def test_mixed_dynamic_fused_average_bit_identical(nx, ny, nz, monkeypatch):
     """The local (1, 1) average of M:L and M:M and stage 3 in one pass give the same bits as
     the averages through HBM followed by stage 3 (LSB_NO_DYNFUSE=1): a register-resident
     grid, a generic one whose tiles are partial in x and y (nx = 96, ny = 30), a grid of
-    four x tiles; 65 planes, so the last block of planes holds one."""
-    case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyna{nx}x{ny}", nx=nx, ny=ny, nz=64, dyn_avg=0, pert_rms=1e-2)
+    four x tiles (these three with 4 planes per block and 65 planes, so the last block
+    holds one) and a 512 x 129 x 256 grid with 16 planes per block, as at C4."""
+    case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyna{nx}x{ny}", nx=nx, ny=ny, nz=nz, dyn_avg=0, pert_rms=1e-2)
     u, v, w = L.cases.initial_condition(case)
     out = {}
     for sep in ("0", "1"):
