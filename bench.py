@@ -119,6 +119,34 @@ def compare_forms(states, wall):
 COPY_BYTES = 1 << 30
 
 
+def copy_bandwidth(reps=5, per=20):
+    """The practical streaming ceiling of HBM on this GPU: a plain device-to-device copy
+    (torch copy_) of 1 GiB and of 4 GiB, bytes read + written over its event-timed duration."""
+    import torch
+
+    out = {"copy": "torch copy_, device to device, float64", "unit": "TB/s (read + written)"}
+    for size in (1 << 30, 1 << 32):
+        src = torch.ones(size // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        rates = []
+        for _ in range(reps):
+            e0.record()
+            for _ in range(per):
+                dst.copy_(src)
+            e1.record()
+            torch.cuda.synchronize()
+            rates.append(round(2.0 * size / (e0.elapsed_time(e1) / per * 1e-3) / 1e12, 3))
+        out[f"{size >> 30}_GiB"] = rates
+        del src, dst
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def pmc_probe(case):
     """Known-size copy, then one step of each form (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)."""
     import torch
@@ -611,6 +639,8 @@ def main(argv=None):
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the HBM-traffic table bench.py reads (profiles/..._hbm_traffic.json) from the "
                          "counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE runs of --pmc-probe")
+    ap.add_argument("--copy-bw", action="store_true",
+                    help="the bandwidth of a plain 1 GiB and 4 GiB device copy (the practical HBM ceiling)")
     ap.add_argument("--cpu-sample-ny", type=int, default=16)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--cpu-worker", nargs=3, metavar=("CASE", "NY", "STEPS"), help=argparse.SUPPRESS)
@@ -623,6 +653,8 @@ def main(argv=None):
         print("bench.py: the library runs the whole domain on one GPU; runs on several GPUs are not "
               "supported (use --gpus 1)", file=sys.stderr)
         return 2
+    if args.copy_bw:
+        return copy_bandwidth()
     import numpy as np
     import lsbflow_amd as L
     case = L.cases.CASES[args.case]
