@@ -22,6 +22,7 @@
 // (0,0), (1,0), (2,0), (1,1), (2,1), (2,2).
 
 #include <cmath>
+#include <cstdlib>
 
 #include "lsb_internal.h"
 #include "physical.h"
@@ -59,6 +60,59 @@ __device__ __forceinline__ void stf(float* p, Off bo, float v) {
 
 }  // namespace
 
+// stage 1 at one point of plane k (column i) from u' and g'; out: nlt (3), p (6), |S|,
+// S' (6), L1 (6) in that order
+__device__ __forceinline__ void dynm_stage1_math(const DynMixArgs& a, int k, int i, const double (&up)[3],
+                                                 const double (&g)[3][3], float (&out)[22]) {
+    const double U = a.ubar[k];
+    double u[3];
+    u[0] = up[0] + U;
+    u[1] = up[1];
+    u[2] = up[2];
+    // advective term -1/2 u_m du_l/dx_m with the full gradient (g'_02 + ubarp), sponge
+    const double g02 = g[0][2] + a.ubarp[k];
+    const double lam = a.lam ? a.lam[i] : 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * ((l == 0 && m == 2) ? g02 : g[l][m]);
+        if (a.lam) s = s - lam * up[l];  // lambda (target - u): target = ubar for u, 0 for v, w
+        out[l] = (float)s;
+    }
+    // perturbation products: u_l u_m - U_l U_m (only U_0 = ubar is nonzero)
+    out[3] = (float)(up[0] * (u[0] + U));
+    out[4] = (float)(up[1] * u[0]);
+    out[5] = (float)(up[2] * u[0]);
+    out[6] = (float)(up[1] * up[1]);
+    out[7] = (float)(up[2] * up[1]);
+    out[8] = (float)(up[2] * up[2]);
+    // strain of the perturbation and of the full field
+    double sp[6], sn = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sp[q] = 0.5 * (g[kL[q]][kM[q]] + g[kM[q]][kL[q]]);
+    const double su = 0.5 * a.ubarp[k];  // base strain S^U_20 = S^U_02
+    double sf[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        sf[q] = sp[q] + (q == 2 ? su : 0.0);
+        const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;  // off-diagonals appear twice
+        sn = sn + w * 2.0 * sf[q] * sf[q];
+    }
+    sn = sqrt(sn);
+    const double dl = a.delta[k];
+    out[9] = (float)sn;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        out[10 + q] = (float)sp[q];
+        out[16 + q] = (float)(dl * dl * sn * sf[q]);
+    }
+}
+
+__device__ __forceinline__ float* dynm_stage1_dst(const DynMixArgs& a, int n) {
+    return n < 3 ? a.nlt[n] : n < 9 ? a.p[n - 3] : n == 9 ? a.snorm : n < 16 ? a.sp[n - 10] : a.l1[n - 16];
+}
+
 template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,55 +121,54 @@ __global__ void __launch_bounds__(256) k_dynm_stage1(DDims d, DynMixArgs a) {
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
         const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
-        const double U = a.ubar[k];
-        double up[3], u[3], g[3][3];
+        double up[3], g[3][3];
 #pragma unroll
         for (int l = 0; l < 3; ++l) up[l] = (double)ldf(a.ui[l], o);
-        u[0] = up[0] + U;
-        u[1] = up[1];
-        u[2] = up[2];
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
             for (int m = 0; m < 3; ++m) g[l][m] = (double)ldf(a.g[l + 3 * m], o);
-        // advective term -1/2 u_m du_l/dx_m with the full gradient (g'_02 + ubarp), sponge
-        const double g02 = g[0][2] + a.ubarp[k];
-        const double lam = a.lam ? a.lam[i] : 0.0;
+        float out[22];
+        dynm_stage1_math(a, k, i, up, g, out);
+#pragma unroll
+        for (int n = 0; n < 22; ++n) stf(dynm_stage1_dst(a, n), o, out[n]);
+    }
+}
+
+// The same with two adjacent x points per thread: 8-byte loads and stores (nx and nxpp
+// even, so every pair is 8-byte aligned), the per-point expressions unchanged.
+__global__ void __launch_bounds__(256) k_dynm_stage1x2(DDims d, DynMixArgs a) {
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const unsigned o = ((unsigned)row * (unsigned)d.nxpp + (unsigned)i) * (unsigned)sizeof(float);
+        auto ld2 = [&](const float* p) {
+            return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(p) + o);
+        };
+        double up[2][3], g[2][3][3];
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) s = s - 0.5 * u[m] * ((l == 0 && m == 2) ? g02 : g[l][m]);
-            if (a.lam) s = s - lam * up[l];  // lambda (target - u): target = ubar for u, 0 for v, w
-            stf(a.nlt[l], o, (float)s);
+            const float2 v = ld2(a.ui[l]);
+            up[0][l] = (double)v.x;
+            up[1][l] = (double)v.y;
         }
-        // perturbation products: u_l u_m - U_l U_m (only U_0 = ubar is nonzero)
-        stf(a.p[0], o, (float)(up[0] * (u[0] + U)));
-        stf(a.p[1], o, (float)(up[1] * u[0]));
-        stf(a.p[2], o, (float)(up[2] * u[0]));
-        stf(a.p[3], o, (float)(up[1] * up[1]));
-        stf(a.p[4], o, (float)(up[2] * up[1]));
-        stf(a.p[5], o, (float)(up[2] * up[2]));
-        // strain of the perturbation and of the full field
-        double sp[6], sn = 0.0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sp[q] = 0.5 * (g[kL[q]][kM[q]] + g[kM[q]][kL[q]]);
-        const double su = 0.5 * a.ubarp[k];  // base strain S^U_20 = S^U_02
-        double sf[6];
+        for (int l = 0; l < 3; ++l)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            sf[q] = sp[q] + (q == 2 ? su : 0.0);
-            const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;  // off-diagonals appear twice
-            sn = sn + w * 2.0 * sf[q] * sf[q];
-        }
-        sn = sqrt(sn);
-        const double dl = a.delta[k];
-        stf(a.snorm, o, (float)sn);
+            for (int m = 0; m < 3; ++m) {
+                const float2 v = ld2(a.g[l + 3 * m]);
+                g[0][l][m] = (double)v.x;
+                g[1][l][m] = (double)v.y;
+            }
+        float out[2][22];
+        dynm_stage1_math(a, k, i, up[0], g[0], out[0]);
+        dynm_stage1_math(a, k, i + 1, up[1], g[1], out[1]);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            stf(a.sp[q], o, (float)sp[q]);
-            stf(a.l1[q], o, (float)(dl * dl * sn * sf[q]));
-        }
+        for (int n = 0; n < 22; ++n)
+            *reinterpret_cast<float2*>(reinterpret_cast<char*>(dynm_stage1_dst(a, n)) + o) =
+                make_float2(out[0][n], out[1][n]);
     }
 }
 
@@ -399,6 +452,12 @@ template int ph_plane_mean2<float>(lsb_ctx*, const float*, const float*, double*
 
 size_t ph_plane_mean2_scratch(const lsb_ctx* c) { return (size_t)c->nzp * kPlaneBlocks * 2; }
 
+// LSB_NO_DYNX2=1: stage 1 one point per thread (for tests)
+static bool stage1_single() {
+    const char* e = std::getenv("LSB_NO_DYNX2");
+    return e && e[0] == '1';
+}
+
 int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s) {
     const DDims d = ddims(c);
     const dim3 g = row_grid(d);
@@ -410,7 +469,10 @@ int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s) {
     const bool small = (double)d.ny * d.nzp * d.nxpp * sizeof(float) < 4294967296.0;
     switch (stage) {
         case 1:
-            if (small)
+            if (small && (d.nx % 2) == 0 && (d.nxpp % 2) == 0 && !stage1_single()) {
+                const dim3 g2((unsigned)((d.nx / 2 + 255) / 256), g.y);
+                hipLaunchKernelGGL(k_dynm_stage1x2, g2, dim3(256), 0, s, d, a);
+            } else if (small)
                 hipLaunchKernelGGL(k_dynm_stage1<unsigned>, g, dim3(256), 0, s, d, a);
             else
                 hipLaunchKernelGGL(k_dynm_stage1<size_t>, g, dim3(256), 0, s, d, a);

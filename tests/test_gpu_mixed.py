@@ -149,6 +149,31 @@ def test_mixed_dynamic_fused_average_bit_identical(nx, ny, nz, monkeypatch):
     print("fused (1, 1) average + stage 3 bit-identical:", case.name, flush=True)
 
 
+@pytest.mark.parametrize("nx,ny,forcing", [(128, 64, 0), (96, 30, 1)])
+def test_mixed_dynamic_stage1_pairs_bit_identical(nx, ny, forcing, monkeypatch):
+    """Stage 1 of the mixed dynamic model with two x points per thread (8-byte loads and
+    stores) gives the same bits as one point per thread (LSB_NO_DYNX2=1), with and
+    without the sponge term."""
+    case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyn1x{nx}x{ny}", nx=nx, ny=ny, nz=64, dyn_avg=0, pert_rms=1e-2)
+    u, v, w = L.cases.initial_condition(case)
+    out = {}
+    for single in ("0", "1"):
+        monkeypatch.setenv("LSB_NO_DYNX2", single)
+        c = setup_case(case, precision=1, forcing=forcing)
+        try:
+            if forcing:
+                c.set_sponge(0.3, 0.25, 0.70)
+            c.state_set(u, v, w)
+            c.timestep(10)
+            out[single] = c.state_get()
+        finally:
+            c.close()
+    for k in VEL + ("pf", "nlnif", "nlnmif"):
+        assert np.all(np.isfinite(out["0"][k])), k
+        assert np.array_equal(out["0"][k], out["1"][k]), k
+    print("stage 1, two points per thread, bit-identical:", case.name, "forcing", forcing, flush=True)
+
+
 @pytest.mark.parametrize("dealias", [2, 0])
 def test_mixed_temperature_buoyancy(dealias):
     """The temperature path in the mixed form: its chain (grad T, the products with the
