@@ -172,6 +172,45 @@ __global__ void __launch_bounds__(256) k_dynm_stage1x2(DDims d, DynMixArgs a) {
     }
 }
 
+// stage 2 at one point of plane k from its 21 filtered values x: S' (6), L1 (6), u' (3),
+// p (6); returns M:L and M:M
+__device__ __forceinline__ void dynm_stage2_math(const DynMixArgs& a, int nzp, int k, const double (&x)[21],
+                                                 float& num_o, float& den_o) {
+    const double Ub = a.base[k];
+    const double SUh = a.base[nzp + k];
+    const double LU = a.base[2 * nzp + k];
+    double sh[6], snh = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        sh[q] = x[q] + (q == 2 ? SUh : 0.0);
+        const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;
+        snh = snh + w * 2.0 * sh[q] * sh[q];
+    }
+    snh = sqrt(snh);
+    const double cd = 2.0 * a.delta[k];
+    double uh[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) uh[l] = x[12 + l];
+    const double Uh[3] = {Ub, 0.0, 0.0};
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int l = kL[q], m = kM[q];
+        const double mij = 2.0 * (x[6 + q] - cd * cd * snh * sh[q]);
+        double lij = x[15 + q] - (Uh[l] * uh[m] + uh[l] * Uh[m] + uh[l] * uh[m]);
+        if (q == 0) lij = lij + LU;
+        const double w = (l == m) ? 1.0 : 2.0;
+        num = num + w * mij * lij;
+        den = den + w * mij * mij;
+    }
+    num_o = (float)num;
+    den_o = (float)den;
+}
+
+__device__ __forceinline__ const float* dynm_stage2_src(const DynMixArgs& a, int n) {
+    return n < 6 ? a.sph[n] : n < 12 ? a.l1h[n - 6] : n < 15 ? a.uh[n - 12] : a.ph[n - 15];
+}
+
 template <class Off>
 __global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,35 +219,36 @@ __global__ void __launch_bounds__(256) k_dynm_stage2(DDims d, DynMixArgs a) {
     for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
         const int k = row / d.ny;
         const Off o = ((Off)row * (Off)d.nxpp + (Off)i) * (Off)sizeof(float);
-        const double Ub = a.base[k];
-        const double SUh = a.base[d.nzp + k];
-        const double LU = a.base[2 * d.nzp + k];
-        double sh[6], snh = 0.0;
+        double x[21];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            sh[q] = (double)ldf(a.sph[q], o) + (q == 2 ? SUh : 0.0);
-            const double w = (kL[q] == kM[q]) ? 1.0 : 2.0;
-            snh = snh + w * 2.0 * sh[q] * sh[q];
+        for (int n = 0; n < 21; ++n) x[n] = (double)ldf(dynm_stage2_src(a, n), o);
+        float num, den;
+        dynm_stage2_math(a, d.nzp, k, x, num, den);
+        stf(a.num, o, num);
+        stf(a.den, o, den);
+    }
+}
+
+// The same with two adjacent x points per thread (8-byte loads and stores).
+__global__ void __launch_bounds__(256) k_dynm_stage2x2(DDims d, DynMixArgs a) {
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= d.nx) return;
+    const int nrows = d.ny * d.nzp;
+    for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+        const int k = row / d.ny;
+        const unsigned o = ((unsigned)row * (unsigned)d.nxpp + (unsigned)i) * (unsigned)sizeof(float);
+        double x[2][21];
+#pragma unroll
+        for (int n = 0; n < 21; ++n) {
+            const float2 v = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(dynm_stage2_src(a, n)) + o);
+            x[0][n] = (double)v.x;
+            x[1][n] = (double)v.y;
         }
-        snh = sqrt(snh);
-        const double cd = 2.0 * a.delta[k];
-        double uh[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) uh[l] = (double)ldf(a.uh[l], o);
-        const double Uh[3] = {Ub, 0.0, 0.0};
-        double num = 0.0, den = 0.0;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int l = kL[q], m = kM[q];
-            const double mij = 2.0 * ((double)ldf(a.l1h[q], o) - cd * cd * snh * sh[q]);
-            double lij = (double)ldf(a.ph[q], o) - (Uh[l] * uh[m] + uh[l] * Uh[m] + uh[l] * uh[m]);
-            if (q == 0) lij = lij + LU;
-            const double w = (l == m) ? 1.0 : 2.0;
-            num = num + w * mij * lij;
-            den = den + w * mij * mij;
-        }
-        stf(a.num, o, (float)num);
-        stf(a.den, o, (float)den);
+        float num[2], den[2];
+        dynm_stage2_math(a, d.nzp, k, x[0], num[0], den[0]);
+        dynm_stage2_math(a, d.nzp, k, x[1], num[1], den[1]);
+        *reinterpret_cast<float2*>(reinterpret_cast<char*>(a.num) + o) = make_float2(num[0], num[1]);
+        *reinterpret_cast<float2*>(reinterpret_cast<char*>(a.den) + o) = make_float2(den[0], den[1]);
     }
 }
 
@@ -452,7 +492,7 @@ template int ph_plane_mean2<float>(lsb_ctx*, const float*, const float*, double*
 
 size_t ph_plane_mean2_scratch(const lsb_ctx* c) { return (size_t)c->nzp * kPlaneBlocks * 2; }
 
-// LSB_NO_DYNX2=1: stage 1 one point per thread (for tests)
+// LSB_NO_DYNX2=1: stages 1 and 2 one point per thread (for tests)
 static bool stage1_single() {
     const char* e = std::getenv("LSB_NO_DYNX2");
     return e && e[0] == '1';
@@ -478,7 +518,10 @@ int ph_dyn_mixed(lsb_ctx* c, int stage, const DynMixArgs& a, hipStream_t s) {
                 hipLaunchKernelGGL(k_dynm_stage1<size_t>, g, dim3(256), 0, s, d, a);
             break;
         case 2:
-            if (small)
+            if (small && (d.nx % 2) == 0 && (d.nxpp % 2) == 0 && !stage1_single()) {
+                const dim3 g2((unsigned)((d.nx / 2 + 255) / 256), g.y);
+                hipLaunchKernelGGL(k_dynm_stage2x2, g2, dim3(256), 0, s, d, a);
+            } else if (small)
                 hipLaunchKernelGGL(k_dynm_stage2<unsigned>, g, dim3(256), 0, s, d, a);
             else
                 hipLaunchKernelGGL(k_dynm_stage2<size_t>, g, dim3(256), 0, s, d, a);

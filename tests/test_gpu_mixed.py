@@ -151,8 +151,8 @@ def test_mixed_dynamic_fused_average_bit_identical(nx, ny, nz, monkeypatch):
 
 @pytest.mark.parametrize("nx,ny,forcing", [(128, 64, 0), (96, 30, 1)])
 def test_mixed_dynamic_stage1_pairs_bit_identical(nx, ny, forcing, monkeypatch):
-    """Stage 1 of the mixed dynamic model with two x points per thread (8-byte loads and
-    stores) gives the same bits as one point per thread (LSB_NO_DYNX2=1), with and
+    """Stages 1 and 2 of the mixed dynamic model with two x points per thread (8-byte loads
+    and stores) give the same bits as one point per thread (LSB_NO_DYNX2=1), with and
     without the sponge term."""
     case = L.cases.CASES["c2_dyn_les"].with_(name=f"dyn1x{nx}x{ny}", nx=nx, ny=ny, nz=64, dyn_avg=0, pert_rms=1e-2)
     u, v, w = L.cases.initial_condition(case)
@@ -171,7 +171,7 @@ def test_mixed_dynamic_stage1_pairs_bit_identical(nx, ny, forcing, monkeypatch):
     for k in VEL + ("pf", "nlnif", "nlnmif"):
         assert np.all(np.isfinite(out["0"][k])), k
         assert np.array_equal(out["0"][k], out["1"][k]), k
-    print("stage 1, two points per thread, bit-identical:", case.name, "forcing", forcing, flush=True)
+    print("stages 1 and 2, two points per thread, bit-identical:", case.name, "forcing", forcing, flush=True)
 
 
 @pytest.mark.parametrize("dealias", [2, 0])
